@@ -1,0 +1,136 @@
+/*
+ * mhd_mi355x.h -- C ABI of the MI355X (gfx950) 2D resistive-MHD step.
+ *
+ * Drop-in replacement for the step interface of the reference solver
+ * (RikibabManos/Magnetohydrodynamics_Simulator; citations relative to its
+ * repository).  The reference exposes header-only C++ free functions that
+ * mutate caller-owned std::vector<double> fields of a padded `Grid`
+ * (include/Grid.hpp:6-62).  This ABI keeps that padded row-major layout at the
+ * boundary -- every host array is (ny + 4) x (nx + 4) doubles, index
+ * j * (nx + 4) + i, two ghost layers (Grid.hpp:29-58) -- so a host Grid's
+ * .data() pointers pass straight through.
+ *
+ * Conventions
+ *   - every entry point returns 0 (MHD_OK) or a positive MHD_E* code;
+ *     mhd_last_error(ctx) describes the last failure on that context
+ *     (mhd_last_error(NULL) the last failed mhd_create on this thread);
+ *   - the library owns all device memory and never keeps a host pointer
+ *     after a call returns;
+ *   - a context is used from one host thread at a time (the reference is
+ *     single-threaded and non-reentrant as well);
+ *   - all arithmetic is IEEE binary64 on the GPU; there is no CPU fallback:
+ *     without a usable gfx950 device mhd_create fails with MHD_EHIP.
+ */
+#ifndef MHD_MI355X_H
+#define MHD_MI355X_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MHD_OK 0
+#define MHD_EINVAL 1       /* bad argument (size, pointer, option) */
+#define MHD_EHIP 2         /* HIP runtime error or no device */
+#define MHD_ENONFINITE 3   /* a non-finite value was produced (CFL speed) */
+#define MHD_ECG 4          /* CG reached max iterations (only if opts.cg_fail_is_error) */
+#define MHD_ENOMEM 5       /* device allocation failed */
+
+typedef struct mhd_ctx mhd_ctx;
+
+/* Solver options.  Defaults reproduce the reference's hard-coded constants. */
+typedef struct mhd_opts {
+  double gamma;            /* adiabatic index; reference: 5/3 (Grid.hpp:104, HLL.hpp:31) -- informational,
+                              the kernels are compiled for 5/3 and mhd_create rejects other values */
+  double cfl;              /* effective Courant number; reference 0.5 * 0.9 = 0.45 (diagnostics.hpp:82-84) */
+  double cg_tol;           /* relative CG tolerance; reference: Eigen default DBL_EPSILON */
+  long long cg_max_iter;   /* <= 0: 2 N, Eigen's default (N = nx * ny corners) */
+  int cg_fail_is_error;    /* 0 (reference): a non-converged solve is reported only in the stats */
+  int march_rows;          /* rows per work item of the explicit-stage kernel; 0 = auto */
+} mhd_opts;
+
+/* Per-step statistics (the reference computes none of these; CG failure is
+ * silent there, solvers.hpp:469). */
+typedef struct mhd_step_stats {
+  int cg_iters[2];         /* Eigen iteration count of the two implicit solves */
+  int cg_spmv[2];          /* operator applications actually performed */
+  double cg_error[2];      /* sqrt(|r|^2 / |b|^2) at exit (Eigen's tol_error) */
+  double dt;               /* the dt the step used */
+} mhd_step_stats;
+
+/* Fills *opts with the defaults above. */
+int mhd_default_opts(mhd_opts* opts);
+
+/* Creates a context for an nx x ny periodic grid with cell sizes dx, dy and
+ * resistivity eta (Grid ctor, Grid.hpp:29-54; cell_width/cell_height/
+ * resistivity as set by src/integrator_main.cpp:37-39).  opts may be NULL. */
+int mhd_create(int nx, int ny, double dx, double dy, double eta, const mhd_opts* opts, mhd_ctx** out);
+int mhd_destroy(mhd_ctx* ctx);
+const char* mhd_last_error(const mhd_ctx* ctx);
+
+/* Copies the conserved state from padded host arrays (ghosts ignored) and
+ * applies the density floor, as the driver's first fillGhostPeriodic +
+ * computePrimitives does (src/integrator_main.cpp:183-184, Grid.hpp:115). */
+int mhd_upload(mhd_ctx* ctx, const double* rho, const double* mx, const double* my,
+               const double* E, const double* bxf, const double* byf);
+
+/* Copies the state back into padded host arrays with ghosts filled
+ * periodically (Grid::fillGhostPeriodic, Grid.hpp:64-80).  Any pointer may be
+ * NULL.  vx, vy, p, bxc, byc are the primitives of Grid::computePrimitives
+ * (Grid.hpp:100-136), computed on the device. */
+int mhd_download(mhd_ctx* ctx, double* rho, double* mx, double* my, double* E,
+                 double* bxf, double* byf, double* vx, double* vy, double* p,
+                 double* bxc, double* byc);
+
+/* CFL_condition_checker (include/diagnostics.hpp:44-88) on the current state,
+ * with the Courant number of opts.cfl. */
+int mhd_cfl_dt(mhd_ctx* ctx, double* dt);
+
+/* generate_A_matrix + solver.compute (include/solvers.hpp:404-443,
+ * src/integrator_main.cpp:189-191): sets alpha of the operator I - alpha L.
+ * The driver passes alpha = dt * eta * gamma_ARS. */
+int mhd_build_operator(mhd_ctx* ctx, double alpha);
+
+/* IMEX (include/solvers.hpp:491-613): one ARS(2,2,2) step of size dt.  The
+ * implicit solves use the operator of the last mhd_build_operator call, as
+ * the reference's solver does.  stats may be NULL. */
+int mhd_imex_step(mhd_ctx* ctx, double dt, mhd_step_stats* stats);
+
+/* Unit entry points for parity tests. */
+/* explicit_rhs (solvers.hpp:393-402): dU/dt of the current state into padded
+ * arrays (physical cells written, ghosts left untouched). */
+int mhd_explicit_rhs(mhd_ctx* ctx, double* drho, double* dmx, double* dmy, double* dE,
+                     double* dbxf, double* dbyf);
+/* resistive_B_term (solvers.hpp:445-489) of the current state's face B.
+ * implicit != 0 solves with the operator of mhd_build_operator.  iters may be NULL. */
+int mhd_resistive_term(mhd_ctx* ctx, int implicit, double* bx_rate, double* by_rate, int* iters);
+/* The CG solve alone (Eigen ConjugateGradient::solve, solvers.hpp:469) on a
+ * compact nx*ny right-hand side. */
+int mhd_cg_solve(mhd_ctx* ctx, const double* b, double* x, int* iters, double* error);
+/* calculate_current_div_B (include/diagnostics.hpp:15-42). */
+int mhd_divb(mhd_ctx* ctx, double* rms, double* maxabs);
+
+/* Driver loop (src/integrator_main.cpp:183-243): CFL time step (or the
+ * constant dt_override when it is non-zero), the clamp of the last step to
+ * t_end, the operator rebuild when dt changes by more than 1e-10 relative,
+ * then IMEX; stops at t_end or after max_steps steps (max_steps < 0: no
+ * limit).  step_dt / cg_iters (2 per step) may be NULL, else hold max_steps
+ * entries.  *t_out receives the final time. */
+int mhd_run(mhd_ctx* ctx, double t_end, double dt_override, int max_steps,
+            int* steps_out, double* t_out, double* step_dt, int* cg_iters);
+
+/* Instrumentation (no reference counterpart).  With profiling on, each
+ * kernel launch of the context is bracketed by HIP events on its stream;
+ * mhd_profile_get returns launch count and summed milliseconds per kernel id.
+ * Kernel ids: 0 stage1, 1 stage2, 2 cg_init, 3 cg_pass_a, 4 cg_pass_b,
+ * 5 implicit_finalize1, 6 implicit_finalize2, 7 cfl, 8 other. */
+int mhd_profile_enable(mhd_ctx* ctx, int on);
+int mhd_profile_get(mhd_ctx* ctx, int kernel_id, long long* launches, double* total_ms);
+/* Bytes-calibration helper for hardware counters: one plain device copy
+ * kernel of nbytes (rounded down to doubles), run synchronously. */
+int mhd_calibration_copy(mhd_ctx* ctx, long long nbytes);
+int mhd_synchronize(mhd_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,709 @@
+// mhd_capi.hip -- the C ABI (include/mhd_mi355x.h) over the gfx950 kernels.
+//
+// Host-side orchestration of one ARS(2,2,2) step (include/solvers.hpp:491-613
+// of the reference) on device-resident compact fields:
+//   stage 1       U -> f1 = L(U), S = U + dt g f1                 (1 kernel)
+//   implicit 1    CG on eta J(S.B), S.B += dt g rate(x)           (init, 2/iter, 1)
+//   stage 2       U <- (U + dt d f1) + dt (1-d) L(S) [+ dt (1-g) g2(S)]   (1 kernel)
+//   implicit 2    CG on eta J(U.B), B^{n+1} = U.B + dt g g3(B3)   (init, 2/iter, 1)
+// with g = (2 - sqrt 2)/2 and d = 1 - 1/(2g) (solvers.hpp:41-42).  The stage-1
+// explicit resistive term g1 has weight 0 in every combination
+// (A_im[3] = A_im[6] = b_im[0] = 0) and is not evaluated.
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/mhd_mi355x.h"
+#include "mhd_kernels.hpp"
+
+using namespace mhd;
+
+namespace {
+
+constexpr int kNumKernelIds = 9;
+
+thread_local std::string g_create_error;
+
+struct Tableau {
+  // IMEX_Butcher_Tableus, solvers.hpp:39-66
+  double gamma = (2.0 - std::sqrt(2.0)) / 2.0;
+  double delta = 1.0 - (1.0 / (2.0 * gamma));
+  double A_ex3 = gamma, A_ex6 = delta, A_ex7 = 1.0 - delta;
+  double A_im4 = gamma, A_im7 = 1.0 - gamma;
+  double b_im2 = gamma;
+};
+
+}  // namespace
+
+struct mhd_ctx {
+  int nx = 0, ny = 0;
+  long long n = 0;
+  double dx = 0, dy = 0, eta = 0;
+  mhd_opts o{};
+  Tableau tab;
+  hipStream_t stream = nullptr;
+  double* U[6] = {};
+  double* S[6] = {};
+  double* F1[6] = {};
+  double* x = nullptr;
+  double* r = nullptr;
+  double* p[2] = {};
+  double* part = nullptr;
+  CgState* st = nullptr;
+  unsigned int* counter = nullptr;
+  double* dout = nullptr;
+  CgState* st_host = nullptr;    // pinned
+  double* out_host = nullptr;    // pinned
+  double alpha = 0.0;
+  bool have_op = false;
+  int last_spmv = 1;
+  int red_blocks = 1;
+  int stage_rows = 16;
+  std::string err;
+  bool prof = false;
+  long long prof_n[kNumKernelIds] = {};
+  double prof_ms[kNumKernelIds] = {};
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> pending[kNumKernelIds];
+};
+
+namespace {
+
+int fail(mhd_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  else g_create_error = msg;
+  return code;
+}
+
+#define HIP_OR_FAIL(ctx, call)                                                                  \
+  do {                                                                                          \
+    hipError_t e_ = (call);                                                                     \
+    if (e_ != hipSuccess)                                                                       \
+      return fail((ctx), MHD_EHIP, std::string(#call) + ": " + hipGetErrorString(e_));          \
+  } while (0)
+
+// Launch helper: optional event bracketing for per-kernel timing.
+template <typename F>
+int launch(mhd_ctx* c, int id, F&& body) {
+  hipEvent_t a = nullptr, b = nullptr;
+  if (c->prof) {
+    HIP_OR_FAIL(c, hipEventCreate(&a));
+    HIP_OR_FAIL(c, hipEventCreate(&b));
+    HIP_OR_FAIL(c, hipEventRecord(a, c->stream));
+  }
+  body();
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(c, MHD_EHIP, std::string("kernel launch: ") + hipGetErrorString(e));
+  if (c->prof) {
+    HIP_OR_FAIL(c, hipEventRecord(b, c->stream));
+    c->pending[id].emplace_back(a, b);
+  }
+  return MHD_OK;
+}
+
+int flush_profile(mhd_ctx* c) {
+  HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
+  for (int k = 0; k < kNumKernelIds; ++k) {
+    for (auto& pr : c->pending[k]) {
+      float ms = 0.f;
+      HIP_OR_FAIL(c, hipEventElapsedTime(&ms, pr.first, pr.second));
+      c->prof_ms[k] += ms;
+      c->prof_n[k] += 1;
+      (void)hipEventDestroy(pr.first);
+      (void)hipEventDestroy(pr.second);
+    }
+    c->pending[k].clear();
+  }
+  return MHD_OK;
+}
+
+int grid_1d(long long n) {
+  long long b = (n + 255) / 256;
+  if (b > 8192) b = 8192;
+  if (b < 1) b = 1;
+  return (int)b;
+}
+
+StageArgs stage_args(mhd_ctx* c) {
+  StageArgs a{};
+  a.nx = c->nx;
+  a.ny = c->ny;
+  a.rows = c->stage_rows;
+  a.dx = c->dx;
+  a.dy = c->dy;
+  a.inv_dx = 1.0 / c->dx;
+  a.inv_dy = 1.0 / c->dy;
+  a.eta = c->eta;
+  return a;
+}
+
+int run_stage(mhd_ctx* c, int mode, const StageArgs& a, int id) {
+  dim3 grid((c->nx + kStripOut - 1) / kStripOut, (c->ny + a.rows - 1) / a.rows);
+  return launch(c, id, [&] {
+    if (mode == 0) hipLaunchKernelGGL(k_stage<0>, grid, dim3(64), 0, c->stream, a);
+    else if (mode == 1) hipLaunchKernelGGL(k_stage<1>, grid, dim3(64), 0, c->stream, a);
+    else hipLaunchKernelGGL(k_stage<2>, grid, dim3(64), 0, c->stream, a);
+  });
+}
+
+CgArgs cg_args(mhd_ctx* c, const double* bx, const double* by) {
+  CgArgs a{};
+  a.nx = c->nx;
+  a.ny = c->ny;
+  a.dx = c->dx;
+  a.dy = c->dy;
+  a.eta = c->eta;
+  const double cx = c->alpha / (c->dx * c->dx);
+  const double cy = c->alpha / (c->dy * c->dy);
+  a.cx = cx;
+  a.cy = cy;
+  a.diag = 1 + 2 * (cx + cy);
+  a.invd = 1.0 / a.diag;
+  a.tol = c->o.cg_tol;
+  a.bx = bx;
+  a.by = by;
+  a.r = c->r;
+  a.p0 = c->p[0];
+  a.x = c->x;
+  a.part = c->part;
+  a.st = c->st;
+  return a;
+}
+
+// One conjugate-gradient solve; the right-hand side is eta J(bx, by) or bin.
+// Passes are enqueued in chunks and the device-side done flag is read back
+// between chunks; passes launched after convergence return immediately.
+int cg_run(mhd_ctx* c, const double* bx, const double* by, const double* bin, CgState* out) {
+  if (!c->have_op) return fail(c, MHD_EINVAL, "implicit solve before mhd_build_operator");
+  CgArgs a = cg_args(c, bx, by);
+  const long long maxit = (c->o.cg_max_iter > 0) ? c->o.cg_max_iter : 2LL * c->n;
+  *c->st_host = CgState{};
+  c->st_host->max_iters = maxit;
+  HIP_OR_FAIL(c, hipMemcpyAsync(c->st, c->st_host, sizeof(CgState), hipMemcpyHostToDevice, c->stream));
+  const int nb = c->red_blocks;
+  int rc = launch(c, 2, [&] { hipLaunchKernelGGL(k_cg_init, dim3(nb), dim3(kRedThreads), 0, c->stream, a, bin, 0); });
+  if (rc) return rc;
+  long long it = 0;
+  for (;;) {
+    int chunk = (it == 0) ? (c->last_spmv > 0 ? c->last_spmv : 1) : 2;
+    for (int k = 0; k < chunk; ++k, ++it) {
+      if (it == 0) {
+        rc = launch(c, 3, [&] {
+          hipLaunchKernelGGL(k_cg_pass_a<true>, dim3(nb), dim3(kRedThreads), 0, c->stream, a, c->p[0], c->p[1]);
+        });
+        if (rc) return rc;
+        rc = launch(c, 4, [&] {
+          hipLaunchKernelGGL(k_cg_pass_b<true>, dim3(nb), dim3(kRedThreads), 0, c->stream, a, (const double*)c->p[0]);
+        });
+      } else {
+        double* pold = c->p[(it - 1) & 1];
+        double* pnew = c->p[it & 1];
+        rc = launch(c, 3, [&] {
+          hipLaunchKernelGGL(k_cg_pass_a<false>, dim3(nb), dim3(kRedThreads), 0, c->stream, a, (const double*)pold, pnew);
+        });
+        if (rc) return rc;
+        rc = launch(c, 4, [&] {
+          hipLaunchKernelGGL(k_cg_pass_b<false>, dim3(nb), dim3(kRedThreads), 0, c->stream, a, (const double*)pnew);
+        });
+      }
+      if (rc) return rc;
+    }
+    HIP_OR_FAIL(c, hipMemcpyAsync(c->st_host, c->st, sizeof(CgState), hipMemcpyDeviceToHost, c->stream));
+    HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
+    if (c->st_host->done) break;
+    if (it > maxit + 4) return fail(c, MHD_EHIP, "CG state machine did not terminate");
+  }
+  *out = *c->st_host;
+  if (out->spmv > 0) c->last_spmv = out->spmv;
+  if (out->done == 2 && c->o.cg_fail_is_error) return fail(c, MHD_ECG, "CG reached the iteration limit");
+  return MHD_OK;
+}
+
+double cg_error(const CgState& s) {
+  if (s.done == 3) return 0.0;
+  return std::sqrt(s.rn2 / s.rhs_norm2);
+}
+
+FinArgs fin_args(mhd_ctx* c, const double* bx, const double* by, double* bxo, double* byo, double coef) {
+  FinArgs f{};
+  f.nx = c->nx;
+  f.ny = c->ny;
+  f.dx = c->dx;
+  f.dy = c->dy;
+  f.eta = c->eta;
+  f.c = coef;
+  f.x = c->x;
+  f.bx = bx;
+  f.by = by;
+  f.bx_out = bxo;
+  f.by_out = byo;
+  f.st = c->st;
+  return f;
+}
+
+DiagArgs diag_args(mhd_ctx* c) {
+  DiagArgs d{};
+  d.nx = c->nx;
+  d.ny = c->ny;
+  d.dx = c->dx;
+  d.dy = c->dy;
+  for (int q = 0; q < 6; ++q) d.s[q] = c->U[q];
+  d.part = c->part;
+  d.counter = c->counter;
+  d.out = c->dout;
+  return d;
+}
+
+// Host-side periodic ghost fill of one padded array (Grid::fillOneField, Grid.hpp:143-161).
+void fill_ghosts(double* f, int nx, int ny) {
+  const int ng = 2, nxt = nx + 4, nyt = ny + 4;
+  for (int j = ng; j < nyt - ng; ++j)
+    for (int g = 0; g < ng; ++g) {
+      f[j * nxt + g] = f[j * nxt + nx + g];
+      f[j * nxt + nx + ng + g] = f[j * nxt + ng + g];
+    }
+  for (int i = 0; i < nxt; ++i)
+    for (int g = 0; g < ng; ++g) {
+      f[g * nxt + i] = f[(ny + g) * nxt + i];
+      f[(ny + ng + g) * nxt + i] = f[(ng + g) * nxt + i];
+    }
+}
+
+int to_host_padded(mhd_ctx* c, const double* dev, double* host, bool ghosts) {
+  const size_t nxt = (size_t)c->nx + 4;
+  HIP_OR_FAIL(c, hipMemcpy2DAsync(host + 2 * nxt + 2, nxt * sizeof(double), dev, (size_t)c->nx * sizeof(double),
+                                  (size_t)c->nx * sizeof(double), (size_t)c->ny, hipMemcpyDeviceToHost, c->stream));
+  HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
+  if (ghosts) fill_ghosts(host, c->nx, c->ny);
+  return MHD_OK;
+}
+
+int check_ctx(mhd_ctx* c) { return c ? MHD_OK : MHD_EINVAL; }
+
+}  // namespace
+
+extern "C" {
+
+int mhd_default_opts(mhd_opts* o) {
+  if (!o) return MHD_EINVAL;
+  std::memset(o, 0, sizeof *o);
+  o->gamma = 5.0 / 3.0;
+  o->cfl = 0.45;
+  o->cg_tol = DBL_EPSILON;
+  o->cg_max_iter = 0;
+  o->cg_fail_is_error = 0;
+  o->march_rows = 0;
+  return MHD_OK;
+}
+
+const char* mhd_last_error(const mhd_ctx* c) { return c ? c->err.c_str() : g_create_error.c_str(); }
+
+int mhd_create(int nx, int ny, double dx, double dy, double eta, const mhd_opts* opts, mhd_ctx** out) {
+  if (!out) return fail(nullptr, MHD_EINVAL, "out is NULL");
+  *out = nullptr;
+  if (nx < 4 || ny < 4) return fail(nullptr, MHD_EINVAL, "nx and ny must be >= 4");
+  if ((long long)nx * ny > (1LL << 30)) return fail(nullptr, MHD_EINVAL, "nx * ny exceeds 2^30 cells");
+  if (!(dx > 0) || !(dy > 0) || !std::isfinite(dx) || !std::isfinite(dy))
+    return fail(nullptr, MHD_EINVAL, "dx and dy must be positive and finite");
+  if (!(eta >= 0) || !std::isfinite(eta)) return fail(nullptr, MHD_EINVAL, "eta must be >= 0 and finite");
+  mhd_opts o;
+  mhd_default_opts(&o);
+  if (opts) o = *opts;
+  if (o.gamma != 5.0 / 3.0) return fail(nullptr, MHD_EINVAL, "only gamma = 5/3 is compiled in");
+  if (!(o.cfl > 0) || !(o.cg_tol >= 0)) return fail(nullptr, MHD_EINVAL, "cfl must be > 0 and cg_tol >= 0");
+  if (o.march_rows < 0) return fail(nullptr, MHD_EINVAL, "march_rows must be >= 0");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(nullptr, MHD_EHIP, "no HIP device available");
+
+  mhd_ctx* c = new mhd_ctx();
+  c->nx = nx;
+  c->ny = ny;
+  c->n = (long long)nx * ny;
+  c->dx = dx;
+  c->dy = dy;
+  c->eta = eta;
+  c->o = o;
+  auto bail = [&](const std::string& m, int code) {
+    g_create_error = m;
+    mhd_destroy(c);
+    return code;
+  };
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return bail("hipStreamCreate failed", MHD_EHIP);
+  const size_t fb = (size_t)c->n * sizeof(double);
+  for (int q = 0; q < 6; ++q) {
+    if (hipMalloc(&c->U[q], fb) != hipSuccess || hipMalloc(&c->S[q], fb) != hipSuccess ||
+        hipMalloc(&c->F1[q], fb) != hipSuccess)
+      return bail("device allocation failed", MHD_ENOMEM);
+  }
+  if (hipMalloc(&c->x, fb) != hipSuccess || hipMalloc(&c->r, fb) != hipSuccess || hipMalloc(&c->p[0], fb) != hipSuccess ||
+      hipMalloc(&c->p[1], fb) != hipSuccess || hipMalloc(&c->part, kMaxRedBlocks * 2 * sizeof(double)) != hipSuccess ||
+      hipMalloc(&c->st, sizeof(CgState)) != hipSuccess || hipMalloc(&c->counter, 64) != hipSuccess ||
+      hipMalloc(&c->dout, 4 * sizeof(double)) != hipSuccess)
+    return bail("device allocation failed", MHD_ENOMEM);
+  if (hipHostMalloc(&c->st_host, sizeof(CgState)) != hipSuccess || hipHostMalloc(&c->out_host, 4 * sizeof(double)) != hipSuccess)
+    return bail("pinned allocation failed", MHD_ENOMEM);
+  for (int q = 0; q < 6; ++q) {
+    if (hipMemsetAsync(c->U[q], 0, fb, c->stream) != hipSuccess) return bail("hipMemset failed", MHD_EHIP);
+  }
+  CgState zero{};
+  if (hipMemcpyAsync(c->st, &zero, sizeof zero, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+      hipMemsetAsync(c->counter, 0, 64, c->stream) != hipSuccess)
+    return bail("device init failed", MHD_EHIP);
+  // reduction grid: one block per row band, a multiple of 8 (one band per XCD)
+  int nb = ny < kMaxRedBlocks ? ny : kMaxRedBlocks;
+  if (nb >= 8) nb &= ~7;
+  c->red_blocks = nb;
+  // stage kernel: rows per work item so that ~4096 wavefronts are launched
+  if (o.march_rows > 0) {
+    c->stage_rows = o.march_rows;
+  } else {
+    long long strips = (nx + kStripOut - 1) / kStripOut;
+    long long rows = (ny * strips + 4095) / 4096;
+    if (rows < 8) rows = 8;
+    if (rows > 256) rows = 256;
+    c->stage_rows = (int)rows;
+  }
+  if (hipStreamSynchronize(c->stream) != hipSuccess) return bail("device init failed", MHD_EHIP);
+  *out = c;
+  return MHD_OK;
+}
+
+int mhd_destroy(mhd_ctx* c) {
+  if (!c) return MHD_EINVAL;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (int k = 0; k < kNumKernelIds; ++k) {
+    for (auto& pr : c->pending[k]) {
+      (void)hipEventDestroy(pr.first);
+      (void)hipEventDestroy(pr.second);
+    }
+  }
+  for (int q = 0; q < 6; ++q) {
+    if (c->U[q]) (void)hipFree(c->U[q]);
+    if (c->S[q]) (void)hipFree(c->S[q]);
+    if (c->F1[q]) (void)hipFree(c->F1[q]);
+  }
+  double* bufs[] = {c->x, c->r, c->p[0], c->p[1], c->part, c->dout};
+  for (double* b : bufs)
+    if (b) (void)hipFree(b);
+  if (c->st) (void)hipFree(c->st);
+  if (c->counter) (void)hipFree(c->counter);
+  if (c->st_host) (void)hipHostFree(c->st_host);
+  if (c->out_host) (void)hipHostFree(c->out_host);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return MHD_OK;
+}
+
+int mhd_upload(mhd_ctx* c, const double* rho, const double* mx, const double* my, const double* E,
+               const double* bxf, const double* byf) {
+  if (check_ctx(c)) return MHD_EINVAL;
+  const double* src[6] = {rho, mx, my, E, bxf, byf};
+  for (int q = 0; q < 6; ++q)
+    if (!src[q]) return fail(c, MHD_EINVAL, "mhd_upload: NULL field");
+  const size_t nxt = (size_t)c->nx + 4;
+  for (int q = 0; q < 6; ++q)
+    HIP_OR_FAIL(c, hipMemcpy2DAsync(c->U[q], (size_t)c->nx * sizeof(double), src[q] + 2 * nxt + 2, nxt * sizeof(double),
+                                    (size_t)c->nx * sizeof(double), (size_t)c->ny, hipMemcpyHostToDevice, c->stream));
+  int rc = launch(c, 8, [&] {
+    hipLaunchKernelGGL(k_floor_rho, dim3(grid_1d(c->n)), dim3(256), 0, c->stream, c->U[0], (int)c->n);
+  });
+  if (rc) return rc;
+  HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
+  return MHD_OK;
+}
+
+int mhd_download(mhd_ctx* c, double* rho, double* mx, double* my, double* E, double* bxf, double* byf, double* vx,
+                 double* vy, double* p, double* bxc, double* byc) {
+  if (check_ctx(c)) return MHD_EINVAL;
+  double* dst[6] = {rho, mx, my, E, bxf, byf};
+  for (int q = 0; q < 6; ++q)
+    if (dst[q]) {
+      int rc = to_host_padded(c, c->U[q], dst[q], true);
+      if (rc) return rc;
+    }
+  double* pd[5] = {vx, vy, p, bxc, byc};
+  bool any = false;
+  for (double* q : pd) any = any || q;
+  if (any) {
+    DiagArgs d = diag_args(c);
+    int rc = launch(c, 8, [&] {
+      hipLaunchKernelGGL(k_prims, dim3(grid_1d(c->n)), dim3(256), 0, c->stream, d, c->F1[0], c->F1[1], c->F1[2], c->F1[3],
+                         c->F1[4]);
+    });
+    if (rc) return rc;
+    for (int q = 0; q < 5; ++q)
+      if (pd[q]) {
+        rc = to_host_padded(c, c->F1[q], pd[q], true);
+        if (rc) return rc;
+      }
+  }
+  return MHD_OK;
+}
+
+static int max_speed(mhd_ctx* c, double* maxs, double* nbad) {
+  DiagArgs d = diag_args(c);
+  int rc = launch(c, 7, [&] {
+    hipLaunchKernelGGL(k_cfl, dim3(c->red_blocks), dim3(kRedThreads), 0, c->stream, d);
+  });
+  if (rc) return rc;
+  HIP_OR_FAIL(c, hipMemcpyAsync(c->out_host, c->dout, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
+  *maxs = c->out_host[0];
+  *nbad = c->out_host[1];
+  return MHD_OK;
+}
+
+int mhd_cfl_dt(mhd_ctx* c, double* dt) {
+  if (check_ctx(c) || !dt) return fail(c, MHD_EINVAL, "mhd_cfl_dt: bad argument");
+  double maxs = 0, nbad = 0;
+  int rc = max_speed(c, &maxs, &nbad);
+  if (rc) return rc;
+  // CFL_condition_checker, diagnostics.hpp:52-86, with the Courant number opts.cfl
+  const double h = (c->dy < c->dx) ? c->dy : c->dx;
+  double dt_res = INFINITY;
+  if (c->eta != 0.0) dt_res = h * h / (4.0 * c->eta);
+  const double courant = c->o.cfl / 0.9;
+  double dt_hyp = courant * h / maxs;
+  double lim = (dt_res < dt_hyp) ? dt_res : dt_hyp;
+  *dt = lim * 0.9;
+  if (nbad > 0) return fail(c, MHD_ENONFINITE, "non-finite wave speed in " + std::to_string((long long)nbad) + " cells");
+  return MHD_OK;
+}
+
+int mhd_build_operator(mhd_ctx* c, double alpha) {
+  if (check_ctx(c)) return MHD_EINVAL;
+  if (!std::isfinite(alpha) || alpha < 0) return fail(c, MHD_EINVAL, "alpha must be finite and >= 0");
+  c->alpha = alpha;
+  c->have_op = true;
+  return MHD_OK;
+}
+
+int mhd_imex_step(mhd_ctx* c, double dt, mhd_step_stats* stats) {
+  if (check_ctx(c)) return MHD_EINVAL;
+  if (!std::isfinite(dt)) return fail(c, MHD_EINVAL, "dt must be finite");
+  mhd_step_stats st{};
+  st.dt = dt;
+  const Tableau& T = c->tab;
+  // stage 1: f1 = L(U), S = U + dt A_ex[3] f1 (solvers.hpp:495-523)
+  StageArgs a = stage_args(c);
+  for (int q = 0; q < 6; ++q) {
+    a.s[q] = c->U[q];
+    a.o1[q] = c->F1[q];
+    a.o2[q] = c->S[q];
+  }
+  a.c_a = dt * T.A_ex3;
+  int rc = run_stage(c, 1, a, 0);
+  if (rc) return rc;
+  const double diag_coeff = dt * T.A_im4;
+  if (c->eta > 0.0) {
+    // implicit 1: x = CG(eta J(S.B)), S.B += dt A_im[4] rate(x) (solvers.hpp:526-529)
+    CgState cs;
+    rc = cg_run(c, c->S[4], c->S[5], nullptr, &cs);
+    if (rc) return rc;
+    st.cg_iters[0] = cs.iters;
+    st.cg_spmv[0] = cs.spmv;
+    st.cg_error[0] = cg_error(cs);
+    FinArgs f = fin_args(c, c->S[4], c->S[5], c->S[4], c->S[5], diag_coeff);
+    rc = launch(c, 5, [&] { hipLaunchKernelGGL(k_finalize1, dim3(grid_1d(c->n)), dim3(256), 0, c->stream, f); });
+    if (rc) return rc;
+  }
+  // stage 2: U3* = (U + dt A_ex[6] f1) + dt A_ex[7] L(S) [+ dt A_im[7] g2] (solvers.hpp:535-566)
+  StageArgs b = stage_args(c);
+  for (int q = 0; q < 6; ++q) {
+    b.s[q] = c->S[q];
+    b.u[q] = c->U[q];
+    b.f1[q] = c->F1[q];
+    b.o1[q] = c->U[q];
+  }
+  b.c_a = dt * T.A_ex6;
+  b.c_b = dt * T.A_ex7;
+  b.c_g = dt * T.A_im7;
+  rc = run_stage(c, 2, b, 1);
+  if (rc) return rc;
+  if (c->eta > 0.0) {
+    // implicit 2 and the final recombination (solvers.hpp:571-608)
+    CgState cs;
+    rc = cg_run(c, c->U[4], c->U[5], nullptr, &cs);
+    if (rc) return rc;
+    st.cg_iters[1] = cs.iters;
+    st.cg_spmv[1] = cs.spmv;
+    st.cg_error[1] = cg_error(cs);
+    FinArgs f = fin_args(c, c->U[4], c->U[5], c->S[4], c->S[5], diag_coeff);
+    rc = launch(c, 6, [&] { hipLaunchKernelGGL(k_finalize2, dim3(grid_1d(c->n)), dim3(256), 0, c->stream, f); });
+    if (rc) return rc;
+    std::swap(c->U[4], c->S[4]);
+    std::swap(c->U[5], c->S[5]);
+  }
+  if (stats) *stats = st;
+  return MHD_OK;
+}
+
+int mhd_explicit_rhs(mhd_ctx* c, double* drho, double* dmx, double* dmy, double* dE, double* dbxf, double* dbyf) {
+  if (check_ctx(c)) return MHD_EINVAL;
+  double* dst[6] = {drho, dmx, dmy, dE, dbxf, dbyf};
+  StageArgs a = stage_args(c);
+  for (int q = 0; q < 6; ++q) {
+    a.s[q] = c->U[q];
+    a.o1[q] = c->F1[q];
+  }
+  int rc = run_stage(c, 0, a, 8);
+  if (rc) return rc;
+  for (int q = 0; q < 6; ++q)
+    if (dst[q]) {
+      rc = to_host_padded(c, c->F1[q], dst[q], false);
+      if (rc) return rc;
+    }
+  return MHD_OK;
+}
+
+int mhd_resistive_term(mhd_ctx* c, int implicit, double* bx_rate, double* by_rate, int* iters) {
+  if (check_ctx(c) || !bx_rate || !by_rate) return fail(c, MHD_EINVAL, "mhd_resistive_term: bad argument");
+  int rc;
+  FinArgs f = fin_args(c, c->U[4], c->U[5], c->F1[0], c->F1[1], 0.0);
+  int zero = 0;
+  if (implicit) {
+    CgState cs;
+    rc = cg_run(c, c->U[4], c->U[5], nullptr, &cs);
+    if (rc) return rc;
+    if (iters) *iters = cs.iters;
+    zero = cs.spmv == 0;
+  } else {
+    CgArgs a = cg_args(c, c->U[4], c->U[5]);
+    rc = launch(c, 8, [&] {
+      hipLaunchKernelGGL(k_cg_init, dim3(c->red_blocks), dim3(kRedThreads), 0, c->stream, a, (const double*)nullptr, 1);
+    });
+    if (rc) return rc;
+    f.x = c->r;
+    if (iters) *iters = 0;
+  }
+  rc = launch(c, 8, [&] { hipLaunchKernelGGL(k_rate, dim3(grid_1d(c->n)), dim3(256), 0, c->stream, f, zero); });
+  if (rc) return rc;
+  rc = to_host_padded(c, c->F1[0], bx_rate, false);
+  if (rc) return rc;
+  return to_host_padded(c, c->F1[1], by_rate, false);
+}
+
+int mhd_cg_solve(mhd_ctx* c, const double* b, double* x, int* iters, double* error) {
+  if (check_ctx(c) || !b || !x) return fail(c, MHD_EINVAL, "mhd_cg_solve: bad argument");
+  const size_t fb = (size_t)c->n * sizeof(double);
+  HIP_OR_FAIL(c, hipMemcpyAsync(c->F1[0], b, fb, hipMemcpyHostToDevice, c->stream));
+  CgState cs;
+  int rc = cg_run(c, c->U[4], c->U[5], c->F1[0], &cs);
+  if (rc) return rc;
+  if (cs.spmv == 0) {
+    std::memset(x, 0, fb);
+  } else {
+    HIP_OR_FAIL(c, hipMemcpyAsync(x, c->x, fb, hipMemcpyDeviceToHost, c->stream));
+    HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
+  }
+  if (iters) *iters = cs.iters;
+  if (error) *error = cg_error(cs);
+  return MHD_OK;
+}
+
+int mhd_divb(mhd_ctx* c, double* rms, double* maxabs) {
+  if (check_ctx(c) || !rms || !maxabs) return fail(c, MHD_EINVAL, "mhd_divb: bad argument");
+  DiagArgs d = diag_args(c);
+  int rc = launch(c, 8, [&] { hipLaunchKernelGGL(k_divb, dim3(c->red_blocks), dim3(kRedThreads), 0, c->stream, d); });
+  if (rc) return rc;
+  HIP_OR_FAIL(c, hipMemcpyAsync(c->out_host, c->dout, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
+  *rms = std::sqrt(c->out_host[0] / (double)(c->nx * c->ny));
+  *maxabs = c->out_host[1];
+  return MHD_OK;
+}
+
+int mhd_run(mhd_ctx* c, double t_end, double dt_override, int max_steps, int* steps_out, double* t_out,
+            double* step_dt, int* cg_iters) {
+  if (check_ctx(c)) return MHD_EINVAL;
+  if (!std::isfinite(t_end) || !std::isfinite(dt_override) || dt_override < 0)
+    return fail(c, MHD_EINVAL, "mhd_run: bad t_end or dt_override");
+  const double A_im4 = c->tab.A_im4;
+  double opt;
+  int rc = mhd_cfl_dt(c, &opt);                                   // integrator_main.cpp:186
+  if (rc) return rc;
+  double dt0 = (dt_override == 0.0) ? opt : dt_override;
+  double mc = dt0 * c->eta;
+  rc = mhd_build_operator(c, mc * A_im4);                         // :189-191
+  if (rc) return rc;
+  double prev = dt0, t = 0.0;
+  int steps = 0;
+  while (t < t_end && (max_steps < 0 || steps < max_steps)) {     // :207
+    // :211-212; with an override the CFL value only feeds the reference's warning print
+    double dt = dt_override;
+    if (dt_override == 0.0) {
+      rc = mhd_cfl_dt(c, &dt);
+      if (rc) return rc;
+    }
+    if (t + dt > t_end) dt = t_end - t;                           // :214
+    if (c->eta > 0.0) {
+      if (std::fabs(dt - prev) > 1e-10 * dt) {                     // :219
+        double m = dt * c->eta;
+        rc = mhd_build_operator(c, m * A_im4);
+        if (rc) return rc;
+      }
+    }
+    prev = dt;
+    mhd_step_stats st;
+    rc = mhd_imex_step(c, dt, &st);                               // :230
+    if (rc) return rc;
+    t += dt;
+    if (step_dt) step_dt[steps] = dt;
+    if (cg_iters) {
+      cg_iters[2 * steps] = st.cg_iters[0];
+      cg_iters[2 * steps + 1] = st.cg_iters[1];
+    }
+    steps++;
+  }
+  if (steps_out) *steps_out = steps;
+  if (t_out) *t_out = t;
+  return MHD_OK;
+}
+
+int mhd_profile_enable(mhd_ctx* c, int on) {
+  if (check_ctx(c)) return MHD_EINVAL;
+  int rc = flush_profile(c);
+  if (rc) return rc;
+  c->prof = on != 0;
+  for (int k = 0; k < kNumKernelIds; ++k) {
+    c->prof_n[k] = 0;
+    c->prof_ms[k] = 0.0;
+  }
+  return MHD_OK;
+}
+
+int mhd_profile_get(mhd_ctx* c, int id, long long* launches, double* total_ms) {
+  if (check_ctx(c) || id < 0 || id >= kNumKernelIds) return fail(c, MHD_EINVAL, "mhd_profile_get: bad argument");
+  int rc = flush_profile(c);
+  if (rc) return rc;
+  if (launches) *launches = c->prof_n[id];
+  if (total_ms) *total_ms = c->prof_ms[id];
+  return MHD_OK;
+}
+
+int mhd_calibration_copy(mhd_ctx* c, long long nbytes) {
+  if (check_ctx(c) || nbytes <= 0) return fail(c, MHD_EINVAL, "mhd_calibration_copy: bad argument");
+  long long n = nbytes / (long long)sizeof(double);
+  double *src = nullptr, *dst = nullptr;
+  HIP_OR_FAIL(c, hipMalloc(&src, n * sizeof(double)));
+  HIP_OR_FAIL(c, hipMalloc(&dst, n * sizeof(double)));
+  HIP_OR_FAIL(c, hipMemsetAsync(src, 0, n * sizeof(double), c->stream));
+  int rc = launch(c, 8, [&] { hipLaunchKernelGGL(k_copy, dim3(grid_1d(n)), dim3(256), 0, c->stream, (const double*)src, dst, n); });
+  HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
+  (void)hipFree(src);
+  (void)hipFree(dst);
+  return rc;
+}
+
+int mhd_synchronize(mhd_ctx* c) {
+  if (check_ctx(c)) return MHD_EINVAL;
+  HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
+  return MHD_OK;
+}
+
+}  // extern "C"

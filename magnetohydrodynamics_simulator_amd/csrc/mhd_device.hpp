@@ -1,0 +1,175 @@
+// mhd_device.hpp -- per-cell / per-face FP64 device math of the MHD step.
+//
+// These are the scalar building blocks that the fused stage kernel strings
+// together.  Each follows a reference function (paths relative to the
+// reference repository) and keeps its operation order, except where the
+// reference's own -ffast-math build (CMakeLists.txt:21) already rewrites a
+// division: GCC's -freciprocal-math turns x / c for a constant c into x * (1/c)
+// and replaces three or more divisions by the same value with one reciprocal
+// and multiplies; the same rewrites are made here, so both sides round alike
+// to within an ulp per operation.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace mhd {
+
+constexpr double kGamma = 5.0 / 3.0;                    // Grid.hpp:104, HLL.hpp:31
+constexpr double kGammaM1 = kGamma - 1.0;                // (gamma - 1)
+constexpr double kInvGammaM1 = 1.0 / (kGamma - 1.0);     // x / (gamma - 1.0) -> x * (1 / (gamma - 1))
+constexpr double kFloor = 1e-8;                          // Grid.hpp:115,124
+constexpr double kTie = 1e-12;                           // solvers.hpp:326,333
+
+// Primitive state, PrimitiveState (HLL.hpp:12-19).
+struct Prim {
+  double rho, vx, vy, p, bx, by;
+};
+
+// Flux through one face: mass, x/y momentum, energy and the one non-zero
+// induction component (F_by on x-faces, G_bx on y-faces); the normal
+// induction flux is identically zero (HLL.hpp:76,99,166,212).
+struct Flux {
+  double rho, mx, my, E, b;
+};
+
+// minmod, MUSCL.hpp:11-17.
+__device__ __forceinline__ double minmod(double a, double b) {
+  return (a * b > 0.0) ? ((fabs(a) < fabs(b)) ? a : b) : 0.0;
+}
+
+// MUSCLReconstructOne, MUSCL.hpp:26-47: undivided minmod slope, L = low face, R = high face.
+__device__ __forceinline__ void muscl(double fm, double f0, double fp, double& L, double& R) {
+  double s = minmod(f0 - fm, fp - f0);
+  L = f0 - 0.5 * s;
+  R = f0 + 0.5 * s;
+}
+
+// std::min / std::max semantics.
+__device__ __forceinline__ double dmin(double a, double b) { return (b < a) ? b : a; }
+__device__ __forceinline__ double dmax(double a, double b) { return (a < b) ? b : a; }
+
+// Grid::computePrimitives for one cell, Grid.hpp:111-126.  rho is the stored
+// density; the velocities use it before the floor is applied.
+__device__ __forceinline__ void primitives(double rho, double mx, double my, double E, double bxc, double byc,
+                                           double& rho_f, double& vx, double& vy, double& p) {
+  vx = mx / rho;
+  vy = my / rho;
+  rho_f = (rho < kFloor) ? kFloor : rho;
+  double kinE = 0.5 * rho_f * (vx * vx + vy * vy);
+  double magE = (bxc * bxc + byc * byc) / 2;
+  double intE = E - kinE - magE;
+  p = kGammaM1 * intE;
+  if (p < kFloor) p = kFloor;
+}
+
+// getFastWaveSpeed, HLL.hpp:107-122 (three divisions by rho -> one reciprocal).
+__device__ __forceinline__ double fast_speed(const Prim& s, double bn) {
+  double B_sq = s.bx * s.bx + s.by * s.by;
+  double inv_rho = 1.0 / s.rho;
+  double sound_v_sq = (kGamma * s.p) * inv_rho;
+  double tot_alfven_v_sq = B_sq * inv_rho;
+  double norm_alfven_v_sq = (bn * bn) * inv_rho;
+  double sum = tot_alfven_v_sq + sound_v_sq;
+  double term = sum * sum - 4.0 * sound_v_sq * norm_alfven_v_sq;
+  return sqrt(0.5 * (sum + sqrt(dmax(0.0, term))));
+}
+
+// convertPrimitiveX (HLL.hpp:61-81) and the conserved energy of
+// ConservedState::fromPrimitive (HLL.hpp:31-47) for one state.
+__device__ __forceinline__ void phys_flux_x(const Prim& s, Flux& F, double& E_cons) {
+  double v_sq = s.vx * s.vx + s.vy * s.vy;
+  double B_sq = s.bx * s.bx + s.by * s.by;
+  double P_tot = s.p + 0.5 * B_sq;
+  double v_dot_B = s.vx * s.bx + s.vy * s.by;
+  double pe = s.p * kInvGammaM1;
+  double E = pe + 0.5 * (s.rho * v_sq + B_sq);
+  F.rho = s.rho * s.vx;
+  F.mx = s.rho * s.vx * s.vx + P_tot - s.bx * s.bx;
+  F.my = s.rho * s.vx * s.vy - s.bx * s.by;
+  F.E = (E + P_tot) * s.vx - s.bx * v_dot_B;
+  F.b = s.vx * s.by - s.vy * s.bx;
+  E_cons = pe + 0.5 * s.rho * v_sq + 0.5 * B_sq;
+}
+
+// convertPrimitiveY (HLL.hpp:84-104) and fromPrimitive's energy.
+__device__ __forceinline__ void phys_flux_y(const Prim& s, Flux& F, double& E_cons) {
+  double v_sq = s.vx * s.vx + s.vy * s.vy;
+  double B_sq = s.bx * s.bx + s.by * s.by;
+  double P_tot = s.p + 0.5 * B_sq;
+  double v_dot_B = s.vx * s.bx + s.vy * s.by;
+  double pe = s.p * kInvGammaM1;
+  double E = pe + 0.5 * (s.rho * v_sq + B_sq);
+  F.rho = s.rho * s.vy;
+  F.my = s.rho * s.vy * s.vy + P_tot - s.by * s.by;
+  F.mx = s.rho * s.vx * s.vy - s.bx * s.by;
+  F.E = (E + P_tot) * s.vy - s.by * v_dot_B;
+  F.b = s.vy * s.bx - s.vx * s.by;
+  E_cons = pe + 0.5 * s.rho * v_sq + 0.5 * B_sq;
+}
+
+// computeHLLFluxX, HLL.hpp:126-172.  L/R are the states left/right of the
+// face; both carry the same face-centred normal field in .bx.
+__device__ __forceinline__ Flux hll_x(const Prim& L, const Prim& R) {
+  Flux FL, FR;
+  double EL, ER;
+  phys_flux_x(L, FL, EL);
+  phys_flux_x(R, FR, ER);
+  double cL = fast_speed(L, L.bx);
+  double cR = fast_speed(R, R.bx);
+  double sL = dmin(L.vx - cL, R.vx - cR);
+  double sR = dmax(L.vx + cL, R.vx + cR);
+  if (sL >= 0) return FL;
+  if (sR <= 0) return FR;
+  double inv_S = 1.0 / (sR - sL);
+  double sLR = sL * sR;
+  Flux H;
+  H.rho = (sR * FL.rho - sL * FR.rho + sLR * (R.rho - L.rho)) * inv_S;
+  H.mx = (sR * FL.mx - sL * FR.mx + sLR * (R.rho * R.vx - L.rho * L.vx)) * inv_S;
+  H.my = (sR * FL.my - sL * FR.my + sLR * (R.rho * R.vy - L.rho * L.vy)) * inv_S;
+  H.E = (sR * FL.E - sL * FR.E + sLR * (ER - EL)) * inv_S;
+  H.b = (sR * FL.b - sL * FR.b + sLR * (R.by - L.by)) * inv_S;
+  return H;
+}
+
+// computeHLLFluxY, HLL.hpp:177-218.  D/U are the states below/above the face;
+// both carry the same face-centred normal field in .by.
+__device__ __forceinline__ Flux hll_y(const Prim& D, const Prim& U) {
+  Flux FD, FU;
+  double ED, EU;
+  phys_flux_y(D, FD, ED);
+  phys_flux_y(U, FU, EU);
+  double cD = fast_speed(D, D.by);
+  double cU = fast_speed(U, U.by);
+  double sD = dmin(D.vy - cD, U.vy - cU);
+  double sU = dmax(D.vy + cD, U.vy + cU);
+  if (sD >= 0) return FD;
+  if (sU <= 0) return FU;
+  double inv_S = 1.0 / (sU - sD);
+  double sDU = sD * sU;
+  Flux H;
+  H.rho = (sU * FD.rho - sD * FU.rho + sDU * (U.rho - D.rho)) * inv_S;
+  H.mx = (sU * FD.mx - sD * FU.mx + sDU * (U.rho * U.vx - D.rho * D.vx)) * inv_S;
+  H.my = (sU * FD.my - sD * FU.my + sDU * (U.rho * U.vy - D.rho * D.vy)) * inv_S;
+  H.E = (sU * FD.E - sD * FU.E + sDU * (EU - ED)) * inv_S;
+  H.b = (sU * FD.b - sD * FU.b + sDU * (U.bx - D.bx)) * inv_S;
+  return H;
+}
+
+// Upwind corner E_z, cell_corner_electric_fields (solvers.hpp:314-340).
+// eN/eS/eE/eW are the four face electric fields around the corner, vxc/vyc
+// the four-cell average velocity.  The reference's abs() is a double abs in
+// its real build (Eigen includes the libstdc++ <stdlib.h> wrapper), hence fabs.
+__device__ __forceinline__ double corner_ez(double eN, double eS, double eE, double eW, double vxc, double vyc) {
+  double ezy = (fabs(vyc) < kTie) ? 0.5 * (eS + eN) : ((vyc > 0.0) ? eS : eN);
+  double ezx = (fabs(vxc) < kTie) ? 0.5 * (eE + eW) : ((vxc > 0.0) ? eW : eE);
+  return 0.5 * (ezx + ezy);
+}
+
+// Periodic index helpers.
+__device__ __forceinline__ int pmod(int a, int n) {
+  int r = a % n;
+  return r < 0 ? r + n : r;
+}
+__device__ __forceinline__ int wrap1(int a, int n) { return a < 0 ? a + n : (a >= n ? a - n : a); }
+
+}  // namespace mhd

@@ -1,0 +1,80 @@
+// mhd_kernels.hpp -- kernel argument blocks and launch constants shared by the
+// kernels (mhd_kernels.hip) and the C-ABI host code (mhd_capi.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace mhd {
+
+constexpr int kStripOut = 60;     // output columns per 64-lane strip of the stage kernel
+constexpr int kRedThreads = 256;  // threads per block of the reduction kernels
+constexpr int kMaxRedBlocks = 2048;
+
+// Explicit-stage kernel arguments.  MODE 0: o1 = L(s).  MODE 1: o1 = f1 = L(s),
+// o2 = s + c_a f1.  MODE 2: o1 = (u + c_a f1) + c_b L(s) [+ c_g g2(s) on B].
+struct StageArgs {
+  const double* s[6];   // state whose fluxes are computed (halo reads)
+  const double* u[6];   // MODE 2: U^n
+  const double* f1[6];  // MODE 2: f1
+  double* o1[6];
+  double* o2[6];
+  int nx, ny, rows;
+  double dx, dy, inv_dx, inv_dy, eta;
+  double c_a, c_b, c_g;
+};
+
+// Conjugate-gradient state, device resident.
+struct CgState {
+  double rhs_norm2, threshold, abs_new, alpha, beta, rn2, pap;
+  long long max_iters;
+  int iters, spmv, done;   // done: 0 running, 1 converged, 2 max iterations, 3 zero rhs
+  unsigned int counter;    // block arrivals of the running reduction
+};
+
+struct CgArgs {
+  int nx, ny;
+  double dx, dy, eta;
+  double invd, diag, cx, cy, tol;
+  const double* bx;  // face B for the right-hand side
+  const double* by;
+  double* r;
+  double* p0;
+  double* x;
+  double* part;
+  CgState* st;
+};
+
+struct FinArgs {
+  int nx, ny;
+  double dx, dy, eta, c;
+  const double* x;
+  const double* bx;
+  const double* by;
+  double* bx_out;
+  double* by_out;
+  const CgState* st;
+};
+
+struct DiagArgs {
+  int nx, ny;
+  double dx, dy;
+  const double* s[6];
+  double* part;
+  unsigned int* counter;
+  double* out;
+};
+
+template <int MODE> __global__ void k_stage(StageArgs a);
+__global__ void k_cg_init(CgArgs a, const double* bin, int explicit_only);
+template <bool FIRST> __global__ void k_cg_pass_a(CgArgs a, const double* pold, double* pnew);
+template <bool FIRST> __global__ void k_cg_pass_b(CgArgs a, const double* pv);
+__global__ void k_finalize1(FinArgs a);
+__global__ void k_finalize2(FinArgs a);
+__global__ void k_rate(FinArgs a, int zero);
+__global__ void k_cfl(DiagArgs a);
+__global__ void k_divb(DiagArgs a);
+__global__ void k_prims(DiagArgs a, double* vx, double* vy, double* p, double* bxc, double* byc);
+__global__ void k_floor_rho(double* rho, int n);
+__global__ void k_copy(const double* src, double* dst, long long n);
+
+}  // namespace mhd

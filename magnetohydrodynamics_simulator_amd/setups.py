@@ -1,0 +1,192 @@
+"""Initial conditions as padded (ny+4, nx+4) float64 arrays.
+
+Restates the reference's problem setups (include/benchmarks.hpp of
+RikibabManos/Magnetohydrodynamics_Simulator) with numpy, plus the seeded
+setups of the ensemble and column workloads.  Every function returns a dict of
+the six conserved fields rho, mx, my, E, bxf, byf in the reference Grid's
+padded layout (ghost cells hold the setup evaluated at ghost coordinates; the
+solver ignores them and refills them periodically).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+PI_REF = 3.14159265359   # truncated pi of the reference (integrator_main.cpp:36, benchmarks.hpp:88,...)
+GAMMA = 5.0 / 3.0
+NG = 2
+
+
+def grid_spacing(n: int) -> float:
+    """cell_width = 2 * pi / n as set by src/integrator_main.cpp:37-38."""
+    return 2 * PI_REF / n
+
+
+def _from_primitive(rho, vx, vy, p, bx, by):
+    """ConservedState::fromPrimitive, include/HLL.hpp:31-47."""
+    v_sq = vx * vx + vy * vy
+    b_sq = bx * bx + by * by
+    E = p / (GAMMA - 1.0) + 0.5 * rho * v_sq + 0.5 * b_sq
+    return {"rho": rho * np.ones_like(E), "mx": rho * vx, "my": rho * vy, "E": E,
+            "bxf": bx * np.ones_like(E), "byf": by * np.ones_like(E)}
+
+
+def _centres(nx, ny, dx, dy):
+    i = np.arange(nx + 4, dtype=np.float64)
+    j = np.arange(ny + 4, dtype=np.float64)
+    x = (i - NG + 0.5) * dx
+    y = (j - NG + 0.5) * dy
+    return np.meshgrid(x, y)
+
+
+def _corners(nx, ny, dx, dy, extra=1):
+    i = np.arange(nx + 4 + extra, dtype=np.float64)
+    j = np.arange(ny + 4 + extra, dtype=np.float64)
+    return np.meshgrid((i - NG) * dx, (j - NG) * dy)
+
+
+def _face_b_from_potential(Az, dx, dy):
+    """bx = (Az(i,j+1) - Az(i,j))/dy, by = -(Az(i+1,j) - Az(i,j))/dx on the padded grid."""
+    bx = (Az[1:, :-1] - Az[:-1, :-1]) / dy
+    by = -(Az[:-1, 1:] - Az[:-1, :-1]) / dx
+    return bx, by
+
+
+def orszag_tang(nx: int, ny: int | None = None, dx: float | None = None, dy: float | None = None):
+    """orszag_tang_initial_conditions, benchmarks.hpp:15-76: rho = 25/9,
+    p = 5/3, v = (-sin y, sin x), B = curl of Az = cos(2x)/2 + cos y at corners."""
+    ny = ny or nx
+    dx = dx or grid_spacing(nx)
+    dy = dy or grid_spacing(ny)
+    XC, YC = _corners(nx, ny, dx, dy)
+    Az = 0.5 * np.cos(2 * XC) + np.cos(YC)
+    bx, by = _face_b_from_potential(Az, dx, dy)
+    X, Y = _centres(nx, ny, dx, dy)
+    return _from_primitive(25.0 / 9.0, -np.sin(Y), np.sin(X), 5.0 / 3.0, bx, by)
+
+
+def decaying_sine(nx: int, ny: int | None = None, dx: float | None = None, dy: float | None = None):
+    """decaying_magnetic_sine_wave_initial_conditions, benchmarks.hpp:352-417:
+    Az = cos(kx x) sin(ky y) / kx at corners, rho = p = 1, v = 0."""
+    ny = ny or nx
+    dx = dx or grid_spacing(nx)
+    dy = dy or grid_spacing(ny)
+    kx = 2 * PI_REF / (nx * dx)
+    ky = 2 * PI_REF / (ny * dy)
+    XC, YC = _corners(nx, ny, dx, dy)
+    Az = (1.0 / kx) * np.cos(kx * XC) * np.sin(ky * YC)
+    bx, by = _face_b_from_potential(Az, dx, dy)
+    z = np.zeros_like(bx)
+    return _from_primitive(1.0, z, z, 1.0, bx, by)
+
+
+def alfven(nx: int, ny: int | None = None, dx: float | None = None, dy: float | None = None):
+    """alfven_wave_test_initial_values, benchmarks.hpp:78-126."""
+    ny = ny or nx
+    dx = dx or grid_spacing(nx)
+    dy = dy or grid_spacing(ny)
+    kx = 2.0 * PI_REF / (nx * dx)
+    X, _ = _centres(nx, ny, dx, dy)
+    by = 1e-3 * np.sin(kx * X)
+    vy = -by / np.sqrt(1.0)
+    return _from_primitive(1.0, np.zeros_like(X), vy, 1.0, np.ones_like(X), by)
+
+
+def gaussian_pulse(nx: int, ny: int | None = None, dx: float | None = None, dy: float | None = None):
+    """gaussian_density_pulse_test_initial_values, benchmarks.hpp:128-176
+    (y_mid uses nx, as in the reference)."""
+    ny = ny or nx
+    dx = dx or grid_spacing(nx)
+    dy = dy or grid_spacing(ny)
+    X, Y = _centres(nx, ny, dx, dy)
+    x_mid, y_mid = 0.5 * nx * dx, 0.5 * nx * dy
+    d2 = (X - x_mid) * (X - x_mid) + (Y - y_mid) * (Y - y_mid)
+    rho = 1.0 + np.exp(-5 * d2 / 4.0)
+    one = np.ones_like(X)
+    return _from_primitive(rho, one, one, 1.0, 0 * one, 0 * one)
+
+
+def acoustic(nx: int, ny: int | None = None, dx: float | None = None, dy: float | None = None):
+    """acoustic_wave_test_initial_values, benchmarks.hpp:178-231."""
+    ny = ny or nx
+    dx = dx or grid_spacing(nx)
+    dy = dy or grid_spacing(ny)
+    i = np.arange(nx + 4, dtype=np.float64)
+    x = np.broadcast_to((i - NG) * dx, (ny + 4, nx + 4))
+    W = dx * nx
+    p_bg = 1.0 / GAMMA
+    cs = np.sqrt(GAMMA * p_bg / 1.0)
+    xs = 1e-4 * np.sin(4 * PI_REF * x / W)
+    rho = 1.0 + xs + 0.0
+    vx = (cs / 1.0) * xs
+    p = p_bg + (cs * cs) * (xs + 0.0)
+    z = np.zeros_like(rho)
+    return _from_primitive(rho, vx, z, p, z, z)
+
+
+def sausage(nx: int, ny: int | None = None, dx: float | None = None, dy: float | None = None,
+            b0: float = 5.0, p0: float = 0.1, amp: float = 0.15):
+    """sausage_instability_initial_conditions, benchmarks.hpp:233-293
+    (x_mid from nxt, as in the reference)."""
+    ny = ny or nx
+    dx = dx or grid_spacing(nx)
+    dy = dy or grid_spacing(ny)
+    X, Y = _centres(nx, ny, dx, dy)
+    k = 2.0 * PI_REF / (ny * dy)
+    x_mid = 0.5 * (nx + 4) * dx
+    r = X - x_mid
+    pert = 1.0 - amp * np.cos(k * Y)
+    rho = np.maximum(1.0 * np.exp(-(r * r) / (0.04 * pert)) + 0.1, 1e-4)
+    p = np.maximum(p0 * np.exp(-(r * r) / (0.04 * pert)) + 0.1, 1e-4)
+    by = b0 * (r / np.sqrt(0.04)) * np.exp(-(r * r) / 0.04)
+    z = np.zeros_like(rho)
+    return _from_primitive(rho, z, z, p, z, by)
+
+
+def kink(nx: int, ny: int | None = None, dx: float | None = None, dy: float | None = None):
+    """kink_instability_initial_conditions, benchmarks.hpp:295-350."""
+    ny = ny or nx
+    dx = dx or grid_spacing(nx)
+    dy = dy or grid_spacing(ny)
+    X, Y = _centres(nx, ny, dx, dy)
+    r = X - 0.5 * (nx + 4) * dx
+    env = np.exp(-(r * r) / 0.04)
+    z = np.zeros_like(env)
+    return _from_primitive(env + 0.1, 0.05 * np.sin(2.0 * PI_REF * Y) * env, z, env + 0.1, z, env)
+
+
+def fourier_ensemble_member(n: int, rng: np.random.Generator, modes: int = 8, amp: float = 1e-2, kmax: int = 4):
+    """Ensemble member on [0, 2 pi)^2: rho = 1, p = 1, B = (1, 0.5) plus seeded
+    Fourier perturbations of the velocity and of B (B through a corner vector
+    potential, so div B = 0 to round-off)."""
+    dx = grid_spacing(n)
+    X, Y = _centres(n, n, dx, dx)
+    XC, YC = _corners(n, n, dx, dx)
+    L = n * dx
+    vx = np.zeros_like(X); vy = np.zeros_like(X)
+    Az = np.zeros_like(XC)
+    for _ in range(modes):
+        while True:
+            kx, ky = rng.integers(-kmax, kmax + 1, size=2)
+            if 0 < kx * kx + ky * ky <= kmax * kmax:
+                break
+        ph = rng.uniform(0, 2 * np.pi, size=3)
+        a = amp * rng.uniform(-1, 1, size=3)
+        arg = 2 * np.pi * (kx * X + ky * Y) / L
+        vx += a[0] * np.sin(arg + ph[0])
+        vy += a[1] * np.sin(arg + ph[1])
+        k2 = (2 * np.pi / L) * np.sqrt(kx * kx + ky * ky)
+        Az += (a[2] / k2) * np.cos(2 * np.pi * (kx * XC + ky * YC) / L + ph[2])
+    Az += 1.0 * YC - 0.5 * XC           # uniform B = (1, 0.5)
+    bx, by = _face_b_from_potential(Az, dx, dx)
+    return _from_primitive(1.0, vx, vy, 1.0, bx, by)
+
+
+SETUPS = {
+    "orszag_tang": orszag_tang,
+    "decaying_sine": decaying_sine,
+    "alfven": alfven,
+    "gaussian": gaussian_pulse,
+    "acoustic": acoustic,
+    "sausage": sausage,
+    "kink": kink,
+}

@@ -1,0 +1,213 @@
+"""Host-side mirror of the reference's step interface over the gfx950 C ABI.
+
+The reference (RikibabManos/Magnetohydrodynamics_Simulator; citations relative
+to its repository) is a header-only C++ program whose hot path is a set of
+free functions over a padded ``Grid`` (include/Grid.hpp:6-62):
+``IMEX`` (include/solvers.hpp:491), ``explicit_rhs`` (:393),
+``resistive_B_term`` (:445), ``generate_A_matrix`` (:404),
+``CFL_condition_checker`` and ``calculate_current_div_B``
+(include/diagnostics.hpp:44,15).  This module keeps those names and argument
+meanings; the Eigen solver and the scratch structs become a ``Context`` that
+owns the device state.  All arrays at this boundary are padded (ny+4, nx+4)
+float64, exactly the reference Grid's storage.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import _lib
+
+GAMMA_ARS = (2.0 - math.sqrt(2.0)) / 2.0      # IMEX_Butcher_Tableus::gamma, solvers.hpp:41
+PI_REF = 3.14159265359                         # integrator_main.cpp:36
+
+_dp = C.POINTER(C.c_double)
+
+
+def _ptr(a: np.ndarray | None):
+    if a is None:
+        return None
+    if a.dtype != np.float64 or not a.flags.c_contiguous:
+        raise TypeError("arrays must be C-contiguous float64")
+    return a.ctypes.data_as(_dp)
+
+
+class Grid:
+    """Host copy of the reference Grid: 12 padded float64 arrays, ng = 2."""
+
+    FIELDS = ("rho", "mx", "my", "E", "bxc", "byc", "bxf", "byf", "Ez", "vx", "vy", "p")
+
+    def __init__(self, nx: int, ny: int, ng: int = 2):
+        if ng != 2:
+            raise ValueError("the MUSCL path uses ng = 2 (Grid.hpp:29)")
+        self.nx, self.ny, self.ng = nx, ny, ng
+        self.nxt, self.nyt = nx + 2 * ng, ny + 2 * ng
+        self.cell_width = 2 * PI_REF / nx
+        self.cell_height = 2 * PI_REF / ny
+        self.resistivity = 0.0
+        for f in self.FIELDS:
+            setattr(self, f, np.zeros((self.nyt, self.nxt)))
+
+    def indexC(self, i: int, j: int) -> int:
+        return j * self.nxt + i
+
+    def physical(self, name: str) -> np.ndarray:
+        return getattr(self, name)[2:-2, 2:-2]
+
+    def copy(self) -> "Grid":
+        g = Grid(self.nx, self.ny, self.ng)
+        g.cell_width, g.cell_height, g.resistivity = self.cell_width, self.cell_height, self.resistivity
+        for f in self.FIELDS:
+            setattr(g, f, getattr(self, f).copy())
+        return g
+
+
+class Context:
+    """Device state + implicit operator of one run (replaces the reference's
+    integrator_reserved_memory, ReconstructedValues, the two scratch Grids and
+    the Eigen ConjugateGradient object)."""
+
+    def __init__(self, nx: int, ny: int, dx: float, dy: float, eta: float = 0.0, *, cfl: float = 0.45,
+                 cg_tol: float | None = None, cg_max_iter: int = 0, cg_fail_is_error: bool = False,
+                 march_rows: int = 0):
+        self.lib = _lib.load()
+        o = _lib.Opts()
+        _lib.check(self.lib.mhd_default_opts(C.byref(o)))
+        o.cfl = cfl
+        if cg_tol is not None:
+            o.cg_tol = cg_tol
+        o.cg_max_iter = cg_max_iter
+        o.cg_fail_is_error = int(cg_fail_is_error)
+        o.march_rows = march_rows
+        h = C.c_void_p()
+        _lib.check(self.lib.mhd_create(nx, ny, dx, dy, eta, C.byref(o), C.byref(h)), None)
+        self.h = h
+        self.nx, self.ny, self.dx, self.dy, self.eta = nx, ny, dx, dy, eta
+        self.nxt, self.nyt = nx + 4, ny + 4
+        self.opts = o
+
+    @classmethod
+    def for_grid(cls, g: Grid, **kw) -> "Context":
+        return cls(g.nx, g.ny, g.cell_width, g.cell_height, g.resistivity, **kw)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.mhd_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        _lib.check(rc, self.h)
+
+    def _padded(self):
+        return np.zeros((self.nyt, self.nxt))
+
+    # ---- state transfer ----
+    def upload(self, rho, mx, my, E, bxf, byf):
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (rho, mx, my, E, bxf, byf)]
+        for a in arrs:
+            if a.shape != (self.nyt, self.nxt):
+                raise ValueError(f"expected padded shape {(self.nyt, self.nxt)}, got {a.shape}")
+        self._check(self.lib.mhd_upload(self.h, *[_ptr(a) for a in arrs]))
+
+    def upload_grid(self, g: Grid):
+        self.upload(g.rho, g.mx, g.my, g.E, g.bxf, g.byf)
+
+    def download(self, primitives: bool = False) -> dict:
+        names = ["rho", "mx", "my", "E", "bxf", "byf"]
+        out = {n: self._padded() for n in names}
+        prim = ["vx", "vy", "p", "bxc", "byc"]
+        if primitives:
+            out.update({n: self._padded() for n in prim})
+        ptrs = [_ptr(out[n]) for n in names] + [(_ptr(out[n]) if primitives else None) for n in prim]
+        self._check(self.lib.mhd_download(self.h, *ptrs))
+        return out
+
+    def download_grid(self, g: Grid):
+        d = self.download(primitives=True)
+        for k, v in d.items():
+            getattr(g, k)[...] = v
+
+    # ---- the reference's hot-path entry points ----
+    def CFL_condition_checker(self) -> float:
+        dt = C.c_double()
+        self._check(self.lib.mhd_cfl_dt(self.h, C.byref(dt)))
+        return dt.value
+
+    def generate_A_matrix(self, alpha: float):
+        self._check(self.lib.mhd_build_operator(self.h, alpha))
+
+    def IMEX(self, dt: float) -> _lib.StepStats:
+        st = _lib.StepStats()
+        self._check(self.lib.mhd_imex_step(self.h, dt, C.byref(st)))
+        return st
+
+    def explicit_rhs(self):
+        outs = [self._padded() for _ in range(6)]
+        self._check(self.lib.mhd_explicit_rhs(self.h, *[_ptr(o) for o in outs]))
+        return outs
+
+    def resistive_B_term(self, implicit: bool):
+        bx, by = self._padded(), self._padded()
+        it = C.c_int()
+        self._check(self.lib.mhd_resistive_term(self.h, int(implicit), _ptr(bx), _ptr(by), C.byref(it)))
+        return bx, by, it.value
+
+    def cg_solve(self, b: np.ndarray):
+        b = np.ascontiguousarray(b, dtype=np.float64).ravel()
+        if b.size != self.nx * self.ny:
+            raise ValueError("b must hold nx * ny corner values")
+        x = np.zeros_like(b)
+        it, err = C.c_int(), C.c_double()
+        self._check(self.lib.mhd_cg_solve(self.h, _ptr(b), _ptr(x), C.byref(it), C.byref(err)))
+        return x, it.value, err.value
+
+    def calculate_current_div_B(self):
+        rms, mx = C.c_double(), C.c_double()
+        self._check(self.lib.mhd_divb(self.h, C.byref(rms), C.byref(mx)))
+        return rms.value, mx.value
+
+    def run(self, t_end: float, dt_override: float = 0.0, max_steps: int = -1):
+        """The driver loop of src/integrator_main.cpp:183-243 (no snapshots)."""
+        cap = max_steps if max_steps > 0 else 1_000_000
+        dts = np.zeros(cap)
+        its = np.zeros(2 * cap, dtype=np.int32)
+        steps, t = C.c_int(), C.c_double()
+        self._check(self.lib.mhd_run(self.h, t_end, dt_override, max_steps, C.byref(steps), C.byref(t),
+                                     _ptr(dts), its.ctypes.data_as(C.POINTER(C.c_int))))
+        n = steps.value
+        return n, dts[:n].copy(), its[: 2 * n].reshape(n, 2).copy(), t.value
+
+    # ---- instrumentation ----
+    def profile(self, on: bool):
+        self._check(self.lib.mhd_profile_enable(self.h, int(on)))
+
+    def profile_get(self, kernel: str):
+        n, ms = C.c_longlong(), C.c_double()
+        self._check(self.lib.mhd_profile_get(self.h, _lib.KERNEL_IDS[kernel], C.byref(n), C.byref(ms)))
+        return n.value, ms.value
+
+    def synchronize(self):
+        self._check(self.lib.mhd_synchronize(self.h))
+
+    def calibration_copy(self, nbytes: int):
+        self._check(self.lib.mhd_calibration_copy(self.h, nbytes))
+
+
+def operator_alpha(dt: float, eta: float) -> float:
+    """alpha of generate_A_matrix as the driver forms it:
+    (dt * resistivity) * A_im[4] (src/integrator_main.cpp:189-190)."""
+    return (dt * eta) * GAMMA_ARS

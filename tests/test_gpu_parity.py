@@ -1,0 +1,166 @@
+"""GPU parity: the gfx950 path against the CPU oracle on identical inputs.
+
+Every case feeds the same padded arrays to the oracle (oracle/mhd_oracle.c,
+the restatement of the reference's solver) and to the HIP library through its
+C ABI, then compares per field with relative L1 = sum|gpu - cpu| / sum|cpu|
+over the physical cells.
+
+Tolerances (FP64 path; the reference build itself is -ffast-math, so the
+contract is a tolerance, not bit equality):
+  * single operator applications: relative L1 <= 1e-13
+  * full runs: relative L1 <= 1e-10 per field,
+    |div B|_inf <= 1e-11, CG iteration counts equal within +-1 per solve.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from helpers import CONSERVED, ic_arrays, interior, make_gpu, make_oracle, rel_l1, symmetry_residual
+
+pytestmark = pytest.mark.gpu
+
+TOL_OP = 1e-13
+TOL_RUN = 1e-10
+
+
+def _say(msg):
+    print(f"\n[case] {msg}", flush=True)
+
+
+@pytest.mark.parametrize("name,nx,ny,seed", [
+    ("orszag_tang", 64, 64, None),
+    ("orszag_tang", 48, 36, 1),
+    ("kink", 40, 70, 2),
+    ("decaying_sine", 5, 7, 3),
+])
+def test_explicit_rhs(name, nx, ny, seed):
+    _say(f"explicit_rhs {name} {nx}x{ny} seed={seed}")
+    ic, dx, dy = ic_arrays(name, nx, ny, seed=seed, amp=1e-2 if seed else 0.0)
+    sim = make_oracle(ic, nx, ny, dx, dy, 0.0)
+    ref = sim.explicit_rhs()
+    with make_gpu(ic, nx, ny, dx, dy, 0.0) as ctx:
+        got = ctx.explicit_rhs()
+    for k, (g, r) in enumerate(zip(got, ref)):
+        e = rel_l1(interior(g), interior(r))
+        assert e <= TOL_OP, (CONSERVED[k], e)
+
+
+def test_cfl_and_divb():
+    _say("cfl/divb orszag_tang 96x80")
+    nx, ny = 96, 80
+    ic, dx, dy = ic_arrays("orszag_tang", nx, ny, seed=5, amp=1e-3)
+    sim = make_oracle(ic, nx, ny, dx, dy, 1e-2)
+    sim.fill_ghost()
+    sim.primitives()
+    for cfl in (0.45, 0.4):
+        with make_gpu(ic, nx, ny, dx, dy, 1e-2, cfl=cfl) as ctx:
+            dt = ctx.CFL_condition_checker()
+            rms, mx = ctx.calculate_current_div_B()
+        assert abs(dt - sim.cfl(cfl)) <= 1e-14 * dt
+    rr, rm = sim.divb()
+    assert abs(mx - rm) <= 1e-12 * max(rm, 1e-300) + 1e-300
+    assert abs(rms - rr) <= 1e-12 * max(rr, 1e-300) + 1e-300
+
+
+@pytest.mark.parametrize("nx,ny,alpha", [(32, 32, 1e-3), (40, 24, 5e-2), (64, 48, 1.0)])
+def test_cg_solve(nx, ny, alpha):
+    _say(f"cg_solve {nx}x{ny} alpha={alpha}")
+    rng = np.random.default_rng(nx + ny)
+    dx, dy = 2 * math.pi / nx, 2 * math.pi / ny
+    b = rng.standard_normal(nx * ny)
+    sim = po_sim = make_oracle({k: np.zeros((ny + 4, nx + 4)) for k in CONSERVED}, nx, ny, dx, dy, 0.0, tol=1e-10)
+    sim.build_operator(alpha)
+    xr, itr, _, er = po_sim.cg_solve(b)
+    with make_gpu({k: np.zeros((ny + 4, nx + 4)) for k in CONSERVED}, nx, ny, dx, dy, 0.0, tol=1e-10) as ctx:
+        ctx.generate_A_matrix(alpha)
+        xg, itg, eg = ctx.cg_solve(b)
+    assert abs(itg - itr) <= 1, (itg, itr)
+    if itg == itr:
+        assert rel_l1(xg, xr) <= 1e-13
+    else:
+        assert rel_l1(xg, xr) <= 1e-9
+
+
+@pytest.mark.parametrize("eta", [1e-2, 0.5])
+def test_resistive_term(eta):
+    _say(f"resistive_term eta={eta}")
+    nx, ny = 48, 40
+    ic, dx, dy = ic_arrays("orszag_tang", nx, ny, seed=7, amp=1e-2)
+    sim = make_oracle(ic, nx, ny, dx, dy, eta, tol=1e-12)
+    sim.fill_ghost()
+    alpha = 1e-3 * eta * 0.2928932188134524
+    sim.build_operator(alpha)
+    with make_gpu(ic, nx, ny, dx, dy, eta, tol=1e-12) as ctx:
+        ctx.generate_A_matrix(alpha)
+        for implicit in (False, True):
+            gx, gy, gi = ctx.resistive_B_term(implicit)
+            rx, ry, ri = sim.resistive_term(implicit)
+            assert abs(gi - ri) <= 1
+            assert rel_l1(interior(gx), interior(rx)) <= 1e-12
+            assert rel_l1(interior(gy), interior(ry)) <= 1e-12
+
+
+@pytest.mark.parametrize("eta,tol", [(0.0, None), (1e-2, 1e-10), (1e-3, None)])
+def test_imex_single_step(eta, tol):
+    _say(f"imex one step OT 64x56 eta={eta} tol={tol}")
+    nx, ny = 64, 56
+    ic, dx, dy = ic_arrays("orszag_tang", nx, ny)
+    sim = make_oracle(ic, nx, ny, dx, dy, eta, tol=tol)
+    sim.fill_ghost()
+    sim.primitives()
+    dt = sim.cfl(0.45)
+    alpha = (dt * eta) * 0.2928932188134524
+    sim.build_operator(alpha)
+    st_r = sim.imex(dt)
+    with make_gpu(ic, nx, ny, dx, dy, eta, tol=tol) as ctx:
+        ctx.generate_A_matrix(alpha)
+        st_g = ctx.IMEX(dt)
+        got = ctx.download()
+    for k in CONSERVED:
+        e = rel_l1(interior(got[k]), sim.interior(k))
+        assert e <= TOL_OP, (k, e)
+    for s in range(2):
+        assert abs(st_g.cg_iters[s] - st_r.cg_iters[s]) <= 1
+
+
+def test_config1_orszag_tang_128():
+    """Plain method, CPU-runnable workload: OT 128^2, eta = 0, CFL 0.4, to t = 0.5."""
+    _say("config1 OT 128^2 to t=0.5")
+    n = 128
+    ic, dx, dy = ic_arrays("orszag_tang", n, n)
+    sim = make_oracle(ic, n, n, dx, dy, 0.0)
+    nr, dts_r, _, tr = sim.run(0.5, cfl=0.4)
+    with make_gpu(ic, n, n, dx, dy, 0.0, cfl=0.4) as ctx:
+        ng, dts_g, _, tg = ctx.run(0.5)
+        got = ctx.download()
+        _, divmax = ctx.calculate_current_div_B()
+    assert ng == nr and tg == pytest.approx(tr, abs=1e-15)
+    assert np.allclose(dts_g, dts_r, rtol=1e-12, atol=0)
+    for k in CONSERVED:
+        e = rel_l1(interior(got[k]), sim.interior(k))
+        assert e <= TOL_RUN, (k, e)
+    assert divmax <= 1e-11
+    ref_fields = {k: sim.field(k) for k in CONSERVED}
+    assert symmetry_residual(got) <= 1e-9
+    assert abs(symmetry_residual(got) - symmetry_residual(ref_fields)) <= 1e-9
+
+
+def test_config2_resistive_small():
+    """Plain method at scale, reduced: OT 192^2, eta = 1e-3, CG tol 1e-10, CFL 0.4, 10 steps."""
+    _say("config2-reduced OT 192^2 eta=1e-3 10 steps")
+    n = 192
+    ic, dx, dy = ic_arrays("orszag_tang", n, n)
+    sim = make_oracle(ic, n, n, dx, dy, 1e-3, tol=1e-10)
+    nr, dts_r, it_r, _ = sim.run(10.0, cfl=0.4, max_steps=10)
+    with make_gpu(ic, n, n, dx, dy, 1e-3, tol=1e-10, cfl=0.4) as ctx:
+        ng, dts_g, it_g, _ = ctx.run(10.0, max_steps=10)
+        got = ctx.download()
+        _, divmax = ctx.calculate_current_div_B()
+    assert ng == nr == 10
+    assert np.abs(it_g - it_r).max() <= 1, (it_g, it_r)
+    assert it_g.min() > 0
+    for k in CONSERVED:
+        e = rel_l1(interior(got[k]), sim.interior(k))
+        assert e <= TOL_RUN, (k, e)
+    assert divmax <= 1e-11
