@@ -1,0 +1,185 @@
+"""Throughput of the FP64 ARS(2,2,2) IMEX step on one MI355X.
+
+Workload: resistive Orszag-Tang vortex, 4096 x 4096 cells, eta = 1e-3,
+Jacobi-PCG relative tolerance 1e-10, CFL 0.4 -- the largest single-GPU
+configuration of the "plain method at scale" workload.  A step is one pass of
+the reference driver loop (src/integrator_main.cpp:207-231): CFL time step,
+operator rebuild when dt changed, one IMEX step with its two implicit solves.
+Inputs are resident in HBM before the timed region.
+
+Prints one JSON line: Mcell-updates/s (nx * ny * steps / s / 1e6) with
+  roofline      the dominant kernel's algorithmic bytes per launch over its
+                mean launch time (HIP events on the library's stream), against
+                the 8 TB/s HBM peak;
+  cpu_baseline  the CPU restatement of the reference step (oracle/, built with
+                the reference's -O3 -ffast-math and -march=native on this host),
+                single-threaded like the reference, on a bounded sample of the
+                same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+# Algorithmic HBM bytes per cell of each kernel (FP64 fields, 8 B each):
+#   stage1   reads U (6 fields), writes f1 (6) and U2* (6)           = 144
+#   stage2   reads U2 (6), U^n (6), f1 (6), writes U3* (6)           = 192
+#   cg_init  reads face B (2), writes r, p                            = 32
+#   cg_pass_a (iteration > 1) reads r, p_old, writes p_new           = 24;  first iteration reads p = 8
+#   cg_pass_b (iteration > 1) reads p, x, r, writes x, r             = 40;  first iteration (x = 0) = 32
+#   finalize1 / finalize2  read x and face B (3), write face B (2)   = 40
+#   cfl      reads U (6)                                              = 48
+STAGE_BYTES = {"stage1": 144, "stage2": 192, "cg_init": 32, "implicit_finalize1": 40,
+               "implicit_finalize2": 40, "cfl": 48}
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=4096)
+    ap.add_argument("--eta", type=float, default=1e-3)
+    ap.add_argument("--cg-tol", type=float, default=1e-10)
+    ap.add_argument("--cfl", type=float, default=0.4)
+    ap.add_argument("--cpu-steps", type=int, default=2, help="CPU-baseline sample (steps at --n)")
+    ap.add_argument("--no-cpu", action="store_true")
+    return ap.parse_args()
+
+
+def cpu_baseline(n: int, eta: float, tol: float, cfl: float, steps: int) -> dict:
+    """Time the oracle's reference-faithful driver loop on one host core."""
+    from magnetohydrodynamics_simulator_amd import setups
+    from oracle import pyoracle as po
+
+    lib_path, flags = None, "-O3 -ffast-math -march=x86-64-v3 (prebuilt)"
+    out_dir = os.path.join(ROOT, "gpurun_out") if os.path.isdir(os.path.join(ROOT, "gpurun_out")) else "/tmp"
+    try:
+        po.build(fast_native=True, out_dir=out_dir)
+        lib_path, flags = os.path.join(out_dir, "libmhd_oracle_native.so"), "-O3 -ffast-math -march=native"
+    except Exception:
+        pass
+    lib = po.load("fast", path=lib_path) if lib_path else po.load("fast")
+    dx = setups.grid_spacing(n)
+    ic = setups.orszag_tang(n)
+    sim = po.Sim(n, n, dx, dx, eta, lib=lib)
+    sim.set_cg(tol, -1)
+    for k in ("rho", "mx", "my", "E", "bxf", "byf"):
+        sim.field(k)[...] = ic[k]
+    del ic
+    t0 = time.perf_counter()
+    sim.run(1e9, cfl=cfl, max_steps=0, snapshot_every=0)           # driver set-up only
+    t_setup = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    nsteps, _, its, _ = sim.run(1e9, cfl=cfl, max_steps=steps, snapshot_every=0)
+    t_run = time.perf_counter() - t0 - t_setup
+    sim.close()
+    cpu = subprocess.run(["lscpu"], capture_output=True, text=True).stdout
+    model = next((ln.split(":", 1)[1].strip() for ln in cpu.splitlines() if ln.startswith("Model name")), "unknown")
+    return {
+        "value": n * n * nsteps / t_run / 1e6,
+        "unit": "Mcell-updates/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"orszag_tang {n}x{n} eta={eta} cg_tol={tol} cfl={cfl}: {nsteps} driver steps "
+                  f"(CFL + CSR rebuild + IMEX), {t_run:.1f} s on 1 thread of {model} "
+                  f"(nproc {os.cpu_count()}); oracle built {flags}; CG iters {its.tolist()}",
+    }
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU only (got --gpus {a.gpus}); multi-GPU runs are not part of this build",
+              file=sys.stderr)
+        sys.exit(2)
+    from magnetohydrodynamics_simulator_amd import Context, setups
+
+    n = a.n
+    dx = setups.grid_spacing(n)
+    ic = setups.orszag_tang(n)
+    ctx = Context(n, n, dx, dx, a.eta, cfl=a.cfl, cg_tol=a.cg_tol)
+    ctx.upload(*[ic[k] for k in ("rho", "mx", "my", "E", "bxf", "byf")])
+    del ic
+    # warm-up: W driver steps
+    if a.warmup > 0:
+        ctx.run(1e9, max_steps=a.warmup)
+    ctx.synchronize()
+    ctx.profile(True)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    steps, dts, its, _ = ctx.run(1e9, max_steps=a.steps)
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    elapsed = t1 - t0
+    prof = {k: ctx.profile_get(k) for k in ("stage1", "stage2", "cg_init", "cg_pass_a", "cg_pass_b",
+                                            "implicit_finalize1", "implicit_finalize2", "cfl")}
+    ctx.profile(False)
+    rms, divmax = ctx.calculate_current_div_B()
+    ctx.close()
+
+    N = n * n
+    solves = 2 * steps
+    spmv = int(its.sum()) + solves          # converged solves: products = iterations + 1
+    kb, kbytes = {}, {}
+    for k, (cnt, ms) in prof.items():
+        if k in STAGE_BYTES:
+            byt = STAGE_BYTES[k] * N * cnt
+        elif k == "cg_pass_a":
+            byt = N * (8 * solves + 24 * (spmv - solves))
+        else:
+            byt = N * (32 * solves + 40 * (spmv - solves))
+        kbytes[k] = byt
+        kb[k] = {"launches": cnt, "ms_total": round(ms, 3), "ms_per_step": round(ms / steps, 4),
+                 "GBs": round(byt / (ms * 1e-3) / 1e9, 1) if ms > 0 else None}
+    dom = max(prof, key=lambda k: prof[k][1])
+    dom_cnt, dom_ms = prof[dom]
+    # launches that found the solve converged return at once and move no bytes;
+    # the per-launch figure averages over every launch of the kernel
+    bytes_per_launch = kbytes[dom] / dom_cnt
+    achieved = bytes_per_launch / (dom_ms * 1e-3 / dom_cnt) / 1e9
+    step_bytes = N * (336 + 144 + 64 * (spmv / steps))
+
+    line = {
+        "metric": "FP64 Mcell-updates/s per IMEX step",
+        "value": round(N * steps / elapsed / 1e6, 2),
+        "unit": "Mcell-updates/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(elapsed / steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"resistive Orszag-Tang {n}x{n}, eta={a.eta}, Jacobi-PCG tol {a.cg_tol}, CFL {a.cfl}",
+                   "nx": n, "ny": n, "eta": a.eta, "cg_tol": a.cg_tol, "cfl": a.cfl,
+                   "parallelism": "single-gpu"},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+        "step_roofline": {"algorithmic_bytes_per_step": int(step_bytes),
+                          "frac": round(step_bytes / (elapsed / steps) / 1e9 / HBM_PEAK_GBS, 4)},
+        "cg_spmv_per_step": round(spmv / steps, 2),
+        "divb_max": divmax,
+        "kernels": kb,
+    }
+    if not a.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(n, a.eta, a.cg_tol, a.cfl, a.cpu_steps)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -41,9 +41,15 @@ def test_explicit_rhs(name, nx, ny, seed):
     ref = sim.explicit_rhs()
     with make_gpu(ic, nx, ny, dx, dy, 0.0) as ctx:
         got = ctx.explicit_rhs()
+    # a field whose rate vanishes analytically (d rho/dt = -rho div v = 0 for
+    # the Orszag-Tang start) holds only round-off: compare it on the scale of
+    # the other rates instead of its own
+    scale = max(np.abs(interior(r)).mean() for r in ref)
     for k, (g, r) in enumerate(zip(got, ref)):
-        e = rel_l1(interior(g), interior(r))
-        assert e <= TOL_OP, (CONSERVED[k], e)
+        gi, ri = interior(g), interior(r)
+        num = np.abs(gi - ri).sum()
+        den = max(np.abs(ri).sum(), 1e-3 * scale * ri.size)
+        assert num / den <= TOL_OP, (CONSERVED[k], num / den)
 
 
 def test_cfl_and_divb():
