@@ -54,16 +54,16 @@ struct mhd_ctx {
   double* x = nullptr;
   double* r = nullptr;
   double* p[2] = {};
-  double* part = nullptr;
+  double* part1 = nullptr;       // CG: one partial per tile
+  double* part2 = nullptr;       // CG: two partials per tile
+  double* part = nullptr;        // diagnostics: two partials per tile
   CgState* st = nullptr;
-  unsigned int* counter = nullptr;
-  double* dout = nullptr;
   CgState* st_host = nullptr;    // pinned
-  double* out_host = nullptr;    // pinned
+  double* part_host = nullptr;   // pinned, two partials per tile
   double alpha = 0.0;
   bool have_op = false;
   int last_spmv = 1;
-  int red_blocks = 1;
+  int tiles = 1, tiles_y = 1, tile_rows = 4;   // 2D sweeps: kTileX columns x tile_rows rows per tile
   int stage_rows = 16;
   std::string err;
   bool prof = false;
@@ -170,14 +170,19 @@ CgArgs cg_args(mhd_ctx* c, const double* bx, const double* by) {
   a.r = c->r;
   a.p0 = c->p[0];
   a.x = c->x;
-  a.part = c->part;
+  a.part1 = c->part1;
+  a.part2 = c->part2;
   a.st = c->st;
+  a.ty = c->tile_rows;
+  a.tiles_y = c->tiles_y;
   return a;
 }
 
 // One conjugate-gradient solve; the right-hand side is eta J(bx, by) or bin.
-// Passes are enqueued in chunks and the device-side done flag is read back
-// between chunks; passes launched after convergence return immediately.
+// Kernel order: init, A(0), then B(it), A(it + 1) pairs.  Each A decides on
+// convergence from the partials of the B before it; passes launched after
+// the decision return at once.  The host reads the state back after each
+// chunk of pairs; the first chunk is as long as the previous solve's count.
 int cg_run(mhd_ctx* c, const double* bx, const double* by, const double* bin, CgState* out) {
   if (!c->have_op) return fail(c, MHD_EINVAL, "implicit solve before mhd_build_operator");
   CgArgs a = cg_args(c, bx, by);
@@ -185,32 +190,32 @@ int cg_run(mhd_ctx* c, const double* bx, const double* by, const double* bin, Cg
   *c->st_host = CgState{};
   c->st_host->max_iters = maxit;
   HIP_OR_FAIL(c, hipMemcpyAsync(c->st, c->st_host, sizeof(CgState), hipMemcpyHostToDevice, c->stream));
-  const int nb = c->red_blocks;
-  int rc = launch(c, 2, [&] { hipLaunchKernelGGL(k_cg_init, dim3(nb), dim3(kRedThreads), 0, c->stream, a, bin, 0); });
+  const int nb = c->tiles;
+  int rc = launch(c, 2, [&] { hipLaunchKernelGGL(k_cg_init, dim3(nb), dim3(kTileX), 0, c->stream, a, bin, 0); });
+  if (rc) return rc;
+  rc = launch(c, 3, [&] {
+    hipLaunchKernelGGL(k_cg_pass_a<true>, dim3(nb), dim3(kTileX), 0, c->stream, a, (const double*)c->p[0], c->p[1], 0);
+  });
   if (rc) return rc;
   long long it = 0;
   for (;;) {
-    int chunk = (it == 0) ? (c->last_spmv > 0 ? c->last_spmv : 1) : 2;
+    const int chunk = (it == 0) ? (c->last_spmv > 0 ? c->last_spmv : 1) : 1;
     for (int k = 0; k < chunk; ++k, ++it) {
-      if (it == 0) {
-        rc = launch(c, 3, [&] {
-          hipLaunchKernelGGL(k_cg_pass_a<true>, dim3(nb), dim3(kRedThreads), 0, c->stream, a, c->p[0], c->p[1]);
-        });
-        if (rc) return rc;
+      // B(it) uses the direction formed by A(it): p[0] for it = 0, else p[it & 1]
+      const double* pv = (it == 0) ? c->p[0] : c->p[it & 1];
+      if (it == 0)
+        rc = launch(c, 4, [&] { hipLaunchKernelGGL(k_cg_pass_b<true>, dim3(nb), dim3(kTileX), 0, c->stream, a, pv, 0); });
+      else
         rc = launch(c, 4, [&] {
-          hipLaunchKernelGGL(k_cg_pass_b<true>, dim3(nb), dim3(kRedThreads), 0, c->stream, a, (const double*)c->p[0]);
+          hipLaunchKernelGGL(k_cg_pass_b<false>, dim3(nb), dim3(kTileX), 0, c->stream, a, pv, (int)it);
         });
-      } else {
-        double* pold = c->p[(it - 1) & 1];
-        double* pnew = c->p[it & 1];
-        rc = launch(c, 3, [&] {
-          hipLaunchKernelGGL(k_cg_pass_a<false>, dim3(nb), dim3(kRedThreads), 0, c->stream, a, (const double*)pold, pnew);
-        });
-        if (rc) return rc;
-        rc = launch(c, 4, [&] {
-          hipLaunchKernelGGL(k_cg_pass_b<false>, dim3(nb), dim3(kRedThreads), 0, c->stream, a, (const double*)pnew);
-        });
-      }
+      if (rc) return rc;
+      // A(it + 1): p_new = invd r + beta p_old, p_old = p[0] for it = 0, else p[it & 1]
+      const double* pold = c->p[it & 1];
+      double* pnew = c->p[(it + 1) & 1];
+      rc = launch(c, 3, [&] {
+        hipLaunchKernelGGL(k_cg_pass_a<false>, dim3(nb), dim3(kTileX), 0, c->stream, a, pold, pnew, (int)(it + 1));
+      });
       if (rc) return rc;
     }
     HIP_OR_FAIL(c, hipMemcpyAsync(c->st_host, c->st, sizeof(CgState), hipMemcpyDeviceToHost, c->stream));
@@ -243,6 +248,8 @@ FinArgs fin_args(mhd_ctx* c, const double* bx, const double* by, double* bxo, do
   f.bx_out = bxo;
   f.by_out = byo;
   f.st = c->st;
+  f.ty = c->tile_rows;
+  f.tiles_y = c->tiles_y;
   return f;
 }
 
@@ -254,9 +261,20 @@ DiagArgs diag_args(mhd_ctx* c) {
   d.dy = c->dy;
   for (int q = 0; q < 6; ++q) d.s[q] = c->U[q];
   d.part = c->part;
-  d.counter = c->counter;
-  d.out = c->dout;
+  d.ty = c->tile_rows;
+  d.tiles_y = c->tiles_y;
   return d;
+}
+
+// Launches a per-tile diagnostic kernel and copies its partials to the host.
+template <typename K>
+int diag_partials(mhd_ctx* c, K kernel, int id) {
+  DiagArgs d = diag_args(c);
+  int rc = launch(c, id, [&] { hipLaunchKernelGGL(kernel, dim3(c->tiles), dim3(kTileX), 0, c->stream, d); });
+  if (rc) return rc;
+  HIP_OR_FAIL(c, hipMemcpyAsync(c->part_host, c->part, 2 * sizeof(double) * c->tiles, hipMemcpyDeviceToHost, c->stream));
+  HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
+  return MHD_OK;
 }
 
 // Host-side periodic ghost fill of one padded array (Grid::fillOneField, Grid.hpp:143-161).
@@ -340,33 +358,58 @@ int mhd_create(int nx, int ny, double dx, double dy, double eta, const mhd_opts*
         hipMalloc(&c->F1[q], fb) != hipSuccess)
       return bail("device allocation failed", MHD_ENOMEM);
   }
+  // 2D sweeps: tiles of kTileX columns x tile_rows rows, about kTargetTiles of them
+  {
+    const int tiles_x = (nx + kTileX - 1) / kTileX;
+    int want_y = kTargetTiles / tiles_x;
+    if (want_y < 1) want_y = 1;
+    int rows = (ny + want_y - 1) / want_y;
+    if (rows < 4) rows = 4;
+    c->tile_rows = rows;
+    c->tiles_y = (ny + rows - 1) / rows;
+    c->tiles = tiles_x * c->tiles_y;
+  }
+  const size_t pb = (size_t)c->tiles * 2 * sizeof(double);
   if (hipMalloc(&c->x, fb) != hipSuccess || hipMalloc(&c->r, fb) != hipSuccess || hipMalloc(&c->p[0], fb) != hipSuccess ||
-      hipMalloc(&c->p[1], fb) != hipSuccess || hipMalloc(&c->part, kMaxRedBlocks * 2 * sizeof(double)) != hipSuccess ||
-      hipMalloc(&c->st, sizeof(CgState)) != hipSuccess || hipMalloc(&c->counter, 64) != hipSuccess ||
-      hipMalloc(&c->dout, 4 * sizeof(double)) != hipSuccess)
+      hipMalloc(&c->p[1], fb) != hipSuccess || hipMalloc(&c->part, pb) != hipSuccess ||
+      hipMalloc(&c->part1, pb) != hipSuccess || hipMalloc(&c->part2, pb) != hipSuccess ||
+      hipMalloc(&c->st, sizeof(CgState)) != hipSuccess)
     return bail("device allocation failed", MHD_ENOMEM);
-  if (hipHostMalloc(&c->st_host, sizeof(CgState)) != hipSuccess || hipHostMalloc(&c->out_host, 4 * sizeof(double)) != hipSuccess)
+  if (hipHostMalloc(&c->st_host, sizeof(CgState)) != hipSuccess || hipHostMalloc(&c->part_host, pb) != hipSuccess)
     return bail("pinned allocation failed", MHD_ENOMEM);
   for (int q = 0; q < 6; ++q) {
     if (hipMemsetAsync(c->U[q], 0, fb, c->stream) != hipSuccess) return bail("hipMemset failed", MHD_EHIP);
   }
-  CgState zero{};
-  if (hipMemcpyAsync(c->st, &zero, sizeof zero, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
-      hipMemsetAsync(c->counter, 0, 64, c->stream) != hipSuccess)
+  *c->st_host = CgState{};
+  if (hipMemcpyAsync(c->st, c->st_host, sizeof(CgState), hipMemcpyHostToDevice, c->stream) != hipSuccess)
     return bail("device init failed", MHD_EHIP);
-  // reduction grid: one block per row band, a multiple of 8 (one band per XCD)
-  int nb = ny < kMaxRedBlocks ? ny : kMaxRedBlocks;
-  if (nb >= 8) nb &= ~7;
-  c->red_blocks = nb;
-  // stage kernel: rows per work item so that ~4096 wavefronts are launched
+  // stage kernel: one wavefront per strip segment of stage_rows rows.  Pick
+  // the segment length so that the launch fills the resident wavefront slots
+  // of the device a whole number of times (a partial last round would idle
+  // most of the chip), with segments of at least 32 rows.
   if (o.march_rows > 0) {
     c->stage_rows = o.march_rows;
   } else {
-    long long strips = (nx + kStripOut - 1) / kStripOut;
-    long long rows = (ny * strips + 4095) / 4096;
-    if (rows < 8) rows = 8;
-    if (rows > 256) rows = 256;
-    c->stage_rows = (int)rows;
+    int dev = 0, ncu = 256, per_cu = 12;
+    (void)hipGetDevice(&dev);
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) == hipSuccess) ncu = prop.multiProcessorCount;
+    int occ = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_stage<2>, 64, 0) == hipSuccess && occ > 0) per_cu = occ;
+    const long long slots = (long long)ncu * per_cu;
+    const long long strips = (nx + kStripOut - 1) / kStripOut;
+    long long best_rows = ny, best_cost = -1;
+    for (long long rounds = 1; rounds <= 64; ++rounds) {
+      long long segs = (slots * rounds) / strips;
+      if (segs < 1) continue;
+      long long rows = (ny + segs - 1) / segs;
+      if (rows < 32 && rounds > 1) break;
+      long long waves = strips * ((ny + rows - 1) / rows);
+      long long r_used = (waves + slots - 1) / slots;
+      long long cost = r_used * (rows + 4);   // row steps of the slowest slot
+      if (best_cost < 0 || cost < best_cost) { best_cost = cost; best_rows = rows; }
+    }
+    c->stage_rows = (int)(best_rows < 4 ? 4 : best_rows);
   }
   if (hipStreamSynchronize(c->stream) != hipSuccess) return bail("device init failed", MHD_EHIP);
   *out = c;
@@ -387,13 +430,12 @@ int mhd_destroy(mhd_ctx* c) {
     if (c->S[q]) (void)hipFree(c->S[q]);
     if (c->F1[q]) (void)hipFree(c->F1[q]);
   }
-  double* bufs[] = {c->x, c->r, c->p[0], c->p[1], c->part, c->dout};
+  double* bufs[] = {c->x, c->r, c->p[0], c->p[1], c->part, c->part1, c->part2};
   for (double* b : bufs)
     if (b) (void)hipFree(b);
   if (c->st) (void)hipFree(c->st);
-  if (c->counter) (void)hipFree(c->counter);
   if (c->st_host) (void)hipHostFree(c->st_host);
-  if (c->out_host) (void)hipHostFree(c->out_host);
+  if (c->part_host) (void)hipHostFree(c->part_host);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return MHD_OK;
@@ -446,15 +488,17 @@ int mhd_download(mhd_ctx* c, double* rho, double* mx, double* my, double* E, dou
 }
 
 static int max_speed(mhd_ctx* c, double* maxs, double* nbad) {
-  DiagArgs d = diag_args(c);
-  int rc = launch(c, 7, [&] {
-    hipLaunchKernelGGL(k_cfl, dim3(c->red_blocks), dim3(kRedThreads), 0, c->stream, d);
-  });
+  int rc = diag_partials(c, k_cfl, 7);
   if (rc) return rc;
-  HIP_OR_FAIL(c, hipMemcpyAsync(c->out_host, c->dout, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-  HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
-  *maxs = c->out_host[0];
-  *nbad = c->out_host[1];
+  // exact: a max (NaN skipped as the reference's `>` skips it) and a count
+  double m = 1e-8, bad = 0.0;
+  for (int b = 0; b < c->tiles; ++b) {
+    const double v = c->part_host[2 * b];
+    if (v > m) m = v;
+    bad += c->part_host[2 * b + 1];
+  }
+  *maxs = m;
+  *nbad = bad;
   return MHD_OK;
 }
 
@@ -509,7 +553,7 @@ int mhd_imex_step(mhd_ctx* c, double dt, mhd_step_stats* stats) {
     st.cg_spmv[0] = cs.spmv;
     st.cg_error[0] = cg_error(cs);
     FinArgs f = fin_args(c, c->S[4], c->S[5], c->S[4], c->S[5], diag_coeff);
-    rc = launch(c, 5, [&] { hipLaunchKernelGGL(k_finalize1, dim3(grid_1d(c->n)), dim3(256), 0, c->stream, f); });
+    rc = launch(c, 5, [&] { hipLaunchKernelGGL(k_finalize1, dim3(c->tiles), dim3(kTileX), 0, c->stream, f); });
     if (rc) return rc;
   }
   // stage 2: U3* = (U + dt A_ex[6] f1) + dt A_ex[7] L(S) [+ dt A_im[7] g2] (solvers.hpp:535-566)
@@ -534,7 +578,7 @@ int mhd_imex_step(mhd_ctx* c, double dt, mhd_step_stats* stats) {
     st.cg_spmv[1] = cs.spmv;
     st.cg_error[1] = cg_error(cs);
     FinArgs f = fin_args(c, c->U[4], c->U[5], c->S[4], c->S[5], diag_coeff);
-    rc = launch(c, 6, [&] { hipLaunchKernelGGL(k_finalize2, dim3(grid_1d(c->n)), dim3(256), 0, c->stream, f); });
+    rc = launch(c, 6, [&] { hipLaunchKernelGGL(k_finalize2, dim3(c->tiles), dim3(kTileX), 0, c->stream, f); });
     if (rc) return rc;
     std::swap(c->U[4], c->S[4]);
     std::swap(c->U[5], c->S[5]);
@@ -575,7 +619,7 @@ int mhd_resistive_term(mhd_ctx* c, int implicit, double* bx_rate, double* by_rat
   } else {
     CgArgs a = cg_args(c, c->U[4], c->U[5]);
     rc = launch(c, 8, [&] {
-      hipLaunchKernelGGL(k_cg_init, dim3(c->red_blocks), dim3(kRedThreads), 0, c->stream, a, (const double*)nullptr, 1);
+      hipLaunchKernelGGL(k_cg_init, dim3(c->tiles), dim3(kTileX), 0, c->stream, a, (const double*)nullptr, 1);
     });
     if (rc) return rc;
     f.x = c->r;
@@ -608,13 +652,15 @@ int mhd_cg_solve(mhd_ctx* c, const double* b, double* x, int* iters, double* err
 
 int mhd_divb(mhd_ctx* c, double* rms, double* maxabs) {
   if (check_ctx(c) || !rms || !maxabs) return fail(c, MHD_EINVAL, "mhd_divb: bad argument");
-  DiagArgs d = diag_args(c);
-  int rc = launch(c, 8, [&] { hipLaunchKernelGGL(k_divb, dim3(c->red_blocks), dim3(kRedThreads), 0, c->stream, d); });
+  int rc = diag_partials(c, k_divb, 8);
   if (rc) return rc;
-  HIP_OR_FAIL(c, hipMemcpyAsync(c->out_host, c->dout, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-  HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
-  *rms = std::sqrt(c->out_host[0] / (double)(c->nx * c->ny));
-  *maxabs = c->out_host[1];
+  double sum = 0.0, mx = 0.0;
+  for (int b = 0; b < c->tiles; ++b) {      // fixed order: deterministic
+    sum += c->part_host[2 * b];
+    if (c->part_host[2 * b + 1] > mx) mx = c->part_host[2 * b + 1];
+  }
+  *rms = std::sqrt(sum / (double)(c->nx * c->ny));
+  *maxabs = mx;
   return MHD_OK;
 }
 

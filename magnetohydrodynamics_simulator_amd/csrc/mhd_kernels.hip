@@ -208,15 +208,18 @@ __global__ __launch_bounds__(64) void k_stage(StageArgs a) {
 template __global__ void k_stage<0>(StageArgs);
 template __global__ void k_stage<1>(StageArgs);
 template __global__ void k_stage<2>(StageArgs);
-
 // ---------------------------------------------------------------------------
-// Block reductions and the last-block-done pattern.  Partial sums are
-// combined in a fixed order, so every reduction is deterministic.
+// Reductions.  A kernel that produces a global sum writes one partial per
+// block; the kernel that consumes the sum re-reduces all partials itself, in
+// a fixed order, in every block.  The kernel boundary is the only
+// synchronisation: no device-scope fence or atomic is used (on gfx950 a
+// device-scope release writes back the whole L2 of the issuing XCD).  Fixed
+// order makes every reduction deterministic.
 // ---------------------------------------------------------------------------
 
 template <int NV>
 __device__ __forceinline__ void block_sum(double (&v)[NV], double* lds) {
-  // wave-level tree (fixed order), then the 4 wave results in order
+  // wave-level tree (fixed order), then the wave results in order; result in thread 0
 #pragma unroll
   for (int q = 0; q < NV; ++q) {
 #pragma unroll
@@ -237,6 +240,7 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double* lds) {
       v[q] = t;
     }
   }
+  __syncthreads();
 }
 
 __device__ __forceinline__ double max_ignoring_nan(double m, double v) { return (v > m) ? v : m; }
@@ -253,24 +257,13 @@ __device__ __forceinline__ void block_max(double& v, double* lds) {
     for (int ww = 1; ww < nw; ++ww) t = max_ignoring_nan(t, lds[ww]);
     v = t;
   }
-}
-
-// Returns true in thread 0 of the block that finished last.
-__device__ __forceinline__ bool arrive_last(unsigned int* counter, unsigned int nblocks, int* lds_flag) {
-  __threadfence();
-  if (threadIdx.x == 0) {
-    unsigned int prev = atomicAdd(counter, 1u);
-    *lds_flag = (prev == nblocks - 1) ? 1 : 0;
-  }
   __syncthreads();
-  bool last = *lds_flag != 0;
-  if (last) __threadfence();
-  return last;
 }
 
-// Sum NV interleaved partial arrays part[b*NV+q] over nb blocks, fixed order.
+// Sum of NV interleaved partial arrays part[b * NV + q] over nb blocks, in a
+// fixed order; every thread of the block receives the totals.
 template <int NV>
-__device__ void final_sum(const double* part, int nb, double (&out)[NV], double* lds) {
+__device__ void sum_partials(const double* __restrict__ part, int nb, double (&out)[NV], double* lds) {
   double v[NV];
 #pragma unroll
   for (int q = 0; q < NV; ++q) v[q] = 0.0;
@@ -278,218 +271,232 @@ __device__ void final_sum(const double* part, int nb, double (&out)[NV], double*
 #pragma unroll
     for (int q = 0; q < NV; ++q) v[q] += part[b * NV + q];
   }
-  __syncthreads();
   block_sum<NV>(v, lds);
+  if (threadIdx.x == 0) {
 #pragma unroll
-  for (int q = 0; q < NV; ++q) out[q] = v[q];
+    for (int q = 0; q < NV; ++q) lds[8 + q] = v[q];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < NV; ++q) out[q] = lds[8 + q];
+  __syncthreads();
 }
 
 // Virtual block index: consecutive workgroup ids are dealt to the 8 XCDs in
-// turn; remapping gives each XCD (and its L2) a contiguous band of rows.
+// turn; remapping gives each XCD (and its L2) a contiguous run of tiles.
 __device__ __forceinline__ int xcd_block(int b, int nb) {
   if ((nb & 7) != 0) return b;
   return (b & 7) * (nb >> 3) + (b >> 3);
 }
 
-// ---------------------------------------------------------------------------
-// Matrix-free Jacobi-preconditioned CG on the corner unknowns, restating
-// Eigen's ConjugateGradient (zero initial guess, |r|^2 < max(tol^2|b|^2,
-// DBL_MIN), iteration count of completed non-final iterations) on the
-// operator of generate_A_matrix (solvers.hpp:404-443):
-//   (A x)_ij = d x_ij - cx (x_{i-1,j} + x_{i+1,j}) - cy (x_{i,j-1} + x_{i,j+1}),
-//   d = 1 + 2 (cx + cy), cx = alpha / dx^2, cy = alpha / dy^2, periodic.
-// The Jacobi preconditioner is the constant 1/d.  Each iteration is two
-// passes with one grid-wide reduction each; the stencil product is recomputed
-// in the second pass instead of being stored.
-// ---------------------------------------------------------------------------
+// Tile of a 2D sweep: 256 consecutive columns x `ty` rows; tiles are numbered
+// with y fastest so that vertically adjacent tiles run on the same XCD.
+struct Tile {
+  int vb, i, ic, im, ip, y0, y1;
+  bool valid;
+};
 
-__device__ __forceinline__ void rows_of_block(int vb, int nb, int ny, int& r0, int& r1) {
-  int per = (ny + nb - 1) / nb;
-  r0 = vb * per;
-  r1 = min(r0 + per, ny);
+__device__ __forceinline__ Tile tile_of(int nx, int ny, int ty, int tiles_y) {
+  Tile t;
+  t.vb = xcd_block(blockIdx.x, gridDim.x);
+  const int bx = t.vb / tiles_y, by = t.vb - bx * tiles_y;
+  t.i = bx * kTileX + threadIdx.x;
+  t.valid = t.i < nx;
+  t.ic = t.valid ? t.i : nx - 1;
+  t.im = (t.ic == 0) ? nx - 1 : t.ic - 1;
+  t.ip = (t.ic == nx - 1) ? 0 : t.ic + 1;
+  t.y0 = by * ty;
+  t.y1 = min(t.y0 + ty, ny);
+  return t;
 }
 
-// b = eta * J(B) at corners (resistive_B_term's first loop, solvers.hpp:450-463)
-// or b = bin; then r = b, p = invd * r; sums |b|^2 and r.p.
-__global__ __launch_bounds__(kRedThreads) void k_cg_init(CgArgs a, const double* __restrict__ bin, int explicit_only) {
+// ---------------------------------------------------------------------------
+// Matrix-free Jacobi-preconditioned CG on the corner unknowns, restating
+// Eigen's ConjugateGradient (zero initial guess, stop when
+// |r|^2 < max(tol^2 |b|^2, DBL_MIN), iteration count = completed non-final
+// iterations, maxIter = 2 N) on the operator of generate_A_matrix
+// (solvers.hpp:404-443):
+//   (A x)_ij = d x_ij - cx (x_{i-1,j} + x_{i+1,j}) - cy (x_{i,j-1} + x_{i,j+1}),
+//   d = 1 + 2 (cx + cy), cx = alpha / dx^2, cy = alpha / dy^2, periodic.
+// The Jacobi preconditioner is the constant 1/d.  Kernel sequence of a solve:
+//   init, A(0), B(0), A(1), B(1), ...
+// A(it) reduces the partials of the previous kernel, takes the convergence
+// decision and beta, then forms p = invd r + beta p_old and sums p . Ap.
+// B(it) forms alpha, x += alpha p, r -= alpha Ap and sums |r|^2, r . (invd r).
+// The stencil product is recomputed in B instead of being stored.  Block 0
+// records the scalar state; scalars written by a kernel are read only by
+// later kernels (abs_new is double-buffered by iteration parity).
+// ---------------------------------------------------------------------------
+
+// A p with the accumulation order of a row of the assembled operator
+// (south, west, centre, east, north), identical in both passes.
+__device__ __forceinline__ double stencil(double ps, double pw, double pc, double pe, double pn, double cx,
+                                          double cy, double d) {
+  double q = -cy * ps;
+  q = fma(-cx, pw, q);
+  q = fma(d, pc, q);
+  q = fma(-cx, pe, q);
+  q = fma(-cy, pn, q);
+  return q;
+}
+
+// b = eta J(B) at corners (resistive_B_term's first loop, solvers.hpp:450-463)
+// or b = bin; r = b, p = invd b; partials |b|^2 and b . p.
+__global__ __launch_bounds__(kTileX) void k_cg_init(CgArgs a, const double* __restrict__ bin, int explicit_only) {
   __shared__ double lds[16];
-  __shared__ int lflag;
   const int nx = a.nx, ny = a.ny;
-  const int vb = xcd_block(blockIdx.x, gridDim.x);
-  int r0, r1;
-  rows_of_block(vb, gridDim.x, ny, r0, r1);
+  const Tile t = tile_of(nx, ny, a.ty, a.tiles_y);
   double acc[2] = {0.0, 0.0};
-  for (int j = r0; j < r1; ++j) {
-    const int jm = (j == 0) ? ny - 1 : j - 1;
-    for (int i = threadIdx.x; i < nx; i += blockDim.x) {
-      const int im = (i == 0) ? nx - 1 : i - 1;
-      const int o = j * nx + i;
+  if (t.y0 < ny) {
+    double bx_below = bin ? 0.0 : a.bx[(t.y0 == 0 ? ny - 1 : t.y0 - 1) * nx + t.ic];
+    for (int j = t.y0; j < t.y1; ++j) {
+      const int o = j * nx + t.ic;
       double b;
       if (bin) {
         b = bin[o];
       } else {
-        double dBy_dx = (a.by[o] - a.by[j * nx + im]) / a.dx;
-        double dBx_dy = (a.bx[o] - a.bx[jm * nx + i]) / a.dy;
+        const double bxc = a.bx[o];
+        const double dBy_dx = (a.by[o] - a.by[j * nx + t.im]) / a.dx;
+        const double dBx_dy = (bxc - bx_below) / a.dy;
         b = a.eta * (dBy_dx - dBx_dy);
+        bx_below = bxc;
       }
-      a.r[o] = b - 0.0;
-      if (!explicit_only) {
-        double p = a.invd * b;
-        a.p0[o] = p;
-        acc[0] += b * b;
-        acc[1] += b * p;
+      if (t.valid) {
+        a.r[o] = b - 0.0;
+        if (!explicit_only) {
+          const double p = a.invd * b;
+          a.p0[o] = p;
+          acc[0] += b * b;
+          acc[1] += b * p;
+        }
       }
     }
   }
   if (explicit_only) return;
   block_sum<2>(acc, lds);
   if (threadIdx.x == 0) {
-    a.part[vb * 2 + 0] = acc[0];
-    a.part[vb * 2 + 1] = acc[1];
-  }
-  if (!arrive_last(&a.st->counter, gridDim.x, &lflag)) return;
-  double tot[2];
-  final_sum<2>(a.part, gridDim.x, tot, lds);
-  if (threadIdx.x == 0) {
-    CgState* st = a.st;
-    st->counter = 0;
-    st->iters = 0;
-    st->spmv = 0;
-    st->rhs_norm2 = tot[0];
-    st->rn2 = tot[0];
-    st->abs_new = tot[1];
-    st->alpha = 0.0;
-    st->beta = 0.0;
-    if (tot[0] == 0.0) {
-      st->done = 3;                       // x = 0, iters = 0, error = 0
-    } else {
-      double thr = dmax(a.tol * a.tol * tot[0], 2.2250738585072014e-308);
-      st->threshold = thr;
-      if (tot[0] < thr) st->done = 1;     // residual below threshold at x = 0
-      else if (st->max_iters <= 0) st->done = 2;
-      else st->done = 0;
-    }
+    a.part2[t.vb * 2 + 0] = acc[0];
+    a.part2[t.vb * 2 + 1] = acc[1];
   }
 }
 
-// Pass A: p_new = invd * r + beta * p_old (p_new = p for the first iteration),
-// q = A p_new, sum p_new . q -> alpha = absNew / (p . q).
 template <bool FIRST>
-__global__ __launch_bounds__(kRedThreads) void k_cg_pass_a(CgArgs a, const double* __restrict__ pold,
-                                                          double* __restrict__ pnew) {
+__global__ __launch_bounds__(kTileX) void k_cg_pass_a(CgArgs a, const double* __restrict__ pold,
+                                                      double* __restrict__ pnew, int it) {
   __shared__ double lds[16];
-  __shared__ int lflag;
-  if (a.st->done) return;
+  CgState* st = a.st;
+  if (st->done) return;
+  double rr[2];
+  sum_partials<2>(a.part2, gridDim.x, rr, lds);
+  const Tile t = tile_of(a.nx, a.ny, a.ty, a.tiles_y);
+  const bool lead = (t.vb == 0) && (threadIdx.x == 0);
+  double beta = 0.0;
+  if (FIRST) {
+    const double rhs = rr[0];
+    const double thr = dmax(a.tol * a.tol * rhs, 2.2250738585072014e-308);
+    int done = 0;
+    if (rhs == 0.0) done = 3;                 // x = 0, no iteration
+    else if (rhs < thr) done = 1;             // residual below threshold at x = 0
+    else if (st->max_iters <= 0) done = 2;
+    if (lead) {
+      st->rhs_norm2 = rhs;
+      st->threshold = thr;
+      st->rn2 = rhs;
+      st->abs_new[0] = rr[1];
+      st->iters = 0;
+      if (done) st->done = done;
+    }
+    if (done) return;
+  } else {
+    const double rn2 = rr[0];
+    if (rn2 < st->threshold) {                // converged after B(it - 1)
+      if (lead) { st->rn2 = rn2; st->iters = it - 1; st->done = 1; }
+      return;
+    }
+    const double abs_old = st->abs_new[(it - 1) & 1];
+    beta = rr[1] / abs_old;
+    if (lead) { st->rn2 = rn2; st->iters = it; st->abs_new[it & 1] = rr[1]; st->beta = beta; }
+    if (it >= st->max_iters) {
+      if (lead) st->done = 2;
+      return;
+    }
+  }
   const int nx = a.nx, ny = a.ny;
-  const double beta = FIRST ? 0.0 : a.st->beta;
   const double invd = a.invd, d = a.diag, cx = a.cx, cy = a.cy;
-  const int vb = xcd_block(blockIdx.x, gridDim.x);
-  int r0, r1;
-  rows_of_block(vb, gridDim.x, ny, r0, r1);
   double acc[1] = {0.0};
-  for (int j = r0; j < r1; ++j) {
-    const int jm = (j == 0) ? ny - 1 : j - 1;
-    const int jp = (j == ny - 1) ? 0 : j + 1;
-    for (int i = threadIdx.x; i < nx; i += blockDim.x) {
-      const int im = (i == 0) ? nx - 1 : i - 1;
-      const int ip = (i == nx - 1) ? 0 : i + 1;
-      const int o = j * nx + i;
-      double pc, ps, pw, pe, pn;
-      if (FIRST) {
-        pc = pold[o]; ps = pold[jm * nx + i]; pw = pold[j * nx + im]; pe = pold[j * nx + ip]; pn = pold[jp * nx + i];
-      } else {
-        pc = invd * a.r[o] + beta * pold[o];
-        ps = invd * a.r[jm * nx + i] + beta * pold[jm * nx + i];
-        pw = invd * a.r[j * nx + im] + beta * pold[j * nx + im];
-        pe = invd * a.r[j * nx + ip] + beta * pold[j * nx + ip];
-        pn = invd * a.r[jp * nx + i] + beta * pold[jp * nx + i];
-        pnew[o] = pc;
+  if (t.y0 < ny) {
+    auto pv = [&](int o) -> double { return FIRST ? pold[o] : fma(beta, pold[o], invd * a.r[o]); };
+    const int jm0 = (t.y0 == 0) ? ny - 1 : t.y0 - 1;
+    double pm = pv(jm0 * nx + t.ic);
+    double pc = pv(t.y0 * nx + t.ic);
+    for (int j = t.y0; j < t.y1; ++j) {
+      const int jp = (j == ny - 1) ? 0 : j + 1;
+      const double pn = pv(jp * nx + t.ic);
+      const double pw = pv(j * nx + t.im);
+      const double pe = pv(j * nx + t.ip);
+      const double q = stencil(pm, pw, pc, pe, pn, cx, cy, d);
+      if (t.valid) {
+        if (!FIRST) pnew[j * nx + t.ic] = pc;
+        acc[0] += pc * q;
       }
-      double q = -cy * ps;
-      q += -cx * pw;
-      q += d * pc;
-      q += -cx * pe;
-      q += -cy * pn;
-      acc[0] += pc * q;
+      pm = pc;
+      pc = pn;
     }
   }
   block_sum<1>(acc, lds);
-  if (threadIdx.x == 0) a.part[vb] = acc[0];
-  if (!arrive_last(&a.st->counter, gridDim.x, &lflag)) return;
-  double tot[1];
-  final_sum<1>(a.part, gridDim.x, tot, lds);
-  if (threadIdx.x == 0) {
-    a.st->counter = 0;
-    a.st->pap = tot[0];
-    a.st->alpha = a.st->abs_new / tot[0];
-  }
+  if (threadIdx.x == 0) a.part1[t.vb] = acc[0];
 }
 
-// Pass B: q = A p, x += alpha p, r -= alpha q, sums |r|^2 and r . (invd r);
-// convergence test and beta.
 template <bool FIRST>
-__global__ __launch_bounds__(kRedThreads) void k_cg_pass_b(CgArgs a, const double* __restrict__ pv) {
+__global__ __launch_bounds__(kTileX) void k_cg_pass_b(CgArgs a, const double* __restrict__ pv, int it) {
   __shared__ double lds[16];
-  __shared__ int lflag;
-  if (a.st->done) return;
+  CgState* st = a.st;
+  if (st->done) return;
+  double pq[1];
+  sum_partials<1>(a.part1, gridDim.x, pq, lds);
+  const Tile t = tile_of(a.nx, a.ny, a.ty, a.tiles_y);
+  const double alpha = st->abs_new[it & 1] / pq[0];
+  if (t.vb == 0 && threadIdx.x == 0) {
+    st->pap = pq[0];
+    st->alpha = alpha;
+    st->spmv = it + 1;
+  }
   const int nx = a.nx, ny = a.ny;
-  const double alpha = a.st->alpha;
   const double invd = a.invd, d = a.diag, cx = a.cx, cy = a.cy;
-  const int vb = xcd_block(blockIdx.x, gridDim.x);
-  int r0, r1;
-  rows_of_block(vb, gridDim.x, ny, r0, r1);
   double acc[2] = {0.0, 0.0};
-  for (int j = r0; j < r1; ++j) {
-    const int jm = (j == 0) ? ny - 1 : j - 1;
-    const int jp = (j == ny - 1) ? 0 : j + 1;
-    for (int i = threadIdx.x; i < nx; i += blockDim.x) {
-      const int im = (i == 0) ? nx - 1 : i - 1;
-      const int ip = (i == nx - 1) ? 0 : i + 1;
-      const int o = j * nx + i;
-      const double pc = pv[o];
-      double q = -cy * pv[jm * nx + i];
-      q += -cx * pv[j * nx + im];
-      q += d * pc;
-      q += -cx * pv[j * nx + ip];
-      q += -cy * pv[jp * nx + i];
-      const double xo = FIRST ? 0.0 : a.x[o];
-      a.x[o] = xo + alpha * pc;
-      const double rn = a.r[o] - alpha * q;
-      a.r[o] = rn;
-      const double z = invd * rn;
-      acc[0] += rn * rn;
-      acc[1] += rn * z;
+  if (t.y0 < ny) {
+    const int jm0 = (t.y0 == 0) ? ny - 1 : t.y0 - 1;
+    double pm = pv[jm0 * nx + t.ic];
+    double pc = pv[t.y0 * nx + t.ic];
+    for (int j = t.y0; j < t.y1; ++j) {
+      const int jp = (j == ny - 1) ? 0 : j + 1;
+      const int o = j * nx + t.ic;
+      const double pn = pv[jp * nx + t.ic];
+      const double q = stencil(pm, pv[j * nx + t.im], pc, pv[j * nx + t.ip], pn, cx, cy, d);
+      if (t.valid) {
+        const double xo = FIRST ? 0.0 : a.x[o];
+        a.x[o] = xo + alpha * pc;
+        const double rn = a.r[o] - alpha * q;
+        a.r[o] = rn;
+        const double z = invd * rn;
+        acc[0] += rn * rn;
+        acc[1] += rn * z;
+      }
+      pm = pc;
+      pc = pn;
     }
   }
   block_sum<2>(acc, lds);
   if (threadIdx.x == 0) {
-    a.part[vb * 2 + 0] = acc[0];
-    a.part[vb * 2 + 1] = acc[1];
-  }
-  if (!arrive_last(&a.st->counter, gridDim.x, &lflag)) return;
-  double tot[2];
-  final_sum<2>(a.part, gridDim.x, tot, lds);
-  if (threadIdx.x == 0) {
-    CgState* st = a.st;
-    st->counter = 0;
-    st->spmv += 1;
-    st->rn2 = tot[0];
-    if (tot[0] < st->threshold) {
-      st->done = 1;
-    } else {
-      double abs_old = st->abs_new;
-      st->abs_new = tot[1];
-      st->beta = tot[1] / abs_old;
-      st->iters += 1;
-      if (st->iters >= st->max_iters) st->done = 2;
-    }
+    a.part2[t.vb * 2 + 0] = acc[0];
+    a.part2[t.vb * 2 + 1] = acc[1];
   }
 }
 
-template __global__ void k_cg_pass_a<true>(CgArgs, const double*, double*);
-template __global__ void k_cg_pass_a<false>(CgArgs, const double*, double*);
-template __global__ void k_cg_pass_b<true>(CgArgs, const double*);
-template __global__ void k_cg_pass_b<false>(CgArgs, const double*);
+template __global__ void k_cg_pass_a<true>(CgArgs, const double*, double*, int);
+template __global__ void k_cg_pass_a<false>(CgArgs, const double*, double*, int);
+template __global__ void k_cg_pass_b<true>(CgArgs, const double*, int);
+template __global__ void k_cg_pass_b<false>(CgArgs, const double*, int);
 
 // ---------------------------------------------------------------------------
 // Implicit-stage finalisers.
@@ -497,67 +504,77 @@ template __global__ void k_cg_pass_b<false>(CgArgs, const double*);
 
 // rate = -curl_z(x): rate_x(i,j) = -(x(i,j+1) - x(i,j))/dy, rate_y = (x(i+1,j) - x(i,j))/dx
 // (resistive_B_term's second loop, solvers.hpp:472-488).
-__device__ __forceinline__ double rate_x(const double* x, int nx, int i, int j, int jp, double dy) {
-  return -(x[jp * nx + i] - x[j * nx + i]) / dy;
-}
-__device__ __forceinline__ double rate_y(const double* x, int nx, int i, int j, int ip, double dx) {
-  return (x[j * nx + ip] - x[j * nx + i]) / dx;
-}
 
 // B += rate(x) * c (solvers.hpp:528-529).  x is taken as 0 when CG did no
 // product (Eigen returns x = 0), which leaves B unchanged.
-__global__ __launch_bounds__(256) void k_finalize1(FinArgs a) {
+__global__ __launch_bounds__(kTileX) void k_finalize1(FinArgs a) {
   if (a.st->spmv == 0) return;
   const int nx = a.nx, ny = a.ny;
-  const int n = nx * ny;
-  for (int o = blockIdx.x * blockDim.x + threadIdx.x; o < n; o += gridDim.x * blockDim.x) {
-    const int j = o / nx, i = o - j * nx;
+  const Tile t = tile_of(nx, ny, a.ty, a.tiles_y);
+  if (t.y0 >= ny || !t.valid) return;
+  double xc = a.x[t.y0 * nx + t.ic];
+  for (int j = t.y0; j < t.y1; ++j) {
     const int jp = (j == ny - 1) ? 0 : j + 1;
-    const int ip = (i == nx - 1) ? 0 : i + 1;
-    a.bx_out[o] = a.bx[o] + rate_x(a.x, nx, i, j, jp, a.dy) * a.c;
-    a.by_out[o] = a.by[o] + rate_y(a.x, nx, i, j, ip, a.dx) * a.c;
+    const int o = j * nx + t.ic;
+    const double xn = a.x[jp * nx + t.ic];
+    const double xe = a.x[j * nx + t.ip];
+    a.bx_out[o] = a.bx[o] + (-(xn - xc) / a.dy) * a.c;
+    a.by_out[o] = a.by[o] + ((xe - xc) / a.dx) * a.c;
+    xc = xn;
   }
 }
 
-// Final recombination of the B field (solvers.hpp:568-608): with B3 = B3* +
-// c rate(x), B^{n+1} = B3* + c g3, where g3 = -curl(eta J(B3)) is the explicit
-// resistive term of B3 (solvers.hpp:579) -- the reference re-evaluates g3
-// rather than copying B3.
-__device__ __forceinline__ double b3x(const FinArgs& a, bool xv, int i, int j) {
-  const int ny = a.ny, nx = a.nx;
-  const int jp = (j == ny - 1) ? 0 : j + 1;
-  const double r = xv ? rate_x(a.x, nx, i, j, jp, a.dy) : -(0.0 - 0.0) / a.dy;
-  return a.bx[j * nx + i] + r * a.c;
+// Final recombination of the B field (solvers.hpp:568-608): with
+// B3 = B3* + c rate(x), B^{n+1} = B3* + c g3, where g3 = -curl(eta J(B3)) is
+// the explicit resistive term of B3 (solvers.hpp:579) -- the reference
+// re-evaluates g3 rather than copying B3.  Each thread owns column i and
+// marches up its tile keeping J3 of columns i and i+1 in a sliding window.
+__device__ __forceinline__ double fin_x(const FinArgs& a, bool xv, int col, int row) {
+  return xv ? a.x[row * a.nx + col] : 0.0;
 }
-__device__ __forceinline__ double b3y(const FinArgs& a, bool xv, int i, int j) {
-  const int nx = a.nx;
-  const int ip = (i == nx - 1) ? 0 : i + 1;
-  const double r = xv ? rate_y(a.x, nx, i, j, ip, a.dx) : (0.0 - 0.0) / a.dx;
-  return a.by[j * nx + i] + r * a.c;
+
+// B3 components from x and B3*: bx3(i, j) (left face) and by3(i, j) (bottom face)
+__device__ __forceinline__ double fin_bx3(const FinArgs& a, bool xv, int i, int j) {
+  const int jp = (j == a.ny - 1) ? 0 : j + 1;
+  const double r = -(fin_x(a, xv, i, jp) - fin_x(a, xv, i, j)) / a.dy;
+  return a.bx[j * a.nx + i] + r * a.c;
 }
-__device__ __forceinline__ double j3(const FinArgs& a, bool xv, int i, int j) {
+__device__ __forceinline__ double fin_by3(const FinArgs& a, bool xv, int i, int j) {
+  const int ip = (i == a.nx - 1) ? 0 : i + 1;
+  const double r = (fin_x(a, xv, ip, j) - fin_x(a, xv, i, j)) / a.dx;
+  return a.by[j * a.nx + i] + r * a.c;
+}
+// J3(i, j) given bx3(i, j-1) (carried)
+__device__ __forceinline__ double fin_j3(const FinArgs& a, bool xv, int i, int j, double bx3_below, double& bx3_here) {
   const int im = (i == 0) ? a.nx - 1 : i - 1;
-  const int jm = (j == 0) ? a.ny - 1 : j - 1;
-  const double dBy_dx = (b3y(a, xv, i, j) - b3y(a, xv, im, j)) / a.dx;
-  const double dBx_dy = (b3x(a, xv, i, j) - b3x(a, xv, i, jm)) / a.dy;
+  bx3_here = fin_bx3(a, xv, i, j);
+  const double dBy_dx = (fin_by3(a, xv, i, j) - fin_by3(a, xv, im, j)) / a.dx;
+  const double dBx_dy = (bx3_here - bx3_below) / a.dy;
   return a.eta * (dBy_dx - dBx_dy);
 }
 
-__global__ __launch_bounds__(256) void k_finalize2(FinArgs a) {
+__global__ __launch_bounds__(kTileX) void k_finalize2(FinArgs a) {
   const bool xv = a.st->spmv != 0;
   const int nx = a.nx, ny = a.ny;
-  const int n = nx * ny;
-  for (int o = blockIdx.x * blockDim.x + threadIdx.x; o < n; o += gridDim.x * blockDim.x) {
-    const int j = o / nx, i = o - j * nx;
+  const Tile t = tile_of(nx, ny, a.ty, a.tiles_y);
+  if (t.y0 >= ny || !t.valid) return;
+  const int i0 = t.ic, i1 = t.ip;
+  const int jm = (t.y0 == 0) ? ny - 1 : t.y0 - 1;
+  // prologue: J3 at row y0 for columns i and i+1
+  double b0 = fin_bx3(a, xv, i0, jm), b1 = fin_bx3(a, xv, i1, jm);
+  double j0 = fin_j3(a, xv, i0, t.y0, b0, b0);
+  double j1 = fin_j3(a, xv, i1, t.y0, b1, b1);
+  for (int j = t.y0; j < t.y1; ++j) {
     const int jp = (j == ny - 1) ? 0 : j + 1;
-    const int ip = (i == nx - 1) ? 0 : i + 1;
-    const double jc = j3(a, xv, i, j);
-    const double jt = j3(a, xv, i, jp);
-    const double jr = j3(a, xv, ip, j);
-    const double g3x = -(jt - jc) / a.dy;
-    const double g3y = (jr - jc) / a.dx;
+    double nb0, nb1;
+    const double jn0 = fin_j3(a, xv, i0, jp, b0, nb0);
+    const double jn1 = fin_j3(a, xv, i1, jp, b1, nb1);
+    const double g3x = -(jn0 - j0) / a.dy;
+    const double g3y = (j1 - j0) / a.dx;
+    const int o = j * nx + i0;
     a.bx_out[o] = a.bx[o] + g3x * a.c;
     a.by_out[o] = a.by[o] + g3y * a.c;
+    j0 = jn0; j1 = jn1; b0 = nb0; b1 = nb1;
   }
 }
 
@@ -573,8 +590,8 @@ __global__ __launch_bounds__(256) void k_rate(FinArgs a, int zero) {
       a.bx_out[o] = -(0.0 - 0.0) / a.dy;
       a.by_out[o] = (0.0 - 0.0) / a.dx;
     } else {
-      a.bx_out[o] = rate_x(a.x, nx, i, j, jp, a.dy);
-      a.by_out[o] = rate_y(a.x, nx, i, j, ip, a.dx);
+      a.bx_out[o] = -(a.x[jp * nx + i] - a.x[j * nx + i]) / a.dy;
+      a.by_out[o] = (a.x[j * nx + ip] - a.x[j * nx + i]) / a.dx;
     }
   }
 }
@@ -583,126 +600,85 @@ __global__ __launch_bounds__(256) void k_rate(FinArgs a, int zero) {
 // Diagnostics and layout helpers.
 // ---------------------------------------------------------------------------
 
-// Primitives of one physical cell of the compact state.
-__device__ __forceinline__ void cell_prims(const DiagArgs& a, int i, int j, double& rho_f, double& vx, double& vy,
-                                           double& p, double& bxc, double& byc) {
-  const int nx = a.nx, ny = a.ny;
-  const int ip = (i == nx - 1) ? 0 : i + 1;
-  const int jp = (j == ny - 1) ? 0 : j + 1;
-  const int o = j * nx + i;
-  bxc = (a.s[4][o] + a.s[4][j * nx + ip]) / 2.0;
-  byc = (a.s[5][o] + a.s[5][jp * nx + i]) / 2.0;
-  primitives(a.s[0][o], a.s[1][o], a.s[2][o], a.s[3][o], bxc, byc, rho_f, vx, vy, p);
-}
-
-// CFL_condition_checker's max wave speed (diagnostics.hpp:58-80); NaN speeds
-// are skipped exactly as the reference's `>` comparison skips them, and
-// counted separately.
-__global__ __launch_bounds__(kRedThreads) void k_cfl(DiagArgs a) {
+// CFL_condition_checker's wave speed (diagnostics.hpp:58-80) per cell, with
+// Grid::computePrimitives' primitives; per-block max (NaN skipped exactly as
+// the reference's `>` comparison skips it) and count of non-finite speeds.
+__global__ __launch_bounds__(kTileX) void k_cfl(DiagArgs a) {
   __shared__ double lds[16];
-  __shared__ int lflag;
   const int nx = a.nx, ny = a.ny;
-  const int vb = xcd_block(blockIdx.x, gridDim.x);
-  int r0, r1;
-  rows_of_block(vb, gridDim.x, ny, r0, r1);
+  const Tile t = tile_of(nx, ny, a.ty, a.tiles_y);
   double m = 1e-8;
-  double bad = 0.0;
-  for (int j = r0; j < r1; ++j) {
-    for (int i = threadIdx.x; i < nx; i += blockDim.x) {
-      double rho, vx, vy, p, bx, by;
-      cell_prims(a, i, j, rho, vx, vy, p, bx, by);
+  double bad[1] = {0.0};
+  if (t.y0 < ny && t.valid) {
+    double byc_lo = a.s[5][t.y0 * nx + t.ic];
+    for (int j = t.y0; j < t.y1; ++j) {
+      const int jp = (j == ny - 1) ? 0 : j + 1;
+      const int o = j * nx + t.ic;
+      const double by_hi = a.s[5][jp * nx + t.ic];
+      const double bxc = (a.s[4][o] + a.s[4][j * nx + t.ip]) / 2.0;
+      const double byc = (byc_lo + by_hi) / 2.0;
+      byc_lo = by_hi;
+      double rho, vx, vy, p;
+      primitives(a.s[0][o], a.s[1][o], a.s[2][o], a.s[3][o], bxc, byc, rho, vx, vy, p);
       rho = dmax(rho, 1e-8);
       p = dmax(p, 1e-8);
-      double ss = kGamma * p / rho;
-      double as = (bx * bx + by * by) / rho;
-      double ms = sqrt(ss + as);
-      double fs = sqrt(vx * vx + vy * vy);
-      double cur = ms + fs;
+      const double ss = kGamma * p / rho;
+      const double as = (bxc * bxc + byc * byc) / rho;
+      const double cur = sqrt(ss + as) + sqrt(vx * vx + vy * vy);
       if (cur > m) m = cur;
-      if (!(fabs(cur) <= 1.7976931348623157e308)) bad += 1.0;
+      if (!(fabs(cur) <= 1.7976931348623157e308)) bad[0] += 1.0;
     }
   }
-  double b1[1] = {bad};
   block_max(m, lds);
-  __syncthreads();
-  block_sum<1>(b1, lds + 8);
+  block_sum<1>(bad, lds + 8);
   if (threadIdx.x == 0) {
-    a.part[vb * 2 + 0] = m;
-    a.part[vb * 2 + 1] = b1[0];
-  }
-  if (!arrive_last(a.counter, gridDim.x, &lflag)) return;
-  double mm = 1e-8, nb = 0.0;
-  for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x) {
-    mm = max_ignoring_nan(mm, a.part[b * 2]);
-    nb += a.part[b * 2 + 1];
-  }
-  __syncthreads();
-  block_max(mm, lds);
-  __syncthreads();
-  double nb1[1] = {nb};
-  block_sum<1>(nb1, lds + 8);
-  if (threadIdx.x == 0) {
-    *a.counter = 0;
-    a.out[0] = mm;
-    a.out[1] = nb1[0];
+    a.part[t.vb * 2 + 0] = m;
+    a.part[t.vb * 2 + 1] = bad[0];
   }
 }
 
-// calculate_current_div_B (diagnostics.hpp:15-42): sum of squares and max |div|.
-__global__ __launch_bounds__(kRedThreads) void k_divb(DiagArgs a) {
+// calculate_current_div_B (diagnostics.hpp:15-42): per-block sum of squares and max |div|.
+__global__ __launch_bounds__(kTileX) void k_divb(DiagArgs a) {
   __shared__ double lds[16];
-  __shared__ int lflag;
   const int nx = a.nx, ny = a.ny;
-  const int vb = xcd_block(blockIdx.x, gridDim.x);
-  int r0, r1;
-  rows_of_block(vb, gridDim.x, ny, r0, r1);
+  const Tile t = tile_of(nx, ny, a.ty, a.tiles_y);
   double ss[1] = {0.0};
   double m = 0.0;
-  for (int j = r0; j < r1; ++j) {
-    const int jp = (j == ny - 1) ? 0 : j + 1;
-    for (int i = threadIdx.x; i < nx; i += blockDim.x) {
-      const int ip = (i == nx - 1) ? 0 : i + 1;
-      const int o = j * nx + i;
-      double xd = (a.s[4][j * nx + ip] - a.s[4][o]) / a.dx;
-      double yd = (a.s[5][jp * nx + i] - a.s[5][o]) / a.dy;
-      double t = xd + yd;
-      ss[0] += t * t;
-      if (fabs(t) > m) m = fabs(t);
+  if (t.y0 < ny && t.valid) {
+    double by_lo = a.s[5][t.y0 * nx + t.ic];
+    for (int j = t.y0; j < t.y1; ++j) {
+      const int jp = (j == ny - 1) ? 0 : j + 1;
+      const int o = j * nx + t.ic;
+      const double by_hi = a.s[5][jp * nx + t.ic];
+      const double xd = (a.s[4][j * nx + t.ip] - a.s[4][o]) / a.dx;
+      const double yd = (by_hi - by_lo) / a.dy;
+      by_lo = by_hi;
+      const double tt = xd + yd;
+      ss[0] += tt * tt;
+      if (fabs(tt) > m) m = fabs(tt);
     }
   }
   block_sum<1>(ss, lds);
-  __syncthreads();
   block_max(m, lds + 8);
   if (threadIdx.x == 0) {
-    a.part[vb * 2 + 0] = ss[0];
-    a.part[vb * 2 + 1] = m;
-  }
-  if (!arrive_last(a.counter, gridDim.x, &lflag)) return;
-  double s1[1] = {0.0};
-  double mm = 0.0;
-  for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x) {
-    s1[0] += a.part[b * 2];
-    mm = max_ignoring_nan(mm, a.part[b * 2 + 1]);
-  }
-  __syncthreads();
-  block_sum<1>(s1, lds);
-  __syncthreads();
-  block_max(mm, lds + 8);
-  if (threadIdx.x == 0) {
-    *a.counter = 0;
-    a.out[0] = s1[0];
-    a.out[1] = mm;
+    a.part[t.vb * 2 + 0] = ss[0];
+    a.part[t.vb * 2 + 1] = m;
   }
 }
 
 // Primitives of every cell, for mhd_download.
 __global__ __launch_bounds__(256) void k_prims(DiagArgs a, double* vx, double* vy, double* p, double* bxc,
                                                double* byc) {
-  const int n = a.nx * a.ny;
+  const int nx = a.nx, ny = a.ny;
+  const int n = nx * ny;
   for (int o = blockIdx.x * blockDim.x + threadIdx.x; o < n; o += gridDim.x * blockDim.x) {
-    const int j = o / a.nx, i = o - j * a.nx;
-    double r, u, v, pp, bx, by;
-    cell_prims(a, i, j, r, u, v, pp, bx, by);
+    const int j = o / nx, i = o - j * nx;
+    const int ip = (i == nx - 1) ? 0 : i + 1;
+    const int jp = (j == ny - 1) ? 0 : j + 1;
+    const double bx = (a.s[4][o] + a.s[4][j * nx + ip]) / 2.0;
+    const double by = (a.s[5][o] + a.s[5][jp * nx + i]) / 2.0;
+    double r, u, v, pp;
+    primitives(a.s[0][o], a.s[1][o], a.s[2][o], a.s[3][o], bx, by, r, u, v, pp);
     vx[o] = u; vy[o] = v; p[o] = pp; bxc[o] = bx; byc[o] = by;
   }
 }

@@ -7,8 +7,8 @@
 namespace mhd {
 
 constexpr int kStripOut = 60;     // output columns per 64-lane strip of the stage kernel
-constexpr int kRedThreads = 256;  // threads per block of the reduction kernels
-constexpr int kMaxRedBlocks = 2048;
+constexpr int kTileX = 256;       // columns (= threads) per tile of the 2D sweeps
+constexpr int kTargetTiles = 2048;
 
 // Explicit-stage kernel arguments.  MODE 0: o1 = L(s).  MODE 1: o1 = f1 = L(s),
 // o2 = s + c_a f1.  MODE 2: o1 = (u + c_a f1) + c_b L(s) [+ c_g g2(s) on B].
@@ -23,16 +23,16 @@ struct StageArgs {
   double c_a, c_b, c_g;
 };
 
-// Conjugate-gradient state, device resident.
+// Conjugate-gradient state, device resident.  Written by block 0 of a pass
+// kernel, read by later kernels; abs_new is double-buffered by iteration parity.
 struct CgState {
-  double rhs_norm2, threshold, abs_new, alpha, beta, rn2, pap;
+  double rhs_norm2, threshold, abs_new[2], alpha, beta, rn2, pap;
   long long max_iters;
   int iters, spmv, done;   // done: 0 running, 1 converged, 2 max iterations, 3 zero rhs
-  unsigned int counter;    // block arrivals of the running reduction
 };
 
 struct CgArgs {
-  int nx, ny;
+  int nx, ny, ty, tiles_y;
   double dx, dy, eta;
   double invd, diag, cx, cy, tol;
   const double* bx;  // face B for the right-hand side
@@ -40,12 +40,13 @@ struct CgArgs {
   double* r;
   double* p0;
   double* x;
-  double* part;
+  double* part1;   // one partial per tile (p . Ap)
+  double* part2;   // two partials per tile (|r|^2, r . z or |b|^2, b . p)
   CgState* st;
 };
 
 struct FinArgs {
-  int nx, ny;
+  int nx, ny, ty, tiles_y;
   double dx, dy, eta, c;
   const double* x;
   const double* bx;
@@ -56,18 +57,16 @@ struct FinArgs {
 };
 
 struct DiagArgs {
-  int nx, ny;
+  int nx, ny, ty, tiles_y;
   double dx, dy;
   const double* s[6];
-  double* part;
-  unsigned int* counter;
-  double* out;
+  double* part;    // two partials per tile
 };
 
 template <int MODE> __global__ void k_stage(StageArgs a);
 __global__ void k_cg_init(CgArgs a, const double* bin, int explicit_only);
-template <bool FIRST> __global__ void k_cg_pass_a(CgArgs a, const double* pold, double* pnew);
-template <bool FIRST> __global__ void k_cg_pass_b(CgArgs a, const double* pv);
+template <bool FIRST> __global__ void k_cg_pass_a(CgArgs a, const double* pold, double* pnew, int it);
+template <bool FIRST> __global__ void k_cg_pass_b(CgArgs a, const double* pv, int it);
 __global__ void k_finalize1(FinArgs a);
 __global__ void k_finalize2(FinArgs a);
 __global__ void k_rate(FinArgs a, int zero);

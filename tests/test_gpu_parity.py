@@ -170,3 +170,39 @@ def test_config2_resistive_small():
         e = rel_l1(interior(got[k]), sim.interior(k))
         assert e <= TOL_RUN, (k, e)
     assert divmax <= 1e-11
+
+
+def test_cg_iteration_semantics():
+    """Eigen's edge cases: zero right-hand side -> x = 0 with 0 iterations;
+    maxIter = 1 -> one product, iteration count 1 (ConjugateGradient.h)."""
+    _say("cg iteration semantics 16x16")
+    nx = ny = 16
+    z = {k: np.zeros((ny + 4, nx + 4)) for k in CONSERVED}
+    with make_gpu(z, nx, ny, 0.4, 0.4, 0.0, tol=1e-30) as ctx:
+        ctx.generate_A_matrix(0.1)
+        x, it, err = ctx.cg_solve(np.zeros(nx * ny))
+        assert (it, err) == (0, 0.0) and not x.any()
+    b = np.random.default_rng(0).standard_normal(nx * ny)
+    sim = make_oracle(z, nx, ny, 0.4, 0.4, 0.0)
+    sim.set_cg(1e-30, 1)
+    sim.build_operator(0.1)
+    xr, itr, spr, _ = sim.cg_solve(b)
+    with make_gpu(z, nx, ny, 0.4, 0.4, 0.0, tol=1e-30, cg_max_iter=1) as ctx:
+        ctx.generate_A_matrix(0.1)
+        x, it, err = ctx.cg_solve(b)
+    assert it == itr == 1
+    assert rel_l1(x, xr) <= 1e-13
+
+
+def test_determinism():
+    """Fixed-order reductions: two identical runs agree bit for bit."""
+    _say("determinism OT 256^2 eta=1e-3 3 steps")
+    n = 256
+    ic, dx, dy = ic_arrays("orszag_tang", n, n)
+    outs = []
+    for _ in range(2):
+        with make_gpu(ic, n, n, dx, dy, 1e-3, tol=1e-10, cfl=0.4) as ctx:
+            ctx.run(10.0, max_steps=3)
+            outs.append(ctx.download())
+    for k in CONSERVED:
+        assert np.array_equal(outs[0][k], outs[1][k]), k
