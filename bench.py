@@ -34,15 +34,31 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 # Algorithmic HBM bytes per cell of each kernel (FP64 fields, 8 B each):
-#   stage1   reads U (6 fields), writes f1 (6) and U2* (6)           = 144
-#   stage2   reads U2 (6), U^n (6), f1 (6), writes U3* (6)           = 192
-#   cg_init  reads face B (2), writes r, p                            = 32
-#   cg_pass_a (iteration > 1) reads r, p_old, writes p_new           = 24;  first iteration reads p = 8
-#   cg_pass_b (iteration > 1) reads p, x, r, writes x, r             = 40;  first iteration (x = 0) = 32
-#   finalize1 / finalize2  read x and face B (3), write face B (2)   = 40
-#   cfl      reads U (6)                                              = 48
-STAGE_BYTES = {"stage1": 144, "stage2": 192, "cg_init": 32, "implicit_finalize1": 40,
+#   stage1     reads U (6 fields), writes f1 (6) and U2* (6)                   = 144
+#   stage2     reads U2 (6), U^n (6), f1 (6), writes U3* (6)                   = 192
+#   cg_start   reads face B (2), writes r, p (first product from registers)    = 32
+#   cg_pass_a  reads r, p_old, writes p_new                                    = 24 (iterations >= 2)
+#   cg_pass_b  reads p, x, r, writes x, r                                      = 40 (first: x = 0, 32)
+#   finalize1 / finalize2  read x and face B (3 fields), write face B (2)      = 40
+#   cfl        reads U (6)                                                     = 48
+# Per IMEX step the fixed algorithmic figure is N (336 + 144 [eta > 0] + 64 n)
+# with n the operator products of both solves (the byte model of the
+# north-star workload: 336 explicit, 2 x 72 implicit set-up/finish, 64 per
+# CG iteration).
+STAGE_BYTES = {"stage1": 144, "stage2": 192, "cg_start": 32, "implicit_finalize1": 40,
                "implicit_finalize2": 40, "cfl": 48}
+KERNELS = ("stage1", "stage2", "cg_start", "cg_pass_a", "cg_pass_b", "implicit_finalize1",
+           "implicit_finalize2", "cfl")
+
+
+def kernel_bytes(k: str, N: int, launches: int, solves: int, spmv: int) -> float:
+    """Algorithmic bytes moved by all launches of kernel k in a run with
+    `solves` CG solves and `spmv` operator products in total."""
+    if k in STAGE_BYTES:
+        return STAGE_BYTES[k] * N * launches
+    if k == "cg_pass_a":             # one direction update per product after the first of a solve
+        return 24.0 * N * (spmv - solves)
+    return N * (32.0 * solves + 40.0 * (spmv - solves))   # cg_pass_b
 
 
 def parse():
@@ -117,40 +133,36 @@ def main():
     if a.warmup > 0:
         ctx.run(1e9, max_steps=a.warmup)
     ctx.synchronize()
+    N = n * n
+    # untimed pass: every kernel bracketed by events -> per-kernel breakdown
     ctx.profile(True)
+    psteps, _, pits, _ = ctx.run(1e9, max_steps=max(3, min(a.steps, 10)))
+    prof = {k: ctx.profile_get(k) for k in KERNELS}
+    psolves, pspmv = 2 * psteps, int(pits.sum()) + 2 * psteps
+    kb = {}
+    for k, (cnt, ms) in prof.items():
+        byt = kernel_bytes(k, N, cnt, psolves, pspmv)
+        kb[k] = {"launches_per_step": round(cnt / psteps, 2), "ms_per_step": round(ms / psteps, 4),
+                 "GBs": round(byt / (ms * 1e-3) / 1e9, 1) if ms > 0 else None}
+    dom = max(prof, key=lambda k: prof[k][1])
+    # timed region: only the dominant kernel's launches carry events
+    ctx.profile([dom])
     ctx.synchronize()
     t0 = time.perf_counter()
     steps, dts, its, _ = ctx.run(1e9, max_steps=a.steps)
     ctx.synchronize()
     t1 = time.perf_counter()
     elapsed = t1 - t0
-    prof = {k: ctx.profile_get(k) for k in ("stage1", "stage2", "cg_init", "cg_pass_a", "cg_pass_b",
-                                            "implicit_finalize1", "implicit_finalize2", "cfl")}
+    dom_cnt, dom_ms = ctx.profile_get(dom)
     ctx.profile(False)
     rms, divmax = ctx.calculate_current_div_B()
     ctx.close()
 
-    N = n * n
     solves = 2 * steps
     spmv = int(its.sum()) + solves          # converged solves: products = iterations + 1
-    kb, kbytes = {}, {}
-    for k, (cnt, ms) in prof.items():
-        if k in STAGE_BYTES:
-            byt = STAGE_BYTES[k] * N * cnt
-        elif k == "cg_pass_a":
-            byt = N * (8 * solves + 24 * (spmv - solves))
-        else:
-            byt = N * (32 * solves + 40 * (spmv - solves))
-        kbytes[k] = byt
-        kb[k] = {"launches": cnt, "ms_total": round(ms, 3), "ms_per_step": round(ms / steps, 4),
-                 "GBs": round(byt / (ms * 1e-3) / 1e9, 1) if ms > 0 else None}
-    dom = max(prof, key=lambda k: prof[k][1])
-    dom_cnt, dom_ms = prof[dom]
-    # launches that found the solve converged return at once and move no bytes;
-    # the per-launch figure averages over every launch of the kernel
-    bytes_per_launch = kbytes[dom] / dom_cnt
+    bytes_per_launch = kernel_bytes(dom, N, dom_cnt, solves, spmv) / dom_cnt
     achieved = bytes_per_launch / (dom_ms * 1e-3 / dom_cnt) / 1e9
-    step_bytes = N * (336 + 144 + 64 * (spmv / steps))
+    step_bytes = N * (336 + (144 if a.eta > 0 else 0) + 64 * (spmv / steps))
 
     line = {
         "metric": "FP64 Mcell-updates/s per IMEX step",
@@ -169,7 +181,8 @@ def main():
                    "nx": n, "ny": n, "eta": a.eta, "cg_tol": a.cg_tol, "cfl": a.cfl,
                    "parallelism": "single-gpu"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "bytes_per_launch": int(bytes_per_launch), "ms_per_launch": round(dom_ms / dom_cnt, 4)},
         "step_roofline": {"algorithmic_bytes_per_step": int(step_bytes),
                           "frac": round(step_bytes / (elapsed / steps) / 1e9 / HBM_PEAK_GBS, 4)},
         "cg_spmv_per_step": round(spmv / steps, 2),

@@ -118,12 +118,14 @@ int mhd_divb(mhd_ctx* ctx, double* rms, double* maxabs);
 int mhd_run(mhd_ctx* ctx, double t_end, double dt_override, int max_steps,
             int* steps_out, double* t_out, double* step_dt, int* cg_iters);
 
-/* Instrumentation (no reference counterpart).  With profiling on, each
- * kernel launch of the context is bracketed by HIP events on its stream;
- * mhd_profile_get returns launch count and summed milliseconds per kernel id.
- * Kernel ids: 0 stage1, 1 stage2, 2 cg_init, 3 cg_pass_a, 4 cg_pass_b,
- * 5 implicit_finalize1, 6 implicit_finalize2, 7 cfl, 8 other. */
-int mhd_profile_enable(mhd_ctx* ctx, int on);
+/* Instrumentation (no reference counterpart).  mask selects kernel ids
+ * (bit k = id k, 0 = off, 0x1ff = all); each launch of a selected kernel is
+ * bracketed by HIP events on the context's stream, and mhd_profile_get
+ * returns the launch count and summed milliseconds per id since the last
+ * mhd_profile_enable.  Kernel ids: 0 stage1, 1 stage2, 2 cg_start,
+ * 3 cg_pass_a, 4 cg_pass_b, 5 implicit_finalize1, 6 implicit_finalize2,
+ * 7 cfl, 8 other. */
+int mhd_profile_enable(mhd_ctx* ctx, int mask);
 int mhd_profile_get(mhd_ctx* ctx, int kernel_id, long long* launches, double* total_ms);
 /* Bytes-calibration helper for hardware counters: one plain device copy
  * kernel of nbytes (rounded down to doubles), run synchronously. */

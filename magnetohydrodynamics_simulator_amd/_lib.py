@@ -14,7 +14,7 @@ LIB_PATH = os.path.join(PKG, "libmhd_mi355x.so")
 MHD_OK, MHD_EINVAL, MHD_EHIP, MHD_ENONFINITE, MHD_ECG, MHD_ENOMEM = range(6)
 ERROR_NAMES = {0: "MHD_OK", 1: "MHD_EINVAL", 2: "MHD_EHIP", 3: "MHD_ENONFINITE", 4: "MHD_ECG", 5: "MHD_ENOMEM"}
 
-KERNEL_IDS = {"stage1": 0, "stage2": 1, "cg_init": 2, "cg_pass_a": 3, "cg_pass_b": 4,
+KERNEL_IDS = {"stage1": 0, "stage2": 1, "cg_start": 2, "cg_pass_a": 3, "cg_pass_b": 4,
               "implicit_finalize1": 5, "implicit_finalize2": 6, "cfl": 7, "other": 8}
 
 

@@ -54,8 +54,9 @@ struct mhd_ctx {
   double* x = nullptr;
   double* r = nullptr;
   double* p[2] = {};
-  double* part1 = nullptr;       // CG: one partial per tile
-  double* part2 = nullptr;       // CG: two partials per tile
+  double* part_s = nullptr;      // CG start kernel: three partials per tile
+  double* part_a = nullptr;      // CG pass A: one partial per tile
+  double* part_b = nullptr;      // CG pass B: two partials per tile
   double* part = nullptr;        // diagnostics: two partials per tile
   CgState* st = nullptr;
   CgState* st_host = nullptr;    // pinned
@@ -66,7 +67,7 @@ struct mhd_ctx {
   int tiles = 1, tiles_y = 1, tile_rows = 4;   // 2D sweeps: kTileX columns x tile_rows rows per tile
   int stage_rows = 16;
   std::string err;
-  bool prof = false;
+  unsigned prof_mask = 0;        // kernel ids whose launches are bracketed by events
   long long prof_n[kNumKernelIds] = {};
   double prof_ms[kNumKernelIds] = {};
   std::vector<std::pair<hipEvent_t, hipEvent_t>> pending[kNumKernelIds];
@@ -91,7 +92,8 @@ int fail(mhd_ctx* c, int code, const std::string& msg) {
 template <typename F>
 int launch(mhd_ctx* c, int id, F&& body) {
   hipEvent_t a = nullptr, b = nullptr;
-  if (c->prof) {
+  const bool prof = (c->prof_mask >> id) & 1u;
+  if (prof) {
     HIP_OR_FAIL(c, hipEventCreate(&a));
     HIP_OR_FAIL(c, hipEventCreate(&b));
     HIP_OR_FAIL(c, hipEventRecord(a, c->stream));
@@ -99,7 +101,7 @@ int launch(mhd_ctx* c, int id, F&& body) {
   body();
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(c, MHD_EHIP, std::string("kernel launch: ") + hipGetErrorString(e));
-  if (c->prof) {
+  if (prof) {
     HIP_OR_FAIL(c, hipEventRecord(b, c->stream));
     c->pending[id].emplace_back(a, b);
   }
@@ -170,8 +172,9 @@ CgArgs cg_args(mhd_ctx* c, const double* bx, const double* by) {
   a.r = c->r;
   a.p0 = c->p[0];
   a.x = c->x;
-  a.part1 = c->part1;
-  a.part2 = c->part2;
+  a.part_s = c->part_s;
+  a.part_a = c->part_a;
+  a.part_b = c->part_b;
   a.st = c->st;
   a.ty = c->tile_rows;
   a.tiles_y = c->tiles_y;
@@ -179,10 +182,11 @@ CgArgs cg_args(mhd_ctx* c, const double* bx, const double* by) {
 }
 
 // One conjugate-gradient solve; the right-hand side is eta J(bx, by) or bin.
-// Kernel order: init, A(0), then B(it), A(it + 1) pairs.  Each A decides on
-// convergence from the partials of the B before it; passes launched after
-// the decision return at once.  The host reads the state back after each
-// chunk of pairs; the first chunk is as long as the previous solve's count.
+// Kernel order: start (b, r, p, first product), B(0), then A(it), B(it)
+// pairs.  Each A takes the convergence decision on the partials of the B
+// before it; kernels launched after the decision return at once.  The host
+// reads the state back after each chunk; the first chunk is as long as the
+// previous solve's product count, so a steady run syncs once per solve.
 int cg_run(mhd_ctx* c, const double* bx, const double* by, const double* bin, CgState* out) {
   if (!c->have_op) return fail(c, MHD_EINVAL, "implicit solve before mhd_build_operator");
   CgArgs a = cg_args(c, bx, by);
@@ -191,36 +195,42 @@ int cg_run(mhd_ctx* c, const double* bx, const double* by, const double* bin, Cg
   c->st_host->max_iters = maxit;
   HIP_OR_FAIL(c, hipMemcpyAsync(c->st, c->st_host, sizeof(CgState), hipMemcpyHostToDevice, c->stream));
   const int nb = c->tiles;
-  int rc = launch(c, 2, [&] { hipLaunchKernelGGL(k_cg_init, dim3(nb), dim3(kTileX), 0, c->stream, a, bin, 0); });
-  if (rc) return rc;
-  rc = launch(c, 3, [&] {
-    hipLaunchKernelGGL(k_cg_pass_a<true>, dim3(nb), dim3(kTileX), 0, c->stream, a, (const double*)c->p[0], c->p[1], 0);
+  int rc = launch(c, 2, [&] {
+    if (bin) hipLaunchKernelGGL(k_cg_start<false>, dim3(nb), dim3(kTileX), 0, c->stream, a, bin);
+    else hipLaunchKernelGGL(k_cg_start<true>, dim3(nb), dim3(kTileX), 0, c->stream, a, bin);
   });
   if (rc) return rc;
-  long long it = 0;
+  rc = launch(c, 4, [&] {
+    hipLaunchKernelGGL(k_cg_pass_b<true>, dim3(nb), dim3(kTileX), 0, c->stream, a, (const double*)c->p[0], 0);
+  });
+  if (rc) return rc;
+  // the direction of iteration it lives in p[it & 1] (p[0] from the start kernel)
+  long long it = 1;
   for (;;) {
-    const int chunk = (it == 0) ? (c->last_spmv > 0 ? c->last_spmv : 1) : 1;
+    const int chunk = (it == 1) ? (c->last_spmv > 0 ? c->last_spmv : 1) : 1;
     for (int k = 0; k < chunk; ++k, ++it) {
-      // B(it) uses the direction formed by A(it): p[0] for it = 0, else p[it & 1]
-      const double* pv = (it == 0) ? c->p[0] : c->p[it & 1];
-      if (it == 0)
-        rc = launch(c, 4, [&] { hipLaunchKernelGGL(k_cg_pass_b<true>, dim3(nb), dim3(kTileX), 0, c->stream, a, pv, 0); });
-      else
-        rc = launch(c, 4, [&] {
-          hipLaunchKernelGGL(k_cg_pass_b<false>, dim3(nb), dim3(kTileX), 0, c->stream, a, pv, (int)it);
-        });
-      if (rc) return rc;
-      // A(it + 1): p_new = invd r + beta p_old, p_old = p[0] for it = 0, else p[it & 1]
-      const double* pold = c->p[it & 1];
-      double* pnew = c->p[(it + 1) & 1];
+      const double* pold = c->p[(it - 1) & 1];
+      double* pnew = c->p[it & 1];
       rc = launch(c, 3, [&] {
-        hipLaunchKernelGGL(k_cg_pass_a<false>, dim3(nb), dim3(kTileX), 0, c->stream, a, pold, pnew, (int)(it + 1));
+        hipLaunchKernelGGL(k_cg_pass_a, dim3(nb), dim3(kTileX), 0, c->stream, a, pold, pnew, (int)it);
+      });
+      if (rc) return rc;
+      if (k + 1 == chunk) break;   // the chunk ends with an A: it holds the decision
+      rc = launch(c, 4, [&] {
+        hipLaunchKernelGGL(k_cg_pass_b<false>, dim3(nb), dim3(kTileX), 0, c->stream, a, (const double*)pnew, (int)it);
       });
       if (rc) return rc;
     }
     HIP_OR_FAIL(c, hipMemcpyAsync(c->st_host, c->st, sizeof(CgState), hipMemcpyDeviceToHost, c->stream));
     HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
     if (c->st_host->done) break;
+    // not converged: run B(it) for the A just launched, then continue
+    const double* pv = c->p[it & 1];
+    rc = launch(c, 4, [&] {
+      hipLaunchKernelGGL(k_cg_pass_b<false>, dim3(nb), dim3(kTileX), 0, c->stream, a, pv, (int)it);
+    });
+    if (rc) return rc;
+    ++it;
     if (it > maxit + 4) return fail(c, MHD_EHIP, "CG state machine did not terminate");
   }
   *out = *c->st_host;
@@ -372,7 +382,8 @@ int mhd_create(int nx, int ny, double dx, double dy, double eta, const mhd_opts*
   const size_t pb = (size_t)c->tiles * 2 * sizeof(double);
   if (hipMalloc(&c->x, fb) != hipSuccess || hipMalloc(&c->r, fb) != hipSuccess || hipMalloc(&c->p[0], fb) != hipSuccess ||
       hipMalloc(&c->p[1], fb) != hipSuccess || hipMalloc(&c->part, pb) != hipSuccess ||
-      hipMalloc(&c->part1, pb) != hipSuccess || hipMalloc(&c->part2, pb) != hipSuccess ||
+      hipMalloc(&c->part_s, pb * 2) != hipSuccess || hipMalloc(&c->part_a, pb) != hipSuccess ||
+      hipMalloc(&c->part_b, pb) != hipSuccess ||
       hipMalloc(&c->st, sizeof(CgState)) != hipSuccess)
     return bail("device allocation failed", MHD_ENOMEM);
   if (hipHostMalloc(&c->st_host, sizeof(CgState)) != hipSuccess || hipHostMalloc(&c->part_host, pb) != hipSuccess)
@@ -430,7 +441,7 @@ int mhd_destroy(mhd_ctx* c) {
     if (c->S[q]) (void)hipFree(c->S[q]);
     if (c->F1[q]) (void)hipFree(c->F1[q]);
   }
-  double* bufs[] = {c->x, c->r, c->p[0], c->p[1], c->part, c->part1, c->part2};
+  double* bufs[] = {c->x, c->r, c->p[0], c->p[1], c->part, c->part_s, c->part_a, c->part_b};
   for (double* b : bufs)
     if (b) (void)hipFree(b);
   if (c->st) (void)hipFree(c->st);
@@ -618,9 +629,7 @@ int mhd_resistive_term(mhd_ctx* c, int implicit, double* bx_rate, double* by_rat
     zero = cs.spmv == 0;
   } else {
     CgArgs a = cg_args(c, c->U[4], c->U[5]);
-    rc = launch(c, 8, [&] {
-      hipLaunchKernelGGL(k_cg_init, dim3(c->tiles), dim3(kTileX), 0, c->stream, a, (const double*)nullptr, 1);
-    });
+    rc = launch(c, 8, [&] { hipLaunchKernelGGL(k_cg_rhs, dim3(c->tiles), dim3(kTileX), 0, c->stream, a); });
     if (rc) return rc;
     f.x = c->r;
     if (iters) *iters = 0;
@@ -711,11 +720,11 @@ int mhd_run(mhd_ctx* c, double t_end, double dt_override, int max_steps, int* st
   return MHD_OK;
 }
 
-int mhd_profile_enable(mhd_ctx* c, int on) {
+int mhd_profile_enable(mhd_ctx* c, int mask) {
   if (check_ctx(c)) return MHD_EINVAL;
   int rc = flush_profile(c);
   if (rc) return rc;
-  c->prof = on != 0;
+  c->prof_mask = (unsigned)mask & ((1u << kNumKernelIds) - 1u);
   for (int k = 0; k < kNumKernelIds; ++k) {
     c->prof_n[k] = 0;
     c->prof_ms[k] = 0.0;

@@ -329,7 +329,7 @@ __device__ __forceinline__ Tile tile_of(int nx, int ny, int ty, int tiles_y) {
 // ---------------------------------------------------------------------------
 
 // A p with the accumulation order of a row of the assembled operator
-// (south, west, centre, east, north), identical in both passes.
+// (south, west, centre, east, north), identical in every pass.
 __device__ __forceinline__ double stencil(double ps, double pw, double pc, double pe, double pn, double cx,
                                           double cy, double d) {
   double q = -cy * ps;
@@ -340,59 +340,151 @@ __device__ __forceinline__ double stencil(double ps, double pw, double pc, doubl
   return q;
 }
 
-// b = eta J(B) at corners (resistive_B_term's first loop, solvers.hpp:450-463)
-// or b = bin; r = b, p = invd b; partials |b|^2 and b . p.
-__global__ __launch_bounds__(kTileX) void k_cg_init(CgArgs a, const double* __restrict__ bin, int explicit_only) {
+// Columns of a tile: `w` valid columns starting at x0 and the periodic left
+// and right halo columns xl, xr.
+__device__ __forceinline__ void tile_columns(int nx, int ty, int tiles_y, int& x0, int& w, int& xl, int& xr) {
+  const int vb = xcd_block(blockIdx.x, gridDim.x);
+  const int bx = vb / tiles_y;
+  x0 = bx * kTileX;
+  w = min(kTileX, nx - x0);
+  xl = (x0 == 0) ? nx - 1 : x0 - 1;
+  xr = (x0 + w >= nx) ? x0 + w - nx : x0 + w;
+  (void)ty;
+}
+
+// eta J(B) at corner (i, j) (resistive_B_term's first loop, solvers.hpp:450-463).
+__device__ __forceinline__ double rhs_at(const CgArgs& a, int i, int j) {
+  const int nx = a.nx, ny = a.ny;
+  const int im = (i == 0) ? nx - 1 : i - 1;
+  const int jm = (j == 0) ? ny - 1 : j - 1;
+  const double dBy_dx = (a.by[j * nx + i] - a.by[j * nx + im]) / a.dx;
+  const double dBx_dy = (a.bx[j * nx + i] - a.bx[jm * nx + i]) / a.dy;
+  return a.eta * (dBy_dx - dBx_dy);
+}
+
+// Start of a solve: b = eta J(B) (or bin), r = b, p = invd b, and the first
+// product q = A p; partials |b|^2, b . p, p . q.
+template <bool FROM_B>
+__global__ __launch_bounds__(kTileX) void k_cg_start(CgArgs a, const double* __restrict__ bin) {
+  __shared__ double prow[2][kTileX + 2];
   __shared__ double lds[16];
   const int nx = a.nx, ny = a.ny;
   const Tile t = tile_of(nx, ny, a.ty, a.tiles_y);
-  double acc[2] = {0.0, 0.0};
-  if (t.y0 < ny) {
-    double bx_below = bin ? 0.0 : a.bx[(t.y0 == 0 ? ny - 1 : t.y0 - 1) * nx + t.ic];
-    for (int j = t.y0; j < t.y1; ++j) {
+  int x0, w, xl, xr;
+  tile_columns(nx, a.ty, a.tiles_y, x0, w, xl, xr);
+  const int tid = threadIdx.x;
+  const double invd = a.invd, d = a.diag, cx = a.cx, cy = a.cy;
+  auto bv = [&](int i, int j) -> double { return FROM_B ? rhs_at(a, i, j) : bin[j * nx + i]; };
+  double acc[3] = {0.0, 0.0, 0.0};
+  const int jm0 = (t.y0 == 0) ? ny - 1 : t.y0 - 1;
+  double bm = bv(t.ic, jm0);
+  double bc = bv(t.ic, t.y0);
+  for (int j = t.y0; j < t.y1; ++j) {
+    const int jp = (j == ny - 1) ? 0 : j + 1;
+    const double bn = bv(t.ic, jp);
+    const int buf = (j - t.y0) & 1;
+    const double pc = invd * bc;
+    if (t.valid) prow[buf][tid + 1] = pc;
+    if (tid == 0) prow[buf][0] = invd * bv(xl, j);
+    if (tid == w - 1) prow[buf][w + 1] = invd * bv(xr, j);
+    __syncthreads();
+    const double q = stencil(invd * bm, prow[buf][tid], pc, prow[buf][tid + 2], invd * bn, cx, cy, d);
+    if (t.valid) {
       const int o = j * nx + t.ic;
-      double b;
-      if (bin) {
-        b = bin[o];
-      } else {
-        const double bxc = a.bx[o];
-        const double dBy_dx = (a.by[o] - a.by[j * nx + t.im]) / a.dx;
-        const double dBx_dy = (bxc - bx_below) / a.dy;
-        b = a.eta * (dBy_dx - dBx_dy);
-        bx_below = bxc;
-      }
-      if (t.valid) {
-        a.r[o] = b - 0.0;
-        if (!explicit_only) {
-          const double p = a.invd * b;
-          a.p0[o] = p;
-          acc[0] += b * b;
-          acc[1] += b * p;
-        }
-      }
+      a.r[o] = bc - 0.0;
+      a.p0[o] = pc;
+      acc[0] += bc * bc;
+      acc[1] += bc * pc;
+      acc[2] += pc * q;
     }
+    bm = bc;
+    bc = bn;
   }
-  if (explicit_only) return;
-  block_sum<2>(acc, lds);
-  if (threadIdx.x == 0) {
-    a.part2[t.vb * 2 + 0] = acc[0];
-    a.part2[t.vb * 2 + 1] = acc[1];
+  block_sum<3>(acc, lds);
+  if (tid == 0) {
+    a.part_s[t.vb * 3 + 0] = acc[0];
+    a.part_s[t.vb * 3 + 1] = acc[1];
+    a.part_s[t.vb * 3 + 2] = acc[2];
   }
 }
 
-template <bool FIRST>
+// Right-hand side only (explicit resistive term): r = eta J(B).
+__global__ __launch_bounds__(kTileX) void k_cg_rhs(CgArgs a) {
+  const int nx = a.nx, ny = a.ny;
+  const Tile t = tile_of(nx, ny, a.ty, a.tiles_y);
+  if (!t.valid) return;
+  for (int j = t.y0; j < t.y1; ++j) a.r[j * nx + t.ic] = rhs_at(a, t.ic, j) - 0.0;
+}
+
+// Pass A(it), it >= 1: convergence decision on the partials of B(it - 1),
+// beta, p_new = invd r + beta p_old, q = A p_new; partial p_new . q.
 __global__ __launch_bounds__(kTileX) void k_cg_pass_a(CgArgs a, const double* __restrict__ pold,
                                                       double* __restrict__ pnew, int it) {
+  __shared__ double prow[2][kTileX + 2];
   __shared__ double lds[16];
   CgState* st = a.st;
   if (st->done) return;
   double rr[2];
-  sum_partials<2>(a.part2, gridDim.x, rr, lds);
+  sum_partials<2>(a.part_b, gridDim.x, rr, lds);
   const Tile t = tile_of(a.nx, a.ny, a.ty, a.tiles_y);
   const bool lead = (t.vb == 0) && (threadIdx.x == 0);
-  double beta = 0.0;
+  const double rn2 = rr[0];
+  if (rn2 < st->threshold) {                // converged after B(it - 1)
+    if (lead) { st->rn2 = rn2; st->iters = it - 1; st->done = 1; }
+    return;
+  }
+  const double beta = rr[1] / st->abs_new[(it - 1) & 1];
+  if (lead) { st->rn2 = rn2; st->iters = it; st->abs_new[it & 1] = rr[1]; st->beta = beta; }
+  if (it >= st->max_iters) {
+    if (lead) st->done = 2;
+    return;
+  }
+  const int nx = a.nx, ny = a.ny;
+  int x0, w, xl, xr;
+  tile_columns(nx, a.ty, a.tiles_y, x0, w, xl, xr);
+  const int tid = threadIdx.x;
+  const double invd = a.invd, d = a.diag, cx = a.cx, cy = a.cy;
+  auto pv = [&](int o) -> double { return fma(beta, pold[o], invd * a.r[o]); };
+  double acc[1] = {0.0};
+  const int jm0 = (t.y0 == 0) ? ny - 1 : t.y0 - 1;
+  double pm = pv(jm0 * nx + t.ic);
+  double pc = pv(t.y0 * nx + t.ic);
+  for (int j = t.y0; j < t.y1; ++j) {
+    const int jp = (j == ny - 1) ? 0 : j + 1;
+    const double pn = pv(jp * nx + t.ic);
+    const int buf = (j - t.y0) & 1;
+    if (t.valid) prow[buf][tid + 1] = pc;
+    if (tid == 0) prow[buf][0] = pv(j * nx + xl);
+    if (tid == w - 1) prow[buf][w + 1] = pv(j * nx + xr);
+    __syncthreads();
+    const double q = stencil(pm, prow[buf][tid], pc, prow[buf][tid + 2], pn, cx, cy, d);
+    if (t.valid) {
+      pnew[j * nx + t.ic] = pc;
+      acc[0] += pc * q;
+    }
+    pm = pc;
+    pc = pn;
+  }
+  block_sum<1>(acc, lds);
+  if (tid == 0) a.part_a[t.vb] = acc[0];
+}
+
+// Pass B(it): alpha = absNew / (p . Ap), x += alpha p, r -= alpha Ap;
+// partials |r|^2 and r . (invd r).  B(0) also takes the start-of-solve
+// decisions of Eigen's conjugate_gradient from the partials of the start kernel.
+template <bool FIRST>
+__global__ __launch_bounds__(kTileX) void k_cg_pass_b(CgArgs a, const double* __restrict__ pv, int it) {
+  __shared__ double prow[2][kTileX + 2];
+  __shared__ double lds[16];
+  CgState* st = a.st;
+  if (st->done) return;
+  const Tile t = tile_of(a.nx, a.ny, a.ty, a.tiles_y);
+  const bool lead = (t.vb == 0) && (threadIdx.x == 0);
+  double alpha, pap;
   if (FIRST) {
-    const double rhs = rr[0];
+    double s3[3];
+    sum_partials<3>(a.part_s, gridDim.x, s3, lds);
+    const double rhs = s3[0];
     const double thr = dmax(a.tol * a.tol * rhs, 2.2250738585072014e-308);
     int done = 0;
     if (rhs == 0.0) done = 3;                 // x = 0, no iteration
@@ -402,99 +494,64 @@ __global__ __launch_bounds__(kTileX) void k_cg_pass_a(CgArgs a, const double* __
       st->rhs_norm2 = rhs;
       st->threshold = thr;
       st->rn2 = rhs;
-      st->abs_new[0] = rr[1];
+      st->abs_new[0] = s3[1];
       st->iters = 0;
       if (done) st->done = done;
     }
     if (done) return;
+    pap = s3[2];
+    alpha = s3[1] / pap;
   } else {
-    const double rn2 = rr[0];
-    if (rn2 < st->threshold) {                // converged after B(it - 1)
-      if (lead) { st->rn2 = rn2; st->iters = it - 1; st->done = 1; }
-      return;
-    }
-    const double abs_old = st->abs_new[(it - 1) & 1];
-    beta = rr[1] / abs_old;
-    if (lead) { st->rn2 = rn2; st->iters = it; st->abs_new[it & 1] = rr[1]; st->beta = beta; }
-    if (it >= st->max_iters) {
-      if (lead) st->done = 2;
-      return;
-    }
+    double s1[1];
+    sum_partials<1>(a.part_a, gridDim.x, s1, lds);
+    pap = s1[0];
+    alpha = st->abs_new[it & 1] / pap;
   }
-  const int nx = a.nx, ny = a.ny;
-  const double invd = a.invd, d = a.diag, cx = a.cx, cy = a.cy;
-  double acc[1] = {0.0};
-  if (t.y0 < ny) {
-    auto pv = [&](int o) -> double { return FIRST ? pold[o] : fma(beta, pold[o], invd * a.r[o]); };
-    const int jm0 = (t.y0 == 0) ? ny - 1 : t.y0 - 1;
-    double pm = pv(jm0 * nx + t.ic);
-    double pc = pv(t.y0 * nx + t.ic);
-    for (int j = t.y0; j < t.y1; ++j) {
-      const int jp = (j == ny - 1) ? 0 : j + 1;
-      const double pn = pv(jp * nx + t.ic);
-      const double pw = pv(j * nx + t.im);
-      const double pe = pv(j * nx + t.ip);
-      const double q = stencil(pm, pw, pc, pe, pn, cx, cy, d);
-      if (t.valid) {
-        if (!FIRST) pnew[j * nx + t.ic] = pc;
-        acc[0] += pc * q;
-      }
-      pm = pc;
-      pc = pn;
-    }
-  }
-  block_sum<1>(acc, lds);
-  if (threadIdx.x == 0) a.part1[t.vb] = acc[0];
-}
-
-template <bool FIRST>
-__global__ __launch_bounds__(kTileX) void k_cg_pass_b(CgArgs a, const double* __restrict__ pv, int it) {
-  __shared__ double lds[16];
-  CgState* st = a.st;
-  if (st->done) return;
-  double pq[1];
-  sum_partials<1>(a.part1, gridDim.x, pq, lds);
-  const Tile t = tile_of(a.nx, a.ny, a.ty, a.tiles_y);
-  const double alpha = st->abs_new[it & 1] / pq[0];
-  if (t.vb == 0 && threadIdx.x == 0) {
-    st->pap = pq[0];
+  if (lead) {
+    st->pap = pap;
     st->alpha = alpha;
     st->spmv = it + 1;
   }
   const int nx = a.nx, ny = a.ny;
+  int x0, w, xl, xr;
+  tile_columns(nx, a.ty, a.tiles_y, x0, w, xl, xr);
+  const int tid = threadIdx.x;
   const double invd = a.invd, d = a.diag, cx = a.cx, cy = a.cy;
   double acc[2] = {0.0, 0.0};
-  if (t.y0 < ny) {
-    const int jm0 = (t.y0 == 0) ? ny - 1 : t.y0 - 1;
-    double pm = pv[jm0 * nx + t.ic];
-    double pc = pv[t.y0 * nx + t.ic];
-    for (int j = t.y0; j < t.y1; ++j) {
-      const int jp = (j == ny - 1) ? 0 : j + 1;
+  const int jm0 = (t.y0 == 0) ? ny - 1 : t.y0 - 1;
+  double pm = pv[jm0 * nx + t.ic];
+  double pc = pv[t.y0 * nx + t.ic];
+  for (int j = t.y0; j < t.y1; ++j) {
+    const int jp = (j == ny - 1) ? 0 : j + 1;
+    const double pn = pv[jp * nx + t.ic];
+    const int buf = (j - t.y0) & 1;
+    if (t.valid) prow[buf][tid + 1] = pc;
+    if (tid == 0) prow[buf][0] = pv[j * nx + xl];
+    if (tid == w - 1) prow[buf][w + 1] = pv[j * nx + xr];
+    __syncthreads();
+    const double q = stencil(pm, prow[buf][tid], pc, prow[buf][tid + 2], pn, cx, cy, d);
+    if (t.valid) {
       const int o = j * nx + t.ic;
-      const double pn = pv[jp * nx + t.ic];
-      const double q = stencil(pm, pv[j * nx + t.im], pc, pv[j * nx + t.ip], pn, cx, cy, d);
-      if (t.valid) {
-        const double xo = FIRST ? 0.0 : a.x[o];
-        a.x[o] = xo + alpha * pc;
-        const double rn = a.r[o] - alpha * q;
-        a.r[o] = rn;
-        const double z = invd * rn;
-        acc[0] += rn * rn;
-        acc[1] += rn * z;
-      }
-      pm = pc;
-      pc = pn;
+      const double xo = FIRST ? 0.0 : a.x[o];
+      a.x[o] = xo + alpha * pc;
+      const double rn = a.r[o] - alpha * q;
+      a.r[o] = rn;
+      const double z = invd * rn;
+      acc[0] += rn * rn;
+      acc[1] += rn * z;
     }
+    pm = pc;
+    pc = pn;
   }
   block_sum<2>(acc, lds);
-  if (threadIdx.x == 0) {
-    a.part2[t.vb * 2 + 0] = acc[0];
-    a.part2[t.vb * 2 + 1] = acc[1];
+  if (tid == 0) {
+    a.part_b[t.vb * 2 + 0] = acc[0];
+    a.part_b[t.vb * 2 + 1] = acc[1];
   }
 }
 
-template __global__ void k_cg_pass_a<true>(CgArgs, const double*, double*, int);
-template __global__ void k_cg_pass_a<false>(CgArgs, const double*, double*, int);
+template __global__ void k_cg_start<true>(CgArgs, const double*);
+template __global__ void k_cg_start<false>(CgArgs, const double*);
 template __global__ void k_cg_pass_b<true>(CgArgs, const double*, int);
 template __global__ void k_cg_pass_b<false>(CgArgs, const double*, int);
 
@@ -527,54 +584,83 @@ __global__ __launch_bounds__(kTileX) void k_finalize1(FinArgs a) {
 // Final recombination of the B field (solvers.hpp:568-608): with
 // B3 = B3* + c rate(x), B^{n+1} = B3* + c g3, where g3 = -curl(eta J(B3)) is
 // the explicit resistive term of B3 (solvers.hpp:579) -- the reference
-// re-evaluates g3 rather than copying B3.  Each thread owns column i and
-// marches up its tile keeping J3 of columns i and i+1 in a sliding window.
-__device__ __forceinline__ double fin_x(const FinArgs& a, bool xv, int col, int row) {
-  return xv ? a.x[row * a.nx + col] : 0.0;
-}
+// re-evaluates g3 rather than copying B3.  Each thread marches its column up
+// the tile computing J3 = eta J(B3) at the corners of the next row; J3 of the
+// column to the right comes from an LDS row written one step earlier (the
+// thread at the right edge of the tile also runs the halo column).
+struct Fin2Chain {
+  double bx3_below;   // bx3(col, row - 1)
+  double bxs, bys;    // B3* faces of the last row computed
+};
 
-// B3 components from x and B3*: bx3(i, j) (left face) and by3(i, j) (bottom face)
-__device__ __forceinline__ double fin_bx3(const FinArgs& a, bool xv, int i, int j) {
-  const int jp = (j == a.ny - 1) ? 0 : j + 1;
-  const double r = -(fin_x(a, xv, i, jp) - fin_x(a, xv, i, j)) / a.dy;
-  return a.bx[j * a.nx + i] + r * a.c;
-}
-__device__ __forceinline__ double fin_by3(const FinArgs& a, bool xv, int i, int j) {
-  const int ip = (i == a.nx - 1) ? 0 : i + 1;
-  const double r = (fin_x(a, xv, ip, j) - fin_x(a, xv, i, j)) / a.dx;
-  return a.by[j * a.nx + i] + r * a.c;
-}
-// J3(i, j) given bx3(i, j-1) (carried)
-__device__ __forceinline__ double fin_j3(const FinArgs& a, bool xv, int i, int j, double bx3_below, double& bx3_here) {
-  const int im = (i == 0) ? a.nx - 1 : i - 1;
-  bx3_here = fin_bx3(a, xv, i, j);
-  const double dBy_dx = (fin_by3(a, xv, i, j) - fin_by3(a, xv, im, j)) / a.dx;
-  const double dBx_dy = (bx3_here - bx3_below) / a.dy;
+// J3(col, row) given chain.bx3_below = bx3(col, row - 1); advances the chain.
+__device__ __forceinline__ double fin2_j3(const FinArgs& a, bool xv, int col, int row, Fin2Chain& ch) {
+  const int nx = a.nx, ny = a.ny;
+  const int cm = (col == 0) ? nx - 1 : col - 1;
+  const int cp = (col == nx - 1) ? 0 : col + 1;
+  const int rp = (row == ny - 1) ? 0 : row + 1;
+  const double xc = xv ? a.x[row * nx + col] : 0.0;
+  const double xn = xv ? a.x[rp * nx + col] : 0.0;
+  const double xe = xv ? a.x[row * nx + cp] : 0.0;
+  const double xw = xv ? a.x[row * nx + cm] : 0.0;
+  const double bxs = a.bx[row * nx + col];
+  const double bys = a.by[row * nx + col];
+  const double bysw = a.by[row * nx + cm];
+  const double bx3 = bxs + (-(xn - xc) / a.dy) * a.c;           // bx3(col, row)
+  const double by3 = bys + ((xe - xc) / a.dx) * a.c;            // by3(col, row)
+  const double by3w = bysw + ((xc - xw) / a.dx) * a.c;          // by3(col - 1, row)
+  const double dBy_dx = (by3 - by3w) / a.dx;
+  const double dBx_dy = (bx3 - ch.bx3_below) / a.dy;
+  ch.bx3_below = bx3;
+  ch.bxs = bxs;
+  ch.bys = bys;
   return a.eta * (dBy_dx - dBx_dy);
 }
 
+__device__ __forceinline__ Fin2Chain fin2_begin(const FinArgs& a, bool xv, int col, int row0) {
+  const int nx = a.nx, ny = a.ny;
+  const int rm = (row0 == 0) ? ny - 1 : row0 - 1;
+  const double xc = xv ? a.x[rm * nx + col] : 0.0;
+  const double xn = xv ? a.x[row0 * nx + col] : 0.0;
+  Fin2Chain ch;
+  ch.bx3_below = a.bx[rm * nx + col] + (-(xn - xc) / a.dy) * a.c;
+  ch.bxs = ch.bys = 0.0;
+  return ch;
+}
+
 __global__ __launch_bounds__(kTileX) void k_finalize2(FinArgs a) {
+  __shared__ double jrow[2][kTileX + 2];
   const bool xv = a.st->spmv != 0;
   const int nx = a.nx, ny = a.ny;
   const Tile t = tile_of(nx, ny, a.ty, a.tiles_y);
-  if (t.y0 >= ny || !t.valid) return;
-  const int i0 = t.ic, i1 = t.ip;
-  const int jm = (t.y0 == 0) ? ny - 1 : t.y0 - 1;
-  // prologue: J3 at row y0 for columns i and i+1
-  double b0 = fin_bx3(a, xv, i0, jm), b1 = fin_bx3(a, xv, i1, jm);
-  double j0 = fin_j3(a, xv, i0, t.y0, b0, b0);
-  double j1 = fin_j3(a, xv, i1, t.y0, b1, b1);
+  int x0, w, xl, xr;
+  tile_columns(nx, a.ty, a.tiles_y, x0, w, xl, xr);
+  (void)xl;
+  const int tid = threadIdx.x;
+  const bool edge = (tid == w - 1);
+  Fin2Chain own = fin2_begin(a, xv, t.ic, t.y0);
+  Fin2Chain hal = fin2_begin(a, xv, xr, t.y0);
+  double jc = fin2_j3(a, xv, t.ic, t.y0, own);
+  if (t.valid) jrow[0][tid + 1] = jc;
+  if (edge) jrow[0][w + 1] = fin2_j3(a, xv, xr, t.y0, hal);
+  __syncthreads();
   for (int j = t.y0; j < t.y1; ++j) {
+    const int buf = (j - t.y0) & 1;
     const int jp = (j == ny - 1) ? 0 : j + 1;
-    double nb0, nb1;
-    const double jn0 = fin_j3(a, xv, i0, jp, b0, nb0);
-    const double jn1 = fin_j3(a, xv, i1, jp, b1, nb1);
-    const double g3x = -(jn0 - j0) / a.dy;
-    const double g3y = (j1 - j0) / a.dx;
-    const int o = j * nx + i0;
-    a.bx_out[o] = a.bx[o] + g3x * a.c;
-    a.by_out[o] = a.by[o] + g3y * a.c;
-    j0 = jn0; j1 = jn1; b0 = nb0; b1 = nb1;
+    const double bxs = own.bxs, bys = own.bys;   // B3* faces of row j
+    const double jn = fin2_j3(a, xv, t.ic, jp, own);
+    if (t.valid) jrow[buf ^ 1][tid + 1] = jn;
+    if (edge) jrow[buf ^ 1][w + 1] = fin2_j3(a, xv, xr, jp, hal);
+    const double jr = jrow[buf][tid + 2];        // J3(i + 1, j)
+    if (t.valid) {
+      const int o = j * nx + t.ic;
+      const double g3x = -(jn - jc) / a.dy;
+      const double g3y = (jr - jc) / a.dx;
+      a.bx_out[o] = bxs + g3x * a.c;
+      a.by_out[o] = bys + g3y * a.c;
+    }
+    __syncthreads();
+    jc = jn;
   }
 }
 

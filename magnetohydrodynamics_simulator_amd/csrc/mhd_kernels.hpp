@@ -40,8 +40,9 @@ struct CgArgs {
   double* r;
   double* p0;
   double* x;
-  double* part1;   // one partial per tile (p . Ap)
-  double* part2;   // two partials per tile (|r|^2, r . z or |b|^2, b . p)
+  double* part_s;  // start kernel: three partials per tile (|b|^2, b . p, p . Ap)
+  double* part_a;  // pass A: one partial per tile (p . Ap)
+  double* part_b;  // pass B: two partials per tile (|r|^2, r . z)
   CgState* st;
 };
 
@@ -64,8 +65,9 @@ struct DiagArgs {
 };
 
 template <int MODE> __global__ void k_stage(StageArgs a);
-__global__ void k_cg_init(CgArgs a, const double* bin, int explicit_only);
-template <bool FIRST> __global__ void k_cg_pass_a(CgArgs a, const double* pold, double* pnew, int it);
+template <bool FROM_B> __global__ void k_cg_start(CgArgs a, const double* bin);
+__global__ void k_cg_rhs(CgArgs a);
+__global__ void k_cg_pass_a(CgArgs a, const double* pold, double* pnew, int it);
 template <bool FIRST> __global__ void k_cg_pass_b(CgArgs a, const double* pv, int it);
 __global__ void k_finalize1(FinArgs a);
 __global__ void k_finalize2(FinArgs a);

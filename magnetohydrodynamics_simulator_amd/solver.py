@@ -192,8 +192,18 @@ class Context:
         return n, dts[:n].copy(), its[: 2 * n].reshape(n, 2).copy(), t.value
 
     # ---- instrumentation ----
-    def profile(self, on: bool):
-        self._check(self.lib.mhd_profile_enable(self.h, int(on)))
+    def profile(self, kernels=None):
+        """Bracket launches of the named kernels (all if True, none if
+        False/None) with HIP events; counters restart from zero."""
+        if kernels is True:
+            mask = (1 << len(_lib.KERNEL_IDS)) - 1
+        elif not kernels:
+            mask = 0
+        else:
+            mask = 0
+            for k in ([kernels] if isinstance(kernels, str) else kernels):
+                mask |= 1 << _lib.KERNEL_IDS[k]
+        self._check(self.lib.mhd_profile_enable(self.h, mask))
 
     def profile_get(self, kernel: str):
         n, ms = C.c_longlong(), C.c_double()
