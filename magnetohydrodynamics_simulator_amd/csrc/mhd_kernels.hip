@@ -352,15 +352,26 @@ __device__ __forceinline__ void tile_columns(int nx, int ty, int tiles_y, int& x
   (void)ty;
 }
 
-// eta J(B) at corner (i, j) (resistive_B_term's first loop, solvers.hpp:450-463).
+// eta J(B) at corner (i, j) (resistive_B_term's first loop, solvers.hpp:450-463)
+// from the face values by(i, j), by(i-1, j), bx(i, j), bx(i, j-1).
+__device__ __forceinline__ double rhs_from(const CgArgs& a, double by_c, double by_w, double bx_c, double bx_s) {
+  const double dBy_dx = (by_c - by_w) / a.dx;
+  const double dBx_dy = (bx_c - bx_s) / a.dy;
+  return a.eta * (dBy_dx - dBx_dy);
+}
 __device__ __forceinline__ double rhs_at(const CgArgs& a, int i, int j) {
   const int nx = a.nx, ny = a.ny;
   const int im = (i == 0) ? nx - 1 : i - 1;
   const int jm = (j == 0) ? ny - 1 : j - 1;
-  const double dBy_dx = (a.by[j * nx + i] - a.by[j * nx + im]) / a.dx;
-  const double dBx_dy = (a.bx[j * nx + i] - a.bx[jm * nx + i]) / a.dy;
-  return a.eta * (dBy_dx - dBx_dy);
+  return rhs_from(a, a.by[j * nx + i], a.by[j * nx + im], a.bx[j * nx + i], a.bx[jm * nx + i]);
 }
+
+__device__ __forceinline__ int wrap_row(int j, int ny) { return j < 0 ? j + ny : (j >= ny ? j - ny : j); }
+
+// The 2D sweeps below march a tile upwards one row per iteration.  Global
+// loads are issued two rows ahead of their use, so each wave keeps a row of
+// loads in flight across the LDS exchange and the arithmetic of the current
+// row (the barrier only waits for LDS traffic).
 
 // Start of a solve: b = eta J(B) (or bin), r = b, p = invd b, and the first
 // product q = A p; partials |b|^2, b . p, p . q.
@@ -373,20 +384,38 @@ __global__ __launch_bounds__(kTileX) void k_cg_start(CgArgs a, const double* __r
   int x0, w, xl, xr;
   tile_columns(nx, a.ty, a.tiles_y, x0, w, xl, xr);
   const int tid = threadIdx.x;
+  const bool left = (tid == 0), right = (tid == w - 1);
   const double invd = a.invd, d = a.diag, cx = a.cx, cy = a.cy;
-  auto bv = [&](int i, int j) -> double { return FROM_B ? rhs_at(a, i, j) : bin[j * nx + i]; };
+  // b of row j for the own column, with bx of the row below carried
+  double bx_carry = FROM_B ? a.bx[wrap_row(t.y0 - 2, ny) * nx + t.ic] : 0.0;
+  auto own_b = [&](int j) -> double {
+    const int r = wrap_row(j, ny);
+    if (!FROM_B) return bin[r * nx + t.ic];
+    const double bxc = a.bx[r * nx + t.ic];
+    const double v = rhs_from(a, a.by[r * nx + t.ic], a.by[r * nx + t.im], bxc, bx_carry);
+    bx_carry = bxc;
+    return v;
+  };
+  auto halo_b = [&](int col, int j) -> double {
+    const int r = wrap_row(j, ny);
+    return FROM_B ? rhs_at(a, col, r) : bin[r * nx + col];
+  };
   double acc[3] = {0.0, 0.0, 0.0};
-  const int jm0 = (t.y0 == 0) ? ny - 1 : t.y0 - 1;
-  double bm = bv(t.ic, jm0);
-  double bc = bv(t.ic, t.y0);
+  double bm = own_b(t.y0 - 1);
+  double bc = own_b(t.y0);
+  double bn = own_b(t.y0 + 1);
+  double hl = left ? halo_b(xl, t.y0) : 0.0;
+  double hr = right ? halo_b(xr, t.y0) : 0.0;
   for (int j = t.y0; j < t.y1; ++j) {
-    const int jp = (j == ny - 1) ? 0 : j + 1;
-    const double bn = bv(t.ic, jp);
+    // loads for the next row
+    const double bnn = (j + 1 < t.y1) ? own_b(j + 2) : 0.0;
+    const double hl_n = (left && j + 1 < t.y1) ? halo_b(xl, j + 1) : 0.0;
+    const double hr_n = (right && j + 1 < t.y1) ? halo_b(xr, j + 1) : 0.0;
     const int buf = (j - t.y0) & 1;
     const double pc = invd * bc;
     if (t.valid) prow[buf][tid + 1] = pc;
-    if (tid == 0) prow[buf][0] = invd * bv(xl, j);
-    if (tid == w - 1) prow[buf][w + 1] = invd * bv(xr, j);
+    if (left) prow[buf][0] = invd * hl;
+    if (right) prow[buf][w + 1] = invd * hr;
     __syncthreads();
     const double q = stencil(invd * bm, prow[buf][tid], pc, prow[buf][tid + 2], invd * bn, cx, cy, d);
     if (t.valid) {
@@ -399,6 +428,9 @@ __global__ __launch_bounds__(kTileX) void k_cg_start(CgArgs a, const double* __r
     }
     bm = bc;
     bc = bn;
+    bn = bnn;
+    hl = hl_n;
+    hr = hr_n;
   }
   block_sum<3>(acc, lds);
   if (tid == 0) {
@@ -443,19 +475,30 @@ __global__ __launch_bounds__(kTileX) void k_cg_pass_a(CgArgs a, const double* __
   int x0, w, xl, xr;
   tile_columns(nx, a.ty, a.tiles_y, x0, w, xl, xr);
   const int tid = threadIdx.x;
+  const bool left = (tid == 0), right = (tid == w - 1);
   const double invd = a.invd, d = a.diag, cx = a.cx, cy = a.cy;
-  auto pv = [&](int o) -> double { return fma(beta, pold[o], invd * a.r[o]); };
+  auto pv = [&](double po, double ro) -> double { return fma(beta, po, invd * ro); };
+  auto at = [&](int col, int j) -> int { return wrap_row(j, ny) * nx + col; };
   double acc[1] = {0.0};
-  const int jm0 = (t.y0 == 0) ? ny - 1 : t.y0 - 1;
-  double pm = pv(jm0 * nx + t.ic);
-  double pc = pv(t.y0 * nx + t.ic);
+  double pm = pv(pold[at(t.ic, t.y0 - 1)], a.r[at(t.ic, t.y0 - 1)]);
+  double pc = pv(pold[at(t.ic, t.y0)], a.r[at(t.ic, t.y0)]);
+  double n_po = pold[at(t.ic, t.y0 + 1)], n_ro = a.r[at(t.ic, t.y0 + 1)];
+  double l_po = left ? pold[at(xl, t.y0)] : 0.0, l_ro = left ? a.r[at(xl, t.y0)] : 0.0;
+  double r_po = right ? pold[at(xr, t.y0)] : 0.0, r_ro = right ? a.r[at(xr, t.y0)] : 0.0;
   for (int j = t.y0; j < t.y1; ++j) {
-    const int jp = (j == ny - 1) ? 0 : j + 1;
-    const double pn = pv(jp * nx + t.ic);
+    const bool more = j + 1 < t.y1;
+    // loads for the next row
+    const double nn_po = more ? pold[at(t.ic, j + 2)] : 0.0;
+    const double nn_ro = more ? a.r[at(t.ic, j + 2)] : 0.0;
+    const double ln_po = (left && more) ? pold[at(xl, j + 1)] : 0.0;
+    const double ln_ro = (left && more) ? a.r[at(xl, j + 1)] : 0.0;
+    const double rn_po = (right && more) ? pold[at(xr, j + 1)] : 0.0;
+    const double rn_ro = (right && more) ? a.r[at(xr, j + 1)] : 0.0;
+    const double pn = pv(n_po, n_ro);
     const int buf = (j - t.y0) & 1;
     if (t.valid) prow[buf][tid + 1] = pc;
-    if (tid == 0) prow[buf][0] = pv(j * nx + xl);
-    if (tid == w - 1) prow[buf][w + 1] = pv(j * nx + xr);
+    if (left) prow[buf][0] = pv(l_po, l_ro);
+    if (right) prow[buf][w + 1] = pv(r_po, r_ro);
     __syncthreads();
     const double q = stencil(pm, prow[buf][tid], pc, prow[buf][tid + 2], pn, cx, cy, d);
     if (t.valid) {
@@ -464,6 +507,9 @@ __global__ __launch_bounds__(kTileX) void k_cg_pass_a(CgArgs a, const double* __
     }
     pm = pc;
     pc = pn;
+    n_po = nn_po; n_ro = nn_ro;
+    l_po = ln_po; l_ro = ln_ro;
+    r_po = rn_po; r_ro = rn_ro;
   }
   block_sum<1>(acc, lds);
   if (tid == 0) a.part_a[t.vb] = acc[0];
@@ -516,25 +562,35 @@ __global__ __launch_bounds__(kTileX) void k_cg_pass_b(CgArgs a, const double* __
   int x0, w, xl, xr;
   tile_columns(nx, a.ty, a.tiles_y, x0, w, xl, xr);
   const int tid = threadIdx.x;
+  const bool left = (tid == 0), right = (tid == w - 1);
   const double invd = a.invd, d = a.diag, cx = a.cx, cy = a.cy;
+  auto at = [&](int col, int j) -> int { return wrap_row(j, ny) * nx + col; };
   double acc[2] = {0.0, 0.0};
-  const int jm0 = (t.y0 == 0) ? ny - 1 : t.y0 - 1;
-  double pm = pv[jm0 * nx + t.ic];
-  double pc = pv[t.y0 * nx + t.ic];
+  double pm = pv[at(t.ic, t.y0 - 1)];
+  double pc = pv[at(t.ic, t.y0)];
+  double pn = pv[at(t.ic, t.y0 + 1)];
+  double xc = FIRST ? 0.0 : a.x[t.y0 * nx + t.ic];
+  double rc = a.r[t.y0 * nx + t.ic];
+  double hl = left ? pv[at(xl, t.y0)] : 0.0;
+  double hr = right ? pv[at(xr, t.y0)] : 0.0;
   for (int j = t.y0; j < t.y1; ++j) {
-    const int jp = (j == ny - 1) ? 0 : j + 1;
-    const double pn = pv[jp * nx + t.ic];
+    const bool more = j + 1 < t.y1;
+    // loads for the next row
+    const double pnn = more ? pv[at(t.ic, j + 2)] : 0.0;
+    const double xn = (!FIRST && more) ? a.x[(j + 1) * nx + t.ic] : 0.0;
+    const double rnx = more ? a.r[(j + 1) * nx + t.ic] : 0.0;
+    const double hl_n = (left && more) ? pv[at(xl, j + 1)] : 0.0;
+    const double hr_n = (right && more) ? pv[at(xr, j + 1)] : 0.0;
     const int buf = (j - t.y0) & 1;
     if (t.valid) prow[buf][tid + 1] = pc;
-    if (tid == 0) prow[buf][0] = pv[j * nx + xl];
-    if (tid == w - 1) prow[buf][w + 1] = pv[j * nx + xr];
+    if (left) prow[buf][0] = hl;
+    if (right) prow[buf][w + 1] = hr;
     __syncthreads();
     const double q = stencil(pm, prow[buf][tid], pc, prow[buf][tid + 2], pn, cx, cy, d);
     if (t.valid) {
       const int o = j * nx + t.ic;
-      const double xo = FIRST ? 0.0 : a.x[o];
-      a.x[o] = xo + alpha * pc;
-      const double rn = a.r[o] - alpha * q;
+      a.x[o] = xc + alpha * pc;
+      const double rn = rc - alpha * q;
       a.r[o] = rn;
       const double z = invd * rn;
       acc[0] += rn * rn;
@@ -542,6 +598,11 @@ __global__ __launch_bounds__(kTileX) void k_cg_pass_b(CgArgs a, const double* __
     }
     pm = pc;
     pc = pn;
+    pn = pnn;
+    xc = xn;
+    rc = rnx;
+    hl = hl_n;
+    hr = hr_n;
   }
   block_sum<2>(acc, lds);
   if (tid == 0) {
@@ -568,16 +629,21 @@ __global__ __launch_bounds__(kTileX) void k_finalize1(FinArgs a) {
   if (a.st->spmv == 0) return;
   const int nx = a.nx, ny = a.ny;
   const Tile t = tile_of(nx, ny, a.ty, a.tiles_y);
-  if (t.y0 >= ny || !t.valid) return;
-  double xc = a.x[t.y0 * nx + t.ic];
+  if (!t.valid) return;
+  auto at = [&](int col, int j) -> int { return wrap_row(j, ny) * nx + col; };
+  double xc = a.x[at(t.ic, t.y0)], xn = a.x[at(t.ic, t.y0 + 1)], xe = a.x[at(t.ip, t.y0)];
+  double bxv = a.bx[t.y0 * nx + t.ic], byv = a.by[t.y0 * nx + t.ic];
   for (int j = t.y0; j < t.y1; ++j) {
-    const int jp = (j == ny - 1) ? 0 : j + 1;
+    const bool more = j + 1 < t.y1;
+    // loads for the next row (rows j + 1 of B are read before row j is written)
+    const double xnn = more ? a.x[at(t.ic, j + 2)] : 0.0;
+    const double xen = more ? a.x[at(t.ip, j + 1)] : 0.0;
+    const double bxn = more ? a.bx[(j + 1) * nx + t.ic] : 0.0;
+    const double byn = more ? a.by[(j + 1) * nx + t.ic] : 0.0;
     const int o = j * nx + t.ic;
-    const double xn = a.x[jp * nx + t.ic];
-    const double xe = a.x[j * nx + t.ip];
-    a.bx_out[o] = a.bx[o] + (-(xn - xc) / a.dy) * a.c;
-    a.by_out[o] = a.by[o] + ((xe - xc) / a.dx) * a.c;
-    xc = xn;
+    a.bx_out[o] = bxv + (-(xn - xc) / a.dy) * a.c;
+    a.by_out[o] = byv + ((xe - xc) / a.dx) * a.c;
+    xc = xn; xn = xnn; xe = xen; bxv = bxn; byv = byn;
   }
 }
 
@@ -695,24 +761,34 @@ __global__ __launch_bounds__(kTileX) void k_cfl(DiagArgs a) {
   const Tile t = tile_of(nx, ny, a.ty, a.tiles_y);
   double m = 1e-8;
   double bad[1] = {0.0};
-  if (t.y0 < ny && t.valid) {
-    double byc_lo = a.s[5][t.y0 * nx + t.ic];
+  if (t.valid) {
+    auto at = [&](int col, int j) -> int { return wrap_row(j, ny) * nx + col; };
+    double by_lo = a.s[5][at(t.ic, t.y0)];
+    int o = t.y0 * nx + t.ic;
+    double rho = a.s[0][o], mx = a.s[1][o], my = a.s[2][o], E = a.s[3][o];
+    double bxl = a.s[4][o], bxr = a.s[4][t.y0 * nx + t.ip], by_hi = a.s[5][at(t.ic, t.y0 + 1)];
     for (int j = t.y0; j < t.y1; ++j) {
-      const int jp = (j == ny - 1) ? 0 : j + 1;
-      const int o = j * nx + t.ic;
-      const double by_hi = a.s[5][jp * nx + t.ic];
-      const double bxc = (a.s[4][o] + a.s[4][j * nx + t.ip]) / 2.0;
-      const double byc = (byc_lo + by_hi) / 2.0;
-      byc_lo = by_hi;
-      double rho, vx, vy, p;
-      primitives(a.s[0][o], a.s[1][o], a.s[2][o], a.s[3][o], bxc, byc, rho, vx, vy, p);
-      rho = dmax(rho, 1e-8);
+      const bool more = j + 1 < t.y1;
+      // loads for the next row
+      const int on = more ? (j + 1) * nx + t.ic : o;
+      const double rho_n = a.s[0][on], mx_n = a.s[1][on], my_n = a.s[2][on], E_n = a.s[3][on];
+      const double bxl_n = a.s[4][on];
+      const double bxr_n = a.s[4][more ? (j + 1) * nx + t.ip : o];
+      const double by_hi_n = more ? a.s[5][at(t.ic, j + 2)] : 0.0;
+      const double bxc = (bxl + bxr) / 2.0;
+      const double byc = (by_lo + by_hi) / 2.0;
+      double rf, vx, vy, p;
+      primitives(rho, mx, my, E, bxc, byc, rf, vx, vy, p);
+      rf = dmax(rf, 1e-8);
       p = dmax(p, 1e-8);
-      const double ss = kGamma * p / rho;
-      const double as = (bxc * bxc + byc * byc) / rho;
+      const double ss = kGamma * p / rf;
+      const double as = (bxc * bxc + byc * byc) / rf;
       const double cur = sqrt(ss + as) + sqrt(vx * vx + vy * vy);
       if (cur > m) m = cur;
       if (!(fabs(cur) <= 1.7976931348623157e308)) bad[0] += 1.0;
+      by_lo = by_hi; by_hi = by_hi_n;
+      rho = rho_n; mx = mx_n; my = my_n; E = E_n; bxl = bxl_n; bxr = bxr_n;
+      o = on;
     }
   }
   block_max(m, lds);
