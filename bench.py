@@ -156,6 +156,13 @@ def main():
     dom_cnt, dom_ms = ctx.profile_get(dom)
     ctx.profile(False)
     rms, divmax = ctx.calculate_current_div_B()
+    # reference point: a plain device copy of 1 GiB (read + write) on the same stream
+    ctx.profile(["other"])
+    for _ in range(3):
+        ctx.calibration_copy(1 << 30)
+    ncopy, ms_copy = ctx.profile_get("other")
+    copy_gbs = 2 * (1 << 30) / (ms_copy / ncopy * 1e-3) / 1e9
+    ctx.profile(False)
     ctx.close()
 
     solves = 2 * steps
@@ -186,6 +193,7 @@ def main():
         "step_roofline": {"algorithmic_bytes_per_step": int(step_bytes),
                           "frac": round(step_bytes / (elapsed / steps) / 1e9 / HBM_PEAK_GBS, 4)},
         "cg_spmv_per_step": round(spmv / steps, 2),
+        "copy_kernel_GBs": round(copy_gbs, 1),
         "divb_max": divmax,
         "kernels": kb,
     }

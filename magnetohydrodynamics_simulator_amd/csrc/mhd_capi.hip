@@ -63,7 +63,7 @@ struct mhd_ctx {
   double* part_host = nullptr;   // pinned, two partials per tile
   double alpha = 0.0;
   bool have_op = false;
-  int last_spmv = 1;
+  int last_spmv[2] = {1, 1};     // products of the previous solve in each of the step's two slots
   int tiles = 1, tiles_y = 1, tile_rows = 4;   // 2D sweeps: kTileX columns x tile_rows rows per tile
   int stage_rows = 16;
   std::string err;
@@ -187,7 +187,7 @@ CgArgs cg_args(mhd_ctx* c, const double* bx, const double* by) {
 // before it; kernels launched after the decision return at once.  The host
 // reads the state back after each chunk; the first chunk is as long as the
 // previous solve's product count, so a steady run syncs once per solve.
-int cg_run(mhd_ctx* c, const double* bx, const double* by, const double* bin, CgState* out) {
+int cg_run(mhd_ctx* c, const double* bx, const double* by, const double* bin, CgState* out, int slot) {
   if (!c->have_op) return fail(c, MHD_EINVAL, "implicit solve before mhd_build_operator");
   CgArgs a = cg_args(c, bx, by);
   const long long maxit = (c->o.cg_max_iter > 0) ? c->o.cg_max_iter : 2LL * c->n;
@@ -207,7 +207,7 @@ int cg_run(mhd_ctx* c, const double* bx, const double* by, const double* bin, Cg
   // the direction of iteration it lives in p[it & 1] (p[0] from the start kernel)
   long long it = 1;
   for (;;) {
-    const int chunk = (it == 1) ? (c->last_spmv > 0 ? c->last_spmv : 1) : 1;
+    const int chunk = (it == 1) ? (c->last_spmv[slot] > 0 ? c->last_spmv[slot] : 1) : 1;
     for (int k = 0; k < chunk; ++k, ++it) {
       const double* pold = c->p[(it - 1) & 1];
       double* pnew = c->p[it & 1];
@@ -234,7 +234,7 @@ int cg_run(mhd_ctx* c, const double* bx, const double* by, const double* bin, Cg
     if (it > maxit + 4) return fail(c, MHD_EHIP, "CG state machine did not terminate");
   }
   *out = *c->st_host;
-  if (out->spmv > 0) c->last_spmv = out->spmv;
+  if (out->spmv > 0) c->last_spmv[slot] = out->spmv;
   if (out->done == 2 && c->o.cg_fail_is_error) return fail(c, MHD_ECG, "CG reached the iteration limit");
   return MHD_OK;
 }
@@ -558,7 +558,7 @@ int mhd_imex_step(mhd_ctx* c, double dt, mhd_step_stats* stats) {
   if (c->eta > 0.0) {
     // implicit 1: x = CG(eta J(S.B)), S.B += dt A_im[4] rate(x) (solvers.hpp:526-529)
     CgState cs;
-    rc = cg_run(c, c->S[4], c->S[5], nullptr, &cs);
+    rc = cg_run(c, c->S[4], c->S[5], nullptr, &cs, 0);
     if (rc) return rc;
     st.cg_iters[0] = cs.iters;
     st.cg_spmv[0] = cs.spmv;
@@ -583,7 +583,7 @@ int mhd_imex_step(mhd_ctx* c, double dt, mhd_step_stats* stats) {
   if (c->eta > 0.0) {
     // implicit 2 and the final recombination (solvers.hpp:571-608)
     CgState cs;
-    rc = cg_run(c, c->U[4], c->U[5], nullptr, &cs);
+    rc = cg_run(c, c->U[4], c->U[5], nullptr, &cs, 1);
     if (rc) return rc;
     st.cg_iters[1] = cs.iters;
     st.cg_spmv[1] = cs.spmv;
@@ -623,7 +623,7 @@ int mhd_resistive_term(mhd_ctx* c, int implicit, double* bx_rate, double* by_rat
   int zero = 0;
   if (implicit) {
     CgState cs;
-    rc = cg_run(c, c->U[4], c->U[5], nullptr, &cs);
+    rc = cg_run(c, c->U[4], c->U[5], nullptr, &cs, 0);
     if (rc) return rc;
     if (iters) *iters = cs.iters;
     zero = cs.spmv == 0;
@@ -646,7 +646,7 @@ int mhd_cg_solve(mhd_ctx* c, const double* b, double* x, int* iters, double* err
   const size_t fb = (size_t)c->n * sizeof(double);
   HIP_OR_FAIL(c, hipMemcpyAsync(c->F1[0], b, fb, hipMemcpyHostToDevice, c->stream));
   CgState cs;
-  int rc = cg_run(c, c->U[4], c->U[5], c->F1[0], &cs);
+  int rc = cg_run(c, c->U[4], c->U[5], c->F1[0], &cs, 0);
   if (rc) return rc;
   if (cs.spmv == 0) {
     std::memset(x, 0, fb);

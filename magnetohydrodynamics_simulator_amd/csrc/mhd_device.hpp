@@ -48,12 +48,43 @@ __device__ __forceinline__ void muscl(double fm, double f0, double fp, double& L
 __device__ __forceinline__ double dmin(double a, double b) { return (b < a) ? b : a; }
 __device__ __forceinline__ double dmax(double a, double b) { return (a < b) ? b : a; }
 
+// 1/x for finite, normal, non-zero x: the hardware estimate (v_rcp_f64)
+// refined by two Newton-Raphson steps r <- r + r (1 - x r), each of which
+// doubles the number of correct bits; within about an ulp of the correctly
+// rounded reciprocal.  The kernels apply it to densities (floored at 1e-8)
+// and to HLL wave-speed spans (> 0 where used), so the scaling and special-
+// case handling of the IEEE division sequence are not needed.
+__device__ __forceinline__ double rcp(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-x, r, 1.0);
+  return fma(r, e, r);
+}
+
+// sqrt(x) for normal x > 0: Goldschmidt's coupled iteration on
+// g ~ sqrt(x), h ~ 1 / (2 sqrt(x)) started from the hardware reciprocal
+// square root (v_rsq_f64), followed by two Newton corrections
+// g <- g + h (x - g^2).
+__device__ __forceinline__ double sqrt_nr(double x) {
+  const double y = __builtin_amdgcn_rsq(x);
+  double g = x * y, h = 0.5 * y;
+  const double r = fma(-h, g, 0.5);
+  g = fma(g, r, g);
+  h = fma(h, r, h);
+  double d = fma(-g, g, x);
+  g = fma(d, h, g);
+  d = fma(-g, g, x);
+  return fma(d, h, g);
+}
+
 // Grid::computePrimitives for one cell, Grid.hpp:111-126.  rho is the stored
 // density; the velocities use it before the floor is applied.
 __device__ __forceinline__ void primitives(double rho, double mx, double my, double E, double bxc, double byc,
                                            double& rho_f, double& vx, double& vy, double& p) {
-  vx = mx / rho;
-  vy = my / rho;
+  const double inv = rcp(rho);
+  vx = mx * inv;
+  vy = my * inv;
   rho_f = (rho < kFloor) ? kFloor : rho;
   double kinE = 0.5 * rho_f * (vx * vx + vy * vy);
   double magE = (bxc * bxc + byc * byc) / 2;
@@ -65,13 +96,16 @@ __device__ __forceinline__ void primitives(double rho, double mx, double my, dou
 // getFastWaveSpeed, HLL.hpp:107-122 (three divisions by rho -> one reciprocal).
 __device__ __forceinline__ double fast_speed(const Prim& s, double bn) {
   double B_sq = s.bx * s.bx + s.by * s.by;
-  double inv_rho = 1.0 / s.rho;
+  double inv_rho = rcp(s.rho);
   double sound_v_sq = (kGamma * s.p) * inv_rho;
   double tot_alfven_v_sq = B_sq * inv_rho;
   double norm_alfven_v_sq = (bn * bn) * inv_rho;
   double sum = tot_alfven_v_sq + sound_v_sq;
   double term = sum * sum - 4.0 * sound_v_sq * norm_alfven_v_sq;
-  return sqrt(0.5 * (sum + sqrt(dmax(0.0, term))));
+  // term >= 0; a term below 2^-1000 adds nothing to sum (sum >= gamma p / rho
+  // with both floors applied), so its root is taken as 0
+  const double root = (term >= 0x1p-1000) ? sqrt_nr(term) : 0.0;
+  return sqrt_nr(0.5 * (sum + root));
 }
 
 // convertPrimitiveX (HLL.hpp:61-81) and the conserved energy of
@@ -120,7 +154,7 @@ __device__ __forceinline__ Flux hll_x(const Prim& L, const Prim& R) {
   double sR = dmax(L.vx + cL, R.vx + cR);
   if (sL >= 0) return FL;
   if (sR <= 0) return FR;
-  double inv_S = 1.0 / (sR - sL);
+  double inv_S = rcp(sR - sL);
   double sLR = sL * sR;
   Flux H;
   H.rho = (sR * FL.rho - sL * FR.rho + sLR * (R.rho - L.rho)) * inv_S;
@@ -144,7 +178,7 @@ __device__ __forceinline__ Flux hll_y(const Prim& D, const Prim& U) {
   double sU = dmax(D.vy + cD, U.vy + cU);
   if (sD >= 0) return FD;
   if (sU <= 0) return FU;
-  double inv_S = 1.0 / (sU - sD);
+  double inv_S = rcp(sU - sD);
   double sDU = sD * sU;
   Flux H;
   H.rho = (sU * FD.rho - sD * FU.rho + sDU * (U.rho - D.rho)) * inv_S;
