@@ -71,19 +71,25 @@ __global__ __launch_bounds__(64) void k_stage(StageArgs a) {
   Carry s = {};
   s.byf_c = a.s[5][wrap1(y0 - 2, ny) * nx + c0];
   s.bxl_b = 0.0;
+  // row loads are issued one row ahead of their use so that their latency
+  // overlaps the flux arithmetic of the current row
+  double n_rho, n_mx, n_my, n_E, n_bxl, n_bxr, n_byn;
+  {
+    const int b0 = wrap1(y0 - 2, ny) * nx;
+    const int b1 = wrap1(y0 - 1, ny) * nx;
+    n_rho = a.s[0][b0 + c0]; n_mx = a.s[1][b0 + c0]; n_my = a.s[2][b0 + c0]; n_E = a.s[3][b0 + c0];
+    n_bxl = a.s[4][b0 + c0]; n_bxr = a.s[4][b0 + c1]; n_byn = a.s[5][b1 + c0];
+  }
 
   for (int k = y0 - 2; k <= y1 + 1; ++k) {
-    const int rk = wrap1(k, ny);
-    const int rk1 = wrap1(k + 1, ny);
-    const int base = rk * nx;
-    // ---- loads of row k ----
-    const double rho = a.s[0][base + c0];
-    const double mx = a.s[1][base + c0];
-    const double my = a.s[2][base + c0];
-    const double E = a.s[3][base + c0];
-    const double bxl = a.s[4][base + c0];
-    const double bxr = a.s[4][base + c1];
-    const double byn = a.s[5][rk1 * nx + c0];
+    // ---- loads of row k (issued last iteration) and of row k + 1 ----
+    const double rho = n_rho, mx = n_mx, my = n_my, E = n_E, bxl = n_bxl, bxr = n_bxr, byn = n_byn;
+    if (k < y1 + 1) {
+      const int bn = wrap1(k + 1, ny) * nx;
+      const int bnn = wrap1(k + 2, ny) * nx;
+      n_rho = a.s[0][bn + c0]; n_mx = a.s[1][bn + c0]; n_my = a.s[2][bn + c0]; n_E = a.s[3][bn + c0];
+      n_bxl = a.s[4][bn + c0]; n_bxr = a.s[4][bn + c1]; n_byn = a.s[5][bnn + c0];
+    }
     // ---- P(k) ----
     const double bxc = (bxl + bxr) / 2.0;
     const double byc = (s.byf_c + byn) / 2.0;

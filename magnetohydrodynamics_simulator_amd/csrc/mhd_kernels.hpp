@@ -8,7 +8,10 @@ namespace mhd {
 
 constexpr int kStripOut = 60;     // output columns per 64-lane strip of the stage kernel
 constexpr int kTileX = 256;       // columns (= threads) per tile of the 2D sweeps
-constexpr int kTargetTiles = 2048;
+#ifndef MHD_TARGET_TILES
+#define MHD_TARGET_TILES 2048
+#endif
+constexpr int kTargetTiles = MHD_TARGET_TILES;   // tiles per 2D sweep (compile-time tuning parameter)
 
 // Explicit-stage kernel arguments.  MODE 0: o1 = L(s).  MODE 1: o1 = f1 = L(s),
 // o2 = s + c_a f1.  MODE 2: o1 = (u + c_a f1) + c_b L(s) [+ c_g g2(s) on B].
