@@ -2,10 +2,13 @@
 //
 // Host-side orchestration of one ARS(2,2,2) step (include/solvers.hpp:491-613
 // of the reference) on device-resident compact fields:
-//   stage 1       U -> f1 = L(U), S = U + dt g f1                 (1 kernel)
-//   implicit 1    CG on eta J(S.B), S.B += dt g rate(x)           (init, 2/iter, 1)
-//   stage 2       U <- (U + dt d f1) + dt (1-d) L(S) [+ dt (1-g) g2(S)]   (1 kernel)
-//   implicit 2    CG on eta J(U.B), B^{n+1} = U.B + dt g g3(B3)   (init, 2/iter, 1)
+//   stage 1       f1 = L(U); W = U + dt d f1 (held in F1), S = U + dt g f1   (1 kernel)
+//   implicit 1    CG on eta J(S.B), S.B += dt g rate(x)           (start, 2/iter, 1)
+//   stage 2       U <- W + dt (1-d) L(S) [+ dt (1-g) g2(S)]        (1 kernel)
+//   implicit 2    CG on eta J(U.B), B^{n+1} = U.B + dt g g3(B3)   (start, 2/iter, 1)
+// W is the reference's grid_whole_step after its f1 term (solvers.hpp:548-553):
+// the same additions in the same order, formed in stage 1 so that stage 2
+// reads one field per cell instead of U^n and f1.
 // with g = (2 - sqrt 2)/2 and d = 1 - 1/(2g) (solvers.hpp:41-42).  The stage-1
 // explicit resistive term g1 has weight 0 in every combination
 // (A_im[3] = A_im[6] = b_im[0] = 0) and is not evaluated.
@@ -544,7 +547,8 @@ int mhd_imex_step(mhd_ctx* c, double dt, mhd_step_stats* stats) {
   mhd_step_stats st{};
   st.dt = dt;
   const Tableau& T = c->tab;
-  // stage 1: f1 = L(U), S = U + dt A_ex[3] f1 (solvers.hpp:495-523)
+  // stage 1: f1 = L(U), S = U + dt A_ex[3] f1 (solvers.hpp:495-523),
+  // W = U + dt A_ex[6] f1 (solvers.hpp:548-553) into F1
   StageArgs a = stage_args(c);
   for (int q = 0; q < 6; ++q) {
     a.s[q] = c->U[q];
@@ -552,6 +556,7 @@ int mhd_imex_step(mhd_ctx* c, double dt, mhd_step_stats* stats) {
     a.o2[q] = c->S[q];
   }
   a.c_a = dt * T.A_ex3;
+  a.c_b = dt * T.A_ex6;
   int rc = run_stage(c, 1, a, 0);
   if (rc) return rc;
   const double diag_coeff = dt * T.A_im4;
@@ -567,15 +572,13 @@ int mhd_imex_step(mhd_ctx* c, double dt, mhd_step_stats* stats) {
     rc = launch(c, 5, [&] { hipLaunchKernelGGL(k_finalize1, dim3(c->tiles), dim3(kTileX), 0, c->stream, f); });
     if (rc) return rc;
   }
-  // stage 2: U3* = (U + dt A_ex[6] f1) + dt A_ex[7] L(S) [+ dt A_im[7] g2] (solvers.hpp:535-566)
+  // stage 2: U3* = W + dt A_ex[7] L(S) [+ dt A_im[7] g2] (solvers.hpp:535-566)
   StageArgs b = stage_args(c);
   for (int q = 0; q < 6; ++q) {
     b.s[q] = c->S[q];
-    b.u[q] = c->U[q];
-    b.f1[q] = c->F1[q];
+    b.u[q] = c->F1[q];
     b.o1[q] = c->U[q];
   }
-  b.c_a = dt * T.A_ex6;
   b.c_b = dt * T.A_ex7;
   b.c_g = dt * T.A_im7;
   rc = run_stage(c, 2, b, 1);

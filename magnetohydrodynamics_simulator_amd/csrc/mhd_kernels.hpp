@@ -13,12 +13,12 @@ constexpr int kTileX = 256;       // columns (= threads) per tile of the 2D swee
 #endif
 constexpr int kTargetTiles = MHD_TARGET_TILES;   // tiles per 2D sweep (compile-time tuning parameter)
 
-// Explicit-stage kernel arguments.  MODE 0: o1 = L(s).  MODE 1: o1 = f1 = L(s),
-// o2 = s + c_a f1.  MODE 2: o1 = (u + c_a f1) + c_b L(s) [+ c_g g2(s) on B].
+// Explicit-stage kernel arguments.  MODE 0: o1 = L(s).
+// MODE 1 (f1 = L(s)): o1 = W = s + c_b f1, o2 = U2* = s + c_a f1.
+// MODE 2 (f2 = L(s)): o1 = U3* = u + c_b f2 [+ c_g g2(s) on B], u = W.
 struct StageArgs {
   const double* s[6];   // state whose fluxes are computed (halo reads)
-  const double* u[6];   // MODE 2: U^n
-  const double* f1[6];  // MODE 2: f1
+  const double* u[6];   // MODE 2: W
   double* o1[6];
   double* o2[6];
   int nx, ny, rows;
