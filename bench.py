@@ -41,14 +41,20 @@ HBM_PEAK_GBS = 8000.0
 #   cg_pass_b  reads p, x, r, writes x, r                                      = 40 (first: x = 0, 32)
 #   finalize1 / finalize2  read x and face B (3 fields), write face B (2)      = 40
 #   cfl        reads U (6)                                                     = 48
-# Per IMEX step the fixed algorithmic figure is N (288 + 144 [eta > 0] + 64 n)
-# with n the operator products of both solves: 288 for the two explicit
-# stages, 2 x 72 for the set-up and finish of the two implicit solves and 64
-# per CG iteration (CFL is counted separately, once per step).
+# The per-step figure of the step roofline is the fixed byte model of the
+# reference-faithful data flow, N (336 + 144 [eta > 0] + 64 n) with n the
+# operator products of both solves: 336 for two explicit stages that store
+# f1 (this library stores W = U + dt delta f1 instead and moves 288), 2 x 72
+# for the set-up and finish of the two implicit solves, 64 per CG iteration.
 STAGE_BYTES = {"stage1": 144, "stage2": 144, "cg_start": 32, "implicit_finalize1": 40,
                "implicit_finalize2": 40, "cfl": 48}
 KERNELS = ("stage1", "stage2", "cg_start", "cg_pass_a", "cg_pass_b", "implicit_finalize1",
            "implicit_finalize2", "cfl")
+# device symbol of each kernel id, for the hardware-counter pass
+KERNEL_SYMBOLS = {"stage1": "k_stage<1>", "stage2": "k_stage<2>", "cg_start": "k_cg_start",
+                  "cg_pass_a": "k_cg_pass_a", "cg_pass_b": "k_cg_pass_b", "implicit_finalize1": "k_finalize1",
+                  "implicit_finalize2": "k_finalize2", "cfl": "k_cfl"}
+CALIB_BYTES = 1 << 30          # the copy kernel of the counter calibration reads and writes 1 GiB
 
 
 def kernel_bytes(k: str, N: int, launches: int, solves: int, spmv: int) -> float:
@@ -72,7 +78,78 @@ def parse():
     ap.add_argument("--cfl", type=float, default=0.4)
     ap.add_argument("--cpu-steps", type=int, default=2, help="CPU-baseline sample (steps at --n)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 counter passes")
+    ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     return ap.parse_args()
+
+
+def pmc_child(a) -> None:
+    """Workload of the counter passes: two driver steps, then the 1 GiB
+    calibration copy."""
+    from magnetohydrodynamics_simulator_amd import Context, setups
+
+    n = a.n
+    dx = setups.grid_spacing(n)
+    ic = setups.orszag_tang(n)
+    ctx = Context(n, n, dx, dx, a.eta, cfl=a.cfl, cg_tol=a.cg_tol)
+    ctx.upload(*[ic[k] for k in ("rho", "mx", "my", "E", "bxf", "byf")])
+    ctx.run(1e9, max_steps=2)
+    ctx.calibration_copy(CALIB_BYTES)
+    ctx.synchronize()
+    ctx.close()
+
+
+def _counter_rows(out_dir: str):
+    import csv
+    import glob
+    rows = []
+    for f in glob.glob(os.path.join(out_dir, "**", "*counter_collection.csv"), recursive=True):
+        with open(f, newline="") as fh:
+            rows.extend(csv.DictReader(fh))
+    return rows
+
+
+def hbm_traffic(a, kernel: str) -> dict | None:
+    """HBM bytes per launch of `kernel` from two rocprofv3 counter passes
+    (FETCH_SIZE, then WRITE_SIZE; KiB per dispatch), each scaled by what the
+    same counter reads for the 1 GiB copy kernel.  None if rocprofv3 is
+    unavailable or a pass fails."""
+    import shutil
+    import tempfile
+
+    exe = shutil.which("rocprofv3")
+    if exe is None:
+        return None
+    sym = KERNEL_SYMBOLS[kernel]
+    res = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        out_dir = tempfile.mkdtemp(prefix="mhd_pmc_")
+        cmd = [exe, "--pmc", counter, "--kernel-include-regex", f"{sym}|k_copy", "-d", out_dir, "-o", "pmc",
+               "--output-format", "csv", "--", sys.executable, os.path.abspath(__file__), "--pmc-child",
+               "--n", str(a.n), "--eta", str(a.eta), "--cg-tol", str(a.cg_tol), "--cfl", str(a.cfl)]
+        try:
+            r = subprocess.run(cmd, capture_output=True, text=True, timeout=120, cwd="/tmp")
+        except subprocess.TimeoutExpired:
+            return None
+        if r.returncode != 0:
+            return None
+        vals, copy = [], []
+        for row in _counter_rows(out_dir):
+            name = row.get("Kernel_Name", "")
+            if row.get("Counter_Name") != counter:
+                continue
+            v = float(row.get("Counter_Value", "nan"))
+            if "k_copy" in name:
+                copy.append(v)
+            elif sym in name:
+                vals.append(v)
+        shutil.rmtree(out_dir, ignore_errors=True)
+        if not vals or not copy:
+            return None
+        scale = CALIB_BYTES / (sum(copy) / len(copy) * 1024.0)     # true bytes per counted byte
+        res[counter] = sum(vals) / len(vals) * 1024.0 * scale
+        res[counter + "_scale"] = scale
+    return res
 
 
 def cpu_baseline(n: int, eta: float, tol: float, cfl: float, steps: int) -> dict:
@@ -117,6 +194,9 @@ def cpu_baseline(n: int, eta: float, tol: float, cfl: float, steps: int) -> dict
 
 def main():
     a = parse()
+    if a.pmc_child:
+        pmc_child(a)
+        return
     if a.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {a.gpus}); multi-GPU runs are not part of this build",
               file=sys.stderr)
@@ -169,7 +249,7 @@ def main():
     spmv = int(its.sum()) + solves          # converged solves: products = iterations + 1
     bytes_per_launch = kernel_bytes(dom, N, dom_cnt, solves, spmv) / dom_cnt
     achieved = bytes_per_launch / (dom_ms * 1e-3 / dom_cnt) / 1e9
-    step_bytes = N * (288 + (144 if a.eta > 0 else 0) + 64 * (spmv / steps))
+    step_bytes = N * (336 + (144 if a.eta > 0 else 0) + 64 * (spmv / steps))
 
     line = {
         "metric": "FP64 Mcell-updates/s per IMEX step",
@@ -197,6 +277,14 @@ def main():
         "divb_max": divmax,
         "kernels": kb,
     }
+    if not a.no_traffic:
+        tr = hbm_traffic(a, dom)
+        if tr is not None:
+            line["roofline"]["traffic"] = int(tr["FETCH_SIZE"] + tr["WRITE_SIZE"])
+            line["roofline"]["traffic_detail"] = {
+                "fetch_bytes": int(tr["FETCH_SIZE"]), "write_bytes": int(tr["WRITE_SIZE"]),
+                "fetch_scale": round(tr["FETCH_SIZE_scale"], 4), "write_scale": round(tr["WRITE_SIZE_scale"], 4),
+                "source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, 2 driver steps, scaled by a 1 GiB copy"}
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baseline(n, a.eta, a.cg_tol, a.cfl, a.cpu_steps)
     print(json.dumps(line), flush=True)

@@ -415,17 +415,16 @@ __global__ __launch_bounds__(kTileX) void k_cg_start(CgArgs a, const double* __r
     return FROM_B ? rhs_at(a, col, r) : bin[r * nx + col];
   };
   double acc[3] = {0.0, 0.0, 0.0};
-  const int cl = left ? xl : t.ic, cr = right ? xr : t.ic;
   double bm = own_b(t.y0 - 1);
   double bc = own_b(t.y0);
   double bn = own_b(t.y0 + 1);
-  double hl = halo_b(cl, t.y0);
-  double hr = halo_b(cr, t.y0);
+  double hl = left ? halo_b(xl, t.y0) : 0.0;
+  double hr = right ? halo_b(xr, t.y0) : 0.0;
   for (int j = t.y0; j < t.y1; ++j) {
-    // loads for the next row (unconditional, see pass A)
-    const double bnn = own_b(j + 2);
-    const double hl_n = halo_b(cl, j + 1);
-    const double hr_n = halo_b(cr, j + 1);
+    // loads for the next row
+    const double bnn = (j + 1 < t.y1) ? own_b(j + 2) : 0.0;
+    const double hl_n = (left && j + 1 < t.y1) ? halo_b(xl, j + 1) : 0.0;
+    const double hr_n = (right && j + 1 < t.y1) ? halo_b(xr, j + 1) : 0.0;
     const int buf = (j - t.y0) & 1;
     const double pc = invd * bc;
     if (t.valid) prow[buf][tid + 1] = pc;
@@ -435,7 +434,7 @@ __global__ __launch_bounds__(kTileX) void k_cg_start(CgArgs a, const double* __r
     const double q = stencil(invd * bm, prow[buf][tid], pc, prow[buf][tid + 2], invd * bn, cx, cy, d);
     if (t.valid) {
       const int o = j * nx + t.ic;
-      a.r[o] = bc;
+      a.r[o] = bc - 0.0;
       a.p0[o] = pc;
       acc[0] += bc * bc;
       acc[1] += bc * pc;
@@ -494,21 +493,21 @@ __global__ __launch_bounds__(kTileX) void k_cg_pass_a(CgArgs a, const double* __
   const double invd = a.invd, d = a.diag, cx = a.cx, cy = a.cy;
   auto pv = [&](double po, double ro) -> double { return fma(beta, po, invd * ro); };
   auto at = [&](int col, int j) -> int { return wrap_row(j, ny) * nx + col; };
-  // halo columns: every lane loads one left and one right value (its own
-  // column except at the tile edges) so the loop has no divergent loads
-  const int cl = left ? xl : t.ic, cr = right ? xr : t.ic;
   double acc[1] = {0.0};
   double pm = pv(pold[at(t.ic, t.y0 - 1)], a.r[at(t.ic, t.y0 - 1)]);
   double pc = pv(pold[at(t.ic, t.y0)], a.r[at(t.ic, t.y0)]);
   double n_po = pold[at(t.ic, t.y0 + 1)], n_ro = a.r[at(t.ic, t.y0 + 1)];
-  double l_po = pold[at(cl, t.y0)], l_ro = a.r[at(cl, t.y0)];
-  double r_po = pold[at(cr, t.y0)], r_ro = a.r[at(cr, t.y0)];
+  double l_po = left ? pold[at(xl, t.y0)] : 0.0, l_ro = left ? a.r[at(xl, t.y0)] : 0.0;
+  double r_po = right ? pold[at(xr, t.y0)] : 0.0, r_ro = right ? a.r[at(xr, t.y0)] : 0.0;
   for (int j = t.y0; j < t.y1; ++j) {
-    // loads for the next row (unconditional: past the tile they read valid
-    // periodic rows whose values are not used)
-    const double nn_po = pold[at(t.ic, j + 2)], nn_ro = a.r[at(t.ic, j + 2)];
-    const double ln_po = pold[at(cl, j + 1)], ln_ro = a.r[at(cl, j + 1)];
-    const double rn_po = pold[at(cr, j + 1)], rn_ro = a.r[at(cr, j + 1)];
+    const bool more = j + 1 < t.y1;
+    // loads for the next row
+    const double nn_po = more ? pold[at(t.ic, j + 2)] : 0.0;
+    const double nn_ro = more ? a.r[at(t.ic, j + 2)] : 0.0;
+    const double ln_po = (left && more) ? pold[at(xl, j + 1)] : 0.0;
+    const double ln_ro = (left && more) ? a.r[at(xl, j + 1)] : 0.0;
+    const double rn_po = (right && more) ? pold[at(xr, j + 1)] : 0.0;
+    const double rn_ro = (right && more) ? a.r[at(xr, j + 1)] : 0.0;
     const double pn = pv(n_po, n_ro);
     const int buf = (j - t.y0) & 1;
     if (t.valid) prow[buf][tid + 1] = pc;
@@ -580,22 +579,22 @@ __global__ __launch_bounds__(kTileX) void k_cg_pass_b(CgArgs a, const double* __
   const bool left = (tid == 0), right = (tid == w - 1);
   const double invd = a.invd, d = a.diag, cx = a.cx, cy = a.cy;
   auto at = [&](int col, int j) -> int { return wrap_row(j, ny) * nx + col; };
-  const int cl = left ? xl : t.ic, cr = right ? xr : t.ic;
   double acc[2] = {0.0, 0.0};
   double pm = pv[at(t.ic, t.y0 - 1)];
   double pc = pv[at(t.ic, t.y0)];
   double pn = pv[at(t.ic, t.y0 + 1)];
   double xc = FIRST ? 0.0 : a.x[t.y0 * nx + t.ic];
   double rc = a.r[t.y0 * nx + t.ic];
-  double hl = pv[at(cl, t.y0)];
-  double hr = pv[at(cr, t.y0)];
+  double hl = left ? pv[at(xl, t.y0)] : 0.0;
+  double hr = right ? pv[at(xr, t.y0)] : 0.0;
   for (int j = t.y0; j < t.y1; ++j) {
-    // loads for the next row (unconditional, see pass A)
-    const double pnn = pv[at(t.ic, j + 2)];
-    const double xn = FIRST ? 0.0 : a.x[at(t.ic, j + 1)];
-    const double rnx = a.r[at(t.ic, j + 1)];
-    const double hl_n = pv[at(cl, j + 1)];
-    const double hr_n = pv[at(cr, j + 1)];
+    const bool more = j + 1 < t.y1;
+    // loads for the next row
+    const double pnn = more ? pv[at(t.ic, j + 2)] : 0.0;
+    const double xn = (!FIRST && more) ? a.x[(j + 1) * nx + t.ic] : 0.0;
+    const double rnx = more ? a.r[(j + 1) * nx + t.ic] : 0.0;
+    const double hl_n = (left && more) ? pv[at(xl, j + 1)] : 0.0;
+    const double hr_n = (right && more) ? pv[at(xr, j + 1)] : 0.0;
     const int buf = (j - t.y0) & 1;
     if (t.valid) prow[buf][tid + 1] = pc;
     if (left) prow[buf][0] = hl;
@@ -649,12 +648,12 @@ __global__ __launch_bounds__(kTileX) void k_finalize1(FinArgs a) {
   double xc = a.x[at(t.ic, t.y0)], xn = a.x[at(t.ic, t.y0 + 1)], xe = a.x[at(t.ip, t.y0)];
   double bxv = a.bx[t.y0 * nx + t.ic], byv = a.by[t.y0 * nx + t.ic];
   for (int j = t.y0; j < t.y1; ++j) {
-    // loads for the next row (unconditional; rows j + 1 of B are read before
-    // row j is written, and a row past the tile is never written here)
-    const double xnn = a.x[at(t.ic, j + 2)];
-    const double xen = a.x[at(t.ip, j + 1)];
-    const double bxn = a.bx[at(t.ic, j + 1)];
-    const double byn = a.by[at(t.ic, j + 1)];
+    const bool more = j + 1 < t.y1;
+    // loads for the next row (rows j + 1 of B are read before row j is written)
+    const double xnn = more ? a.x[at(t.ic, j + 2)] : 0.0;
+    const double xen = more ? a.x[at(t.ip, j + 1)] : 0.0;
+    const double bxn = more ? a.bx[(j + 1) * nx + t.ic] : 0.0;
+    const double byn = more ? a.by[(j + 1) * nx + t.ic] : 0.0;
     const int o = j * nx + t.ic;
     a.bx_out[o] = bxv + (-(xn - xc) / a.dy) * a.c;
     a.by_out[o] = byv + ((xe - xc) / a.dx) * a.c;

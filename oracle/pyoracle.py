@@ -28,7 +28,7 @@ class StepStats(C.Structure):
 
 def build(fast_native: bool = False, out_dir: str | None = None) -> None:
     """Compile the oracle libraries with oracle/Makefile (gcc, seconds)."""
-    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    subprocess.run(["make", "-s", "-C", HERE], check=True, stdout=subprocess.DEVNULL)
     if fast_native:
         out = os.path.join(out_dir or HERE, "libmhd_oracle_native.so")
         subprocess.run(["gcc", "-std=c11", "-O3", "-ffast-math", "-march=native", "-fPIC", "-shared",
