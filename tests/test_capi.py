@@ -65,3 +65,56 @@ def test_no_cpu_fallback_without_a_device():
     with pytest.raises(L.MhdError):
         from magnetohydrodynamics_simulator_amd import Context
         Context(64, 64, 0.1, 0.1)
+
+
+CPP_CALL_SITE = r'''
+#include <vector>
+#include "mhd_mi355x.hpp"
+
+// Stand-in with the member names of the reference's Grid (Grid.hpp:9-26).
+struct Grid {
+  int nx, ny, ng = 2;
+  double cell_width, cell_height, resistivity = 0.0;
+  std::vector<double> rho, mx, my, E, bxf, byf, bxc, byc, Ez, vx, vy, p;
+  Grid(int nx_, int ny_) : nx(nx_), ny(ny_), cell_width(0.1), cell_height(0.1) {
+    const size_t n = size_t(nx + 4) * (ny + 4);
+    for (auto* v : {&rho, &mx, &my, &E, &bxf, &byf, &bxc, &byc, &Ez, &vx, &vy, &p}) v->assign(n, 0.0);
+  }
+};
+
+int main() {
+  Grid grid(64, 64);
+  try {
+    mhd_mi355x::Context ctx(grid);
+    double dt = mhd_mi355x::CFL_condition_checker(grid, ctx);
+    mhd_mi355x::generate_A_matrix(ctx, dt * grid.resistivity * 0.2928932188134524);
+    mhd_mi355x::IMEX(grid, ctx, dt);
+    std::vector<double> d[6];
+    for (auto& v : d) v.assign(grid.rho.size(), 0.0);
+    mhd_mi355x::explicit_rhs(grid, ctx, d[0], d[1], d[2], d[3], d[4], d[5]);
+    mhd_mi355x::resistive_B_term(grid, ctx, d[4], d[5], true);
+    auto dv = mhd_mi355x::calculate_current_div_B(grid, ctx);
+    (void)dv;
+  } catch (const mhd_mi355x::Error& e) {
+    return e.code == MHD_EHIP ? 3 : 1;
+  }
+  return 0;
+}
+'''
+
+
+def test_cpp_adapter_compiles(tmp_path):
+    """A reference-style C++ call site compiles against include/mhd_mi355x.hpp
+    and links libmhd_mi355x.so (it is not run: that needs a GPU)."""
+    import shutil
+    import subprocess
+    if shutil.which("g++") is None:
+        pytest.skip("g++ not available")
+    L.load()
+    src = tmp_path / "call_site.cpp"
+    src.write_text(CPP_CALL_SITE)
+    pkg = os.path.dirname(L.LIB_PATH)
+    r = subprocess.run(["g++", "-std=c++17", "-O1", "-I", os.path.join(ROOT, "include"), str(src),
+                        "-o", str(tmp_path / "call_site"), os.path.join(pkg, "libmhd_mi355x.so"),
+                        f"-Wl,-rpath,{pkg}"], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
