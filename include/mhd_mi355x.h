@@ -132,6 +132,42 @@ int mhd_profile_get(mhd_ctx* ctx, int kernel_id, long long* launches, double* to
 int mhd_calibration_copy(mhd_ctx* ctx, long long nbytes);
 int mhd_synchronize(mhd_ctx* ctx);
 
+/* ------------------------------------------------------------------------
+ * Batched ensembles (no reference counterpart: the reference advances one
+ * Grid per process).  R independent periodic runs, each with its own size,
+ * cell size and resistivity, advance together: one launch per kernel covers
+ * every run, and each run follows the reference driver loop
+ * (src/integrator_main.cpp:76-112) on the device -- its own CFL time step
+ * (opts.cfl), the clamp of its last step to its t_end, the operator rebuild
+ * when its dt changes by more than 1e-10 relative -- and the IMEX step of
+ * mhd_imex_step.  Each run's arithmetic is that of a single-run context.
+ * ------------------------------------------------------------------------ */
+typedef struct mhd_batch mhd_batch;
+
+/* nx, ny, dx, dy, eta: arrays of nruns entries.  Every run starts at t = 0
+ * with no end time and no step limit (see mhd_batch_set_schedule). */
+int mhd_batch_create(int nruns, const int* nx, const int* ny, const double* dx, const double* dy,
+                     const double* eta, const mhd_opts* opts, mhd_batch** out);
+int mhd_batch_destroy(mhd_batch* b);
+const char* mhd_batch_last_error(const mhd_batch* b);
+/* Per-run end time, step limit (< 0: none) and constant time step (0: CFL);
+ * any array may be NULL to keep the current values. */
+int mhd_batch_set_schedule(mhd_batch* b, const double* t_end, const int* max_steps, const double* dt_fixed);
+/* State of one run from / to padded host arrays (as mhd_upload / mhd_download);
+ * an upload restarts that run's clock and step count. */
+int mhd_batch_upload(mhd_batch* b, int run, const double* rho, const double* mx, const double* my,
+                     const double* E, const double* bxf, const double* byf);
+int mhd_batch_download(mhd_batch* b, int run, double* rho, double* mx, double* my, double* E,
+                       double* bxf, double* byf);
+/* One driver step of every run that has not reached its end time or step
+ * limit; *n_active (may be NULL) receives the number of runs that stepped. */
+int mhd_batch_step(mhd_batch* b, int* n_active);
+/* mhd_batch_step until no run is active or max_batch_steps (< 0: no limit). */
+int mhd_batch_run(mhd_batch* b, int max_batch_steps, int* steps_done);
+/* Steps taken, time, last dt and the two CG iteration counts of the last step of one run. */
+int mhd_batch_run_info(mhd_batch* b, int run, int* steps, double* t, double* last_dt, int* cg_iters);
+int mhd_batch_synchronize(mhd_batch* b);
+
 #ifdef __cplusplus
 }
 #endif

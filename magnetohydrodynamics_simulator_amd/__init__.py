@@ -6,7 +6,7 @@ the ARS(2,2,2) IMEX step with a matrix-free Jacobi-PCG implicit resistive
 solve.  The compute runs in libmhd_mi355x.so (C ABI: include/mhd_mi355x.h);
 this package is its Python host binding.
 """
-from .solver import GAMMA_ARS, Context, Grid, operator_alpha  # noqa: F401
+from .solver import GAMMA_ARS, Batch, Context, Grid, operator_alpha  # noqa: F401
 from . import setups  # noqa: F401
 
-__all__ = ["Context", "Grid", "GAMMA_ARS", "operator_alpha", "setups"]
+__all__ = ["Batch", "Context", "Grid", "GAMMA_ARS", "operator_alpha", "setups"]

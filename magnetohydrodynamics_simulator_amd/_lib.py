@@ -51,6 +51,16 @@ SIGNATURES = {
     "mhd_profile_get": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_longlong), _dp]),
     "mhd_calibration_copy": (C.c_int, [_vp, C.c_longlong]),
     "mhd_synchronize": (C.c_int, [_vp]),
+    "mhd_batch_create": (C.c_int, [C.c_int, _ip, _ip, _dp, _dp, _dp, C.POINTER(Opts), C.POINTER(_vp)]),
+    "mhd_batch_destroy": (C.c_int, [_vp]),
+    "mhd_batch_last_error": (C.c_char_p, [_vp]),
+    "mhd_batch_set_schedule": (C.c_int, [_vp, _dp, _ip, _dp]),
+    "mhd_batch_upload": (C.c_int, [_vp, C.c_int] + [_dp] * 6),
+    "mhd_batch_download": (C.c_int, [_vp, C.c_int] + [_dp] * 6),
+    "mhd_batch_step": (C.c_int, [_vp, _ip]),
+    "mhd_batch_run": (C.c_int, [_vp, C.c_int, _ip]),
+    "mhd_batch_run_info": (C.c_int, [_vp, C.c_int, _ip, _dp, _dp, _ip]),
+    "mhd_batch_synchronize": (C.c_int, [_vp]),
 }
 
 _LIB: C.CDLL | None = None
@@ -82,4 +92,11 @@ def check(rc: int, ctx=None) -> None:
     if rc != MHD_OK:
         lib = load()
         msg = lib.mhd_last_error(ctx)
+        raise MhdError(rc, msg.decode() if msg else "")
+
+
+def check_batch(rc: int, b=None) -> None:
+    if rc != MHD_OK:
+        lib = load()
+        msg = lib.mhd_batch_last_error(b)
         raise MhdError(rc, msg.decode() if msg else "")

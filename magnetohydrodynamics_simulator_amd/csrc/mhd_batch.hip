@@ -1,0 +1,496 @@
+// mhd_batch.hip -- C ABI of batched ensembles (include/mhd_mi355x.h, mhd_batch_*).
+//
+// Many independent periodic runs of different sizes, resistivities and time
+// steps advance together: every field of every run lives in one pooled array
+// per field (run r at cells [base_r, base_r + nx_r ny_r)), and each kernel of
+// the step is launched once for all runs with a job table that maps blocks to
+// (run, block within the run).  The per-run driver (CFL dt, clamp to t_end,
+// operator rebuild, t and step count) runs on the device (kb_dt), so a step
+// needs no host round trip except the convergence check of each CG solve,
+// which is one for the whole batch.  The sequence of a step is the single-run
+// one (mhd_capi.hip):
+//   kb_cfl, kb_dt, kb_stage<1>, CG (slot 0), kb_finalize1, kb_stage<2>,
+//   CG (slot 1), kb_finalize2
+// and each run's arithmetic is that of the single-run kernels.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/mhd_mi355x.h"
+#include "mhd_kernels.hpp"
+
+using namespace mhd;
+
+namespace {
+
+thread_local std::string g_batch_create_error;
+
+constexpr double kGammaArs = 0.29289321881345254;   // (2 - sqrt 2) / 2, solvers.hpp:41
+
+struct BatchTableau {
+  double gamma = (2.0 - std::sqrt(2.0)) / 2.0;
+  double delta = 1.0 - (1.0 / (2.0 * gamma));
+  double A_ex3 = gamma, A_ex6 = delta, A_ex7 = 1.0 - delta;
+  double A_im4 = gamma, A_im7 = 1.0 - gamma;
+};
+
+}  // namespace
+
+struct mhd_batch {
+  int nruns = 0;
+  long long total = 0;              // cells of all runs
+  int total_tiles = 0;
+  mhd_opts o{};
+  BatchTableau tab;
+  hipStream_t stream = nullptr;
+  std::vector<RunGeom> geom_h;
+  RunGeom* geom = nullptr;
+  RunState* rs = nullptr;
+  RunState* rs_host = nullptr;      // pinned
+  double* U[6] = {};
+  double* S[6] = {};
+  double* W[6] = {};
+  double* x = nullptr;
+  double* r = nullptr;
+  double* p[2] = {};
+  double* part = nullptr;           // CFL: two partials per tile
+  double* part_s = nullptr;
+  double* part_a = nullptr;
+  double* part_b = nullptr;
+  CgState* st[2] = {};
+  CgState* st_host = nullptr;       // pinned, one solve slot (nruns states)
+  int2* stage_jobs = nullptr;
+  int n_stage_jobs = 0;
+  int2* tile_jobs = nullptr;
+  int n_tile_jobs = 0;
+  int stage_rows = 32, tile_rows = 32;
+  int last_spmv[2] = {1, 1};
+  bool any_eta = false;
+  long long steps_taken = 0;
+  std::vector<int> cg_iters_last;   // 2 per run: Eigen iteration counts of the last step
+  std::string err;
+};
+
+namespace {
+
+int bfail(mhd_batch* b, int code, const std::string& msg) {
+  if (b) b->err = msg;
+  else g_batch_create_error = msg;
+  return code;
+}
+
+#define BHIP(b, call)                                                                           \
+  do {                                                                                          \
+    hipError_t e_ = (call);                                                                     \
+    if (e_ != hipSuccess)                                                                       \
+      return bfail((b), MHD_EHIP, std::string(#call) + ": " + hipGetErrorString(e_));           \
+  } while (0)
+
+#define BLAUNCH(b, ...)                                                                         \
+  do {                                                                                          \
+    hipLaunchKernelGGL(__VA_ARGS__);                                                            \
+    hipError_t e_ = hipGetLastError();                                                          \
+    if (e_ != hipSuccess) return bfail((b), MHD_EHIP, std::string("kernel launch: ") + hipGetErrorString(e_)); \
+  } while (0)
+
+// Lays out the runs: pooled offsets, stage segments and 2D tiles, job tables.
+void plan(mhd_batch* b, std::vector<int2>& stage_jobs, std::vector<int2>& tile_jobs) {
+  long long base = 0;
+  int part_base = 0;
+  stage_jobs.clear();
+  tile_jobs.clear();
+  for (int r = 0; r < b->nruns; ++r) {
+    RunGeom& g = b->geom_h[r];
+    g.base = base;
+    base += (long long)g.nx * g.ny;
+    g.rows = std::min(b->stage_rows, g.ny);
+    g.strips = (g.nx + kStripOut - 1) / kStripOut;
+    const int segs = (g.ny + g.rows - 1) / g.rows;
+    for (int w = 0; w < g.strips * segs; ++w) stage_jobs.push_back(make_int2(r, w));
+    g.ty = std::max(4, std::min(b->tile_rows, g.ny));
+    g.tiles_y = (g.ny + g.ty - 1) / g.ty;
+    const int tiles_x = (g.nx + kTileX - 1) / kTileX;
+    g.tiles = tiles_x * g.tiles_y;
+    g.part_base = part_base;
+    part_base += g.tiles;
+    for (int t = 0; t < g.tiles; ++t) tile_jobs.push_back(make_int2(r, t));
+  }
+  b->total = base;
+  b->total_tiles = part_base;
+}
+
+BatchCgArgs cg_args(mhd_batch* b, int slot, const double* bx, const double* by) {
+  BatchCgArgs a{};
+  a.geom = b->geom;
+  a.rs = b->rs;
+  a.jobs = b->tile_jobs;
+  a.bx = bx;
+  a.by = by;
+  a.r = b->r;
+  a.p0 = b->p[0];
+  a.x = b->x;
+  a.part_s = b->part_s;
+  a.part_a = b->part_a;
+  a.part_b = b->part_b;
+  a.st = b->st[slot];
+  a.tol = b->o.cg_tol;
+  return a;
+}
+
+// One CG solve of every active run with eta > 0 (Eigen's algorithm per run,
+// the single-run kernel sequence; see cg_run in mhd_capi.hip).  The host
+// checks all runs' states after each chunk; the first chunk is as long as the
+// longest solve of this slot in the previous step.
+int cg_batch(mhd_batch* b, int slot, const double* bx, const double* by) {
+  BatchCgArgs a = cg_args(b, slot, bx, by);
+  const int nb = b->n_tile_jobs;
+  BLAUNCH(b, kb_cg_start, dim3(nb), dim3(kTileX), 0, b->stream, a);
+  BLAUNCH(b, kb_cg_pass_b<true>, dim3(nb), dim3(kTileX), 0, b->stream, a, (const double*)b->p[0], 0);
+  long long it = 1;
+  long long maxit = 0;
+  for (const RunGeom& g : b->geom_h) maxit = std::max(maxit, g.max_iters);
+  for (;;) {
+    const int chunk = (it == 1) ? std::max(1, b->last_spmv[slot]) : 1;
+    for (int k = 0; k < chunk; ++k, ++it) {
+      const double* pold = b->p[(it - 1) & 1];
+      double* pnew = b->p[it & 1];
+      BLAUNCH(b, kb_cg_pass_a, dim3(nb), dim3(kTileX), 0, b->stream, a, pold, pnew, (int)it);
+      if (k + 1 == chunk) break;
+      BLAUNCH(b, kb_cg_pass_b<false>, dim3(nb), dim3(kTileX), 0, b->stream, a, (const double*)pnew, (int)it);
+    }
+    BHIP(b, hipMemcpyAsync(b->st_host, b->st[slot], sizeof(CgState) * b->nruns, hipMemcpyDeviceToHost, b->stream));
+    BHIP(b, hipStreamSynchronize(b->stream));
+    bool all = true;
+    for (int r = 0; r < b->nruns; ++r) all = all && b->st_host[r].done;
+    if (all) break;
+    const double* pv = b->p[it & 1];
+    BLAUNCH(b, kb_cg_pass_b<false>, dim3(nb), dim3(kTileX), 0, b->stream, a, pv, (int)it);
+    ++it;
+    if (it > maxit + 4) return bfail(b, MHD_EHIP, "CG state machine did not terminate");
+  }
+  int most = 0;
+  for (int r = 0; r < b->nruns; ++r) {
+    const CgState& s = b->st_host[r];
+    if (s.done != 3) most = std::max(most, s.spmv);
+    b->cg_iters_last[2 * r + slot] = s.iters;
+    if (s.done == 2 && b->o.cg_fail_is_error)
+      return bfail(b, MHD_ECG, "run " + std::to_string(r) + ": CG reached the iteration limit");
+  }
+  if (most > 0) b->last_spmv[slot] = most;
+  return MHD_OK;
+}
+
+BatchFinArgs fin_args(mhd_batch* b, int slot, const double* bx, const double* by, double* bxo, double* byo) {
+  BatchFinArgs f{};
+  f.geom = b->geom;
+  f.rs = b->rs;
+  f.jobs = b->tile_jobs;
+  f.x = b->x;
+  f.bx = bx;
+  f.by = by;
+  f.bx_out = bxo;
+  f.by_out = byo;
+  f.st = b->st[slot];
+  f.fc = b->tab.A_im4;
+  return f;
+}
+
+int check_run(mhd_batch* b, int run) {
+  if (!b) return MHD_EINVAL;
+  if (run < 0 || run >= b->nruns) return bfail(b, MHD_EINVAL, "run index out of range");
+  return MHD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mhd_batch_create(int nruns, const int* nx, const int* ny, const double* dx, const double* dy, const double* eta,
+                     const mhd_opts* opts, mhd_batch** out) {
+  if (!out) return bfail(nullptr, MHD_EINVAL, "out is NULL");
+  *out = nullptr;
+  if (nruns < 1 || !nx || !ny || !dx || !dy || !eta) return bfail(nullptr, MHD_EINVAL, "bad run arrays");
+  mhd_opts o;
+  mhd_default_opts(&o);
+  if (opts) o = *opts;
+  if (o.gamma != 5.0 / 3.0) return bfail(nullptr, MHD_EINVAL, "only gamma = 5/3 is compiled in");
+  if (!(o.cfl > 0) || !(o.cg_tol >= 0) || o.march_rows < 0) return bfail(nullptr, MHD_EINVAL, "bad options");
+  long long total = 0;
+  for (int r = 0; r < nruns; ++r) {
+    if (nx[r] < 4 || ny[r] < 4) return bfail(nullptr, MHD_EINVAL, "run " + std::to_string(r) + ": nx, ny must be >= 4");
+    if (!(dx[r] > 0) || !(dy[r] > 0) || !std::isfinite(dx[r]) || !std::isfinite(dy[r]))
+      return bfail(nullptr, MHD_EINVAL, "run " + std::to_string(r) + ": bad dx or dy");
+    if (!(eta[r] >= 0) || !std::isfinite(eta[r])) return bfail(nullptr, MHD_EINVAL, "run " + std::to_string(r) + ": bad eta");
+    total += (long long)nx[r] * ny[r];
+  }
+  if (total > (1LL << 31)) return bfail(nullptr, MHD_EINVAL, "more than 2^31 cells in the batch");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return bfail(nullptr, MHD_EHIP, "no HIP device available");
+
+  mhd_batch* b = new mhd_batch();
+  b->nruns = nruns;
+  b->o = o;
+  if (o.march_rows > 0) b->stage_rows = o.march_rows;
+  b->geom_h.resize(nruns);
+  b->cg_iters_last.assign(2 * nruns, 0);
+  for (int r = 0; r < nruns; ++r) {
+    RunGeom& g = b->geom_h[r];
+    g = RunGeom{};
+    g.nx = nx[r];
+    g.ny = ny[r];
+    g.dx = dx[r];
+    g.dy = dy[r];
+    g.inv_dx = 1.0 / dx[r];
+    g.inv_dy = 1.0 / dy[r];
+    g.eta = eta[r];
+    g.cfl = o.cfl;
+    g.t_end = INFINITY;
+    g.dt_fixed = 0.0;
+    g.max_steps = -1;
+    g.max_iters = (o.cg_max_iter > 0) ? o.cg_max_iter : 2LL * nx[r] * ny[r];
+    b->any_eta = b->any_eta || eta[r] > 0.0;
+  }
+  std::vector<int2> sj, tj;
+  plan(b, sj, tj);
+  auto bail = [&](const std::string& m, int code) {
+    g_batch_create_error = m;
+    mhd_batch_destroy(b);
+    return code;
+  };
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) return bail("hipStreamCreate failed", MHD_EHIP);
+  const size_t fb = (size_t)b->total * sizeof(double);
+  for (int q = 0; q < 6; ++q)
+    if (hipMalloc(&b->U[q], fb) != hipSuccess || hipMalloc(&b->S[q], fb) != hipSuccess || hipMalloc(&b->W[q], fb) != hipSuccess)
+      return bail("device allocation failed", MHD_ENOMEM);
+  const size_t pb = (size_t)b->total_tiles * sizeof(double);
+  if (hipMalloc(&b->x, fb) != hipSuccess || hipMalloc(&b->r, fb) != hipSuccess || hipMalloc(&b->p[0], fb) != hipSuccess ||
+      hipMalloc(&b->p[1], fb) != hipSuccess || hipMalloc(&b->part, 2 * pb) != hipSuccess ||
+      hipMalloc(&b->part_s, 3 * pb) != hipSuccess || hipMalloc(&b->part_a, pb) != hipSuccess ||
+      hipMalloc(&b->part_b, 2 * pb) != hipSuccess || hipMalloc(&b->st[0], sizeof(CgState) * nruns) != hipSuccess ||
+      hipMalloc(&b->st[1], sizeof(CgState) * nruns) != hipSuccess || hipMalloc(&b->geom, sizeof(RunGeom) * nruns) != hipSuccess ||
+      hipMalloc(&b->rs, sizeof(RunState) * nruns) != hipSuccess ||
+      hipMalloc(&b->stage_jobs, sizeof(int2) * sj.size()) != hipSuccess ||
+      hipMalloc(&b->tile_jobs, sizeof(int2) * tj.size()) != hipSuccess)
+    return bail("device allocation failed", MHD_ENOMEM);
+  if (hipHostMalloc(&b->rs_host, sizeof(RunState) * nruns) != hipSuccess ||
+      hipHostMalloc(&b->st_host, sizeof(CgState) * nruns) != hipSuccess)
+    return bail("pinned allocation failed", MHD_ENOMEM);
+  b->n_stage_jobs = (int)sj.size();
+  b->n_tile_jobs = (int)tj.size();
+  for (int q = 0; q < 6; ++q)
+    if (hipMemsetAsync(b->U[q], 0, fb, b->stream) != hipSuccess) return bail("hipMemset failed", MHD_EHIP);
+  std::memset(b->rs_host, 0, sizeof(RunState) * nruns);
+  if (hipMemcpyAsync(b->geom, b->geom_h.data(), sizeof(RunGeom) * nruns, hipMemcpyHostToDevice, b->stream) != hipSuccess ||
+      hipMemcpyAsync(b->rs, b->rs_host, sizeof(RunState) * nruns, hipMemcpyHostToDevice, b->stream) != hipSuccess ||
+      hipMemcpyAsync(b->stage_jobs, sj.data(), sizeof(int2) * sj.size(), hipMemcpyHostToDevice, b->stream) != hipSuccess ||
+      hipMemcpyAsync(b->tile_jobs, tj.data(), sizeof(int2) * tj.size(), hipMemcpyHostToDevice, b->stream) != hipSuccess ||
+      hipStreamSynchronize(b->stream) != hipSuccess)
+    return bail("device init failed", MHD_EHIP);
+  *out = b;
+  return MHD_OK;
+}
+
+int mhd_batch_destroy(mhd_batch* b) {
+  if (!b) return MHD_EINVAL;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  for (int q = 0; q < 6; ++q) {
+    if (b->U[q]) (void)hipFree(b->U[q]);
+    if (b->S[q]) (void)hipFree(b->S[q]);
+    if (b->W[q]) (void)hipFree(b->W[q]);
+  }
+  void* bufs[] = {b->x, b->r, b->p[0], b->p[1], b->part, b->part_s, b->part_a, b->part_b, b->st[0], b->st[1],
+                  b->geom, b->rs, b->stage_jobs, b->tile_jobs};
+  for (void* p : bufs)
+    if (p) (void)hipFree(p);
+  if (b->rs_host) (void)hipHostFree(b->rs_host);
+  if (b->st_host) (void)hipHostFree(b->st_host);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+  return MHD_OK;
+}
+
+const char* mhd_batch_last_error(const mhd_batch* b) { return b ? b->err.c_str() : g_batch_create_error.c_str(); }
+
+int mhd_batch_set_schedule(mhd_batch* b, const double* t_end, const int* max_steps, const double* dt_fixed) {
+  if (!b) return MHD_EINVAL;
+  for (int r = 0; r < b->nruns; ++r) {
+    RunGeom& g = b->geom_h[r];
+    if (t_end) {
+      if (std::isnan(t_end[r])) return bfail(b, MHD_EINVAL, "t_end is NaN");
+      g.t_end = t_end[r];
+    }
+    if (max_steps) g.max_steps = max_steps[r];
+    if (dt_fixed) {
+      if (!(dt_fixed[r] >= 0) || !std::isfinite(dt_fixed[r])) return bfail(b, MHD_EINVAL, "bad dt_fixed");
+      g.dt_fixed = dt_fixed[r];
+    }
+  }
+  BHIP(b, hipMemcpyAsync(b->geom, b->geom_h.data(), sizeof(RunGeom) * b->nruns, hipMemcpyHostToDevice, b->stream));
+  BHIP(b, hipStreamSynchronize(b->stream));
+  return MHD_OK;
+}
+
+int mhd_batch_upload(mhd_batch* b, int run, const double* rho, const double* mx, const double* my, const double* E,
+                     const double* bxf, const double* byf) {
+  int rc = check_run(b, run);
+  if (rc) return rc;
+  const double* src[6] = {rho, mx, my, E, bxf, byf};
+  for (int q = 0; q < 6; ++q)
+    if (!src[q]) return bfail(b, MHD_EINVAL, "mhd_batch_upload: NULL field");
+  const RunGeom& g = b->geom_h[run];
+  const size_t nxt = (size_t)g.nx + 4;
+  for (int q = 0; q < 6; ++q)
+    BHIP(b, hipMemcpy2DAsync(b->U[q] + g.base, (size_t)g.nx * sizeof(double), src[q] + 2 * nxt + 2, nxt * sizeof(double),
+                             (size_t)g.nx * sizeof(double), (size_t)g.ny, hipMemcpyHostToDevice, b->stream));
+  const long long n = (long long)g.nx * g.ny;
+  const int blocks = (int)std::min<long long>((n + 255) / 256, 4096);
+  BLAUNCH(b, k_floor_rho, dim3(blocks), dim3(256), 0, b->stream, b->U[0] + g.base, (int)n);
+  // a fresh state restarts the run's driver
+  RunState z{};
+  BHIP(b, hipMemcpyAsync(b->rs + run, &z, sizeof(RunState), hipMemcpyHostToDevice, b->stream));
+  BHIP(b, hipStreamSynchronize(b->stream));
+  return MHD_OK;
+}
+
+int mhd_batch_download(mhd_batch* b, int run, double* rho, double* mx, double* my, double* E, double* bxf, double* byf) {
+  int rc = check_run(b, run);
+  if (rc) return rc;
+  const RunGeom& g = b->geom_h[run];
+  const size_t nxt = (size_t)g.nx + 4;
+  double* dst[6] = {rho, mx, my, E, bxf, byf};
+  for (int q = 0; q < 6; ++q) {
+    if (!dst[q]) continue;
+    BHIP(b, hipMemcpy2DAsync(dst[q] + 2 * nxt + 2, nxt * sizeof(double), b->U[q] + g.base, (size_t)g.nx * sizeof(double),
+                             (size_t)g.nx * sizeof(double), (size_t)g.ny, hipMemcpyDeviceToHost, b->stream));
+  }
+  BHIP(b, hipStreamSynchronize(b->stream));
+  // periodic ghosts (Grid::fillOneField, Grid.hpp:143-161)
+  const int nx = g.nx, ny = g.ny, nyt = ny + 4;
+  for (int q = 0; q < 6; ++q) {
+    double* f = dst[q];
+    if (!f) continue;
+    for (int j = 2; j < nyt - 2; ++j)
+      for (int k = 0; k < 2; ++k) {
+        f[j * nxt + k] = f[j * nxt + nx + k];
+        f[j * nxt + nx + 2 + k] = f[j * nxt + 2 + k];
+      }
+    for (size_t i = 0; i < nxt; ++i)
+      for (int k = 0; k < 2; ++k) {
+        f[k * nxt + i] = f[(ny + k) * nxt + i];
+        f[(ny + 2 + k) * nxt + i] = f[(2 + k) * nxt + i];
+      }
+  }
+  return MHD_OK;
+}
+
+int mhd_batch_step(mhd_batch* b, int* n_active) {
+  if (!b) return MHD_EINVAL;
+  const BatchTableau& T = b->tab;
+  const int nt = b->n_tile_jobs;
+  // driver: CFL partials, then dt / operator / activity per run
+  BatchDiagArgs d{};
+  d.geom = b->geom;
+  d.rs = b->rs;
+  d.jobs = b->tile_jobs;
+  for (int q = 0; q < 6; ++q) d.s[q] = b->U[q];
+  d.part = b->part;
+  BLAUNCH(b, kb_cfl, dim3(nt), dim3(kTileX), 0, b->stream, d);
+  BLAUNCH(b, kb_dt, dim3(b->nruns), dim3(256), 0, b->stream, (const RunGeom*)b->geom, b->rs, (const double*)b->part,
+          b->st[0], b->st[1], b->nruns, T.A_im4);
+  BHIP(b, hipMemcpyAsync(b->rs_host, b->rs, sizeof(RunState) * b->nruns, hipMemcpyDeviceToHost, b->stream));
+  // stage 1: W = U + dt delta f1 (into W), U2* = U + dt gamma f1 (into S)
+  BatchStageArgs s1{};
+  s1.geom = b->geom;
+  s1.rs = b->rs;
+  s1.jobs = b->stage_jobs;
+  for (int q = 0; q < 6; ++q) {
+    s1.s[q] = b->U[q];
+    s1.o1[q] = b->W[q];
+    s1.o2[q] = b->S[q];
+  }
+  s1.fa = T.A_ex3;
+  s1.fb = T.A_ex6;
+  BLAUNCH(b, kb_stage<1>, dim3(b->n_stage_jobs), dim3(64), 0, b->stream, s1);
+  if (b->any_eta) {
+    int rc = cg_batch(b, 0, b->S[4], b->S[5]);
+    if (rc) return rc;
+    BatchFinArgs f = fin_args(b, 0, b->S[4], b->S[5], b->S[4], b->S[5]);
+    BLAUNCH(b, kb_finalize1, dim3(nt), dim3(kTileX), 0, b->stream, f);
+  }
+  // stage 2: U3* = W + dt (1 - delta) L(S) [+ dt (1 - gamma) g2] into U
+  BatchStageArgs s2{};
+  s2.geom = b->geom;
+  s2.rs = b->rs;
+  s2.jobs = b->stage_jobs;
+  for (int q = 0; q < 6; ++q) {
+    s2.s[q] = b->S[q];
+    s2.u[q] = b->W[q];
+    s2.o1[q] = b->U[q];
+  }
+  s2.fb = T.A_ex7;
+  s2.fg = T.A_im7;
+  BLAUNCH(b, kb_stage<2>, dim3(b->n_stage_jobs), dim3(64), 0, b->stream, s2);
+  if (b->any_eta) {
+    int rc = cg_batch(b, 1, b->U[4], b->U[5]);
+    if (rc) return rc;
+    // B^{n+1} = B3* + dt gamma g3(B3) into S.B, then S.B becomes the state's B.
+    // Runs with eta = 0 have no implicit step: their B is copied by the
+    // finalizer as well (x = 0 there, so g3 = 0 and c g3 = 0).
+    BatchFinArgs f = fin_args(b, 1, b->U[4], b->U[5], b->S[4], b->S[5]);
+    BLAUNCH(b, kb_finalize2, dim3(nt), dim3(kTileX), 0, b->stream, f);
+    std::swap(b->U[4], b->S[4]);
+    std::swap(b->U[5], b->S[5]);
+  }
+  BHIP(b, hipStreamSynchronize(b->stream));
+  int act = 0;
+  for (int r = 0; r < b->nruns; ++r) {
+    if (b->rs_host[r].nonfinite)
+      return bfail(b, MHD_ENONFINITE, "run " + std::to_string(r) + ": non-finite wave speed");
+    act += b->rs_host[r].active;
+  }
+  b->steps_taken++;
+  if (n_active) *n_active = act;
+  return MHD_OK;
+}
+
+int mhd_batch_run(mhd_batch* b, int max_batch_steps, int* steps_done) {
+  if (!b) return MHD_EINVAL;
+  int k = 0;
+  for (; max_batch_steps < 0 || k < max_batch_steps; ++k) {
+    int act = 0;
+    int rc = mhd_batch_step(b, &act);
+    if (rc) return rc;
+    if (act == 0) break;
+  }
+  if (steps_done) *steps_done = k;
+  return MHD_OK;
+}
+
+int mhd_batch_run_info(mhd_batch* b, int run, int* steps, double* t, double* last_dt, int* cg_iters) {
+  int rc = check_run(b, run);
+  if (rc) return rc;
+  RunState s;
+  BHIP(b, hipMemcpyAsync(&s, b->rs + run, sizeof(RunState), hipMemcpyDeviceToHost, b->stream));
+  BHIP(b, hipStreamSynchronize(b->stream));
+  if (steps) *steps = s.steps;
+  if (t) *t = s.t;
+  if (last_dt) *last_dt = s.dt;
+  if (cg_iters) {
+    cg_iters[0] = b->cg_iters_last[2 * run];
+    cg_iters[1] = b->cg_iters_last[2 * run + 1];
+  }
+  return MHD_OK;
+}
+
+int mhd_batch_synchronize(mhd_batch* b) {
+  if (!b) return MHD_EINVAL;
+  BHIP(b, hipStreamSynchronize(b->stream));
+  return MHD_OK;
+}
+
+}  // extern "C"

@@ -55,12 +55,14 @@ struct Carry {
 __device__ __forceinline__ double sh_up(double v) { return __shfl_up(v, 1, 64); }     // value of lane l-1
 __device__ __forceinline__ double sh_dn(double v) { return __shfl_down(v, 1, 64); }   // value of lane l+1
 
+// One wavefront's work: strip `strip` (columns strip * 60 ...), segment `seg`
+// (rows seg * a.rows ...).
 template <int MODE>
-__global__ __launch_bounds__(64) void k_stage(StageArgs a) {
+__device__ __forceinline__ void stage_body(const StageArgs& a, int strip, int seg) {
   const int lane = threadIdx.x;
   const int nx = a.nx, ny = a.ny;
-  const int x0 = blockIdx.x * kStripOut;
-  const int y0 = blockIdx.y * a.rows;
+  const int x0 = strip * kStripOut;
+  const int y0 = seg * a.rows;
   const int y1 = min(y0 + a.rows, ny);
   const int col = x0 - 2 + lane;
   const int c0 = pmod(col, nx);
@@ -219,6 +221,11 @@ __global__ __launch_bounds__(64) void k_stage(StageArgs a) {
   }
 }
 
+template <int MODE>
+__global__ __launch_bounds__(64) void k_stage(StageArgs a) {
+  stage_body<MODE>(a, blockIdx.x, blockIdx.y);
+}
+
 template __global__ void k_stage<0>(StageArgs);
 template __global__ void k_stage<1>(StageArgs);
 template __global__ void k_stage<2>(StageArgs);
@@ -310,9 +317,9 @@ struct Tile {
   bool valid;
 };
 
-__device__ __forceinline__ Tile tile_of(int nx, int ny, int ty, int tiles_y) {
+__device__ __forceinline__ Tile tile_of(int nx, int ny, int ty, int tiles_y, int vb) {
   Tile t;
-  t.vb = xcd_block(blockIdx.x, gridDim.x);
+  t.vb = vb;
   const int bx = t.vb / tiles_y, by = t.vb - bx * tiles_y;
   t.i = bx * kTileX + threadIdx.x;
   t.valid = t.i < nx;
@@ -356,14 +363,12 @@ __device__ __forceinline__ double stencil(double ps, double pw, double pc, doubl
 
 // Columns of a tile: `w` valid columns starting at x0 and the periodic left
 // and right halo columns xl, xr.
-__device__ __forceinline__ void tile_columns(int nx, int ty, int tiles_y, int& x0, int& w, int& xl, int& xr) {
-  const int vb = xcd_block(blockIdx.x, gridDim.x);
+__device__ __forceinline__ void tile_columns(int nx, int tiles_y, int vb, int& x0, int& w, int& xl, int& xr) {
   const int bx = vb / tiles_y;
   x0 = bx * kTileX;
   w = min(kTileX, nx - x0);
   xl = (x0 == 0) ? nx - 1 : x0 - 1;
   xr = (x0 + w >= nx) ? x0 + w - nx : x0 + w;
-  (void)ty;
 }
 
 // eta J(B) at corner (i, j) (resistive_B_term's first loop, solvers.hpp:450-463)
@@ -390,13 +395,13 @@ __device__ __forceinline__ int wrap_row(int j, int ny) { return j < 0 ? j + ny :
 // Start of a solve: b = eta J(B) (or bin), r = b, p = invd b, and the first
 // product q = A p; partials |b|^2, b . p, p . q.
 template <bool FROM_B>
-__global__ __launch_bounds__(kTileX) void k_cg_start(CgArgs a, const double* __restrict__ bin) {
+__device__ __forceinline__ void cg_start_body(const CgArgs& a, const double* __restrict__ bin, int vb) {
   __shared__ double prow[2][kTileX + 2];
   __shared__ double lds[16];
   const int nx = a.nx, ny = a.ny;
-  const Tile t = tile_of(nx, ny, a.ty, a.tiles_y);
+  const Tile t = tile_of(nx, ny, a.ty, a.tiles_y, vb);
   int x0, w, xl, xr;
-  tile_columns(nx, a.ty, a.tiles_y, x0, w, xl, xr);
+  tile_columns(nx, a.tiles_y, vb, x0, w, xl, xr);
   const int tid = threadIdx.x;
   const bool left = (tid == 0), right = (tid == w - 1);
   const double invd = a.invd, d = a.diag, cx = a.cx, cy = a.cy;
@@ -457,22 +462,22 @@ __global__ __launch_bounds__(kTileX) void k_cg_start(CgArgs a, const double* __r
 // Right-hand side only (explicit resistive term): r = eta J(B).
 __global__ __launch_bounds__(kTileX) void k_cg_rhs(CgArgs a) {
   const int nx = a.nx, ny = a.ny;
-  const Tile t = tile_of(nx, ny, a.ty, a.tiles_y);
+  const Tile t = tile_of(nx, ny, a.ty, a.tiles_y, xcd_block(blockIdx.x, gridDim.x));
   if (!t.valid) return;
   for (int j = t.y0; j < t.y1; ++j) a.r[j * nx + t.ic] = rhs_at(a, t.ic, j) - 0.0;
 }
 
 // Pass A(it), it >= 1: convergence decision on the partials of B(it - 1),
 // beta, p_new = invd r + beta p_old, q = A p_new; partial p_new . q.
-__global__ __launch_bounds__(kTileX) void k_cg_pass_a(CgArgs a, const double* __restrict__ pold,
-                                                      double* __restrict__ pnew, int it) {
+__device__ __forceinline__ void cg_pass_a_body(const CgArgs& a, const double* __restrict__ pold,
+                                               double* __restrict__ pnew, int it, int vb, int nb) {
   __shared__ double prow[2][kTileX + 2];
   __shared__ double lds[16];
   CgState* st = a.st;
   if (st->done) return;
   double rr[2];
-  sum_partials<2>(a.part_b, gridDim.x, rr, lds);
-  const Tile t = tile_of(a.nx, a.ny, a.ty, a.tiles_y);
+  sum_partials<2>(a.part_b, nb, rr, lds);
+  const Tile t = tile_of(a.nx, a.ny, a.ty, a.tiles_y, vb);
   const bool lead = (t.vb == 0) && (threadIdx.x == 0);
   const double rn2 = rr[0];
   if (rn2 < st->threshold) {                // converged after B(it - 1)
@@ -487,7 +492,7 @@ __global__ __launch_bounds__(kTileX) void k_cg_pass_a(CgArgs a, const double* __
   }
   const int nx = a.nx, ny = a.ny;
   int x0, w, xl, xr;
-  tile_columns(nx, a.ty, a.tiles_y, x0, w, xl, xr);
+  tile_columns(nx, a.tiles_y, vb, x0, w, xl, xr);
   const int tid = threadIdx.x;
   const bool left = (tid == 0), right = (tid == w - 1);
   const double invd = a.invd, d = a.diag, cx = a.cx, cy = a.cy;
@@ -533,17 +538,17 @@ __global__ __launch_bounds__(kTileX) void k_cg_pass_a(CgArgs a, const double* __
 // partials |r|^2 and r . (invd r).  B(0) also takes the start-of-solve
 // decisions of Eigen's conjugate_gradient from the partials of the start kernel.
 template <bool FIRST>
-__global__ __launch_bounds__(kTileX) void k_cg_pass_b(CgArgs a, const double* __restrict__ pv, int it) {
+__device__ __forceinline__ void cg_pass_b_body(const CgArgs& a, const double* __restrict__ pv, int it, int vb, int nb) {
   __shared__ double prow[2][kTileX + 2];
   __shared__ double lds[16];
   CgState* st = a.st;
   if (st->done) return;
-  const Tile t = tile_of(a.nx, a.ny, a.ty, a.tiles_y);
+  const Tile t = tile_of(a.nx, a.ny, a.ty, a.tiles_y, vb);
   const bool lead = (t.vb == 0) && (threadIdx.x == 0);
   double alpha, pap;
   if (FIRST) {
     double s3[3];
-    sum_partials<3>(a.part_s, gridDim.x, s3, lds);
+    sum_partials<3>(a.part_s, nb, s3, lds);
     const double rhs = s3[0];
     const double thr = dmax(a.tol * a.tol * rhs, 2.2250738585072014e-308);
     int done = 0;
@@ -563,7 +568,7 @@ __global__ __launch_bounds__(kTileX) void k_cg_pass_b(CgArgs a, const double* __
     alpha = s3[1] / pap;
   } else {
     double s1[1];
-    sum_partials<1>(a.part_a, gridDim.x, s1, lds);
+    sum_partials<1>(a.part_a, nb, s1, lds);
     pap = s1[0];
     alpha = st->abs_new[it & 1] / pap;
   }
@@ -574,7 +579,7 @@ __global__ __launch_bounds__(kTileX) void k_cg_pass_b(CgArgs a, const double* __
   }
   const int nx = a.nx, ny = a.ny;
   int x0, w, xl, xr;
-  tile_columns(nx, a.ty, a.tiles_y, x0, w, xl, xr);
+  tile_columns(nx, a.tiles_y, vb, x0, w, xl, xr);
   const int tid = threadIdx.x;
   const bool left = (tid == 0), right = (tid == w - 1);
   const double invd = a.invd, d = a.diag, cx = a.cx, cy = a.cy;
@@ -625,6 +630,19 @@ __global__ __launch_bounds__(kTileX) void k_cg_pass_b(CgArgs a, const double* __
   }
 }
 
+template <bool FROM_B>
+__global__ __launch_bounds__(kTileX) void k_cg_start(CgArgs a, const double* __restrict__ bin) {
+  cg_start_body<FROM_B>(a, bin, xcd_block(blockIdx.x, gridDim.x));
+}
+__global__ __launch_bounds__(kTileX) void k_cg_pass_a(CgArgs a, const double* __restrict__ pold,
+                                                      double* __restrict__ pnew, int it) {
+  cg_pass_a_body(a, pold, pnew, it, xcd_block(blockIdx.x, gridDim.x), gridDim.x);
+}
+template <bool FIRST>
+__global__ __launch_bounds__(kTileX) void k_cg_pass_b(CgArgs a, const double* __restrict__ pv, int it) {
+  cg_pass_b_body<FIRST>(a, pv, it, xcd_block(blockIdx.x, gridDim.x), gridDim.x);
+}
+
 template __global__ void k_cg_start<true>(CgArgs, const double*);
 template __global__ void k_cg_start<false>(CgArgs, const double*);
 template __global__ void k_cg_pass_b<true>(CgArgs, const double*, int);
@@ -639,10 +657,10 @@ template __global__ void k_cg_pass_b<false>(CgArgs, const double*, int);
 
 // B += rate(x) * c (solvers.hpp:528-529).  x is taken as 0 when CG did no
 // product (Eigen returns x = 0), which leaves B unchanged.
-__global__ __launch_bounds__(kTileX) void k_finalize1(FinArgs a) {
+__device__ __forceinline__ void finalize1_body(const FinArgs& a, int vb) {
   if (a.st->spmv == 0) return;
   const int nx = a.nx, ny = a.ny;
-  const Tile t = tile_of(nx, ny, a.ty, a.tiles_y);
+  const Tile t = tile_of(nx, ny, a.ty, a.tiles_y, vb);
   if (!t.valid) return;
   auto at = [&](int col, int j) -> int { return wrap_row(j, ny) * nx + col; };
   double xc = a.x[at(t.ic, t.y0)], xn = a.x[at(t.ic, t.y0 + 1)], xe = a.x[at(t.ip, t.y0)];
@@ -708,13 +726,13 @@ __device__ __forceinline__ Fin2Chain fin2_begin(const FinArgs& a, bool xv, int c
   return ch;
 }
 
-__global__ __launch_bounds__(kTileX) void k_finalize2(FinArgs a) {
+__device__ __forceinline__ void finalize2_body(const FinArgs& a, int vb) {
   __shared__ double jrow[2][kTileX + 2];
   const bool xv = a.st->spmv != 0;
   const int nx = a.nx, ny = a.ny;
-  const Tile t = tile_of(nx, ny, a.ty, a.tiles_y);
+  const Tile t = tile_of(nx, ny, a.ty, a.tiles_y, vb);
   int x0, w, xl, xr;
-  tile_columns(nx, a.ty, a.tiles_y, x0, w, xl, xr);
+  tile_columns(nx, a.tiles_y, vb, x0, w, xl, xr);
   (void)xl;
   const int tid = threadIdx.x;
   const bool edge = (tid == w - 1);
@@ -744,6 +762,9 @@ __global__ __launch_bounds__(kTileX) void k_finalize2(FinArgs a) {
   }
 }
 
+__global__ __launch_bounds__(kTileX) void k_finalize1(FinArgs a) { finalize1_body(a, xcd_block(blockIdx.x, gridDim.x)); }
+__global__ __launch_bounds__(kTileX) void k_finalize2(FinArgs a) { finalize2_body(a, xcd_block(blockIdx.x, gridDim.x)); }
+
 // rate arrays of resistive_B_term for the unit entry point.
 __global__ __launch_bounds__(256) void k_rate(FinArgs a, int zero) {
   const int nx = a.nx, ny = a.ny;
@@ -769,10 +790,10 @@ __global__ __launch_bounds__(256) void k_rate(FinArgs a, int zero) {
 // CFL_condition_checker's wave speed (diagnostics.hpp:58-80) per cell, with
 // Grid::computePrimitives' primitives; per-block max (NaN skipped exactly as
 // the reference's `>` comparison skips it) and count of non-finite speeds.
-__global__ __launch_bounds__(kTileX) void k_cfl(DiagArgs a) {
+__device__ __forceinline__ void cfl_body(const DiagArgs& a, int vb) {
   __shared__ double lds[16];
   const int nx = a.nx, ny = a.ny;
-  const Tile t = tile_of(nx, ny, a.ty, a.tiles_y);
+  const Tile t = tile_of(nx, ny, a.ty, a.tiles_y, vb);
   double m = 1e-8;
   double bad[1] = {0.0};
   if (t.valid) {
@@ -813,11 +834,13 @@ __global__ __launch_bounds__(kTileX) void k_cfl(DiagArgs a) {
   }
 }
 
+__global__ __launch_bounds__(kTileX) void k_cfl(DiagArgs a) { cfl_body(a, xcd_block(blockIdx.x, gridDim.x)); }
+
 // calculate_current_div_B (diagnostics.hpp:15-42): per-block sum of squares and max |div|.
 __global__ __launch_bounds__(kTileX) void k_divb(DiagArgs a) {
   __shared__ double lds[16];
   const int nx = a.nx, ny = a.ny;
-  const Tile t = tile_of(nx, ny, a.ty, a.tiles_y);
+  const Tile t = tile_of(nx, ny, a.ty, a.tiles_y, xcd_block(blockIdx.x, gridDim.x));
   double ss[1] = {0.0};
   double m = 0.0;
   if (t.y0 < ny && t.valid) {
@@ -868,6 +891,224 @@ __global__ __launch_bounds__(256) void k_floor_rho(double* rho, int n) {
 __global__ __launch_bounds__(256) void k_copy(const double* __restrict__ src, double* __restrict__ dst, long long n) {
   for (long long o = blockIdx.x * (long long)blockDim.x + threadIdx.x; o < n; o += (long long)gridDim.x * blockDim.x)
     dst[o] = src[o];
+}
+
+
+// ---------------------------------------------------------------------------
+// Batched ensembles.  Each kernel below is the single-run kernel of the same
+// name applied to every active run of a batch in one launch: the block reads
+// its (run, block-within-run) job, forms the run's argument block from the
+// pooled arrays and the run's geometry and driver state, and runs the
+// single-run body.  A run whose driver has stopped returns at once.
+// ---------------------------------------------------------------------------
+
+template <int MODE>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void kb_stage(BatchStageArgs b) {
+  const int2 job = b.jobs[blockIdx.x];
+  const RunState& rs = b.rs[job.x];
+  if (!rs.active) return;
+  const RunGeom& g = b.geom[job.x];
+  StageArgs a;
+#pragma unroll
+  for (int q = 0; q < 6; ++q) {
+    a.s[q] = b.s[q] + g.base;
+    a.u[q] = (MODE == 2) ? b.u[q] + g.base : nullptr;
+    a.o1[q] = b.o1[q] + g.base;
+    a.o2[q] = (MODE == 1) ? b.o2[q] + g.base : nullptr;
+  }
+  a.nx = g.nx;
+  a.ny = g.ny;
+  a.rows = g.rows;
+  a.dx = g.dx;
+  a.dy = g.dy;
+  a.inv_dx = g.inv_dx;
+  a.inv_dy = g.inv_dy;
+  a.eta = g.eta;
+  a.c_a = rs.dt * b.fa;
+  a.c_b = rs.dt * b.fb;
+  a.c_g = rs.dt * b.fg;
+  stage_body<MODE>(a, job.y % g.strips, job.y / g.strips);
+}
+template __global__ void kb_stage<1>(BatchStageArgs);
+template __global__ void kb_stage<2>(BatchStageArgs);
+
+__device__ __forceinline__ CgArgs batch_cg_args(const BatchCgArgs& b, int run) {
+  const RunGeom& g = b.geom[run];
+  const RunState& rs = b.rs[run];
+  CgArgs a;
+  a.nx = g.nx;
+  a.ny = g.ny;
+  a.ty = g.ty;
+  a.tiles_y = g.tiles_y;
+  a.dx = g.dx;
+  a.dy = g.dy;
+  a.eta = g.eta;
+  a.invd = rs.invd;
+  a.diag = rs.diag;
+  a.cx = rs.cx;
+  a.cy = rs.cy;
+  a.tol = b.tol;
+  a.bx = b.bx + g.base;
+  a.by = b.by + g.base;
+  a.r = b.r + g.base;
+  a.p0 = b.p0 + g.base;
+  a.x = b.x + g.base;
+  a.part_s = b.part_s + 3LL * g.part_base;
+  a.part_a = b.part_a + g.part_base;
+  a.part_b = b.part_b + 2LL * g.part_base;
+  a.st = b.st + run;
+  return a;
+}
+
+__global__ __launch_bounds__(kTileX) void kb_cg_start(BatchCgArgs b) {
+  const int2 job = b.jobs[xcd_block(blockIdx.x, gridDim.x)];
+  if (!b.rs[job.x].active) return;
+  const CgArgs a = batch_cg_args(b, job.x);
+  cg_start_body<true>(a, nullptr, job.y);
+}
+
+__global__ __launch_bounds__(kTileX) void kb_cg_pass_a(BatchCgArgs b, const double* __restrict__ pold,
+                                                       double* __restrict__ pnew, int it) {
+  const int2 job = b.jobs[xcd_block(blockIdx.x, gridDim.x)];
+  if (b.st[job.x].done) return;
+  const CgArgs a = batch_cg_args(b, job.x);
+  const long long base = b.geom[job.x].base;
+  cg_pass_a_body(a, pold + base, pnew + base, it, job.y, b.geom[job.x].tiles);
+}
+
+template <bool FIRST>
+__global__ __launch_bounds__(kTileX) void kb_cg_pass_b(BatchCgArgs b, const double* __restrict__ pv, int it) {
+  const int2 job = b.jobs[xcd_block(blockIdx.x, gridDim.x)];
+  if (b.st[job.x].done) return;
+  const CgArgs a = batch_cg_args(b, job.x);
+  cg_pass_b_body<FIRST>(a, pv + b.geom[job.x].base, it, job.y, b.geom[job.x].tiles);
+}
+template __global__ void kb_cg_pass_b<true>(BatchCgArgs, const double*, int);
+template __global__ void kb_cg_pass_b<false>(BatchCgArgs, const double*, int);
+
+__device__ __forceinline__ FinArgs batch_fin_args(const BatchFinArgs& b, int run) {
+  const RunGeom& g = b.geom[run];
+  FinArgs f;
+  f.nx = g.nx;
+  f.ny = g.ny;
+  f.ty = g.ty;
+  f.tiles_y = g.tiles_y;
+  f.dx = g.dx;
+  f.dy = g.dy;
+  f.eta = g.eta;
+  f.c = b.rs[run].dt * b.fc;
+  f.x = b.x + g.base;
+  f.bx = b.bx + g.base;
+  f.by = b.by + g.base;
+  f.bx_out = b.bx_out + g.base;
+  f.by_out = b.by_out + g.base;
+  f.st = b.st + run;
+  return f;
+}
+
+__global__ __launch_bounds__(kTileX) void kb_finalize1(BatchFinArgs b) {
+  const int2 job = b.jobs[xcd_block(blockIdx.x, gridDim.x)];
+  if (!b.rs[job.x].active) return;
+  finalize1_body(batch_fin_args(b, job.x), job.y);
+}
+
+// The host swaps the B arrays after this kernel for all runs, so a run that
+// no longer steps copies its B across unchanged.
+__global__ __launch_bounds__(kTileX) void kb_finalize2(BatchFinArgs b) {
+  const int2 job = b.jobs[xcd_block(blockIdx.x, gridDim.x)];
+  const FinArgs f = batch_fin_args(b, job.x);
+  if (b.rs[job.x].active) {
+    finalize2_body(f, job.y);
+    return;
+  }
+  const Tile t = tile_of(f.nx, f.ny, f.ty, f.tiles_y, job.y);
+  if (!t.valid) return;
+  for (int j = t.y0; j < t.y1; ++j) {
+    const int o = j * f.nx + t.ic;
+    f.bx_out[o] = f.bx[o];
+    f.by_out[o] = f.by[o];
+  }
+}
+
+// Per-tile CFL partials of every run that has not reached its end yet (the
+// driver state still holds the previous step's activity; kb_dt decides).
+__global__ __launch_bounds__(kTileX) void kb_cfl(BatchDiagArgs b) {
+  const int2 job = b.jobs[xcd_block(blockIdx.x, gridDim.x)];
+  const RunGeom& g = b.geom[job.x];
+  const RunState& rs = b.rs[job.x];
+  if (!(rs.t < g.t_end) || (g.max_steps >= 0 && rs.steps >= g.max_steps)) return;
+  DiagArgs d;
+  d.nx = g.nx;
+  d.ny = g.ny;
+  d.ty = g.ty;
+  d.tiles_y = g.tiles_y;
+  d.dx = g.dx;
+  d.dy = g.dy;
+#pragma unroll
+  for (int q = 0; q < 6; ++q) d.s[q] = b.s[q] + g.base;
+  d.part = b.part + 2LL * g.part_base;
+  cfl_body(d, job.y);
+}
+
+// Driver step of every run (src/integrator_main.cpp:76-100 of the reference,
+// as mhd_run does it for one run): activity, CFL time step with the Courant
+// number of the run (CFL_condition_checker, diagnostics.hpp:52-86), the clamp
+// of the last step to t_end, the operator rebuild when dt changed by more than
+// 1e-10 relative (alpha = dt eta gamma), then t and the step count advance.
+// Also resets both CG states of the run.  One block per run.
+__global__ __launch_bounds__(256) void kb_dt(const RunGeom* __restrict__ geom, RunState* __restrict__ rs,
+                                             const double* __restrict__ part, CgState* __restrict__ st0,
+                                             CgState* __restrict__ st1, int nruns, double a_im4) {
+  __shared__ double lds[16];
+  const int run = blockIdx.x;
+  if (run >= nruns) return;
+  const RunGeom& g = geom[run];
+  RunState s = rs[run];
+  const bool active = (s.t < g.t_end) && (g.max_steps < 0 || s.steps < g.max_steps);
+  double m = 1e-8;
+  double bad[1] = {0.0};
+  if (active) {
+    for (int b = threadIdx.x; b < g.tiles; b += blockDim.x) {
+      m = max_ignoring_nan(m, part[2LL * (g.part_base + b)]);
+      bad[0] += part[2LL * (g.part_base + b) + 1];
+    }
+  }
+  block_max(m, lds);
+  block_sum<1>(bad, lds + 8);
+  if (threadIdx.x != 0) return;
+  CgState c{};
+  c.max_iters = g.max_iters;
+  if (!active) {
+    s.active = 0;
+    c.done = 3;
+  } else {
+    const double h = (g.dy < g.dx) ? g.dy : g.dx;
+    const double dt_res = (g.eta != 0.0) ? h * h / (4.0 * g.eta) : INFINITY;
+    const double dt_hyp = (g.cfl / 0.9) * h / m;
+    const double dt_cfl = ((dt_res < dt_hyp) ? dt_res : dt_hyp) * 0.9;
+    const double dt0 = (g.dt_fixed > 0.0) ? g.dt_fixed : dt_cfl;
+    if (s.steps == 0) {                      // operator of the first dt, before the loop
+      s.alpha = (dt0 * g.eta) * a_im4;
+      s.prev_dt = dt0;
+    }
+    double dt = dt0;
+    if (s.t + dt > g.t_end) dt = g.t_end - s.t;
+    if (g.eta > 0.0 && fabs(dt - s.prev_dt) > 1e-10 * dt) s.alpha = (dt * g.eta) * a_im4;
+    s.prev_dt = dt;
+    s.dt = dt;
+    s.cx = s.alpha / (g.dx * g.dx);
+    s.cy = s.alpha / (g.dy * g.dy);
+    s.diag = 1 + 2 * (s.cx + s.cy);
+    s.invd = 1.0 / s.diag;
+    s.active = 1;
+    if (g.dt_fixed <= 0.0 && bad[0] > 0.0) s.nonfinite = 1;
+    s.t += dt;
+    s.steps += 1;
+    if (g.eta == 0.0) c.done = 3;            // no implicit solve
+  }
+  rs[run] = s;
+  st0[run] = c;
+  st1[run] = c;
 }
 
 }  // namespace mhd

@@ -67,6 +67,93 @@ struct DiagArgs {
   double* part;    // two partials per tile
 };
 
+// ---------------------------------------------------------------------------
+// Batched ensembles: many independent periodic runs of different sizes share
+// pooled field arrays (run r owns cells [base_r, base_r + nx_r ny_r) of every
+// field) and advance together, one launch per kernel for all runs.  A block
+// finds its run and its block index within the run in a job table.
+// ---------------------------------------------------------------------------
+
+// Geometry, layout and schedule of one run (host-written at creation).
+struct RunGeom {
+  long long base;          // first cell of the run in the pooled arrays
+  int nx, ny;
+  int rows, strips;        // stage kernel: rows per segment, 60-column strips
+  int ty, tiles_y, tiles;  // 2D sweeps: tile rows, tiles per column of tiles, tiles
+  int part_base;           // first tile of the run in the pooled partial arrays
+  double dx, dy, inv_dx, inv_dy, eta;
+  double cfl;              // effective Courant number
+  double t_end;            // the run stops once t >= t_end
+  double dt_fixed;         // > 0: constant time step (the driver's override)
+  long long max_iters;     // CG iteration limit
+  int max_steps;           // < 0: no step limit
+  int pad_;
+};
+
+// Driver state of one run, device resident; kb_dt writes it once per step.
+struct RunState {
+  double t, dt, prev_dt, alpha, cx, cy, diag, invd;
+  int steps, active, nonfinite, pad_;
+};
+
+struct BatchStageArgs {
+  const RunGeom* geom;
+  const RunState* rs;
+  const int2* jobs;        // (run, wave index within the run)
+  const double* s[6];
+  const double* u[6];
+  double* o1[6];
+  double* o2[6];
+  double fa, fb, fg;       // stage coefficients per unit of dt (c = dt * f)
+};
+
+struct BatchCgArgs {
+  const RunGeom* geom;
+  const RunState* rs;
+  const int2* jobs;        // (run, tile index within the run)
+  const double* bx;
+  const double* by;
+  double* r;
+  double* p0;
+  double* x;
+  double* part_s;
+  double* part_a;
+  double* part_b;
+  CgState* st;             // one state per run for this solve slot
+  double tol;
+};
+
+struct BatchFinArgs {
+  const RunGeom* geom;
+  const RunState* rs;
+  const int2* jobs;
+  const double* x;
+  const double* bx;
+  const double* by;
+  double* bx_out;
+  double* by_out;
+  const CgState* st;
+  double fc;               // c = dt * fc
+};
+
+struct BatchDiagArgs {
+  const RunGeom* geom;
+  const RunState* rs;
+  const int2* jobs;
+  const double* s[6];
+  double* part;
+};
+
+template <int MODE> __global__ void kb_stage(BatchStageArgs b);
+__global__ void kb_cg_start(BatchCgArgs b);
+__global__ void kb_cg_pass_a(BatchCgArgs b, const double* pold, double* pnew, int it);
+template <bool FIRST> __global__ void kb_cg_pass_b(BatchCgArgs b, const double* pv, int it);
+__global__ void kb_finalize1(BatchFinArgs b);
+__global__ void kb_finalize2(BatchFinArgs b);
+__global__ void kb_cfl(BatchDiagArgs b);
+__global__ void kb_dt(const RunGeom* geom, RunState* rs, const double* part, CgState* st0, CgState* st1,
+                      int nruns, double a_im4);
+
 template <int MODE> __global__ void k_stage(StageArgs a);
 template <bool FROM_B> __global__ void k_cg_start(CgArgs a, const double* bin);
 __global__ void k_cg_rhs(CgArgs a);

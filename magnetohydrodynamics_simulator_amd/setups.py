@@ -181,6 +181,25 @@ def fourier_ensemble_member(n: int, rng: np.random.Generator, modes: int = 8, am
     return _from_primitive(1.0, vx, vy, 1.0, bx, by)
 
 
+ENSEMBLE_SIDES = (64, 96, 128, 192, 256, 384, 512)
+
+
+def ensemble(nruns: int = 256, seed: int = 0, sides=ENSEMBLE_SIDES, eta_range=(1e-4, 1e-1)):
+    """Seeded ensemble of independent periodic runs: side drawn from `sides`,
+    eta log-uniform in eta_range, initial state fourier_ensemble_member.
+    Returns a list of (n, eta, rng_seed_of_member); build the state with
+    ensemble_state(n, member_seed)."""
+    rng = np.random.default_rng(seed)
+    ns = rng.choice(np.asarray(sides), size=nruns)
+    etas = 10.0 ** rng.uniform(np.log10(eta_range[0]), np.log10(eta_range[1]), size=nruns)
+    member_seeds = rng.integers(0, 2**31 - 1, size=nruns)
+    return [(int(n), float(e), int(ms)) for n, e, ms in zip(ns, etas, member_seeds)]
+
+
+def ensemble_state(n: int, member_seed: int, **kw):
+    return fourier_ensemble_member(n, np.random.default_rng(member_seed), **kw)
+
+
 SETUPS = {
     "orszag_tang": orszag_tang,
     "decaying_sine": decaying_sine,

@@ -217,6 +217,104 @@ class Context:
         self._check(self.lib.mhd_calibration_copy(self.h, nbytes))
 
 
+class Batch:
+    """Batched ensemble of independent periodic runs (mhd_batch_* of the C
+    ABI): one launch per kernel advances every run; each run follows the
+    reference driver loop with its own CFL time step, end time and operator."""
+
+    FIELDS = ("rho", "mx", "my", "E", "bxf", "byf")
+
+    def __init__(self, nx, ny, dx, dy, eta, *, cfl: float = 0.45, cg_tol: float | None = None,
+                 cg_max_iter: int = 0, cg_fail_is_error: bool = False, march_rows: int = 0):
+        self.lib = _lib.load()
+        self.nx = np.ascontiguousarray(nx, dtype=np.int32)
+        self.ny = np.ascontiguousarray(ny, dtype=np.int32)
+        self.dx = np.ascontiguousarray(dx, dtype=np.float64)
+        self.dy = np.ascontiguousarray(dy, dtype=np.float64)
+        self.eta = np.ascontiguousarray(eta, dtype=np.float64)
+        self.nruns = len(self.nx)
+        for a in (self.ny, self.dx, self.dy, self.eta):
+            if len(a) != self.nruns:
+                raise ValueError("run arrays differ in length")
+        o = _lib.Opts()
+        _lib.check(self.lib.mhd_default_opts(C.byref(o)))
+        o.cfl = cfl
+        if cg_tol is not None:
+            o.cg_tol = cg_tol
+        o.cg_max_iter = cg_max_iter
+        o.cg_fail_is_error = int(cg_fail_is_error)
+        o.march_rows = march_rows
+        h = C.c_void_p()
+        ip = C.POINTER(C.c_int)
+        _lib.check_batch(self.lib.mhd_batch_create(self.nruns, self.nx.ctypes.data_as(ip), self.ny.ctypes.data_as(ip),
+                                                   _ptr(self.dx), _ptr(self.dy), _ptr(self.eta), C.byref(o),
+                                                   C.byref(h)), None)
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.mhd_batch_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        _lib.check_batch(rc, self.h)
+
+    def set_schedule(self, t_end=None, max_steps=None, dt_fixed=None):
+        def arr(v, dtype):
+            if v is None:
+                return None
+            return np.ascontiguousarray(np.broadcast_to(np.asarray(v, dtype=dtype), (self.nruns,)))
+        te, ms, df = arr(t_end, np.float64), arr(max_steps, np.int32), arr(dt_fixed, np.float64)
+        self._keep = (te, ms, df)
+        self._check(self.lib.mhd_batch_set_schedule(
+            self.h, _ptr(te), ms.ctypes.data_as(C.POINTER(C.c_int)) if ms is not None else None, _ptr(df)))
+
+    def upload(self, run: int, state: dict):
+        shape = (int(self.ny[run]) + 4, int(self.nx[run]) + 4)
+        arrs = [np.ascontiguousarray(state[k], dtype=np.float64) for k in self.FIELDS]
+        for a in arrs:
+            if a.shape != shape:
+                raise ValueError(f"run {run}: expected padded shape {shape}, got {a.shape}")
+        self._check(self.lib.mhd_batch_upload(self.h, run, *[_ptr(a) for a in arrs]))
+
+    def download(self, run: int) -> dict:
+        shape = (int(self.ny[run]) + 4, int(self.nx[run]) + 4)
+        out = {k: np.zeros(shape) for k in self.FIELDS}
+        self._check(self.lib.mhd_batch_download(self.h, run, *[_ptr(out[k]) for k in self.FIELDS]))
+        return out
+
+    def step(self) -> int:
+        act = C.c_int()
+        self._check(self.lib.mhd_batch_step(self.h, C.byref(act)))
+        return act.value
+
+    def run(self, max_batch_steps: int = -1) -> int:
+        n = C.c_int()
+        self._check(self.lib.mhd_batch_run(self.h, max_batch_steps, C.byref(n)))
+        return n.value
+
+    def run_info(self, run: int):
+        steps, t, dt = C.c_int(), C.c_double(), C.c_double()
+        its = (C.c_int * 2)()
+        self._check(self.lib.mhd_batch_run_info(self.h, run, C.byref(steps), C.byref(t), C.byref(dt), its))
+        return steps.value, t.value, dt.value, (its[0], its[1])
+
+    def synchronize(self):
+        self._check(self.lib.mhd_batch_synchronize(self.h))
+
+
 def operator_alpha(dt: float, eta: float) -> float:
     """alpha of generate_A_matrix as the driver forms it:
     (dt * resistivity) * A_im[4] (src/integrator_main.cpp:189-190)."""
