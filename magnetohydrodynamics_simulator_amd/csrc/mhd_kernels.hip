@@ -962,7 +962,7 @@ __device__ __forceinline__ CgArgs batch_cg_args(const BatchCgArgs& b, int run) {
 
 __global__ __launch_bounds__(kTileX) void kb_cg_start(BatchCgArgs b) {
   const int2 job = b.jobs[xcd_block(blockIdx.x, gridDim.x)];
-  if (!b.rs[job.x].active) return;
+  if (b.st[job.x].done) return;            // stopped run, or eta = 0: no solve
   const CgArgs a = batch_cg_args(b, job.x);
   cg_start_body<true>(a, nullptr, job.y);
 }
