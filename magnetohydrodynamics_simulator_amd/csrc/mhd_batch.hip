@@ -177,7 +177,7 @@ int cg_batch(mhd_batch* b, int slot, const double* bx, const double* by) {
   for (int r = 0; r < b->nruns; ++r) {
     const CgState& s = b->st_host[r];
     if (s.done != 3) most = std::max(most, s.spmv);
-    b->cg_iters_last[2 * r + slot] = s.iters;
+    if (b->rs_host[r].active) b->cg_iters_last[2 * r + slot] = s.iters;   // runs that stepped (rs_host arrived before the first state copy)
     if (s.done == 2 && b->o.cg_fail_is_error)
       return bfail(b, MHD_ECG, "run " + std::to_string(r) + ": CG reached the iteration limit");
   }
