@@ -79,6 +79,9 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=2, help="CPU-baseline sample (steps at --n)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 counter passes")
+    ap.add_argument("--batch-runs", type=int, default=256, help="ensemble size of the batched workload (0: skip)")
+    ap.add_argument("--batch-steps", type=int, default=200)
+    ap.add_argument("--plain-steps", type=int, default=10, help="steps per run of the plain-path comparison")
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     return ap.parse_args()
 
@@ -192,6 +195,58 @@ def cpu_baseline(n: int, eta: float, tol: float, cfl: float, steps: int) -> dict
     }
 
 
+def bench_batch(a) -> dict:
+    """Seeded ensemble of independent periodic runs (sides drawn from
+    {64, ..., 512}, eta log-uniform in [1e-4, 1e-1], rho = p = 1, B = (1, 0.5)
+    plus 8 Fourier modes of amplitude 1e-2, CFL 0.4): all runs in one batch
+    for batch_steps steps, against the plain single-run path on the same runs
+    (one context per run, plain_steps steps each)."""
+    from magnetohydrodynamics_simulator_amd import Batch, Context, setups
+
+    members = setups.ensemble(a.batch_runs, seed=0)
+    ns = [m[0] for m in members]
+    etas = [m[1] for m in members]
+    dxs = [setups.grid_spacing(n) for n in ns]
+    cells = int(sum(n * n for n in ns))
+    states = [setups.ensemble_state(n, ms) for n, _, ms in members]
+    b = Batch(ns, ns, dxs, dxs, etas, cfl=a.cfl, cg_tol=a.cg_tol)
+    for r in range(len(members)):
+        b.upload(r, states[r])
+    b.set_schedule(max_steps=2)                 # warm-up
+    b.run()
+    for r in range(len(members)):               # restart from the initial state
+        b.upload(r, states[r])
+    b.set_schedule(max_steps=a.batch_steps)
+    b.synchronize()
+    t0 = time.perf_counter()
+    done = b.run()
+    b.synchronize()
+    t_batch = time.perf_counter() - t0
+    its = np.array([b.run_info(r)[3] for r in range(len(members))])
+    b.close()
+    # plain path: one context per run, the driver loop of each in turn
+    t_plain = 0.0
+    for (n, eta, _), ic in zip(members, states):
+        dx = setups.grid_spacing(n)
+        with Context(n, n, dx, dx, eta, cfl=a.cfl, cg_tol=a.cg_tol) as ctx:
+            ctx.upload(*[ic[k] for k in ("rho", "mx", "my", "E", "bxf", "byf")])
+            ctx.run(1e9, max_steps=1)
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            ctx.run(1e9, max_steps=a.plain_steps)
+            ctx.synchronize()
+            t_plain += time.perf_counter() - t0
+    v_batch = cells * done / t_batch / 1e6
+    v_plain = cells * a.plain_steps / t_plain / 1e6
+    return {"workload": f"{len(members)} periodic runs, sides {min(ns)}..{max(ns)} (seeded), eta 1e-4..1e-1, "
+                        f"Fourier-perturbed uniform state, CFL {a.cfl}, CG tol {a.cg_tol}",
+            "runs": len(members), "cells": cells, "steps": done,
+            "value": round(v_batch, 2), "unit": "Mcell-updates/s", "ms_per_batch_step": round(t_batch / done * 1e3, 4),
+            "plain_value": round(v_plain, 2), "plain_steps_per_run": a.plain_steps,
+            "speedup_vs_plain": round(v_batch / v_plain, 2),
+            "cg_iters_last_step_mean": round(float(its.mean()), 2)}
+
+
 def main():
     a = parse()
     if a.pmc_child:
@@ -285,6 +340,8 @@ def main():
                 "fetch_bytes": int(tr["FETCH_SIZE"]), "write_bytes": int(tr["WRITE_SIZE"]),
                 "fetch_scale": round(tr["FETCH_SIZE_scale"], 4), "write_scale": round(tr["WRITE_SIZE_scale"], 4),
                 "source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, 2 driver steps, scaled by a 1 GiB copy"}
+    if a.batch_runs > 0:
+        line["batched"] = bench_batch(a)
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baseline(n, a.eta, a.cg_tol, a.cfl, a.cpu_steps)
     print(json.dumps(line), flush=True)
