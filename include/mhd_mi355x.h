@@ -164,6 +164,13 @@ int mhd_batch_download(mhd_batch* b, int run, double* rho, double* mx, double* m
 int mhd_batch_step(mhd_batch* b, int* n_active);
 /* mhd_batch_step until no run is active or max_batch_steps (< 0: no limit). */
 int mhd_batch_run(mhd_batch* b, int max_batch_steps, int* steps_done);
+/* Autotuner: times trial_steps steps of the uploaded batch for each candidate
+ * launch shape (rows per explicit-stage segment x rows per CG / finalize
+ * tile, each in {16, 32, 64}), restores the state, and keeps the fastest
+ * shape; the chosen rows are returned (either pointer may be NULL).  Results
+ * do not depend on the shape beyond the summation order of the CG dot
+ * products. */
+int mhd_batch_autotune(mhd_batch* b, int trial_steps, int* stage_rows, int* tile_rows);
 /* Steps taken, time, last dt and the two CG iteration counts of the last step of one run. */
 int mhd_batch_run_info(mhd_batch* b, int run, int* steps, double* t, double* last_dt, int* cg_iters);
 int mhd_batch_synchronize(mhd_batch* b);

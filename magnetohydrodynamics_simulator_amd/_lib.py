@@ -60,6 +60,7 @@ SIGNATURES = {
     "mhd_batch_step": (C.c_int, [_vp, _ip]),
     "mhd_batch_run": (C.c_int, [_vp, C.c_int, _ip]),
     "mhd_batch_run_info": (C.c_int, [_vp, C.c_int, _ip, _dp, _dp, _ip]),
+    "mhd_batch_autotune": (C.c_int, [_vp, C.c_int, _ip, _ip]),
     "mhd_batch_synchronize": (C.c_int, [_vp]),
 }
 

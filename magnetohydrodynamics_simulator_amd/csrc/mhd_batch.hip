@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -30,7 +31,10 @@ namespace {
 
 thread_local std::string g_batch_create_error;
 
-constexpr double kGammaArs = 0.29289321881345254;   // (2 - sqrt 2) / 2, solvers.hpp:41
+// Candidate launch shapes of the autotuner: rows per stage-kernel segment and
+// rows per 2D tile.  Buffers are sized for the smallest of each.
+constexpr int kStageRowChoices[] = {16, 32, 64};
+constexpr int kTileRowChoices[] = {16, 32, 64};
 
 struct BatchTableau {
   double gamma = (2.0 - std::sqrt(2.0)) / 2.0;
@@ -68,6 +72,7 @@ struct mhd_batch {
   int n_stage_jobs = 0;
   int2* tile_jobs = nullptr;
   int n_tile_jobs = 0;
+  size_t cap_stage_jobs = 0, cap_tiles = 0;   // capacity of the job tables and partial arrays
   int stage_rows = 32, tile_rows = 32;
   int last_spmv[2] = {1, 1};
   bool any_eta = false;
@@ -122,6 +127,24 @@ void plan(mhd_batch* b, std::vector<int2>& stage_jobs, std::vector<int2>& tile_j
   }
   b->total = base;
   b->total_tiles = part_base;
+}
+
+// Lays the runs out with the given launch shape and uploads geometry and job
+// tables (the schedule fields of the geometry are kept).
+int replan(mhd_batch* b, int stage_rows, int tile_rows) {
+  b->stage_rows = stage_rows;
+  b->tile_rows = tile_rows;
+  std::vector<int2> sj, tj;
+  plan(b, sj, tj);
+  if (sj.size() > b->cap_stage_jobs || tj.size() > b->cap_tiles)
+    return bfail(b, MHD_EINVAL, "launch shape exceeds the planned capacity");
+  b->n_stage_jobs = (int)sj.size();
+  b->n_tile_jobs = (int)tj.size();
+  BHIP(b, hipMemcpyAsync(b->geom, b->geom_h.data(), sizeof(RunGeom) * b->nruns, hipMemcpyHostToDevice, b->stream));
+  BHIP(b, hipMemcpyAsync(b->stage_jobs, sj.data(), sizeof(int2) * sj.size(), hipMemcpyHostToDevice, b->stream));
+  BHIP(b, hipMemcpyAsync(b->tile_jobs, tj.data(), sizeof(int2) * tj.size(), hipMemcpyHostToDevice, b->stream));
+  BHIP(b, hipStreamSynchronize(b->stream));
+  return MHD_OK;
 }
 
 BatchCgArgs cg_args(mhd_batch* b, int slot, const double* bx, const double* by) {
@@ -235,7 +258,6 @@ int mhd_batch_create(int nruns, const int* nx, const int* ny, const double* dx, 
   mhd_batch* b = new mhd_batch();
   b->nruns = nruns;
   b->o = o;
-  if (o.march_rows > 0) b->stage_rows = o.march_rows;
   b->geom_h.resize(nruns);
   b->cg_iters_last.assign(2 * nruns, 0);
   for (int r = 0; r < nruns; ++r) {
@@ -256,7 +278,17 @@ int mhd_batch_create(int nruns, const int* nx, const int* ny, const double* dx, 
     b->any_eta = b->any_eta || eta[r] > 0.0;
   }
   std::vector<int2> sj, tj;
+  // capacity: the smallest candidate shapes give the most jobs and tiles
+  b->stage_rows = kStageRowChoices[0];
+  b->tile_rows = kTileRowChoices[0];
   plan(b, sj, tj);
+  b->cap_stage_jobs = std::max(sj.size(), (size_t)1);
+  b->cap_tiles = std::max(tj.size(), (size_t)1);
+  b->stage_rows = (o.march_rows > 0) ? o.march_rows : 32;
+  b->tile_rows = 32;
+  plan(b, sj, tj);
+  b->cap_stage_jobs = std::max(b->cap_stage_jobs, sj.size());
+  b->cap_tiles = std::max(b->cap_tiles, tj.size());
   auto bail = [&](const std::string& m, int code) {
     g_batch_create_error = m;
     mhd_batch_destroy(b);
@@ -267,15 +299,15 @@ int mhd_batch_create(int nruns, const int* nx, const int* ny, const double* dx, 
   for (int q = 0; q < 6; ++q)
     if (hipMalloc(&b->U[q], fb) != hipSuccess || hipMalloc(&b->S[q], fb) != hipSuccess || hipMalloc(&b->W[q], fb) != hipSuccess)
       return bail("device allocation failed", MHD_ENOMEM);
-  const size_t pb = (size_t)b->total_tiles * sizeof(double);
+  const size_t pb = b->cap_tiles * sizeof(double);
   if (hipMalloc(&b->x, fb) != hipSuccess || hipMalloc(&b->r, fb) != hipSuccess || hipMalloc(&b->p[0], fb) != hipSuccess ||
       hipMalloc(&b->p[1], fb) != hipSuccess || hipMalloc(&b->part, 2 * pb) != hipSuccess ||
       hipMalloc(&b->part_s, 3 * pb) != hipSuccess || hipMalloc(&b->part_a, pb) != hipSuccess ||
       hipMalloc(&b->part_b, 2 * pb) != hipSuccess || hipMalloc(&b->st[0], sizeof(CgState) * nruns) != hipSuccess ||
       hipMalloc(&b->st[1], sizeof(CgState) * nruns) != hipSuccess || hipMalloc(&b->geom, sizeof(RunGeom) * nruns) != hipSuccess ||
       hipMalloc(&b->rs, sizeof(RunState) * nruns) != hipSuccess ||
-      hipMalloc(&b->stage_jobs, sizeof(int2) * sj.size()) != hipSuccess ||
-      hipMalloc(&b->tile_jobs, sizeof(int2) * tj.size()) != hipSuccess)
+      hipMalloc(&b->stage_jobs, sizeof(int2) * b->cap_stage_jobs) != hipSuccess ||
+      hipMalloc(&b->tile_jobs, sizeof(int2) * b->cap_tiles) != hipSuccess)
     return bail("device allocation failed", MHD_ENOMEM);
   if (hipHostMalloc(&b->rs_host, sizeof(RunState) * nruns) != hipSuccess ||
       hipHostMalloc(&b->st_host, sizeof(CgState) * nruns) != hipSuccess)
@@ -468,6 +500,80 @@ int mhd_batch_run(mhd_batch* b, int max_batch_steps, int* steps_done) {
     if (act == 0) break;
   }
   if (steps_done) *steps_done = k;
+  return MHD_OK;
+}
+
+int mhd_batch_autotune(mhd_batch* b, int trial_steps, int* stage_rows, int* tile_rows) {
+  if (!b) return MHD_EINVAL;
+  if (trial_steps < 1) return bfail(b, MHD_EINVAL, "trial_steps must be >= 1");
+  // save the state, the driver state and the host-side step bookkeeping
+  const size_t fb = (size_t)b->total * sizeof(double);
+  double* saved[6] = {};
+  RunState* rs_saved = nullptr;
+  auto release = [&]() {
+    for (double* p : saved)
+      if (p) (void)hipFree(p);
+    if (rs_saved) (void)hipFree(rs_saved);
+  };
+  for (int q = 0; q < 6; ++q)
+    if (hipMalloc(&saved[q], fb) != hipSuccess) {
+      release();
+      return bfail(b, MHD_ENOMEM, "autotune: device allocation failed");
+    }
+  if (hipMalloc(&rs_saved, sizeof(RunState) * b->nruns) != hipSuccess) {
+    release();
+    return bfail(b, MHD_ENOMEM, "autotune: device allocation failed");
+  }
+  for (int q = 0; q < 6; ++q)
+    (void)hipMemcpyAsync(saved[q], b->U[q], fb, hipMemcpyDeviceToDevice, b->stream);
+  (void)hipMemcpyAsync(rs_saved, b->rs, sizeof(RunState) * b->nruns, hipMemcpyDeviceToDevice, b->stream);
+  const int spmv_saved[2] = {b->last_spmv[0], b->last_spmv[1]};
+  const std::vector<int> its_saved = b->cg_iters_last;
+  const long long steps_saved = b->steps_taken;
+  std::vector<RunGeom> geom_saved = b->geom_h;
+  // the trial runs every run for trial_steps steps regardless of its schedule
+  for (RunGeom& g : b->geom_h) {
+    g.t_end = INFINITY;
+    g.max_steps = -1;
+  }
+  double best = INFINITY;
+  int best_s = b->stage_rows, best_t = b->tile_rows;
+  int rc = MHD_OK;
+  for (int sr : kStageRowChoices) {
+    for (int tr : kTileRowChoices) {
+      rc = replan(b, sr, tr);
+      if (rc) break;
+      double total = 0.0;
+      for (int k = 0; k < trial_steps + 1 && rc == MHD_OK; ++k) {
+        if (hipStreamSynchronize(b->stream) != hipSuccess) { rc = bfail(b, MHD_EHIP, "autotune: sync failed"); break; }
+        auto t0 = std::chrono::steady_clock::now();
+        rc = mhd_batch_step(b, nullptr);
+        auto t1 = std::chrono::steady_clock::now();
+        if (k > 0) total += std::chrono::duration<double>(t1 - t0).count();   // the first step warms the shape up
+      }
+      for (int q = 0; q < 6; ++q)
+        (void)hipMemcpyAsync(b->U[q], saved[q], fb, hipMemcpyDeviceToDevice, b->stream);
+      (void)hipMemcpyAsync(b->rs, rs_saved, sizeof(RunState) * b->nruns, hipMemcpyDeviceToDevice, b->stream);
+      b->last_spmv[0] = spmv_saved[0];
+      b->last_spmv[1] = spmv_saved[1];
+      if (rc) break;
+      if (total < best) {
+        best = total;
+        best_s = sr;
+        best_t = tr;
+      }
+    }
+    if (rc) break;
+  }
+  b->geom_h = geom_saved;
+  b->cg_iters_last = its_saved;
+  b->steps_taken = steps_saved;
+  int rc2 = replan(b, best_s, best_t);
+  release();
+  if (rc) return rc;
+  if (rc2) return rc2;
+  if (stage_rows) *stage_rows = best_s;
+  if (tile_rows) *tile_rows = best_t;
   return MHD_OK;
 }
 

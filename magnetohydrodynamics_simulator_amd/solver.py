@@ -305,6 +305,13 @@ class Batch:
         self._check(self.lib.mhd_batch_run(self.h, max_batch_steps, C.byref(n)))
         return n.value
 
+    def autotune(self, trial_steps: int = 2):
+        """Pick the fastest launch shape for the uploaded ensemble; returns
+        (stage rows, tile rows).  The state is left as it was."""
+        sr, tr = C.c_int(), C.c_int()
+        self._check(self.lib.mhd_batch_autotune(self.h, trial_steps, C.byref(sr), C.byref(tr)))
+        return sr.value, tr.value
+
     def run_info(self, run: int):
         steps, t, dt = C.c_int(), C.c_double(), C.c_double()
         its = (C.c_int * 2)()

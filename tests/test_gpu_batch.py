@@ -125,3 +125,29 @@ def test_batch_end_times_and_stopped_runs():
                 e = rel_l1(interior(got[k]), interior(sim.field(k)))
                 assert e <= TOL_RUN, (r, k, e)
             sim.close()
+
+
+def test_autotune_keeps_state_and_results():
+    _say("batch autotune: state unchanged, results equal to the untuned batch")
+    tol, cfl, steps = 1e-10, 0.4, 3
+    states = _states()
+    with _batch(states, tol, cfl) as b0:
+        b0.set_schedule(max_steps=steps)
+        b0.run()
+        ref = [b0.download(r) for r in range(len(states))]
+    with _batch(states, tol, cfl) as b:
+        b.set_schedule(max_steps=steps)
+        sr, tr = b.autotune(trial_steps=1)
+        assert sr in (16, 32, 64) and tr in (16, 32, 64)
+        for r in range(len(states)):
+            assert b.run_info(r)[0] == 0
+            got = b.download(r)
+            for k in CONSERVED:
+                want = states[r][0][k] if k != "rho" else np.maximum(states[r][0][k], 1e-8)
+                assert np.array_equal(interior(got[k]), interior(want)), (r, k)
+        b.run()
+        for r in range(len(states)):
+            got = b.download(r)
+            assert b.run_info(r)[0] == steps
+            for k in CONSERVED:
+                assert rel_l1(interior(got[k]), interior(ref[r][k])) <= 1e-12, (r, k)
