@@ -133,6 +133,32 @@ int mhd_calibration_copy(mhd_ctx* ctx, long long nbytes);
 int mhd_synchronize(mhd_ctx* ctx);
 
 /* ------------------------------------------------------------------------
+ * Discrete adjoint (no reference counterpart: the reference has no
+ * derivatives).  The adjoint of the whole IMEX step: reverse-mode
+ * primitives / MUSCL-minmod / HLL / upwind CT at the recorded forward state
+ * (the minmod, HLL and CT branch choices of the forward pass), transposed
+ * CT and resistive stencils, density and pressure floors as 0/1 masks, and
+ * for each implicit solve a second CG solve with the same symmetric operator
+ * (I - alpha Lap).  Requires eta > 0 and a constant time step.
+ * ------------------------------------------------------------------------ */
+
+/* Runs nsteps IMEX steps of constant dt from the current state (operator
+ * alpha = dt eta gamma, as the driver builds it for a fixed dt,
+ * src/integrator_main.cpp:56-60), keeping a copy of the state every
+ * checkpoint_every steps.  The context then holds the final state. */
+int mhd_adjoint_forward(mhd_ctx* ctx, double dt, int nsteps, int checkpoint_every);
+/* Given lam_final = dJ/dU(final) (six padded arrays rho, mx, my, E, bxf, byf),
+ * returns lam0 = dJ/dU(initial) (six padded arrays, ghosts filled; the array
+ * pointer or any entry may be NULL) and *eta_bar = dJ/d eta through every
+ * step of the last mhd_adjoint_forward.  Each checkpoint segment is
+ * recomputed once; the context's state is left at the final state. */
+int mhd_adjoint_backward(mhd_ctx* ctx, const double* const* lam_final, double* const* lam0, double* eta_bar);
+/* Unit entry point: out = L'(U)^T lam for the explicit right-hand side L of
+ * explicit_rhs (solvers.hpp:393-402) at the current state U (padded arrays;
+ * out's physical cells are written). */
+int mhd_explicit_rhs_vjp(mhd_ctx* ctx, const double* const* lam, double* const* out);
+
+/* ------------------------------------------------------------------------
  * Batched ensembles (no reference counterpart: the reference advances one
  * Grid per process).  R independent periodic runs, each with its own size,
  * cell size and resistivity, advance together: one launch per kernel covers

@@ -23,92 +23,22 @@
 #include <vector>
 
 #include "../../include/mhd_mi355x.h"
+#include "mhd_context.hpp"
 #include "mhd_kernels.hpp"
 
 using namespace mhd;
+using namespace mhd_host;
 
 namespace {
-
-constexpr int kNumKernelIds = 9;
-
 thread_local std::string g_create_error;
-
-struct Tableau {
-  // IMEX_Butcher_Tableus, solvers.hpp:39-66
-  double gamma = (2.0 - std::sqrt(2.0)) / 2.0;
-  double delta = 1.0 - (1.0 / (2.0 * gamma));
-  double A_ex3 = gamma, A_ex6 = delta, A_ex7 = 1.0 - delta;
-  double A_im4 = gamma, A_im7 = 1.0 - gamma;
-  double b_im2 = gamma;
-};
-
 }  // namespace
 
-struct mhd_ctx {
-  int nx = 0, ny = 0;
-  long long n = 0;
-  double dx = 0, dy = 0, eta = 0;
-  mhd_opts o{};
-  Tableau tab;
-  hipStream_t stream = nullptr;
-  double* U[6] = {};
-  double* S[6] = {};
-  double* F1[6] = {};
-  double* x = nullptr;
-  double* r = nullptr;
-  double* p[2] = {};
-  double* part_s = nullptr;      // CG start kernel: three partials per tile
-  double* part_a = nullptr;      // CG pass A: one partial per tile
-  double* part_b = nullptr;      // CG pass B: two partials per tile
-  double* part = nullptr;        // diagnostics: two partials per tile
-  CgState* st = nullptr;
-  CgState* st_host = nullptr;    // pinned
-  double* part_host = nullptr;   // pinned, two partials per tile
-  double alpha = 0.0;
-  bool have_op = false;
-  int last_spmv[2] = {1, 1};     // products of the previous solve in each of the step's two slots
-  int tiles = 1, tiles_y = 1, tile_rows = 4;   // 2D sweeps: kTileX columns x tile_rows rows per tile
-  int stage_rows = 16;
-  std::string err;
-  unsigned prof_mask = 0;        // kernel ids whose launches are bracketed by events
-  long long prof_n[kNumKernelIds] = {};
-  double prof_ms[kNumKernelIds] = {};
-  std::vector<std::pair<hipEvent_t, hipEvent_t>> pending[kNumKernelIds];
-};
-
-namespace {
+namespace mhd_host {
 
 int fail(mhd_ctx* c, int code, const std::string& msg) {
   if (c) c->err = msg;
   else g_create_error = msg;
   return code;
-}
-
-#define HIP_OR_FAIL(ctx, call)                                                                  \
-  do {                                                                                          \
-    hipError_t e_ = (call);                                                                     \
-    if (e_ != hipSuccess)                                                                       \
-      return fail((ctx), MHD_EHIP, std::string(#call) + ": " + hipGetErrorString(e_));          \
-  } while (0)
-
-// Launch helper: optional event bracketing for per-kernel timing.
-template <typename F>
-int launch(mhd_ctx* c, int id, F&& body) {
-  hipEvent_t a = nullptr, b = nullptr;
-  const bool prof = (c->prof_mask >> id) & 1u;
-  if (prof) {
-    HIP_OR_FAIL(c, hipEventCreate(&a));
-    HIP_OR_FAIL(c, hipEventCreate(&b));
-    HIP_OR_FAIL(c, hipEventRecord(a, c->stream));
-  }
-  body();
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return fail(c, MHD_EHIP, std::string("kernel launch: ") + hipGetErrorString(e));
-  if (prof) {
-    HIP_OR_FAIL(c, hipEventRecord(b, c->stream));
-    c->pending[id].emplace_back(a, b);
-  }
-  return MHD_OK;
 }
 
 int flush_profile(mhd_ctx* c) {
@@ -314,7 +244,105 @@ int to_host_padded(mhd_ctx* c, const double* dev, double* host, bool ghosts) {
   return MHD_OK;
 }
 
+int from_host_padded(mhd_ctx* c, const double* host, double* dev) {
+  const size_t nxt = (size_t)c->nx + 4;
+  HIP_OR_FAIL(c, hipMemcpy2DAsync(dev, (size_t)c->nx * sizeof(double), host + 2 * nxt + 2, nxt * sizeof(double),
+                                  (size_t)c->nx * sizeof(double), (size_t)c->ny, hipMemcpyHostToDevice, c->stream));
+  return MHD_OK;
+}
+
 int check_ctx(mhd_ctx* c) { return c ? MHD_OK : MHD_EINVAL; }
+
+int copy_fields(mhd_ctx* c, double* const* dst, double* const* src, int n) {
+  const size_t fb = (size_t)c->n * sizeof(double);
+  for (int q = 0; q < n; ++q)
+    HIP_OR_FAIL(c, hipMemcpyAsync(dst[q], src[q], fb, hipMemcpyDeviceToDevice, c->stream));
+  return MHD_OK;
+}
+
+int imex_step(mhd_ctx* c, double dt, mhd_step_stats* stats, StepRecord* rec) {
+  mhd_step_stats st{};
+  st.dt = dt;
+  const Tableau& T = c->tab;
+  int rc;
+  if (rec) {
+    rc = copy_fields(c, rec->Un, c->U, 6);
+    if (rc) return rc;
+    rec->spmv[0] = rec->spmv[1] = 0;
+  }
+  // stage 1: f1 = L(U), S = U + dt A_ex[3] f1 (solvers.hpp:495-523),
+  // W = U + dt A_ex[6] f1 (solvers.hpp:548-553) into F1
+  StageArgs a = stage_args(c);
+  for (int q = 0; q < 6; ++q) {
+    a.s[q] = c->U[q];
+    a.o1[q] = c->F1[q];
+    a.o2[q] = c->S[q];
+  }
+  a.c_a = dt * T.A_ex3;
+  a.c_b = dt * T.A_ex6;
+  rc = run_stage(c, 1, a, 0);
+  if (rc) return rc;
+  const double diag_coeff = dt * T.A_im4;
+  if (c->eta > 0.0) {
+    // implicit 1: x = CG(eta J(S.B)), S.B += dt A_im[4] rate(x) (solvers.hpp:526-529)
+    if (rec) {
+      rc = copy_fields(c, rec->SsB, c->S + 4, 2);
+      if (rc) return rc;
+    }
+    CgState cs;
+    rc = cg_run(c, c->S[4], c->S[5], nullptr, &cs, 0);
+    if (rc) return rc;
+    st.cg_iters[0] = cs.iters;
+    st.cg_spmv[0] = cs.spmv;
+    st.cg_error[0] = cg_error(cs);
+    if (rec) {
+      rec->spmv[0] = cs.spmv;
+      if (cs.spmv > 0) HIP_OR_FAIL(c, hipMemcpyAsync(rec->x1, c->x, (size_t)c->n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    }
+    FinArgs f = fin_args(c, c->S[4], c->S[5], c->S[4], c->S[5], diag_coeff);
+    rc = launch(c, 5, [&] { hipLaunchKernelGGL(k_finalize1, dim3(c->tiles), dim3(kTileX), 0, c->stream, f); });
+    if (rc) return rc;
+  }
+  if (rec) {
+    rc = copy_fields(c, rec->S, c->S, 6);
+    if (rc) return rc;
+  }
+  // stage 2: U3* = W + dt A_ex[7] L(S) [+ dt A_im[7] g2] (solvers.hpp:535-566)
+  StageArgs b = stage_args(c);
+  for (int q = 0; q < 6; ++q) {
+    b.s[q] = c->S[q];
+    b.u[q] = c->F1[q];
+    b.o1[q] = c->U[q];
+  }
+  b.c_b = dt * T.A_ex7;
+  b.c_g = dt * T.A_im7;
+  rc = run_stage(c, 2, b, 1);
+  if (rc) return rc;
+  if (c->eta > 0.0) {
+    // implicit 2 and the final recombination (solvers.hpp:571-608)
+    if (rec) {
+      rc = copy_fields(c, rec->U3sB, c->U + 4, 2);
+      if (rc) return rc;
+    }
+    CgState cs;
+    rc = cg_run(c, c->U[4], c->U[5], nullptr, &cs, 1);
+    if (rc) return rc;
+    st.cg_iters[1] = cs.iters;
+    st.cg_spmv[1] = cs.spmv;
+    st.cg_error[1] = cg_error(cs);
+    if (rec) {
+      rec->spmv[1] = cs.spmv;
+      if (cs.spmv > 0) HIP_OR_FAIL(c, hipMemcpyAsync(rec->x2, c->x, (size_t)c->n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    }
+    FinArgs f = fin_args(c, c->U[4], c->U[5], c->S[4], c->S[5], diag_coeff);
+    rc = launch(c, 6, [&] { hipLaunchKernelGGL(k_finalize2, dim3(c->tiles), dim3(kTileX), 0, c->stream, f); });
+    if (rc) return rc;
+    std::swap(c->U[4], c->S[4]);
+    std::swap(c->U[5], c->S[5]);
+  }
+  if (stats) *stats = st;
+  return MHD_OK;
+}
 
 }  // namespace
 
@@ -433,6 +461,7 @@ int mhd_create(int nx, int ny, double dx, double dy, double eta, const mhd_opts*
 int mhd_destroy(mhd_ctx* c) {
   if (!c) return MHD_EINVAL;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  destroy_adjoint(c);
   for (int k = 0; k < kNumKernelIds; ++k) {
     for (auto& pr : c->pending[k]) {
       (void)hipEventDestroy(pr.first);
@@ -544,61 +573,7 @@ int mhd_build_operator(mhd_ctx* c, double alpha) {
 int mhd_imex_step(mhd_ctx* c, double dt, mhd_step_stats* stats) {
   if (check_ctx(c)) return MHD_EINVAL;
   if (!std::isfinite(dt)) return fail(c, MHD_EINVAL, "dt must be finite");
-  mhd_step_stats st{};
-  st.dt = dt;
-  const Tableau& T = c->tab;
-  // stage 1: f1 = L(U), S = U + dt A_ex[3] f1 (solvers.hpp:495-523),
-  // W = U + dt A_ex[6] f1 (solvers.hpp:548-553) into F1
-  StageArgs a = stage_args(c);
-  for (int q = 0; q < 6; ++q) {
-    a.s[q] = c->U[q];
-    a.o1[q] = c->F1[q];
-    a.o2[q] = c->S[q];
-  }
-  a.c_a = dt * T.A_ex3;
-  a.c_b = dt * T.A_ex6;
-  int rc = run_stage(c, 1, a, 0);
-  if (rc) return rc;
-  const double diag_coeff = dt * T.A_im4;
-  if (c->eta > 0.0) {
-    // implicit 1: x = CG(eta J(S.B)), S.B += dt A_im[4] rate(x) (solvers.hpp:526-529)
-    CgState cs;
-    rc = cg_run(c, c->S[4], c->S[5], nullptr, &cs, 0);
-    if (rc) return rc;
-    st.cg_iters[0] = cs.iters;
-    st.cg_spmv[0] = cs.spmv;
-    st.cg_error[0] = cg_error(cs);
-    FinArgs f = fin_args(c, c->S[4], c->S[5], c->S[4], c->S[5], diag_coeff);
-    rc = launch(c, 5, [&] { hipLaunchKernelGGL(k_finalize1, dim3(c->tiles), dim3(kTileX), 0, c->stream, f); });
-    if (rc) return rc;
-  }
-  // stage 2: U3* = W + dt A_ex[7] L(S) [+ dt A_im[7] g2] (solvers.hpp:535-566)
-  StageArgs b = stage_args(c);
-  for (int q = 0; q < 6; ++q) {
-    b.s[q] = c->S[q];
-    b.u[q] = c->F1[q];
-    b.o1[q] = c->U[q];
-  }
-  b.c_b = dt * T.A_ex7;
-  b.c_g = dt * T.A_im7;
-  rc = run_stage(c, 2, b, 1);
-  if (rc) return rc;
-  if (c->eta > 0.0) {
-    // implicit 2 and the final recombination (solvers.hpp:571-608)
-    CgState cs;
-    rc = cg_run(c, c->U[4], c->U[5], nullptr, &cs, 1);
-    if (rc) return rc;
-    st.cg_iters[1] = cs.iters;
-    st.cg_spmv[1] = cs.spmv;
-    st.cg_error[1] = cg_error(cs);
-    FinArgs f = fin_args(c, c->U[4], c->U[5], c->S[4], c->S[5], diag_coeff);
-    rc = launch(c, 6, [&] { hipLaunchKernelGGL(k_finalize2, dim3(c->tiles), dim3(kTileX), 0, c->stream, f); });
-    if (rc) return rc;
-    std::swap(c->U[4], c->S[4]);
-    std::swap(c->U[5], c->S[5]);
-  }
-  if (stats) *stats = st;
-  return MHD_OK;
+  return imex_step(c, dt, stats, nullptr);
 }
 
 int mhd_explicit_rhs(mhd_ctx* c, double* drho, double* dmx, double* dmy, double* dE, double* dbxf, double* dbyf) {

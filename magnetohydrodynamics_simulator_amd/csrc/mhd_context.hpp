@@ -1,0 +1,118 @@
+// mhd_context.hpp -- the single-run context (struct mhd_ctx of the C ABI) and
+// the host helpers shared by the C-ABI source files (mhd_capi.hip,
+// mhd_adjoint.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/mhd_mi355x.h"
+#include "mhd_kernels.hpp"
+
+namespace mhd_host {
+
+constexpr int kNumKernelIds = 9;
+
+struct Tableau {
+  // IMEX_Butcher_Tableus, solvers.hpp:39-66
+  double gamma = (2.0 - std::sqrt(2.0)) / 2.0;
+  double delta = 1.0 - (1.0 / (2.0 * gamma));
+  double A_ex3 = gamma, A_ex6 = delta, A_ex7 = 1.0 - delta;
+  double A_im4 = gamma, A_im7 = 1.0 - gamma;
+  double b_im2 = gamma;
+};
+
+struct AdjointStore;   // checkpoints and scratch of the adjoint (mhd_adjoint.hip)
+
+// Device buffers a step writes besides the state, for the adjoint's recompute:
+// the state at the start of the step, the stage state after implicit 1, the
+// face B before each implicit solve and the two solutions.
+struct StepRecord {
+  double* Un[6];
+  double* S[6];
+  double* SsB[2];
+  double* U3sB[2];
+  double* x1;
+  double* x2;
+  int spmv[2];
+};
+
+}  // namespace mhd_host
+
+struct mhd_ctx {
+  int nx = 0, ny = 0;
+  long long n = 0;
+  double dx = 0, dy = 0, eta = 0;
+  mhd_opts o{};
+  mhd_host::Tableau tab;
+  hipStream_t stream = nullptr;
+  double* U[6] = {};
+  double* S[6] = {};
+  double* F1[6] = {};
+  double* x = nullptr;
+  double* r = nullptr;
+  double* p[2] = {};
+  double* part_s = nullptr;      // CG start kernel: three partials per tile
+  double* part_a = nullptr;      // CG pass A: one partial per tile
+  double* part_b = nullptr;      // CG pass B: two partials per tile
+  double* part = nullptr;        // diagnostics: two partials per tile
+  mhd::CgState* st = nullptr;
+  mhd::CgState* st_host = nullptr;    // pinned
+  double* part_host = nullptr;   // pinned, two partials per tile
+  double alpha = 0.0;
+  bool have_op = false;
+  int last_spmv[2] = {1, 1};     // products of the previous solve in each of the step's two slots
+  int tiles = 1, tiles_y = 1, tile_rows = 4;   // 2D sweeps: kTileX columns x tile_rows rows per tile
+  int stage_rows = 16;
+  std::string err;
+  unsigned prof_mask = 0;        // kernel ids whose launches are bracketed by events
+  long long prof_n[mhd_host::kNumKernelIds] = {};
+  double prof_ms[mhd_host::kNumKernelIds] = {};
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> pending[mhd_host::kNumKernelIds];
+  mhd_host::AdjointStore* adj = nullptr;
+};
+
+namespace mhd_host {
+
+int fail(mhd_ctx* c, int code, const std::string& msg);
+
+#define HIP_OR_FAIL(ctx, call)                                                                  \
+  do {                                                                                          \
+    hipError_t e_ = (call);                                                                     \
+    if (e_ != hipSuccess)                                                                       \
+      return ::mhd_host::fail((ctx), MHD_EHIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+// Launch helper: optional event bracketing for per-kernel timing.
+template <typename F>
+int launch(mhd_ctx* c, int id, F&& body) {
+  hipEvent_t a = nullptr, b = nullptr;
+  const bool prof = (c->prof_mask >> id) & 1u;
+  if (prof) {
+    HIP_OR_FAIL(c, hipEventCreate(&a));
+    HIP_OR_FAIL(c, hipEventCreate(&b));
+    HIP_OR_FAIL(c, hipEventRecord(a, c->stream));
+  }
+  body();
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(c, MHD_EHIP, std::string("kernel launch: ") + hipGetErrorString(e));
+  if (prof) {
+    HIP_OR_FAIL(c, hipEventRecord(b, c->stream));
+    c->pending[id].emplace_back(a, b);
+  }
+  return MHD_OK;
+}
+
+int grid_1d(long long n);
+int cg_run(mhd_ctx* c, const double* bx, const double* by, const double* bin, mhd::CgState* out, int slot);
+int to_host_padded(mhd_ctx* c, const double* dev, double* host, bool ghosts);
+int from_host_padded(mhd_ctx* c, const double* host, double* dev);
+// One IMEX step (mhd_imex_step); rec, if not null, receives the step's data.
+int imex_step(mhd_ctx* c, double dt, mhd_step_stats* stats, StepRecord* rec);
+void destroy_adjoint(mhd_ctx* c);
+
+}  // namespace mhd_host

@@ -191,6 +191,39 @@ class Context:
         n = steps.value
         return n, dts[:n].copy(), its[: 2 * n].reshape(n, 2).copy(), t.value
 
+    # ---- adjoint ----
+    def _field_ptrs(self, arrs):
+        keep = [np.ascontiguousarray(a, dtype=np.float64) for a in arrs]
+        for a in keep:
+            if a.shape != (self.nyt, self.nxt):
+                raise ValueError(f"expected padded shape {(self.nyt, self.nxt)}, got {a.shape}")
+        return keep, (_dp * 6)(*[_ptr(a) for a in keep])
+
+    def explicit_rhs_vjp(self, lam):
+        """L'(U)^T lam for the explicit right-hand side at the current state;
+        lam and the result are six padded arrays (rho, mx, my, E, bxf, byf)."""
+        keep, lp = self._field_ptrs(lam)
+        outs = [self._padded() for _ in range(6)]
+        op = (_dp * 6)(*[_ptr(o) for o in outs])
+        self._check(self.lib.mhd_explicit_rhs_vjp(self.h, lp, op))
+        return outs
+
+    def adjoint_forward(self, dt: float, nsteps: int, checkpoint_every: int = 10):
+        """nsteps IMEX steps of constant dt from the current state, keeping
+        checkpoints for adjoint_backward."""
+        self._check(self.lib.mhd_adjoint_forward(self.h, dt, nsteps, checkpoint_every))
+
+    def adjoint_backward(self, lam_final: dict):
+        """Adjoint of the last adjoint_forward: given dJ/dU(final) (dict of six
+        padded arrays), returns (dJ/dU(initial) as such a dict, dJ/d eta)."""
+        names = ("rho", "mx", "my", "E", "bxf", "byf")
+        keep, lp = self._field_ptrs([lam_final[k] for k in names])
+        outs = {k: self._padded() for k in names}
+        op = (_dp * 6)(*[_ptr(outs[k]) for k in names])
+        eb = C.c_double()
+        self._check(self.lib.mhd_adjoint_backward(self.h, lp, op, C.byref(eb)))
+        return outs, eb.value
+
     # ---- instrumentation ----
     def profile(self, kernels=None):
         """Bracket launches of the named kernels (all if True, none if

@@ -1,0 +1,99 @@
+"""GPU checks of the discrete adjoint (mhd_adjoint_* and
+mhd_explicit_rhs_vjp of the C ABI) by dot-product tests against central
+finite differences of the GPU forward path on the same inputs:
+  <L'(U)^T lam, v>       vs  <lam, (L(U + e v) - L(U - e v)) / 2e>
+  <lam0, v>              vs  <lamN, (F(U0 + e v) - F(U0 - e v)) / 2e>
+  eta_bar                vs  (J(eta + h) - J(eta - h)) / 2h
+with F = N IMEX steps of constant dt and J = <lamN, U_N>.  States are smooth
+(no limiter kink lies within e of any cell); the CG tolerance is the
+reference default (DBL_EPSILON) so that the solves do not add noise to the
+differences.  Tolerance: 1e-6 relative (the differentiation bar)."""
+import numpy as np
+import pytest
+
+from helpers import CONSERVED, interior
+from magnetohydrodynamics_simulator_amd import Context, setups
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-6
+
+
+def _say(msg):
+    print(f"\n[case] {msg}", flush=True)
+
+
+def _state(n, seed):
+    return setups.ensemble_state(n, seed)
+
+
+def _rand_fields(n, seed, scale=1.0):
+    rng = np.random.default_rng(seed)
+    return [scale * rng.standard_normal((n + 4, n + 4)) for _ in range(6)]
+
+
+def _dot(a, b):
+    return float(sum(np.sum(interior(x) * interior(y)) for x, y in zip(a, b)))
+
+
+def _ctx(ic, n, eta):
+    dx = setups.grid_spacing(n)
+    c = Context(n, n, dx, dx, eta)            # default CG tolerance: DBL_EPSILON
+    c.upload(*[ic[k] for k in CONSERVED])
+    return c
+
+
+def test_explicit_rhs_vjp_dot_product():
+    _say("explicit_rhs_vjp dot product, 40x40 ensemble member")
+    n = 40
+    ic = _state(n, 7)
+    lam = _rand_fields(n, 1)
+    v = _rand_fields(n, 2, 1e-3)
+    with _ctx(ic, n, 0.0) as c:
+        adj = c.explicit_rhs_vjp(lam)
+    eps = 1e-4
+    outs = []
+    for sgn in (1, -1):
+        st = {k: ic[k] + sgn * eps * v[q] for q, k in enumerate(CONSERVED)}
+        with _ctx(st, n, 0.0) as c:
+            outs.append(c.explicit_rhs())
+    jv = [(a - b) / (2 * eps) for a, b in zip(*outs)]
+    lhs, rhs = _dot(adj, v), _dot(lam, jv)
+    assert abs(lhs - rhs) <= TOL * abs(rhs), (lhs, rhs)
+
+
+def _run(ic, n, eta, dt, steps):
+    with _ctx(ic, n, eta) as c:
+        c.run(1e9, dt_override=dt, max_steps=steps)
+        return c.download()
+
+
+@pytest.mark.parametrize("steps,every", [(5, 2), (4, 10)])
+def test_adjoint_run_dot_product(steps, every):
+    _say(f"adjoint of {steps} steps, checkpoints every {every}, 32x32, eta 2e-2")
+    n, eta, dt = 32, 2e-2, 4e-3
+    ic = _state(n, 11)
+    lamN = dict(zip(CONSERVED, _rand_fields(n, 3)))
+    v = _rand_fields(n, 4, 1e-3)
+    with _ctx(ic, n, eta) as c:
+        c.adjoint_forward(dt, steps, every)
+        fin = c.download()
+        lam0, eta_bar = c.adjoint_backward(lamN)
+    ref = _run(ic, n, eta, dt, steps)
+    for k in CONSERVED:       # the recorded forward is the plain forward
+        assert np.array_equal(interior(fin[k]), interior(ref[k])), k
+    eps = 1e-4
+    outs = []
+    for sgn in (1, -1):
+        st = {k: ic[k] + sgn * eps * v[q] for q, k in enumerate(CONSERVED)}
+        outs.append(_run(st, n, eta, dt, steps))
+    jv = [(outs[0][k] - outs[1][k]) / (2 * eps) for k in CONSERVED]
+    lhs = _dot([lam0[k] for k in CONSERVED], v)
+    rhs = _dot([lamN[k] for k in CONSERVED], jv)
+    assert abs(lhs - rhs) <= TOL * abs(rhs), (lhs, rhs)
+    # eta derivative of J = <lamN, U_N>
+    h = 1e-4 * eta
+    jp = _dot([lamN[k] for k in CONSERVED], [_run(ic, n, eta + h, dt, steps)[k] for k in CONSERVED])
+    jm = _dot([lamN[k] for k in CONSERVED], [_run(ic, n, eta - h, dt, steps)[k] for k in CONSERVED])
+    fd = (jp - jm) / (2 * h)
+    assert abs(eta_bar - fd) <= 1e-5 * abs(fd), (eta_bar, fd)
