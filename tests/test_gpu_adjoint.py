@@ -24,7 +24,20 @@ def _say(msg):
 
 
 def _state(n, seed):
-    return setups.ensemble_state(n, seed)
+    """Fourier-perturbed member with smooth non-uniform density and pressure:
+    a uniform field puts every minmod of that variable at its kink
+    (both slopes zero), where central differences do not measure a
+    derivative."""
+    ic = dict(setups.ensemble_state(n, seed))
+    dx = setups.grid_spacing(n)
+    X, Y = np.meshgrid((np.arange(n + 4) - 1.5) * dx, (np.arange(n + 4) - 1.5) * dx)
+    drho = 0.15 * np.sin(X + 2 * Y + 0.4) + 0.1 * np.cos(3 * X - Y + 1.1)
+    dpr = 0.2 * np.cos(2 * X + Y + 0.3) + 0.05 * np.sin(X - 3 * Y + 2.0)
+    ic["rho"] = ic["rho"] * (1 + drho)
+    ic["mx"] = ic["mx"] * (1 + drho)
+    ic["my"] = ic["my"] * (1 + drho)
+    ic["E"] = ic["E"] + dpr / (5.0 / 3.0 - 1.0)
+    return ic
 
 
 def _rand_fields(n, seed, scale=1.0):
