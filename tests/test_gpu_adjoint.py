@@ -7,7 +7,11 @@ finite differences of the GPU forward path on the same inputs:
 with F = N IMEX steps of constant dt and J = <lamN, U_N>.  States are smooth
 (no limiter kink lies within e of any cell); the CG tolerance is the
 reference default (DBL_EPSILON) so that the solves do not add noise to the
-differences.  Tolerance: 1e-6 relative (the differentiation bar)."""
+differences.  Tolerance: 1e-6 relative (the differentiation bar).  Multi-step runs use
+smooth (low-mode) directions: with grid-scale random directions the
+central differences straddle minmod kinks somewhere in the run and the
+comparison measures that noise (about 1e-4 at 4 steps) rather than the
+adjoint, which agrees to 1e-9 per step."""
 import numpy as np
 import pytest
 
@@ -43,6 +47,23 @@ def _state(n, seed):
 def _rand_fields(n, seed, scale=1.0):
     rng = np.random.default_rng(seed)
     return [scale * rng.standard_normal((n + 4, n + 4)) for _ in range(6)]
+
+
+def _smooth_fields(n, seed, scale=1.0):
+    """Low-mode (|k| <= 3) periodic fields: a perturbation of this shape moves
+    the minmod slope differences by O(dx^2) of its size, so the central
+    differences of a multi-step run rarely straddle a limiter kink."""
+    rng = np.random.default_rng(seed)
+    dx = setups.grid_spacing(n)
+    X, Y = np.meshgrid((np.arange(n + 4) - 2) * dx, (np.arange(n + 4) - 2) * dx)
+    out = []
+    for _ in range(6):
+        f = np.zeros_like(X)
+        for _ in range(4):
+            kx, ky = rng.integers(-3, 4, size=2)
+            f += rng.standard_normal() * np.sin(kx * X + ky * Y + rng.uniform(0, 2 * np.pi))
+        out.append(scale * f)
+    return out
 
 
 def _dot(a, b):
@@ -81,13 +102,13 @@ def _run(ic, n, eta, dt, steps):
         return c.download()
 
 
-@pytest.mark.parametrize("steps,every", [(5, 2), (4, 10)])
+@pytest.mark.parametrize("steps,every", [(1, 1), (5, 2), (4, 10)])
 def test_adjoint_run_dot_product(steps, every):
     _say(f"adjoint of {steps} steps, checkpoints every {every}, 32x32, eta 2e-2")
     n, eta, dt = 32, 2e-2, 4e-3
     ic = _state(n, 11)
-    lamN = dict(zip(CONSERVED, _rand_fields(n, 3)))
-    v = _rand_fields(n, 4, 1e-3)
+    lamN = dict(zip(CONSERVED, _smooth_fields(n, 3)))
+    v = _smooth_fields(n, 4, 1e-3)
     with _ctx(ic, n, eta) as c:
         c.adjoint_forward(dt, steps, every)
         fin = c.download()
