@@ -82,6 +82,10 @@ def parse():
     ap.add_argument("--batch-runs", type=int, default=256, help="ensemble size of the batched workload (0: skip)")
     ap.add_argument("--batch-steps", type=int, default=200)
     ap.add_argument("--plain-steps", type=int, default=10, help="steps per run of the plain-path comparison")
+    ap.add_argument("--adjoint-n", type=int, default=512, help="grid side of the differentiation workload (0: skip)")
+    ap.add_argument("--adjoint-steps", type=int, default=50)
+    ap.add_argument("--adjoint-modes", type=int, default=64)
+    ap.add_argument("--fd-modes", type=int, default=64, help="modes checked against central differences")
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     return ap.parse_args()
 
@@ -245,6 +249,78 @@ def bench_batch(a) -> dict:
             "cg_iters_last_step_mean": round(float(its.mean()), 2)}
 
 
+def bench_adjoint(a) -> dict:
+    """Kink column on [0, 1]^2 (axial By core of radius 0.2, p0 = 0.1,
+    B0 = 5, transverse displacement from seeded Fourier modes of amplitude
+    1e-3), eta = 1e-3, frozen dt, loss = magnetic energy of the central
+    window: gradient with respect to the mode amplitudes and eta through the
+    adjoint (checkpoints every 10 steps), its cost against the plain forward
+    run, and central differences (h = 1e-6) of the forward path."""
+    from magnetohydrodynamics_simulator_amd import Context, setups
+
+    n, steps, eta = a.adjoint_n, a.adjoint_steps, 1e-3
+    names = ("rho", "mx", "my", "E", "bxf", "byf")
+    amps, phases = setups.kink_column_modes(a.adjoint_modes, 1e-3, seed=0)
+    st = setups.kink_column(n, amps, phases)
+    dx = 1.0 / n
+
+    def loss_of(amps_, eta_, dt_):
+        s_ = setups.kink_column(n, amps_, phases)
+        c_.upload(*[s_[k] for k in names])
+        c_.run(1e9, dt_override=dt_, max_steps=steps)
+        return setups.window_magnetic_energy(c_.download(), n, dx)[0]
+
+    with Context(n, n, dx, dx, eta, cfl=0.4) as c:
+        c.upload(*[st[k] for k in names])
+        dt = 0.8 * c.CFL_condition_checker()
+        # plain forward (warm, then timed)
+        c.run(1e9, dt_override=dt, max_steps=steps)
+        c.upload(*[st[k] for k in names])
+        c.synchronize()
+        t0 = time.perf_counter()
+        c.run(1e9, dt_override=dt, max_steps=steps)
+        c.synchronize()
+        t_fwd = time.perf_counter() - t0
+        # adjoint: forward with checkpoints, loss on the host, backward
+        c.upload(*[st[k] for k in names])
+        c.adjoint_forward(dt, steps, 10)            # warm-up of the adjoint buffers
+        c.upload(*[st[k] for k in names])
+        c.synchronize()
+        t0 = time.perf_counter()
+        c.adjoint_forward(dt, steps, 10)
+        t1 = time.perf_counter()
+        loss, lam = setups.window_magnetic_energy(c.download(), n, dx)
+        t2 = time.perf_counter()
+        lam0, eta_bar = c.adjoint_backward(lam)
+        t3 = time.perf_counter()
+    t_ckpt, t_bwd = t1 - t0, t3 - t2
+    drho, dE = setups.kink_column_dstate(n, amps, phases)
+    grad = np.array([np.sum(lam0["rho"][2:-2, 2:-2] * drho[k][2:-2, 2:-2]) + np.sum(lam0["E"][2:-2, 2:-2] * dE[k][2:-2, 2:-2])
+                     for k in range(len(amps))])
+    # central differences of the forward path
+    h = 1e-6
+    nfd = min(a.fd_modes, len(amps))
+    fd = np.zeros(nfd)
+    with Context(n, n, dx, dx, eta) as c_:
+        for k in range(nfd):
+            e = np.zeros_like(amps)
+            e[k] = h
+            fd[k] = (loss_of(amps + e, eta, dt) - loss_of(amps - e, eta, dt)) / (2 * h)
+        he = 1e-6
+        fde = (loss_of(amps, eta + he, dt) - loss_of(amps, eta - he, dt)) / (2 * he)
+    rel = np.abs(grad[:nfd] - fd) / np.maximum(np.abs(fd), 1e-300)
+    return {"workload": f"kink column {n}x{n} on [0,1]^2, {len(amps)} displacement modes (amplitude 1e-3), eta {eta}, "
+                        f"{steps} steps of frozen dt {dt:.4e}, loss = window magnetic energy, checkpoints every 10",
+            "loss": loss, "forward_ms": round(t_fwd * 1e3, 3),
+            "adjoint_forward_ms": round(t_ckpt * 1e3, 3), "backward_ms": round(t_bwd * 1e3, 3),
+            "backward_over_forward": round(t_bwd / t_fwd, 3),
+            "gradient_cost_over_forward": round((t_ckpt + t_bwd) / t_fwd, 3),
+            "fd_h": h, "fd_modes_checked": int(nfd),
+            "grad_rel_err_norm": float(np.linalg.norm(grad[:nfd] - fd) / np.linalg.norm(fd)),
+            "grad_rel_err_max": float(rel.max()), "grad_rel_err_median": float(np.median(rel)),
+            "eta_bar": eta_bar, "eta_fd": fde, "eta_rel_err": abs(eta_bar - fde) / abs(fde)}
+
+
 def main():
     a = parse()
     if a.pmc_child:
@@ -340,6 +416,8 @@ def main():
                 "source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, 2 driver steps, scaled by a 1 GiB copy"}
     if a.batch_runs > 0:
         line["batched"] = bench_batch(a)
+    if a.adjoint_n > 0:
+        line["adjoint"] = bench_adjoint(a)
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baseline(n, a.eta, a.cg_tol, a.cfl, a.cpu_steps)
     print(json.dumps(line), flush=True)

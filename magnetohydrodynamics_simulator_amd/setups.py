@@ -181,6 +181,84 @@ def fourier_ensemble_member(n: int, rng: np.random.Generator, modes: int = 8, am
     return _from_primitive(1.0, vx, vy, 1.0, bx, by)
 
 
+# ---------------------------------------------------------------------------
+# Kink column on the unit square (the differentiation workload).  An axial
+# field By = B0 G(x - 1/2) (bx = 0, so div B = 0 exactly) threads a dense core;
+# density and pressure are displaced transversely by
+#   xi(y) = sum_k a_k sin(2 pi k y + phi_k),   k = 1 .. modes,
+# with seeded phases, which breaks the total-pressure balance
+# p + By^2/2 = p0 + B0^2/2 of the undisplaced column.  G is the smooth
+# periodic core profile exp((cos(2 pi s) - 1) / (2 pi^2 R^2)) ~ exp(-s^2/R^2).
+# The state is analytic in the amplitudes; kink_column_dstate gives
+# d(state)/d(a_k) for the gradient with respect to them.
+# ---------------------------------------------------------------------------
+
+KINK_R, KINK_B0, KINK_P0, KINK_RHO_BG = 0.2, 5.0, 0.1, 0.2
+
+
+def _kink_profile(s):
+    g = np.exp((np.cos(2 * np.pi * s) - 1.0) / (2 * np.pi ** 2 * KINK_R ** 2))
+    dg = -g * np.sin(2 * np.pi * s) / (np.pi * KINK_R ** 2)
+    return g, dg
+
+
+def kink_column_modes(modes: int = 64, amp: float = 1e-3, seed: int = 0):
+    """Nominal amplitudes (all amp) and seeded phases of the displacement."""
+    rng = np.random.default_rng(seed)
+    return np.full(modes, amp), rng.uniform(0, 2 * np.pi, size=modes)
+
+
+def _kink_xi(n, amps, phases):
+    dx = 1.0 / n
+    yc = (np.arange(n + 4) - NG + 0.5) * dx
+    k = np.arange(1, len(amps) + 1)
+    sines = np.sin(2 * np.pi * np.outer(yc, k) + phases)        # (ny+4, modes)
+    return sines @ amps, sines
+
+
+def kink_column(n: int, amps, phases):
+    """Padded state of the kink column on [0, 1]^2 (dx = dy = 1/n)."""
+    dx = 1.0 / n
+    xi, _ = _kink_xi(n, np.asarray(amps, dtype=np.float64), phases)
+    xc = (np.arange(n + 4) - NG + 0.5) * dx
+    X = xc[None, :]
+    g, _ = _kink_profile(X - 0.5 - xi[:, None])
+    gb, _ = _kink_profile(X - 0.5 + 0.0 * xi[:, None])
+    rho = KINK_RHO_BG + g
+    p = KINK_P0 + 0.5 * KINK_B0 ** 2 * (1.0 - g * g)
+    by = KINK_B0 * gb
+    z = np.zeros_like(rho)
+    return _from_primitive(rho, z, z, p, z, by)
+
+
+def kink_column_dstate(n: int, amps, phases):
+    """d(state)/d(a_k) for every mode: arrays (modes, ny+4, nx+4) for rho and E
+    (momenta and B do not depend on the amplitudes)."""
+    dx = 1.0 / n
+    xi, sines = _kink_xi(n, np.asarray(amps, dtype=np.float64), phases)
+    xc = (np.arange(n + 4) - NG + 0.5) * dx
+    g, dg = _kink_profile(xc[None, :] - 0.5 - xi[:, None])
+    drho_dxi = -dg                                            # s = x - 1/2 - xi
+    dE_dxi = (0.5 * KINK_B0 ** 2 * 2.0 * g * dg) / (GAMMA - 1.0)
+    drho = sines.T[:, :, None] * drho_dxi[None, :, :]
+    dE = sines.T[:, :, None] * dE_dxi[None, :, :]
+    return drho, dE
+
+
+def window_magnetic_energy(state: dict, n: int, dx: float, lo: float = 0.25, hi: float = 0.75):
+    """Loss of the differentiation workload: (1/2) sum (bxf^2 + byf^2) dx dy
+    over the faces of the central window [lo, hi)^2, and its gradient
+    with respect to the padded state."""
+    i0, i1 = int(round(lo * n)), int(round(hi * n))
+    grad = {k: np.zeros_like(state["rho"]) for k in ("rho", "mx", "my", "E", "bxf", "byf")}
+    loss = 0.0
+    for k in ("bxf", "byf"):
+        w = state[k][NG + i0:NG + i1, NG + i0:NG + i1]
+        loss += 0.5 * float(np.sum(w * w)) * dx * dx
+        grad[k][NG + i0:NG + i1, NG + i0:NG + i1] = w * dx * dx
+    return loss, grad
+
+
 ENSEMBLE_SIDES = (64, 96, 128, 192, 256, 384, 512)
 
 

@@ -131,3 +131,47 @@ def test_adjoint_run_dot_product(steps, every):
     jm = _dot([lamN[k] for k in CONSERVED], [_run(ic, n, eta - h, dt, steps)[k] for k in CONSERVED])
     fd = (jp - jm) / (2 * h)
     assert abs(eta_bar - fd) <= 1e-5 * abs(fd), (eta_bar, fd)
+
+
+def _kink_loss(n, amps, phases, eta, dt, steps):
+    st = setups.kink_column(n, amps, phases)
+    with Context(n, n, 1.0 / n, 1.0 / n, eta) as c:
+        c.upload(*[st[k] for k in CONSERVED])
+        c.run(1e9, dt_override=dt, max_steps=steps)
+        fin = c.download()
+    return setups.window_magnetic_energy(fin, n, 1.0 / n)[0]
+
+
+def test_kink_column_gradient_matches_central_differences():
+    """Reduced differentiation workload: kink column 64x64, 16 displacement
+    modes, eta 1e-3, 12 steps of frozen dt, checkpoints every 5; the loss
+    is the magnetic energy of the central window.  Gradient with respect to
+    the 16 amplitudes (lam0 contracted with the analytic d(state)/d(a_k)) and
+    eta against central differences of the GPU forward."""
+    n, eta, steps = 64, 1e-3, 12
+    _say(f"kink column {n}^2 gradient, 16 modes, {steps} steps")
+    amps, phases = setups.kink_column_modes(16, 1e-3, seed=0)
+    st = setups.kink_column(n, amps, phases)
+    with Context(n, n, 1.0 / n, 1.0 / n, eta, cfl=0.4) as c:
+        c.upload(*[st[k] for k in CONSERVED])
+        dt = 0.8 * c.CFL_condition_checker()
+        c.adjoint_forward(dt, steps, 5)
+        fin = c.download()
+        loss, lam = setups.window_magnetic_energy(fin, n, 1.0 / n)
+        lam0, eta_bar = c.adjoint_backward(lam)
+    drho, dE = setups.kink_column_dstate(n, amps, phases)
+    grad = np.array([np.sum(interior(lam0["rho"]) * interior(drho[k])) + np.sum(interior(lam0["E"]) * interior(dE[k]))
+                     for k in range(len(amps))])
+    assert abs(loss - _kink_loss(n, amps, phases, eta, dt, steps)) <= 1e-13 * loss
+    h = 1e-5
+    fd = np.zeros_like(grad)
+    for k in range(len(amps)):
+        e = np.zeros_like(amps)
+        e[k] = h
+        fd[k] = (_kink_loss(n, amps + e, phases, eta, dt, steps) - _kink_loss(n, amps - e, phases, eta, dt, steps)) / (2 * h)
+    err = np.linalg.norm(grad - fd) / np.linalg.norm(fd)
+    print(f"gradient norm {np.linalg.norm(fd):.6e}, relative error {err:.3e}", flush=True)
+    assert err <= 1e-6, (grad, fd)
+    he = 1e-3 * eta
+    fde = (_kink_loss(n, amps, phases, eta + he, dt, steps) - _kink_loss(n, amps, phases, eta - he, dt, steps)) / (2 * he)
+    assert abs(eta_bar - fde) <= 1e-6 * abs(fde) + 1e-12, (eta_bar, fde)
