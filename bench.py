@@ -264,7 +264,7 @@ def bench_adjoint(a) -> dict:
     st = setups.kink_column(n, amps, phases)
     dx = 1.0 / n
 
-    def loss_of(amps_, eta_, dt_):
+    def loss_of(c_, amps_, dt_):
         s_ = setups.kink_column(n, amps_, phases)
         c_.upload(*[s_[k] for k in names])
         c_.run(1e9, dt_override=dt_, max_steps=steps)
@@ -294,9 +294,7 @@ def bench_adjoint(a) -> dict:
         lam0, eta_bar = c.adjoint_backward(lam)
         t3 = time.perf_counter()
     t_ckpt, t_bwd = t1 - t0, t3 - t2
-    drho, dE = setups.kink_column_dstate(n, amps, phases)
-    grad = np.array([np.sum(lam0["rho"][2:-2, 2:-2] * drho[k][2:-2, 2:-2]) + np.sum(lam0["E"][2:-2, 2:-2] * dE[k][2:-2, 2:-2])
-                     for k in range(len(amps))])
+    grad = setups.kink_column_gradient(lam0, setups.kink_column_dstate(n, amps, phases))
     # central differences of the forward path
     h = 1e-6
     nfd = min(a.fd_modes, len(amps))
@@ -305,9 +303,10 @@ def bench_adjoint(a) -> dict:
         for k in range(nfd):
             e = np.zeros_like(amps)
             e[k] = h
-            fd[k] = (loss_of(amps + e, eta, dt) - loss_of(amps - e, eta, dt)) / (2 * h)
-        he = 1e-6
-        fde = (loss_of(amps, eta + he, dt) - loss_of(amps, eta - he, dt)) / (2 * he)
+            fd[k] = (loss_of(c_, amps + e, dt) - loss_of(c_, amps - e, dt)) / (2 * h)
+    he = 1e-6
+    with Context(n, n, dx, dx, eta + he) as cp, Context(n, n, dx, dx, eta - he) as cm:
+        fde = (loss_of(cp, amps, dt) - loss_of(cm, amps, dt)) / (2 * he)
     rel = np.abs(grad[:nfd] - fd) / np.maximum(np.abs(fd), 1e-300)
     return {"workload": f"kink column {n}x{n} on [0,1]^2, {len(amps)} displacement modes (amplitude 1e-3), eta {eta}, "
                         f"{steps} steps of frozen dt {dt:.4e}, loss = window magnetic energy, checkpoints every 10",

@@ -183,12 +183,17 @@ def fourier_ensemble_member(n: int, rng: np.random.Generator, modes: int = 8, am
 
 # ---------------------------------------------------------------------------
 # Kink column on the unit square (the differentiation workload).  An axial
-# field By = B0 G(x - 1/2) (bx = 0, so div B = 0 exactly) threads a dense core;
-# density and pressure are displaced transversely by
+# field By = B0 G(x - 1/2) threads a dense core; density and pressure are
+# displaced transversely by
 #   xi(y) = sum_k a_k sin(2 pi k y + phi_k),   k = 1 .. modes,
 # with seeded phases, which breaks the total-pressure balance
 # p + By^2/2 = p0 + B0^2/2 of the undisplaced column.  G is the smooth
 # periodic core profile exp((cos(2 pi s) - 1) / (2 pi^2 R^2)) ~ exp(-s^2/R^2).
+# A weak smooth background -- flow of amplitude 0.05, a field from a corner
+# potential of amplitude 0.02 (div B = 0 to round-off), 5 % density and
+# pressure modulation -- keeps every reconstructed field non-uniform along
+# both grid directions: with uniform fields every minmod of that field sits
+# at its kink and the loss is not differentiable in the amplitudes.
 # The state is analytic in the amplitudes; kink_column_dstate gives
 # d(state)/d(a_k) for the gradient with respect to them.
 # ---------------------------------------------------------------------------
@@ -216,33 +221,60 @@ def _kink_xi(n, amps, phases):
     return sines @ amps, sines
 
 
-def kink_column(n: int, amps, phases):
-    """Padded state of the kink column on [0, 1]^2 (dx = dy = 1/n)."""
+def _kink_background(n):
     dx = 1.0 / n
-    xi, _ = _kink_xi(n, np.asarray(amps, dtype=np.float64), phases)
-    xc = (np.arange(n + 4) - NG + 0.5) * dx
-    X = xc[None, :]
-    g, _ = _kink_profile(X - 0.5 - xi[:, None])
-    gb, _ = _kink_profile(X - 0.5 + 0.0 * xi[:, None])
-    rho = KINK_RHO_BG + g
-    p = KINK_P0 + 0.5 * KINK_B0 ** 2 * (1.0 - g * g)
-    by = KINK_B0 * gb
-    z = np.zeros_like(rho)
-    return _from_primitive(rho, z, z, p, z, by)
+    X, Y = _centres(n, n, dx, dx)
+    XC, YC = _corners(n, n, dx, dx)
+    tp = 2 * np.pi
+    vx = 0.05 * np.sin(tp * Y + 0.3) + 0.03 * np.cos(2 * tp * Y + tp * X + 1.0)
+    vy = 0.05 * np.sin(tp * X + 0.7) + 0.02 * np.cos(tp * (X - Y) + 0.4)
+    mod_rho = 1.0 + 0.05 * np.sin(tp * (X + 2 * Y) + 0.2)
+    mod_p = 1.0 + 0.05 * np.cos(tp * (2 * X - Y) + 1.3)
+    Az = 0.02 * np.sin(tp * XC + 0.5) * np.sin(tp * YC + 0.9) / tp
+    bx, by = _face_b_from_potential(Az, dx, dx)
+    return vx, vy, mod_rho, mod_p, bx, by
 
 
-def kink_column_dstate(n: int, amps, phases):
-    """d(state)/d(a_k) for every mode: arrays (modes, ny+4, nx+4) for rho and E
-    (momenta and B do not depend on the amplitudes)."""
+def _kink_fields(n, amps, phases):
     dx = 1.0 / n
     xi, sines = _kink_xi(n, np.asarray(amps, dtype=np.float64), phases)
     xc = (np.arange(n + 4) - NG + 0.5) * dx
-    g, dg = _kink_profile(xc[None, :] - 0.5 - xi[:, None])
-    drho_dxi = -dg                                            # s = x - 1/2 - xi
-    dE_dxi = (0.5 * KINK_B0 ** 2 * 2.0 * g * dg) / (GAMMA - 1.0)
-    drho = sines.T[:, :, None] * drho_dxi[None, :, :]
-    dE = sines.T[:, :, None] * dE_dxi[None, :, :]
-    return drho, dE
+    X = xc[None, :]
+    g, dg = _kink_profile(X - 0.5 - xi[:, None])
+    gb, _ = _kink_profile(X - 0.5 + 0.0 * xi[:, None])
+    vx, vy, mr, mp, bxb, byb = _kink_background(n)
+    rho = (KINK_RHO_BG + g) * mr
+    p = (KINK_P0 + 0.5 * KINK_B0 ** 2 * (1.0 - g * g)) * mp
+    bx = bxb
+    by = KINK_B0 * gb + byb
+    return rho, vx, vy, p, bx, by, g, dg, mr, mp, sines
+
+
+def kink_column(n: int, amps, phases):
+    """Padded state of the kink column on [0, 1]^2 (dx = dy = 1/n)."""
+    rho, vx, vy, p, bx, by, *_ = _kink_fields(n, amps, phases)
+    return _from_primitive(rho, vx, vy, p, bx, by)
+
+
+def kink_column_dstate(n: int, amps, phases):
+    """d(state)/d(a_k) for every mode: dict of arrays (modes, ny+4, nx+4) for
+    rho, mx, my and E (the face field does not depend on the amplitudes)."""
+    rho, vx, vy, p, bx, by, g, dg, mr, mp, sines = _kink_fields(n, amps, phases)
+    drho_dxi = -dg * mr                                       # s = x - 1/2 - xi
+    dp_dxi = 0.5 * KINK_B0 ** 2 * 2.0 * g * dg * mp
+    dE_dxi = dp_dxi / (GAMMA - 1.0) + 0.5 * (vx * vx + vy * vy) * drho_dxi
+    S = sines.T[:, :, None]
+    return {"rho": S * drho_dxi[None], "mx": S * (vx * drho_dxi)[None], "my": S * (vy * drho_dxi)[None],
+            "E": S * dE_dxi[None]}
+
+
+def kink_column_gradient(lam0: dict, dstate: dict) -> np.ndarray:
+    """dJ/d(a_k) = sum over cells of lam0 . d(state)/d(a_k) (physical cells)."""
+    m = dstate["rho"].shape[0]
+    g = np.zeros(m)
+    for k in ("rho", "mx", "my", "E"):
+        g += np.einsum("kij,ij->k", dstate[k][:, NG:-NG, NG:-NG], lam0[k][NG:-NG, NG:-NG])
+    return g
 
 
 def window_magnetic_energy(state: dict, n: int, dx: float, lo: float = 0.25, hi: float = 0.75):
