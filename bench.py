@@ -295,29 +295,35 @@ def bench_adjoint(a) -> dict:
         t3 = time.perf_counter()
     t_ckpt, t_bwd = t1 - t0, t3 - t2
     grad = setups.kink_column_gradient(lam0, setups.kink_column_dstate(n, amps, phases))
-    # central differences of the forward path
-    h = 1e-6
+    # central differences of the forward path at three steps: the upwind E_z
+    # selection of the CT step switches discontinuously where a corner
+    # velocity changes sign, so a large step crosses jumps over a long run,
+    # while a small one carries the differences' rounding (about 1e-9 / h)
     nfd = min(a.fd_modes, len(amps))
-    fd = np.zeros(nfd)
+    fd_report = {}
     with Context(n, n, dx, dx, eta) as c_:
-        for k in range(nfd):
-            e = np.zeros_like(amps)
-            e[k] = h
-            fd[k] = (loss_of(c_, amps + e, dt) - loss_of(c_, amps - e, dt)) / (2 * h)
-    he = 1e-6
+        for h in (1e-6, 1e-7, 1e-8):
+            fd = np.zeros(nfd)
+            for k in range(nfd):
+                e = np.zeros_like(amps)
+                e[k] = h
+                fd[k] = (loss_of(c_, amps + e, dt) - loss_of(c_, amps - e, dt)) / (2 * h)
+            rel = np.abs(grad[:nfd] - fd) / np.maximum(np.abs(fd), 1e-300)
+            fd_report[f"h={h:g}"] = {"rel_err_norm": float(np.linalg.norm(grad[:nfd] - fd) / np.linalg.norm(fd)),
+                                     "rel_err_median": float(np.median(rel)),
+                                     "modes_within_1e-6": int(np.sum(rel <= 1e-6))}
+    he = 1e-4 * eta
     with Context(n, n, dx, dx, eta + he) as cp, Context(n, n, dx, dx, eta - he) as cm:
         fde = (loss_of(cp, amps, dt) - loss_of(cm, amps, dt)) / (2 * he)
-    rel = np.abs(grad[:nfd] - fd) / np.maximum(np.abs(fd), 1e-300)
     return {"workload": f"kink column {n}x{n} on [0,1]^2, {len(amps)} displacement modes (amplitude 1e-3), eta {eta}, "
                         f"{steps} steps of frozen dt {dt:.4e}, loss = window magnetic energy, checkpoints every 10",
             "loss": loss, "forward_ms": round(t_fwd * 1e3, 3),
             "adjoint_forward_ms": round(t_ckpt * 1e3, 3), "backward_ms": round(t_bwd * 1e3, 3),
             "backward_over_forward": round(t_bwd / t_fwd, 3),
             "gradient_cost_over_forward": round((t_ckpt + t_bwd) / t_fwd, 3),
-            "fd_h": h, "fd_modes_checked": int(nfd),
-            "grad_rel_err_norm": float(np.linalg.norm(grad[:nfd] - fd) / np.linalg.norm(fd)),
-            "grad_rel_err_max": float(rel.max()), "grad_rel_err_median": float(np.median(rel)),
-            "eta_bar": eta_bar, "eta_fd": fde, "eta_rel_err": abs(eta_bar - fde) / abs(fde)}
+            "fd_modes_checked": int(nfd), "gradient_norm": float(np.linalg.norm(grad)),
+            "central_differences": fd_report,
+            "eta_bar": eta_bar, "eta_fd_h": he, "eta_fd": fde, "eta_rel_err": abs(eta_bar - fde) / abs(fde)}
 
 
 def main():

@@ -147,7 +147,9 @@ def test_kink_column_gradient_matches_central_differences():
     modes, eta 1e-3, 12 steps of frozen dt, checkpoints every 5; the loss
     is the magnetic energy of the central window.  Gradient with respect to
     the 16 amplitudes (lam0 contracted with the analytic d(state)/d(a_k)) and
-    eta against central differences of the GPU forward."""
+    eta against central differences of the GPU forward (relative 1e-5 on the
+    gradient norm: the differences at h = 1e-7 carry about 1e-9 / h of
+    rounding)."""
     n, eta, steps = 64, 1e-3, 12
     _say(f"kink column {n}^2 gradient, 16 modes, {steps} steps")
     amps, phases = setups.kink_column_modes(16, 1e-3, seed=0)
@@ -161,7 +163,11 @@ def test_kink_column_gradient_matches_central_differences():
         lam0, eta_bar = c.adjoint_backward(lam)
     grad = setups.kink_column_gradient(lam0, setups.kink_column_dstate(n, amps, phases))
     assert abs(loss - _kink_loss(n, amps, phases, eta, dt, steps)) <= 1e-13 * loss
-    h = 1e-6
+    # h = 1e-7: the upwind E_z selection of the CT step (solvers.hpp:326-333)
+    # switches discontinuously where a corner velocity changes sign, and
+    # over a dozen steps a larger step of the amplitudes crosses such jumps;
+    # below 1e-7 the differences' own rounding grows (about 1e-9 / h)
+    h = 1e-7
     fd = np.zeros_like(grad)
     for k in range(len(amps)):
         e = np.zeros_like(amps)
@@ -169,7 +175,8 @@ def test_kink_column_gradient_matches_central_differences():
         fd[k] = (_kink_loss(n, amps + e, phases, eta, dt, steps) - _kink_loss(n, amps - e, phases, eta, dt, steps)) / (2 * h)
     err = np.linalg.norm(grad - fd) / np.linalg.norm(fd)
     print(f"gradient norm {np.linalg.norm(fd):.6e}, relative error {err:.3e}", flush=True)
-    assert err <= 1e-6, (grad, fd)
-    he = 1e-3 * eta
+    assert err <= 1e-5, (grad, fd)
+    he = 1e-4 * eta
     fde = (_kink_loss(n, amps, phases, eta + he, dt, steps) - _kink_loss(n, amps, phases, eta - he, dt, steps)) / (2 * he)
-    assert abs(eta_bar - fde) <= 1e-6 * abs(fde) + 1e-12, (eta_bar, fde)
+    print(f"eta: adjoint {eta_bar:.10e} differences {fde:.10e}", flush=True)
+    assert abs(eta_bar - fde) <= 1e-5 * abs(fde), (eta_bar, fde)
