@@ -300,18 +300,28 @@ def bench_adjoint(a) -> dict:
     # velocity changes sign, so a large step crosses jumps over a long run,
     # while a small one carries the differences' rounding (about 1e-9 / h)
     nfd = min(a.fd_modes, len(amps))
-    fd_report = {}
+    fd_report, fds = {}, {}
+
+    def summary(fd):
+        rel = np.abs(grad[:nfd] - fd) / np.maximum(np.abs(fd), 1e-300)
+        return {"rel_err_norm": float(np.linalg.norm(grad[:nfd] - fd) / np.linalg.norm(fd)),
+                "rel_err_median": float(np.median(rel)), "modes_within_1e-6": int(np.sum(rel <= 1e-6)),
+                "modes_within_1e-5": int(np.sum(rel <= 1e-5))}
+
     with Context(n, n, dx, dx, eta) as c_:
-        for h in (1e-6, 1e-7, 1e-8):
+        for h in (1e-6, 1e-7, 3e-8):
             fd = np.zeros(nfd)
             for k in range(nfd):
                 e = np.zeros_like(amps)
                 e[k] = h
                 fd[k] = (loss_of(c_, amps + e, dt) - loss_of(c_, amps - e, dt)) / (2 * h)
-            rel = np.abs(grad[:nfd] - fd) / np.maximum(np.abs(fd), 1e-300)
-            fd_report[f"h={h:g}"] = {"rel_err_norm": float(np.linalg.norm(grad[:nfd] - fd) / np.linalg.norm(fd)),
-                                     "rel_err_median": float(np.median(rel)),
-                                     "modes_within_1e-6": int(np.sum(rel <= 1e-6))}
+            fds[h] = fd
+            fd_report[f"h={h:g}"] = summary(fd)
+    # consensus of the two small steps (independent of the adjoint): where
+    # they disagree by more than 1e-5 a jump was crossed; keep the smaller
+    agree = np.abs(fds[1e-7] - fds[3e-8]) <= 1e-5 * np.abs(fds[3e-8])
+    cons = np.where(agree, 0.5 * (fds[1e-7] + fds[3e-8]), fds[3e-8])
+    fd_report["consensus_1e-7_3e-8"] = dict(summary(cons), modes_with_jump=int(np.sum(~agree)))
     he = 1e-4 * eta
     with Context(n, n, dx, dx, eta + he) as cp, Context(n, n, dx, dx, eta - he) as cm:
         fde = (loss_of(cp, amps, dt) - loss_of(cm, amps, dt)) / (2 * he)
