@@ -192,13 +192,18 @@ def fourier_ensemble_member(n: int, rng: np.random.Generator, modes: int = 8, am
 # A weak smooth background -- flow of amplitude 0.05, a field from a corner
 # potential of amplitude 0.02 (div B = 0 to round-off), 5 % density and
 # pressure modulation -- keeps every reconstructed field non-uniform along
-# both grid directions: with uniform fields every minmod of that field sits
-# at its kink and the loss is not differentiable in the amplitudes.
+# both grid directions (with a uniform field every minmod of that field sits
+# at its kink), and a uniform drift (0.3, 0.2) keeps every corner velocity
+# away from zero: the upwind E_z selection of the CT step
+# (solvers.hpp:326-333) jumps where a corner velocity changes sign, which
+# would make the loss discontinuous in the amplitudes.  On the periodic box
+# the drift only carries the column along.
 # The state is analytic in the amplitudes; kink_column_dstate gives
 # d(state)/d(a_k) for the gradient with respect to them.
 # ---------------------------------------------------------------------------
 
 KINK_R, KINK_B0, KINK_P0, KINK_RHO_BG = 0.2, 5.0, 0.1, 0.2
+KINK_DRIFT = (0.3, 0.2)
 
 
 def _kink_profile(s):
@@ -226,8 +231,8 @@ def _kink_background(n):
     X, Y = _centres(n, n, dx, dx)
     XC, YC = _corners(n, n, dx, dx)
     tp = 2 * np.pi
-    vx = 0.05 * np.sin(tp * Y + 0.3) + 0.03 * np.cos(2 * tp * Y + tp * X + 1.0)
-    vy = 0.05 * np.sin(tp * X + 0.7) + 0.02 * np.cos(tp * (X - Y) + 0.4)
+    vx = KINK_DRIFT[0] + 0.05 * np.sin(tp * Y + 0.3) + 0.03 * np.cos(2 * tp * Y + tp * X + 1.0)
+    vy = KINK_DRIFT[1] + 0.05 * np.sin(tp * X + 0.7) + 0.02 * np.cos(tp * (X - Y) + 0.4)
     mod_rho = 1.0 + 0.05 * np.sin(tp * (X + 2 * Y) + 0.2)
     mod_p = 1.0 + 0.05 * np.cos(tp * (2 * X - Y) + 1.3)
     Az = 0.02 * np.sin(tp * XC + 0.5) * np.sin(tp * YC + 0.9) / tp
