@@ -48,6 +48,8 @@ namespace mhd {
 //   J(B)(i,j)  = (by(i,j) - by(i-1,j))/dx - (bx(i,j) - bx(i,j-1))/dy   (corner)
 //   R(x)       = (-(x(i,j+1) - x(i,j))/dy, (x(i+1,j) - x(i,j))/dx)    (faces)
 //   Lap(x)     = (x(i-1,j) - 2x + x(i+1,j))/dx^2 + (x(i,j-1) - 2x + x(i,j+1))/dy^2
+// A CG solution x whose solve did no product (spmv = 0, x = 0 in Eigen) is
+// read as zero: kernels that take a CgState pointer check it.
 // ---------------------------------------------------------------------------
 
 struct LinArgs {
@@ -55,13 +57,11 @@ struct LinArgs {
   double idx, idy;
 };
 
-// q = J(bx, by)
-__global__ __launch_bounds__(256) void ka_J(LinArgs g, const double* bx, const double* by, double* q) {
-  const int n = g.nx * g.ny;
-  for (int o = blockIdx.x * blockDim.x + threadIdx.x; o < n; o += gridDim.x * blockDim.x) {
-    const int j = o / g.nx, i = o - j * g.nx;
-    q[o] = (by[o] - by[j * g.nx + wp(i - 1, g.nx)]) * g.idx - (bx[o] - bx[wp(j - 1, g.ny) * g.nx + i]) * g.idy;
-  }
+__device__ __forceinline__ bool x_is_zero(const CgState* st) { return st != nullptr && st->spmv == 0; }
+
+__device__ __forceinline__ double corner_J(const LinArgs& g, const double* bx, const double* by, int i, int j) {
+  const int o = j * g.nx + i;
+  return (by[o] - by[j * g.nx + wp(i - 1, g.nx)]) * g.idx - (bx[o] - bx[wp(j - 1, g.ny) * g.nx + i]) * g.idy;
 }
 
 // (fx, fy) += c J^T(q)
@@ -74,15 +74,31 @@ __global__ __launch_bounds__(256) void ka_Jt(LinArgs g, const double* q, double 
   }
 }
 
-// (fx, fy) = (bx, by) + c R(x)
-__global__ __launch_bounds__(256) void ka_R(LinArgs g, const double* bx, const double* by, const double* x, double c,
-                                            double* fx, double* fy) {
+// t = c J^T(q) and out = base + t
+__global__ __launch_bounds__(256) void ka_Jt2(LinArgs g, const double* q, double c, const double* bx, const double* by,
+                                              double* tx, double* ty, double* ox, double* oy) {
   const int n = g.nx * g.ny;
   for (int o = blockIdx.x * blockDim.x + threadIdx.x; o < n; o += gridDim.x * blockDim.x) {
     const int j = o / g.nx, i = o - j * g.nx;
-    const double xc = x ? x[o] : 0.0;
-    const double xn = x ? x[wp(j + 1, g.ny) * g.nx + i] : 0.0;
-    const double xe = x ? x[j * g.nx + wp(i + 1, g.nx)] : 0.0;
+    const double ax = c * (q[wp(j + 1, g.ny) * g.nx + i] - q[o]) * g.idy;
+    const double ay = c * (q[o] - q[j * g.nx + wp(i + 1, g.nx)]) * g.idx;
+    tx[o] = ax;
+    ty[o] = ay;
+    ox[o] = bx[o] + ax;
+    oy[o] = by[o] + ay;
+  }
+}
+
+// (fx, fy) = (bx, by) + c R(x)
+__global__ __launch_bounds__(256) void ka_R(LinArgs g, const double* bx, const double* by, const double* x,
+                                            const CgState* st, double c, double* fx, double* fy) {
+  const int n = g.nx * g.ny;
+  const bool zx = x_is_zero(st);
+  for (int o = blockIdx.x * blockDim.x + threadIdx.x; o < n; o += gridDim.x * blockDim.x) {
+    const int j = o / g.nx, i = o - j * g.nx;
+    const double xc = zx ? 0.0 : x[o];
+    const double xn = zx ? 0.0 : x[wp(j + 1, g.ny) * g.nx + i];
+    const double xe = zx ? 0.0 : x[j * g.nx + wp(i + 1, g.nx)];
     fx[o] = bx[o] + c * (-(xn - xc) * g.idy);
     fy[o] = by[o] + c * ((xe - xc) * g.idx);
   }
@@ -97,25 +113,12 @@ __global__ __launch_bounds__(256) void ka_Rt(LinArgs g, const double* fx, const 
   }
 }
 
-// partial[b] = sum over the block's cells of w . (a or Lap a)
-__global__ __launch_bounds__(256) void ka_dot(LinArgs g, const double* u, const double* v, int lap, double* partial) {
-  __shared__ double lds[16];
-  const int n = g.nx * g.ny;
-  double acc[1] = {0.0};
-  for (int o = blockIdx.x * blockDim.x + threadIdx.x; o < n; o += gridDim.x * blockDim.x) {
-    double vv = v[o];
-    if (lap) {
-      const int j = o / g.nx, i = o - j * g.nx;
-      vv = (v[j * g.nx + wp(i - 1, g.nx)] - 2.0 * v[o] + v[j * g.nx + wp(i + 1, g.nx)]) * (g.idx * g.idx) +
-           (v[wp(j - 1, g.ny) * g.nx + i] - 2.0 * v[o] + v[wp(j + 1, g.ny) * g.nx + i]) * (g.idy * g.idy);
-    }
-    acc[0] += u[o] * vv;
-  }
-  // fixed-order block reduction
+__device__ __forceinline__ void block_partial(double acc, double* partial) {
+  __shared__ double lds[8];
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1) acc[0] += __shfl_down(acc[0], off, 64);
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-  if (l == 0) lds[w] = acc[0];
+  if (l == 0) lds[w] = acc;
   __syncthreads();
   if (threadIdx.x == 0) {
     double t = lds[0];
@@ -124,29 +127,107 @@ __global__ __launch_bounds__(256) void ka_dot(LinArgs g, const double* u, const 
   }
 }
 
-// acc[0] += scale * sum(partial[0..nb)) in a fixed order (one thread).
-__global__ void ka_accumulate(const double* partial, int nb, double scale, double* acc) {
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    double t = 0.0;
-    for (int b = 0; b < nb; ++b) t += partial[b];
-    acc[0] += scale * t;
+// partial[b] = block sum of u . Lap(v)  (0 if v is a solution of a solve with no product)
+__global__ __launch_bounds__(256) void ka_dot_lap(LinArgs g, const double* u, const double* v, const CgState* st,
+                                                  double* partial) {
+  const int n = g.nx * g.ny;
+  const bool zv = x_is_zero(st);
+  double acc = 0.0;
+  if (!zv) {
+    for (int o = blockIdx.x * blockDim.x + threadIdx.x; o < n; o += gridDim.x * blockDim.x) {
+      const int j = o / g.nx, i = o - j * g.nx;
+      const double lv = (v[j * g.nx + wp(i - 1, g.nx)] - 2.0 * v[o] + v[j * g.nx + wp(i + 1, g.nx)]) * (g.idx * g.idx) +
+                        (v[wp(j - 1, g.ny) * g.nx + i] - 2.0 * v[o] + v[wp(j + 1, g.ny) * g.nx + i]) * (g.idy * g.idy);
+      acc += u[o] * lv;
+    }
+  }
+  block_partial(acc, partial);
+}
+
+// partial[b] = block sum of q . J(bx, by)
+__global__ __launch_bounds__(256) void ka_dot_J(LinArgs g, const double* q, const double* bx, const double* by,
+                                                double* partial) {
+  const int n = g.nx * g.ny;
+  double acc = 0.0;
+  for (int o = blockIdx.x * blockDim.x + threadIdx.x; o < n; o += gridDim.x * blockDim.x) {
+    const int j = o / g.nx, i = o - j * g.nx;
+    acc += q[o] * corner_J(g, bx, by, i, j);
+  }
+  block_partial(acc, partial);
+}
+
+struct EtaScales {
+  double s[6];
+};
+
+// eta_bar += sum_k scale_k * sum_b partial[k * nb + b], in a fixed order.
+__global__ __launch_bounds__(256) void ka_accumulate(const double* partial, int nslots, int nb, EtaScales sc,
+                                                     double* eta_bar) {
+  __shared__ double lds[8];
+  __shared__ double tot[6];
+  for (int k = 0; k < nslots; ++k) {
+    double acc = 0.0;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) acc += partial[k * nb + b];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    if (l == 0) lds[w] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double t = lds[0];
+      for (int ww = 1; ww < (int)(blockDim.x >> 6); ++ww) t += lds[ww];
+      tot[k] = t;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    double e = eta_bar[0];
+    for (int k = 0; k < nslots; ++k) e += sc.s[k] * tot[k];
+    eta_bar[0] = e;
   }
 }
 
-// y = a x + b y on nf fields
-__global__ __launch_bounds__(256) void ka_axpby(int n, double a, const double* x, double b, double* y) {
-  for (int o = blockIdx.x * blockDim.x + threadIdx.x; o < n; o += gridDim.x * blockDim.x) y[o] = a * x[o] + b * y[o];
+// x = 0 where the solve did no product
+__global__ __launch_bounds__(256) void ka_zero_if_no_product(const CgState* st, int n, double* x) {
+  if (st->spmv != 0) return;
+  for (int o = blockIdx.x * blockDim.x + threadIdx.x; o < n; o += gridDim.x * blockDim.x) x[o] = 0.0;
 }
 
-// y = a x + b z (elementwise, fresh output)
-__global__ __launch_bounds__(256) void ka_lin2(int n, double a, const double* x, double b, const double* z, double* y) {
-  for (int o = blockIdx.x * blockDim.x + threadIdx.x; o < n; o += gridDim.x * blockDim.x) y[o] = a * x[o] + b * z[o];
+struct Fields6 {
+  double* f[6];
+};
+
+// Entry of stage 2's adjoint: Wbar = U3*bar with U3*.hbar = lam.h, the density
+// floor of U3* as a mask (U3*.rho = the next state's density), fbar = ca Wbar,
+// Sbar = 0.
+__global__ __launch_bounds__(256) void ka_adj_stage2_in(int n, Fields6 lam, Fields6 wb, Fields6 fb, Fields6 sb,
+                                                        const double* rho_next, double ca) {
+  for (int o = blockIdx.x * blockDim.x + threadIdx.x; o < n; o += gridDim.x * blockDim.x) {
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      double w = (q < 4) ? lam.f[q][o] : wb.f[q][o];
+      if (q == 0 && !(rho_next[o] > kFloor)) w = 0.0;
+      if (q < 4) wb.f[q][o] = w;
+      fb.f[q][o] = ca * w;
+      sb.f[q][o] = 0.0;
+    }
+  }
 }
 
-// y *= (ref > floor): adjoint of the density floor u = max(u, 1e-8)
-__global__ __launch_bounds__(256) void ka_floor_mask(int n, const double* ref, double* y) {
-  for (int o = blockIdx.x * blockDim.x + threadIdx.x; o < n; o += gridDim.x * blockDim.x)
-    if (!(ref[o] > kFloor)) y[o] = 0.0;
+// Exit of stage 1's adjoint: density floor of S* as a mask (S.rho = S*.rho),
+// fbar = cs S*bar + cw Wbar, lam = S*bar + Wbar.
+__global__ __launch_bounds__(256) void ka_adj_stage1_out(int n, Fields6 sb, Fields6 wb, Fields6 fb, Fields6 lam,
+                                                         const double* rho_s, double cs, double cw) {
+  for (int o = blockIdx.x * blockDim.x + threadIdx.x; o < n; o += gridDim.x * blockDim.x) {
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      double sv = sb.f[q][o];
+      if (q == 0 && !(rho_s[o] > kFloor)) sv = 0.0;
+      const double wv = wb.f[q][o];
+      fb.f[q][o] = cs * sv + cw * wv;
+      lam.f[q][o] = sv + wv;
+    }
+  }
 }
 
 __global__ __launch_bounds__(256) void ka_zero(int n, double* y) {
@@ -166,6 +247,7 @@ struct AdjointStore {
   int nsteps = 0, every = 0;
   std::vector<double*> ckpt;          // 6 fields per checkpoint (steps 0, k, 2k, ...)
   std::vector<StepRecord> rec;        // one per step of a segment
+  std::vector<int> fwd_spmv;          // operator products of each forward solve (2 per step)
   std::vector<double*> owned;         // every device buffer of this store
   double* P[7] = {};
   double* xrb[5] = {};
@@ -179,12 +261,18 @@ struct AdjointStore {
   double* wb[6] = {};                 // W_bar = U3*_bar
   double* sb[6] = {};                 // S_bar
   double* fb[6] = {};                 // f_bar (adjoint of an explicit rate)
+  double* keep[6] = {};               // the final state, kept across the recompute
+  double* final_rho = nullptr;
   double* q1 = nullptr;               // corner scratch
-  double* q2 = nullptr;
   double* fx = nullptr;               // face scratch
   double* fy = nullptr;
-  double* partial = nullptr;
+  double* gx = nullptr;
+  double* gy = nullptr;
+  double* partial = nullptr;          // 6 slots of nblk block sums (eta derivative terms)
   double* eta_bar = nullptr;          // device accumulator
+  CgState* states = nullptr;          // final states of every solve of a backward pass (4 per step)
+  CgState* states_host = nullptr;     // pinned mirror
+  int nstates = 0;
   int nblk = 0;
 };
 
@@ -192,6 +280,8 @@ void destroy_adjoint(mhd_ctx* c) {
   if (!c->adj) return;
   for (double* p : c->adj->owned)
     if (p) (void)hipFree(p);
+  if (c->adj->states) (void)hipFree(c->adj->states);
+  if (c->adj->states_host) (void)hipHostFree(c->adj->states_host);
   delete c->adj;
   c->adj = nullptr;
 }
@@ -209,20 +299,20 @@ int alloc(mhd_ctx* c, double** p) {
   return MHD_OK;
 }
 
-int ensure_store(mhd_ctx* c, int seg) {
+int ensure_store(mhd_ctx* c, int seg, int nsteps) {
   if (!c->adj) {
     c->adj = new AdjointStore();
     AdjointStore& A = *c->adj;
-    double** fields[] = {A.P, A.xrb, A.xlb, A.yrb, A.ylb, A.bcb, A.lam, A.wb, A.sb, A.fb};
-    const int counts[] = {7, 5, 5, 5, 5, 2, 6, 6, 6, 6};
-    for (int g = 0; g < 10; ++g)
+    double** fields[] = {A.P, A.xrb, A.xlb, A.yrb, A.ylb, A.bcb, A.lam, A.wb, A.sb, A.fb, A.keep};
+    const int counts[] = {7, 5, 5, 5, 5, 2, 6, 6, 6, 6, 6};
+    for (int g = 0; g < 11; ++g)
       for (int q = 0; q < counts[g]; ++q)
         if (int rc = alloc(c, &fields[g][q])) return rc;
-    double** singles[] = {&A.bnx, &A.bny, &A.q1, &A.q2, &A.fx, &A.fy};
+    double** singles[] = {&A.bnx, &A.bny, &A.q1, &A.fx, &A.fy, &A.gx, &A.gy, &A.final_rho};
     for (double** p : singles)
       if (int rc = alloc(c, p)) return rc;
     A.nblk = (int)std::min<long long>((c->n + 255) / 256, 1024);
-    if (hipMalloc(&A.partial, sizeof(double) * A.nblk) != hipSuccess) return fail(c, MHD_ENOMEM, "adjoint: allocation failed");
+    if (hipMalloc(&A.partial, sizeof(double) * 6 * A.nblk) != hipSuccess) return fail(c, MHD_ENOMEM, "adjoint: allocation failed");
     A.owned.push_back(A.partial);
     if (hipMalloc(&A.eta_bar, sizeof(double)) != hipSuccess) return fail(c, MHD_ENOMEM, "adjoint: allocation failed");
     A.owned.push_back(A.eta_bar);
@@ -242,6 +332,17 @@ int ensure_store(mhd_ctx* c, int seg) {
     if (int rc = alloc(c, &r.x2)) return rc;
     A.rec.push_back(r);
   }
+  if (A.nstates < 4 * nsteps) {
+    if (A.states) (void)hipFree(A.states);
+    if (A.states_host) (void)hipHostFree(A.states_host);
+    A.states = nullptr;
+    A.states_host = nullptr;
+    A.nstates = 0;
+    if (hipMalloc(&A.states, sizeof(CgState) * 4 * nsteps) != hipSuccess ||
+        hipHostMalloc(&A.states_host, sizeof(CgState) * 4 * nsteps) != hipSuccess)
+      return fail(c, MHD_ENOMEM, "adjoint: allocation failed");
+    A.nstates = 4 * nsteps;
+  }
   return MHD_OK;
 }
 
@@ -252,6 +353,12 @@ LinArgs lin_args(mhd_ctx* c) {
   g.idx = 1.0 / c->dx;
   g.idy = 1.0 / c->dy;
   return g;
+}
+
+Fields6 f6(double* const* p) {
+  Fields6 f;
+  for (int q = 0; q < 6; ++q) f.f[q] = p[q];
+  return f;
 }
 
 // out += L'(s)^T lam for the state s (6 device fields).
@@ -290,151 +397,118 @@ int explicit_rhs_vjp(mhd_ctx* c, double* const* s, double* const* lam, double* c
   return launch(c, 8, [&] { hipLaunchKernelGGL(ka_bfaces, dim3(g), dim3(256), 0, c->stream, a); });
 }
 
-// eta_bar += scale * <u, v or Lap v>
-int dot_acc(mhd_ctx* c, const double* u, const double* v, bool lap, double scale) {
-  AdjointStore& A = *c->adj;
-  LinArgs g = lin_args(c);
-  int rc = launch(c, 8, [&] {
-    hipLaunchKernelGGL(ka_dot, dim3(A.nblk), dim3(256), 0, c->stream, g, u, v, lap ? 1 : 0, A.partial);
-  });
-  if (rc) return rc;
-  return launch(c, 8, [&] {
-    hipLaunchKernelGGL(ka_accumulate, dim3(1), dim3(1), 0, c->stream, (const double*)A.partial, A.nblk, scale, A.eta_bar);
-  });
-}
-
-// Adjoint of one implicit solve x = A^-1 eta J(B) (B the face field the
-// solve's right-hand side was formed from, x the forward solution):
-// given xbar (corner field, in q), adds eta J^T A^-1 xbar to (bxb, byb) and
-// the solve's eta derivative <J(B), bbar> + dt gamma <bbar, Lap x> to eta_bar.
-int solve_vjp(mhd_ctx* c, double* xbar, const double* const* B, const double* x, bool have_x, double* bxb,
-              double* byb) {
-  AdjointStore& A = *c->adj;
-  LinArgs g = lin_args(c);
-  const int gr = grid_1d(c->n);
+// bbar = A^-1 xbar (in c->x, zero if the solve did no product); the solve's
+// final state is copied to *state.  spec > 0: speculative (see cg_run).
+int adjoint_solve(mhd_ctx* c, const double* xbar, int spec, CgState* state) {
   CgState cs;
-  int rc = cg_run(c, nullptr, nullptr, xbar, &cs, 0);   // bbar = A^-1 xbar in c->x
+  int rc = cg_run(c, nullptr, nullptr, xbar, &cs, 2, spec, state);
   if (rc) return rc;
-  const double* bbar = c->x;
-  if (cs.spmv == 0) {                                     // zero right-hand side: bbar = 0
-    rc = launch(c, 8, [&] { hipLaunchKernelGGL(ka_zero, dim3(gr), dim3(256), 0, c->stream, (int)c->n, c->x); });
-    if (rc) return rc;
-  }
-  rc = launch(c, 8, [&] { hipLaunchKernelGGL(ka_Jt, dim3(gr), dim3(256), 0, c->stream, g, bbar, c->eta, bxb, byb); });
-  if (rc) return rc;
-  rc = launch(c, 8, [&] { hipLaunchKernelGGL(ka_J, dim3(gr), dim3(256), 0, c->stream, g, B[0], B[1], A.q2); });
-  if (rc) return rc;
-  rc = dot_acc(c, A.q2, bbar, false, 1.0);
-  if (rc) return rc;
-  if (have_x) {
-    rc = dot_acc(c, bbar, x, true, A.dt * c->tab.A_im4);
-    if (rc) return rc;
-  }
-  return MHD_OK;
-}
-
-// Adjoint of the explicit resistive term g = R(eta J(B)) for gbar (faces):
-// adds eta J^T R^T gbar to (bxb, byb) and <J(B), R^T gbar> to eta_bar.
-int resistive_vjp(mhd_ctx* c, const double* gx, const double* gy, double coef, const double* const* B, double* bxb,
-                  double* byb) {
-  AdjointStore& A = *c->adj;
-  LinArgs g = lin_args(c);
+  if (spec == 0) HIP_OR_FAIL(c, hipMemcpyAsync(state, c->st, sizeof(CgState), hipMemcpyDeviceToDevice, c->stream));
   const int gr = grid_1d(c->n);
-  int rc = launch(c, 8, [&] { hipLaunchKernelGGL(ka_Rt, dim3(gr), dim3(256), 0, c->stream, g, gx, gy, coef, A.q1); });
-  if (rc) return rc;
-  rc = launch(c, 8, [&] { hipLaunchKernelGGL(ka_Jt, dim3(gr), dim3(256), 0, c->stream, g, (const double*)A.q1, c->eta, bxb, byb); });
-  if (rc) return rc;
-  rc = launch(c, 8, [&] { hipLaunchKernelGGL(ka_J, dim3(gr), dim3(256), 0, c->stream, g, B[0], B[1], A.q2); });
-  if (rc) return rc;
-  return dot_acc(c, A.q2, A.q1, false, 1.0);
+  return launch(c, 8, [&] {
+    hipLaunchKernelGGL(ka_zero_if_no_product, dim3(gr), dim3(256), 0, c->stream, (const CgState*)c->st, (int)c->n, c->x);
+  });
 }
 
 // lam (adjoint of the state after the step) -> adjoint of the state before it.
-// rec: the step's forward data; rho_next: density after the step.
-int adjoint_step(mhd_ctx* c, const StepRecord& rec, const double* rho_next) {
+// rec: the step's forward data; rho_next: density after the step; spec:
+// speculative iterations of the two adjoint solves (0: host-checked);
+// states: where their final states go.
+int adjoint_step(mhd_ctx* c, const StepRecord& rec, const double* rho_next, int spec, CgState* states) {
   AdjointStore& A = *c->adj;
   const Tableau& T = c->tab;
-  const double dt = A.dt, cg = dt * T.A_im4;
+  const double dt = A.dt, cg = dt * T.A_im4, eta = c->eta;
   const int n = (int)c->n;
   const int gr = grid_1d(c->n);
+  const int nb = A.nblk;
   LinArgs g = lin_args(c);
-  int rc;
   double** lam = A.lam;
+  double* P = A.partial;
+  int rc;
+#define KL(...) do { rc = launch(c, 8, [&] { hipLaunchKernelGGL(__VA_ARGS__); }); if (rc) return rc; } while (0)
   // ---- final recombination: U' = (U3*.h, U3*.B + cg R(eta J(B3))), B3 = U3*.B + cg R(x2)
-  // B3 into (fx, fy)
-  rc = launch(c, 8, [&] {
-    hipLaunchKernelGGL(ka_R, dim3(gr), dim3(256), 0, c->stream, g, (const double*)rec.U3sB[0], (const double*)rec.U3sB[1],
-                       rec.spmv[1] > 0 ? (const double*)rec.x2 : nullptr, cg, A.fx, A.fy);
-  });
+  KL(ka_R, dim3(gr), dim3(256), 0, c->stream, g, (const double*)rec.U3sB[0], (const double*)rec.U3sB[1],
+     (const double*)rec.x2, (const CgState*)rec.st[1], cg, A.fx, A.fy);
+  // g3bar = cg lam.B: q1 = R^T g3bar; B3bar = eta J^T q1 (gx, gy); wb.B = lam.B + B3bar
+  KL(ka_Rt, dim3(gr), dim3(256), 0, c->stream, g, (const double*)lam[4], (const double*)lam[5], cg, A.q1);
+  KL(ka_Jt2, dim3(gr), dim3(256), 0, c->stream, g, (const double*)A.q1, eta, (const double*)lam[4],
+     (const double*)lam[5], A.gx, A.gy, A.wb[4], A.wb[5]);
+  KL(ka_dot_J, dim3(nb), dim3(256), 0, c->stream, g, (const double*)A.q1, (const double*)A.fx, (const double*)A.fy, P + 0 * nb);
+  // x2bar = cg R^T B3bar; b2bar = A^-1 x2bar; U3*.Bbar += eta J^T b2bar
+  KL(ka_Rt, dim3(gr), dim3(256), 0, c->stream, g, (const double*)A.gx, (const double*)A.gy, cg, A.q1);
+  rc = adjoint_solve(c, A.q1, spec, states + 1);
   if (rc) return rc;
-  // B3bar = eta J^T R^T (cg lam.B) into wb.B (zeroed first); eta_bar += <J(B3), R^T(cg lam.B)>
-  for (int q = 4; q < 6; ++q) {
-    rc = launch(c, 8, [&] { hipLaunchKernelGGL(ka_zero, dim3(gr), dim3(256), 0, c->stream, n, A.wb[q]); });
-    if (rc) return rc;
-  }
-  {
-    const double* B3[2] = {A.fx, A.fy};
-    rc = resistive_vjp(c, lam[4], lam[5], cg, B3, A.wb[4], A.wb[5]);
-    if (rc) return rc;
-  }
-  // x2bar = cg R^T(B3bar) -> solve adjoint: U3*.Bbar += eta J^T A^-1 x2bar
-  rc = launch(c, 8, [&] { hipLaunchKernelGGL(ka_Rt, dim3(gr), dim3(256), 0, c->stream, g, (const double*)A.wb[4], (const double*)A.wb[5], cg, A.q1); });
-  if (rc) return rc;
-  // U3*.Bbar = lam.B + B3bar (+ solve term below), accumulated in wb.B
-  for (int q = 4; q < 6; ++q) {
-    rc = launch(c, 8, [&] { hipLaunchKernelGGL(ka_axpby, dim3(gr), dim3(256), 0, c->stream, n, 1.0, (const double*)lam[q], 1.0, A.wb[q]); });
-    if (rc) return rc;
-  }
-  {
-    const double* B[2] = {rec.U3sB[0], rec.U3sB[1]};
-    rc = solve_vjp(c, A.q1, B, rec.x2, rec.spmv[1] > 0, A.wb[4], A.wb[5]);
-    if (rc) return rc;
-  }
-  // hydro: U3*.hbar = lam.h; density floor of U3*
-  for (int q = 0; q < 4; ++q) {
-    rc = launch(c, 8, [&] { hipLaunchKernelGGL(ka_lin2, dim3(gr), dim3(256), 0, c->stream, n, 1.0, (const double*)lam[q], 0.0, (const double*)lam[q], A.wb[q]); });
-    if (rc) return rc;
-  }
-  rc = launch(c, 8, [&] { hipLaunchKernelGGL(ka_floor_mask, dim3(gr), dim3(256), 0, c->stream, n, rho_next, A.wb[0]); });
-  if (rc) return rc;
-  // ---- stage 2: U3* = W + dt(1-delta) L(S) + dt(1-gamma) R(eta J(S.B)); Wbar = U3*bar (wb)
-  // Sbar = eta J^T R^T (dt(1-gamma) U3*.Bbar) + L'(S)^T (dt(1-delta) U3*bar)
-  for (int q = 0; q < 6; ++q) {
-    rc = launch(c, 8, [&] { hipLaunchKernelGGL(ka_zero, dim3(gr), dim3(256), 0, c->stream, n, A.sb[q]); });
-    if (rc) return rc;
-    rc = launch(c, 8, [&] { hipLaunchKernelGGL(ka_lin2, dim3(gr), dim3(256), 0, c->stream, n, dt * T.A_ex7, (const double*)A.wb[q], 0.0, (const double*)A.wb[q], A.fb[q]); });
-    if (rc) return rc;
-  }
-  {
-    const double* SB[2] = {rec.S[4], rec.S[5]};
-    rc = resistive_vjp(c, A.wb[4], A.wb[5], dt * T.A_im7, SB, A.sb[4], A.sb[5]);
-    if (rc) return rc;
-  }
+  KL(ka_Jt, dim3(gr), dim3(256), 0, c->stream, g, (const double*)c->x, eta, A.wb[4], A.wb[5]);
+  KL(ka_dot_J, dim3(nb), dim3(256), 0, c->stream, g, (const double*)c->x, (const double*)rec.U3sB[0],
+     (const double*)rec.U3sB[1], P + 1 * nb);
+  KL(ka_dot_lap, dim3(nb), dim3(256), 0, c->stream, g, (const double*)c->x, (const double*)rec.x2,
+     (const CgState*)rec.st[1], P + 2 * nb);
+  // ---- stage 2: U3* = W + dt(1-delta) L(S) + dt(1-gamma) R(eta J(S.B)); Wbar = U3*bar
+  KL(ka_adj_stage2_in, dim3(gr), dim3(256), 0, c->stream, n, f6(lam), f6(A.wb), f6(A.fb), f6(A.sb), rho_next,
+     dt * T.A_ex7);
+  KL(ka_Rt, dim3(gr), dim3(256), 0, c->stream, g, (const double*)A.wb[4], (const double*)A.wb[5], dt * T.A_im7, A.q1);
+  KL(ka_Jt, dim3(gr), dim3(256), 0, c->stream, g, (const double*)A.q1, eta, A.sb[4], A.sb[5]);
+  KL(ka_dot_J, dim3(nb), dim3(256), 0, c->stream, g, (const double*)A.q1, (const double*)rec.S[4],
+     (const double*)rec.S[5], P + 3 * nb);
   rc = explicit_rhs_vjp(c, rec.S, A.fb, A.sb);
   if (rc) return rc;
-  // ---- implicit 1: S.B = S*.B + cg R(x1), x1 = A^-1 eta J(S*.B): S*bar = Sbar + solve term
-  rc = launch(c, 8, [&] { hipLaunchKernelGGL(ka_Rt, dim3(gr), dim3(256), 0, c->stream, g, (const double*)A.sb[4], (const double*)A.sb[5], cg, A.q1); });
+  // ---- implicit 1: S.B = S*.B + cg R(x1), x1 = A^-1 eta J(S*.B)
+  KL(ka_Rt, dim3(gr), dim3(256), 0, c->stream, g, (const double*)A.sb[4], (const double*)A.sb[5], cg, A.q1);
+  rc = adjoint_solve(c, A.q1, spec, states + 0);
   if (rc) return rc;
-  {
-    const double* B[2] = {rec.SsB[0], rec.SsB[1]};
-    rc = solve_vjp(c, A.q1, B, rec.x1, rec.spmv[0] > 0, A.sb[4], A.sb[5]);
-    if (rc) return rc;
-  }
-  // density floor of S* (S.rho = S*.rho)
-  rc = launch(c, 8, [&] { hipLaunchKernelGGL(ka_floor_mask, dim3(gr), dim3(256), 0, c->stream, n, (const double*)rec.S[0], A.sb[0]); });
-  if (rc) return rc;
+  KL(ka_Jt, dim3(gr), dim3(256), 0, c->stream, g, (const double*)c->x, eta, A.sb[4], A.sb[5]);
+  KL(ka_dot_J, dim3(nb), dim3(256), 0, c->stream, g, (const double*)c->x, (const double*)rec.SsB[0],
+     (const double*)rec.SsB[1], P + 4 * nb);
+  KL(ka_dot_lap, dim3(nb), dim3(256), 0, c->stream, g, (const double*)c->x, (const double*)rec.x1,
+     (const CgState*)rec.st[0], P + 5 * nb);
   // ---- stage 1: S* = U + dt gamma f1, W = U + dt delta f1, f1 = L(U)
-  // Ubar = S*bar + Wbar + L'(U)^T (dt gamma S*bar + dt delta Wbar)
-  for (int q = 0; q < 6; ++q) {
-    rc = launch(c, 8, [&] { hipLaunchKernelGGL(ka_lin2, dim3(gr), dim3(256), 0, c->stream, n, dt * T.A_ex3, (const double*)A.sb[q], dt * T.A_ex6, (const double*)A.wb[q], A.fb[q]); });
-    if (rc) return rc;
-    rc = launch(c, 8, [&] { hipLaunchKernelGGL(ka_lin2, dim3(gr), dim3(256), 0, c->stream, n, 1.0, (const double*)A.sb[q], 1.0, (const double*)A.wb[q], lam[q]); });
-    if (rc) return rc;
-  }
-  return explicit_rhs_vjp(c, rec.Un, A.fb, lam);
+  KL(ka_adj_stage1_out, dim3(gr), dim3(256), 0, c->stream, n, f6(A.sb), f6(A.wb), f6(A.fb), f6(lam),
+     (const double*)rec.S[0], dt * T.A_ex3, dt * T.A_ex6);
+  rc = explicit_rhs_vjp(c, rec.Un, A.fb, lam);
+  if (rc) return rc;
+  // ---- d/d eta: <J(B3), R^T g3bar> + <J(U3*.B), b2bar> + dt gamma <b2bar, Lap x2>
+  //               + <J(S.B), R^T g2bar> + <J(S*.B), b1bar> + dt gamma <b1bar, Lap x1>
+  EtaScales sc{{1.0, 1.0, dt * T.A_im4, 1.0, 1.0, dt * T.A_im4}};
+  KL(ka_accumulate, dim3(1), dim3(256), 0, c->stream, (const double*)P, 6, nb, sc, A.eta_bar);
+#undef KL
+  return MHD_OK;
 }
 
 int check(mhd_ctx* c) { return c ? MHD_OK : MHD_EINVAL; }
+
+// The backward sweep; spec: run every solve speculatively (verified by the caller).
+int backward(mhd_ctx* c, bool spec) {
+  AdjointStore& A = *c->adj;
+  const size_t fb = (size_t)c->n * sizeof(double);
+  int maxsp = 1;
+  for (int v : A.fwd_spmv) maxsp = std::max(maxsp, v);
+  const int spec_adj = spec ? maxsp + maxsp / 2 + 4 : 0;
+  const int nseg = (A.nsteps + A.every - 1) / A.every;
+  const double* rho_next = A.final_rho;
+  for (int sg = nseg - 1; sg >= 0; --sg) {
+    const int s0 = sg * A.every, s1 = std::min(A.nsteps, s0 + A.every);
+    for (int q = 0; q < 6; ++q)
+      HIP_OR_FAIL(c, hipMemcpyAsync(c->U[q], A.ckpt[6 * sg + q], fb, hipMemcpyDeviceToDevice, c->stream));
+    for (int s = s0; s < s1; ++s) {
+      StepRecord& r = A.rec[s - s0];
+      // the recompute repeats the forward solves: their product counts are known
+      r.spec[0] = spec ? std::max(1, A.fwd_spmv[2 * s]) : 0;
+      r.spec[1] = spec ? std::max(1, A.fwd_spmv[2 * s + 1]) : 0;
+      r.st[0] = A.states + 4 * s + 0;
+      r.st[1] = A.states + 4 * s + 1;
+      int rc = imex_step(c, A.dt, nullptr, &r);
+      if (rc) return rc;
+    }
+    for (int s = s1 - 1; s >= s0; --s) {
+      const StepRecord& r = A.rec[s - s0];
+      const double* rn = (s == A.nsteps - 1) ? A.final_rho : (s + 1 < s1 ? A.rec[s + 1 - s0].Un[0] : rho_next);
+      int rc = adjoint_step(c, r, rn, spec_adj, A.states + 4 * s + 2);
+      if (rc) return rc;
+    }
+    rho_next = A.ckpt[6 * sg + 0];
+  }
+  return MHD_OK;
+}
 
 }  // namespace
 
@@ -445,7 +519,7 @@ int mhd_adjoint_forward(mhd_ctx* c, double dt, int nsteps, int checkpoint_every)
   if (!(dt > 0) || !std::isfinite(dt) || nsteps < 1 || checkpoint_every < 1)
     return fail(c, MHD_EINVAL, "mhd_adjoint_forward: need dt > 0, nsteps >= 1, checkpoint_every >= 1");
   if (!(c->eta > 0)) return fail(c, MHD_EINVAL, "mhd_adjoint_forward: the adjoint needs eta > 0");
-  int rc = ensure_store(c, std::min(checkpoint_every, nsteps));
+  int rc = ensure_store(c, std::min(checkpoint_every, nsteps), nsteps);
   if (rc) return rc;
   AdjointStore& A = *c->adj;
   const int nck = (nsteps + checkpoint_every - 1) / checkpoint_every;
@@ -457,6 +531,7 @@ int mhd_adjoint_forward(mhd_ctx* c, double dt, int nsteps, int checkpoint_every)
   A.dt = dt;
   A.nsteps = nsteps;
   A.every = checkpoint_every;
+  A.fwd_spmv.assign(2 * nsteps, 1);
   rc = mhd_build_operator(c, (dt * c->eta) * c->tab.A_im4);   // constant dt: one operator
   if (rc) return rc;
   const size_t fb = (size_t)c->n * sizeof(double);
@@ -464,8 +539,11 @@ int mhd_adjoint_forward(mhd_ctx* c, double dt, int nsteps, int checkpoint_every)
     if (s % checkpoint_every == 0)
       for (int q = 0; q < 6; ++q)
         HIP_OR_FAIL(c, hipMemcpyAsync(A.ckpt[6 * (s / checkpoint_every) + q], c->U[q], fb, hipMemcpyDeviceToDevice, c->stream));
-    rc = imex_step(c, dt, nullptr, nullptr);
+    mhd_step_stats st;
+    rc = imex_step(c, dt, &st, nullptr);
     if (rc) return rc;
+    A.fwd_spmv[2 * s] = st.cg_spmv[0];
+    A.fwd_spmv[2 * s + 1] = st.cg_spmv[1];
   }
   HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
   return MHD_OK;
@@ -480,42 +558,34 @@ int mhd_adjoint_backward(mhd_ctx* c, const double* const* lam_final, double* con
   AdjointStore& A = *c->adj;
   const size_t fb = (size_t)c->n * sizeof(double);
   int rc;
-  for (int q = 0; q < 6; ++q) {
-    rc = from_host_padded(c, lam_final[q], A.lam[q]);
+  // keep the final state (the recompute overwrites U); its density is the
+  // floor reference of the last step
+  for (int q = 0; q < 6; ++q)
+    HIP_OR_FAIL(c, hipMemcpyAsync(A.keep[q], c->U[q], fb, hipMemcpyDeviceToDevice, c->stream));
+  HIP_OR_FAIL(c, hipMemcpyAsync(A.final_rho, c->U[0], fb, hipMemcpyDeviceToDevice, c->stream));
+  // first pass: every solve speculative (no host round trip), verified
+  // afterwards; a solve that did not finish sends the pass round again with
+  // host-checked solves
+  for (int attempt = 0; attempt < 2; ++attempt) {
+    const bool spec = (attempt == 0);
+    for (int q = 0; q < 6; ++q) {
+      rc = from_host_padded(c, lam_final[q], A.lam[q]);
+      if (rc) return rc;
+    }
+    const double zero = 0.0;
+    HIP_OR_FAIL(c, hipMemcpyAsync(A.eta_bar, &zero, sizeof(double), hipMemcpyHostToDevice, c->stream));
+    rc = backward(c, spec);
     if (rc) return rc;
-  }
-  const double zero = 0.0;
-  HIP_OR_FAIL(c, hipMemcpyAsync(A.eta_bar, &zero, sizeof(double), hipMemcpyHostToDevice, c->stream));
-  // the final state (density after the last step) is the current state
-  double* final_rho = nullptr;
-  if ((rc = alloc(c, &final_rho))) return rc;
-  HIP_OR_FAIL(c, hipMemcpyAsync(final_rho, c->U[0], fb, hipMemcpyDeviceToDevice, c->stream));
-  // keep the current (final) state: the recompute overwrites U
-  double* keep[6];
-  for (int q = 0; q < 6; ++q) {
-    if ((rc = alloc(c, &keep[q]))) return rc;
-    HIP_OR_FAIL(c, hipMemcpyAsync(keep[q], c->U[q], fb, hipMemcpyDeviceToDevice, c->stream));
-  }
-  const int nseg = (A.nsteps + A.every - 1) / A.every;
-  const double* rho_next = final_rho;
-  for (int sg = nseg - 1; sg >= 0; --sg) {
-    const int s0 = sg * A.every, s1 = std::min(A.nsteps, s0 + A.every);
-    for (int q = 0; q < 6; ++q)
-      HIP_OR_FAIL(c, hipMemcpyAsync(c->U[q], A.ckpt[6 * sg + q], fb, hipMemcpyDeviceToDevice, c->stream));
-    for (int s = s0; s < s1; ++s) {
-      rc = imex_step(c, A.dt, nullptr, &A.rec[s - s0]);
-      if (rc) return rc;
-    }
-    for (int s = s1 - 1; s >= s0; --s) {
-      const StepRecord& r = A.rec[s - s0];
-      const double* rn = (s == A.nsteps - 1) ? final_rho : (s + 1 < s1 ? A.rec[s + 1 - s0].Un[0] : rho_next);
-      rc = adjoint_step(c, r, rn);
-      if (rc) return rc;
-    }
-    rho_next = A.ckpt[6 * sg + 0];
+    HIP_OR_FAIL(c, hipMemcpyAsync(A.states_host, A.states, sizeof(CgState) * 4 * A.nsteps, hipMemcpyDeviceToHost,
+                                  c->stream));
+    HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
+    bool finished = true;
+    for (int k = 0; k < 4 * A.nsteps; ++k) finished = finished && A.states_host[k].done != 0;
+    if (finished) break;
+    if (!spec) return fail(c, MHD_EHIP, "adjoint: a CG solve did not finish");
   }
   for (int q = 0; q < 6; ++q)
-    HIP_OR_FAIL(c, hipMemcpyAsync(c->U[q], keep[q], fb, hipMemcpyDeviceToDevice, c->stream));
+    HIP_OR_FAIL(c, hipMemcpyAsync(c->U[q], A.keep[q], fb, hipMemcpyDeviceToDevice, c->stream));
   if (lam0)
     for (int q = 0; q < 6; ++q)
       if (lam0[q]) {
@@ -526,21 +596,13 @@ int mhd_adjoint_backward(mhd_ctx* c, const double* const* lam_final, double* con
   HIP_OR_FAIL(c, hipMemcpyAsync(&eb, A.eta_bar, sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
   if (eta_bar) *eta_bar = eb;
-  // release the temporaries
-  auto& own = A.owned;
-  for (double* p : keep) {
-    own.erase(std::remove(own.begin(), own.end(), p), own.end());
-    (void)hipFree(p);
-  }
-  own.erase(std::remove(own.begin(), own.end(), final_rho), own.end());
-  (void)hipFree(final_rho);
   return MHD_OK;
 }
 
 int mhd_explicit_rhs_vjp(mhd_ctx* c, const double* const* lam, double* const* out) {
   if (check(c)) return MHD_EINVAL;
   if (!lam || !out) return fail(c, MHD_EINVAL, "mhd_explicit_rhs_vjp: NULL argument");
-  int rc = ensure_store(c, 0);
+  int rc = ensure_store(c, 0, 0);
   if (rc) return rc;
   AdjointStore& A = *c->adj;
   const int gr = grid_1d(c->n);

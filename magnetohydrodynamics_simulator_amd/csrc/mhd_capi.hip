@@ -120,13 +120,13 @@ CgArgs cg_args(mhd_ctx* c, const double* bx, const double* by) {
 // before it; kernels launched after the decision return at once.  The host
 // reads the state back after each chunk; the first chunk is as long as the
 // previous solve's product count, so a steady run syncs once per solve.
-int cg_run(mhd_ctx* c, const double* bx, const double* by, const double* bin, CgState* out, int slot) {
+int cg_run(mhd_ctx* c, const double* bx, const double* by, const double* bin, CgState* out, int slot, int spec,
+           CgState* spec_out) {
   if (!c->have_op) return fail(c, MHD_EINVAL, "implicit solve before mhd_build_operator");
   CgArgs a = cg_args(c, bx, by);
   const long long maxit = (c->o.cg_max_iter > 0) ? c->o.cg_max_iter : 2LL * c->n;
-  *c->st_host = CgState{};
-  c->st_host->max_iters = maxit;
-  HIP_OR_FAIL(c, hipMemcpyAsync(c->st, c->st_host, sizeof(CgState), hipMemcpyHostToDevice, c->stream));
+  int rc0 = launch(c, 8, [&] { hipLaunchKernelGGL(k_cg_reset, dim3(1), dim3(1), 0, c->stream, c->st, maxit); });
+  if (rc0) return rc0;
   const int nb = c->tiles;
   int rc = launch(c, 2, [&] {
     if (bin) hipLaunchKernelGGL(k_cg_start<false>, dim3(nb), dim3(kTileX), 0, c->stream, a, bin);
@@ -139,6 +139,27 @@ int cg_run(mhd_ctx* c, const double* bx, const double* by, const double* bin, Cg
   if (rc) return rc;
   // the direction of iteration it lives in p[it & 1] (p[0] from the start kernel)
   long long it = 1;
+  if (spec > 0) {
+    // A(1) B(1) ... A(spec) B(spec) A(spec + 1): the last A takes the decision
+    for (; it <= spec + 1; ++it) {
+      const double* pold = c->p[(it - 1) & 1];
+      double* pnew = c->p[it & 1];
+      rc = launch(c, 3, [&] {
+        hipLaunchKernelGGL(k_cg_pass_a, dim3(nb), dim3(kTileX), 0, c->stream, a, pold, pnew, (int)it);
+      });
+      if (rc) return rc;
+      if (it == spec + 1) break;
+      rc = launch(c, 4, [&] {
+        hipLaunchKernelGGL(k_cg_pass_b<false>, dim3(nb), dim3(kTileX), 0, c->stream, a, (const double*)pnew, (int)it);
+      });
+      if (rc) return rc;
+    }
+    if (spec_out) HIP_OR_FAIL(c, hipMemcpyAsync(spec_out, c->st, sizeof(CgState), hipMemcpyDeviceToDevice, c->stream));
+    *out = CgState{};
+    out->done = -1;
+    out->spmv = -1;
+    return MHD_OK;
+  }
   for (;;) {
     const int chunk = (it == 1) ? (c->last_spmv[slot] > 0 ? c->last_spmv[slot] : 1) : 1;
     for (int k = 0; k < chunk; ++k, ++it) {
@@ -173,7 +194,7 @@ int cg_run(mhd_ctx* c, const double* bx, const double* by, const double* bin, Cg
 }
 
 double cg_error(const CgState& s) {
-  if (s.done == 3) return 0.0;
+  if (s.done == 3 || s.done < 0) return 0.0;
   return std::sqrt(s.rn2 / s.rhs_norm2);
 }
 
@@ -290,14 +311,15 @@ int imex_step(mhd_ctx* c, double dt, mhd_step_stats* stats, StepRecord* rec) {
       if (rc) return rc;
     }
     CgState cs;
-    rc = cg_run(c, c->S[4], c->S[5], nullptr, &cs, 0);
+    rc = cg_run(c, c->S[4], c->S[5], nullptr, &cs, 0, rec ? rec->spec[0] : 0, rec ? rec->st[0] : nullptr);
     if (rc) return rc;
     st.cg_iters[0] = cs.iters;
     st.cg_spmv[0] = cs.spmv;
     st.cg_error[0] = cg_error(cs);
     if (rec) {
       rec->spmv[0] = cs.spmv;
-      if (cs.spmv > 0) HIP_OR_FAIL(c, hipMemcpyAsync(rec->x1, c->x, (size_t)c->n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+      if (cs.spmv != 0) HIP_OR_FAIL(c, hipMemcpyAsync(rec->x1, c->x, (size_t)c->n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+      if (rec->spec[0] == 0) HIP_OR_FAIL(c, hipMemcpyAsync(rec->st[0], c->st, sizeof(CgState), hipMemcpyDeviceToDevice, c->stream));
     }
     FinArgs f = fin_args(c, c->S[4], c->S[5], c->S[4], c->S[5], diag_coeff);
     rc = launch(c, 5, [&] { hipLaunchKernelGGL(k_finalize1, dim3(c->tiles), dim3(kTileX), 0, c->stream, f); });
@@ -325,14 +347,15 @@ int imex_step(mhd_ctx* c, double dt, mhd_step_stats* stats, StepRecord* rec) {
       if (rc) return rc;
     }
     CgState cs;
-    rc = cg_run(c, c->U[4], c->U[5], nullptr, &cs, 1);
+    rc = cg_run(c, c->U[4], c->U[5], nullptr, &cs, 1, rec ? rec->spec[1] : 0, rec ? rec->st[1] : nullptr);
     if (rc) return rc;
     st.cg_iters[1] = cs.iters;
     st.cg_spmv[1] = cs.spmv;
     st.cg_error[1] = cg_error(cs);
     if (rec) {
       rec->spmv[1] = cs.spmv;
-      if (cs.spmv > 0) HIP_OR_FAIL(c, hipMemcpyAsync(rec->x2, c->x, (size_t)c->n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+      if (cs.spmv != 0) HIP_OR_FAIL(c, hipMemcpyAsync(rec->x2, c->x, (size_t)c->n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+      if (rec->spec[1] == 0) HIP_OR_FAIL(c, hipMemcpyAsync(rec->st[1], c->st, sizeof(CgState), hipMemcpyDeviceToDevice, c->stream));
     }
     FinArgs f = fin_args(c, c->U[4], c->U[5], c->S[4], c->S[5], diag_coeff);
     rc = launch(c, 6, [&] { hipLaunchKernelGGL(k_finalize2, dim3(c->tiles), dim3(kTileX), 0, c->stream, f); });

@@ -38,7 +38,9 @@ struct StepRecord {
   double* U3sB[2];
   double* x1;
   double* x2;
-  int spmv[2];
+  int spmv[2];              // products of the two solves (-1: not known on the host)
+  int spec[2];              // > 0: run the solve speculatively with this many iterations
+  mhd::CgState* st[2];      // device copies of the two solves' final states
 };
 
 }  // namespace mhd_host
@@ -65,7 +67,8 @@ struct mhd_ctx {
   double* part_host = nullptr;   // pinned, two partials per tile
   double alpha = 0.0;
   bool have_op = false;
-  int last_spmv[2] = {1, 1};     // products of the previous solve in each of the step's two slots
+  int last_spmv[4] = {1, 1, 1, 1};   // products of the previous solve per slot (0, 1: the step's
+                                      // solves; 2, 3: the adjoint's)
   int tiles = 1, tiles_y = 1, tile_rows = 4;   // 2D sweeps: kTileX columns x tile_rows rows per tile
   int stage_rows = 16;
   std::string err;
@@ -108,7 +111,14 @@ int launch(mhd_ctx* c, int id, F&& body) {
 }
 
 int grid_1d(long long n);
-int cg_run(mhd_ctx* c, const double* bx, const double* by, const double* bin, mhd::CgState* out, int slot);
+// One CG solve (Eigen's algorithm) of A x = eta J(bx, by) or A x = bin into
+// c->x.  spec == 0: chunks predicted from the slot's previous solve, the host
+// reads the state after each chunk until the solve is done, *out receives it.
+// spec > 0: spec iterations are launched without any host round trip (the
+// passes of a finished solve return at once), the final device state is
+// copied to spec_out, and out->done is set to -1 (not known).
+int cg_run(mhd_ctx* c, const double* bx, const double* by, const double* bin, mhd::CgState* out, int slot,
+           int spec = 0, mhd::CgState* spec_out = nullptr);
 int to_host_padded(mhd_ctx* c, const double* dev, double* host, bool ghosts);
 int from_host_padded(mhd_ctx* c, const double* host, double* dev);
 // One IMEX step (mhd_imex_step); rec, if not null, receives the step's data.

@@ -882,6 +882,15 @@ __global__ __launch_bounds__(256) void k_prims(DiagArgs a, double* vx, double* v
   }
 }
 
+// Fresh CG state (done = 0, counters zero) with the iteration limit.
+__global__ void k_cg_reset(CgState* st, long long max_iters) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    CgState s{};
+    s.max_iters = max_iters;
+    *st = s;
+  }
+}
+
 // Density floor applied to a freshly uploaded state (Grid.hpp:115).
 __global__ __launch_bounds__(256) void k_floor_rho(double* rho, int n) {
   for (int o = blockIdx.x * blockDim.x + threadIdx.x; o < n; o += gridDim.x * blockDim.x)

@@ -166,6 +166,7 @@ __global__ void k_cfl(DiagArgs a);
 __global__ void k_divb(DiagArgs a);
 __global__ void k_prims(DiagArgs a, double* vx, double* vy, double* p, double* bxc, double* byc);
 __global__ void k_floor_rho(double* rho, int n);
+__global__ void k_cg_reset(CgState* st, long long max_iters);
 __global__ void k_copy(const double* src, double* dst, long long n);
 
 }  // namespace mhd
