@@ -230,6 +230,46 @@ __global__ __launch_bounds__(256) void ka_adj_stage1_out(int n, Fields6 sb, Fiel
   }
 }
 
+// S Jacobi sweeps x <- (b + cx (x_w + x_e) + cy (x_s + x_n)) / d of
+// A x = b, A = d I - cx (E + W) - cy (N + S) (periodic), on T x T tiles
+// staged in LDS with a halo of S corners: each sweep shrinks the region
+// that is still exact by one, so the tile's T x T interior is exact after S
+// sweeps.  xin == nullptr starts from x = 0.
+template <int T, int S>
+__global__ __launch_bounds__(256) void ka_jacobi(LinArgs g, double cx, double cy, double invd, const double* b,
+                                                 const double* xin, double* xout) {
+  constexpr int W = T + 2 * S;
+  __shared__ double xs[2][W * W];
+  __shared__ double bs[W * W];
+  const int tiles_x = (g.nx + T - 1) / T;
+  const int tx = blockIdx.x % tiles_x, ty = blockIdx.x / tiles_x;
+  const int x0 = tx * T - S, y0 = ty * T - S;
+  for (int k = threadIdx.x; k < W * W; k += blockDim.x) {
+    const int i = k % W, j = k / W;
+    const int gi = pmod(x0 + i, g.nx), gj = pmod(y0 + j, g.ny);
+    bs[k] = b[gj * g.nx + gi];
+    xs[0][k] = xin ? xin[gj * g.nx + gi] : 0.0;
+  }
+  __syncthreads();
+  for (int sw = 0; sw < S; ++sw) {
+    const double* cur = xs[sw & 1];
+    double* nxt = xs[(sw & 1) ^ 1];
+    for (int k = threadIdx.x; k < W * W; k += blockDim.x) {
+      const int i = k % W, j = k / W;
+      if (i > sw && i < W - 1 - sw && j > sw && j < W - 1 - sw)
+        nxt[k] = (bs[k] + cx * (cur[k - 1] + cur[k + 1]) + cy * (cur[k - W] + cur[k + W])) * invd;
+    }
+    __syncthreads();
+  }
+  const double* res = xs[S & 1];
+  for (int k = threadIdx.x; k < T * T; k += blockDim.x) {
+    const int i = k % T, j = k / T;
+    const int gi = tx * T + i, gj = ty * T + j;
+    if (gi < g.nx && gj < g.ny) xout[gj * g.nx + gi] = res[(j + S) * W + (i + S)];
+  }
+}
+template __global__ void ka_jacobi<32, 8>(LinArgs, double, double, double, const double*, const double*, double*);
+
 __global__ __launch_bounds__(256) void ka_zero(int n, double* y) {
   for (int o = blockIdx.x * blockDim.x + threadIdx.x; o < n; o += gridDim.x * blockDim.x) y[o] = 0.0;
 }
@@ -270,6 +310,9 @@ struct AdjointStore {
   double* gy = nullptr;
   double* partial = nullptr;          // 6 slots of nblk block sums (eta derivative terms)
   double* eta_bar = nullptr;          // device accumulator
+  std::vector<double*> xs;            // the two implicit solutions of every forward step
+  CgState* fwd_states = nullptr;      // their final solver states (2 per step)
+  double* jac = nullptr;              // Jacobi ping-pong buffer
   CgState* states = nullptr;          // final states of every solve of a backward pass (4 per step)
   CgState* states_host = nullptr;     // pinned mirror
   int nstates = 0;
@@ -281,6 +324,7 @@ void destroy_adjoint(mhd_ctx* c) {
   for (double* p : c->adj->owned)
     if (p) (void)hipFree(p);
   if (c->adj->states) (void)hipFree(c->adj->states);
+  if (c->adj->fwd_states) (void)hipFree(c->adj->fwd_states);
   if (c->adj->states_host) (void)hipHostFree(c->adj->states_host);
   delete c->adj;
   c->adj = nullptr;
@@ -308,7 +352,7 @@ int ensure_store(mhd_ctx* c, int seg, int nsteps) {
     for (int g = 0; g < 11; ++g)
       for (int q = 0; q < counts[g]; ++q)
         if (int rc = alloc(c, &fields[g][q])) return rc;
-    double** singles[] = {&A.bnx, &A.bny, &A.q1, &A.fx, &A.fy, &A.gx, &A.gy, &A.final_rho};
+    double** singles[] = {&A.bnx, &A.bny, &A.q1, &A.fx, &A.fy, &A.gx, &A.gy, &A.final_rho, &A.jac};
     for (double** p : singles)
       if (int rc = alloc(c, p)) return rc;
     A.nblk = (int)std::min<long long>((c->n + 255) / 256, 1024);
@@ -335,13 +379,21 @@ int ensure_store(mhd_ctx* c, int seg, int nsteps) {
   if (A.nstates < 4 * nsteps) {
     if (A.states) (void)hipFree(A.states);
     if (A.states_host) (void)hipHostFree(A.states_host);
+    if (A.fwd_states) (void)hipFree(A.fwd_states);
     A.states = nullptr;
     A.states_host = nullptr;
+    A.fwd_states = nullptr;
     A.nstates = 0;
     if (hipMalloc(&A.states, sizeof(CgState) * 4 * nsteps) != hipSuccess ||
-        hipHostMalloc(&A.states_host, sizeof(CgState) * 4 * nsteps) != hipSuccess)
+        hipHostMalloc(&A.states_host, sizeof(CgState) * 4 * nsteps) != hipSuccess ||
+        hipMalloc(&A.fwd_states, sizeof(CgState) * 2 * nsteps) != hipSuccess)
       return fail(c, MHD_ENOMEM, "adjoint: allocation failed");
     A.nstates = 4 * nsteps;
+  }
+  while ((int)A.xs.size() < 2 * nsteps) {
+    double* p = nullptr;
+    if (int rc = alloc(c, &p)) return rc;
+    A.xs.push_back(p);
   }
   return MHD_OK;
 }
@@ -397,9 +449,45 @@ int explicit_rhs_vjp(mhd_ctx* c, double* const* s, double* const* lam, double* c
   return launch(c, 8, [&] { hipLaunchKernelGGL(ka_bfaces, dim3(g), dim3(256), 0, c->stream, a); });
 }
 
-// bbar = A^-1 xbar (in c->x, zero if the solve did no product); the solve's
-// final state is copied to *state.  spec > 0: speculative (see cg_run).
+// Jacobi sweeps that bring the error of A^-1 below 2^-60 of the solution
+// when the iteration's contraction factor 2 (cx + cy) / d is at most 1/4
+// (the operator of small alpha / h^2), else 0.
+int jacobi_sweeps(mhd_ctx* c) {
+  const double cx = c->alpha / (c->dx * c->dx), cy = c->alpha / (c->dy * c->dy);
+  const double d = 1 + 2 * (cx + cy);
+  const double rho = 2 * (cx + cy) / d;
+  if (!(rho <= 0.25)) return 0;
+  if (rho < 1e-300) return 1;
+  return std::max(1, (int)std::ceil(-60.0 * std::log(2.0) / std::log(rho)));
+}
+
+// bbar = A^-1 xbar in c->x.  A is symmetric, so the adjoint of a solve is a
+// solve with the same operator: by Jacobi sweeps in fused launches of 8 when
+// the operator is close enough to the identity (see jacobi_sweeps), else by
+// CG (zero where the solve does no product; its final state is copied to
+// *state; spec > 0: speculative, see cg_run).
 int adjoint_solve(mhd_ctx* c, const double* xbar, int spec, CgState* state) {
+  const int sweeps = jacobi_sweeps(c);
+  if (sweeps > 0) {
+    AdjointStore& A = *c->adj;
+    const double cx = c->alpha / (c->dx * c->dx), cy = c->alpha / (c->dy * c->dy);
+    const double invd = 1.0 / (1 + 2 * (cx + cy));
+    LinArgs g = lin_args(c);
+    const int tiles = ((c->nx + 31) / 32) * ((c->ny + 31) / 32);
+    const int launches = (sweeps + 7) / 8;
+    // ping-pong so that the last launch writes c->x
+    double* bufs[2] = {(launches & 1) ? c->x : A.jac, (launches & 1) ? A.jac : c->x};
+    const double* in = nullptr;
+    for (int k = 0; k < launches; ++k) {
+      double* out = bufs[k & 1];
+      int rc = launch(c, 8, [&] {
+        hipLaunchKernelGGL((ka_jacobi<32, 8>), dim3(tiles), dim3(256), 0, c->stream, g, cx, cy, invd, xbar, in, out);
+      });
+      if (rc) return rc;
+      in = out;
+    }
+    return MHD_OK;
+  }
   CgState cs;
   int rc = cg_run(c, nullptr, nullptr, xbar, &cs, 2, spec, state);
   if (rc) return rc;
@@ -488,12 +576,13 @@ int backward(mhd_ctx* c, bool spec) {
     for (int q = 0; q < 6; ++q)
       HIP_OR_FAIL(c, hipMemcpyAsync(c->U[q], A.ckpt[6 * sg + q], fb, hipMemcpyDeviceToDevice, c->stream));
     for (int s = s0; s < s1; ++s) {
+      // the recompute takes the implicit solutions the forward pass stored
       StepRecord& r = A.rec[s - s0];
-      // the recompute repeats the forward solves: their product counts are known
-      r.spec[0] = spec ? std::max(1, A.fwd_spmv[2 * s]) : 0;
-      r.spec[1] = spec ? std::max(1, A.fwd_spmv[2 * s + 1]) : 0;
-      r.st[0] = A.states + 4 * s + 0;
-      r.st[1] = A.states + 4 * s + 1;
+      r.use_given = true;
+      r.x1 = A.xs[2 * s];
+      r.x2 = A.xs[2 * s + 1];
+      r.st[0] = A.fwd_states + 2 * s;
+      r.st[1] = A.fwd_states + 2 * s + 1;
       int rc = imex_step(c, A.dt, nullptr, &r);
       if (rc) return rc;
     }
@@ -542,7 +631,12 @@ int mhd_adjoint_forward(mhd_ctx* c, double dt, int nsteps, int checkpoint_every)
       for (int q = 0; q < 6; ++q)
         HIP_OR_FAIL(c, hipMemcpyAsync(A.ckpt[6 * (s / checkpoint_every) + q], c->U[q], fb, hipMemcpyDeviceToDevice, c->stream));
     mhd_step_stats st;
-    rc = imex_step(c, dt, &st, nullptr);
+    StepRecord r{};
+    r.x1 = A.xs[2 * s];
+    r.x2 = A.xs[2 * s + 1];
+    r.st[0] = A.fwd_states + 2 * s;
+    r.st[1] = A.fwd_states + 2 * s + 1;
+    rc = imex_step(c, dt, &st, &r);
     if (rc) return rc;
     A.fwd_spmv[2 * s] = st.cg_spmv[0];
     A.fwd_spmv[2 * s + 1] = st.cg_spmv[1];
@@ -582,7 +676,9 @@ int mhd_adjoint_backward(mhd_ctx* c, const double* const* lam_final, double* con
                                   c->stream));
     HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
     bool finished = true;
-    for (int k = 0; k < 4 * A.nsteps; ++k) finished = finished && A.states_host[k].done != 0;
+    if (jacobi_sweeps(c) == 0)
+      for (int s = 0; s < A.nsteps; ++s)
+        finished = finished && A.states_host[4 * s + 2].done != 0 && A.states_host[4 * s + 3].done != 0;
     if (finished) break;
     if (!spec) return fail(c, MHD_EHIP, "adjoint: a CG solve did not finish");
   }

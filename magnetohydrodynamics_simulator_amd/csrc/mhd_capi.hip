@@ -287,9 +287,11 @@ int imex_step(mhd_ctx* c, double dt, mhd_step_stats* stats, StepRecord* rec) {
   const Tableau& T = c->tab;
   int rc;
   if (rec) {
-    rc = copy_fields(c, rec->Un, c->U, 6);
-    if (rc) return rc;
-    rec->spmv[0] = rec->spmv[1] = 0;
+    if (rec->Un[0]) {
+      rc = copy_fields(c, rec->Un, c->U, 6);
+      if (rc) return rc;
+    }
+    if (!rec->use_given) rec->spmv[0] = rec->spmv[1] = 0;
   }
   // stage 1: f1 = L(U), S = U + dt A_ex[3] f1 (solvers.hpp:495-523),
   // W = U + dt A_ex[6] f1 (solvers.hpp:548-553) into F1
@@ -306,26 +308,31 @@ int imex_step(mhd_ctx* c, double dt, mhd_step_stats* stats, StepRecord* rec) {
   const double diag_coeff = dt * T.A_im4;
   if (c->eta > 0.0) {
     // implicit 1: x = CG(eta J(S.B)), S.B += dt A_im[4] rate(x) (solvers.hpp:526-529)
-    if (rec) {
+    if (rec && rec->SsB[0]) {
       rc = copy_fields(c, rec->SsB, c->S + 4, 2);
       if (rc) return rc;
     }
-    CgState cs;
-    rc = cg_run(c, c->S[4], c->S[5], nullptr, &cs, 0, rec ? rec->spec[0] : 0, rec ? rec->st[0] : nullptr);
-    if (rc) return rc;
-    st.cg_iters[0] = cs.iters;
-    st.cg_spmv[0] = cs.spmv;
-    st.cg_error[0] = cg_error(cs);
-    if (rec) {
-      rec->spmv[0] = cs.spmv;
-      if (cs.spmv != 0) HIP_OR_FAIL(c, hipMemcpyAsync(rec->x1, c->x, (size_t)c->n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
-      if (rec->spec[0] == 0) HIP_OR_FAIL(c, hipMemcpyAsync(rec->st[0], c->st, sizeof(CgState), hipMemcpyDeviceToDevice, c->stream));
-    }
     FinArgs f = fin_args(c, c->S[4], c->S[5], c->S[4], c->S[5], diag_coeff);
+    if (rec && rec->use_given) {
+      f.x = rec->x1;
+      f.st = rec->st[0];
+    } else {
+      CgState cs;
+      rc = cg_run(c, c->S[4], c->S[5], nullptr, &cs, 0, rec ? rec->spec[0] : 0, rec ? rec->st[0] : nullptr);
+      if (rc) return rc;
+      st.cg_iters[0] = cs.iters;
+      st.cg_spmv[0] = cs.spmv;
+      st.cg_error[0] = cg_error(cs);
+      if (rec) {
+        rec->spmv[0] = cs.spmv;
+        if (cs.spmv != 0 && rec->x1) HIP_OR_FAIL(c, hipMemcpyAsync(rec->x1, c->x, (size_t)c->n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        if (rec->spec[0] == 0 && rec->st[0]) HIP_OR_FAIL(c, hipMemcpyAsync(rec->st[0], c->st, sizeof(CgState), hipMemcpyDeviceToDevice, c->stream));
+      }
+    }
     rc = launch(c, 5, [&] { hipLaunchKernelGGL(k_finalize1, dim3(c->tiles), dim3(kTileX), 0, c->stream, f); });
     if (rc) return rc;
   }
-  if (rec) {
+  if (rec && rec->S[0]) {
     rc = copy_fields(c, rec->S, c->S, 6);
     if (rc) return rc;
   }
@@ -342,22 +349,27 @@ int imex_step(mhd_ctx* c, double dt, mhd_step_stats* stats, StepRecord* rec) {
   if (rc) return rc;
   if (c->eta > 0.0) {
     // implicit 2 and the final recombination (solvers.hpp:571-608)
-    if (rec) {
+    if (rec && rec->U3sB[0]) {
       rc = copy_fields(c, rec->U3sB, c->U + 4, 2);
       if (rc) return rc;
     }
-    CgState cs;
-    rc = cg_run(c, c->U[4], c->U[5], nullptr, &cs, 1, rec ? rec->spec[1] : 0, rec ? rec->st[1] : nullptr);
-    if (rc) return rc;
-    st.cg_iters[1] = cs.iters;
-    st.cg_spmv[1] = cs.spmv;
-    st.cg_error[1] = cg_error(cs);
-    if (rec) {
-      rec->spmv[1] = cs.spmv;
-      if (cs.spmv != 0) HIP_OR_FAIL(c, hipMemcpyAsync(rec->x2, c->x, (size_t)c->n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
-      if (rec->spec[1] == 0) HIP_OR_FAIL(c, hipMemcpyAsync(rec->st[1], c->st, sizeof(CgState), hipMemcpyDeviceToDevice, c->stream));
-    }
     FinArgs f = fin_args(c, c->U[4], c->U[5], c->S[4], c->S[5], diag_coeff);
+    if (rec && rec->use_given) {
+      f.x = rec->x2;
+      f.st = rec->st[1];
+    } else {
+      CgState cs;
+      rc = cg_run(c, c->U[4], c->U[5], nullptr, &cs, 1, rec ? rec->spec[1] : 0, rec ? rec->st[1] : nullptr);
+      if (rc) return rc;
+      st.cg_iters[1] = cs.iters;
+      st.cg_spmv[1] = cs.spmv;
+      st.cg_error[1] = cg_error(cs);
+      if (rec) {
+        rec->spmv[1] = cs.spmv;
+        if (cs.spmv != 0 && rec->x2) HIP_OR_FAIL(c, hipMemcpyAsync(rec->x2, c->x, (size_t)c->n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        if (rec->spec[1] == 0 && rec->st[1]) HIP_OR_FAIL(c, hipMemcpyAsync(rec->st[1], c->st, sizeof(CgState), hipMemcpyDeviceToDevice, c->stream));
+      }
+    }
     rc = launch(c, 6, [&] { hipLaunchKernelGGL(k_finalize2, dim3(c->tiles), dim3(kTileX), 0, c->stream, f); });
     if (rc) return rc;
     std::swap(c->U[4], c->S[4]);

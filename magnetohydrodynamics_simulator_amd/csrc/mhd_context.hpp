@@ -31,6 +31,10 @@ struct AdjointStore;   // checkpoints and scratch of the adjoint (mhd_adjoint.hi
 // Device buffers a step writes besides the state, for the adjoint's recompute:
 // the state at the start of the step, the stage state after implicit 1, the
 // face B before each implicit solve and the two solutions.
+// Any pointer may be null (nothing recorded there).  With use_given set the
+// step runs no CG: the two implicit solutions are taken from x1, x2 with the
+// final solver states st[0], st[1] (as a previous run of the same step left
+// them); otherwise the solutions are copied to x1, x2 and the states to st.
 struct StepRecord {
   double* Un[6];
   double* S[6];
@@ -41,6 +45,7 @@ struct StepRecord {
   int spmv[2];              // products of the two solves (-1: not known on the host)
   int spec[2];              // > 0: run the solve speculatively with this many iterations
   mhd::CgState* st[2];      // device copies of the two solves' final states
+  bool use_given;
 };
 
 }  // namespace mhd_host
