@@ -317,6 +317,13 @@ struct AdjointStore {
   CgState* states_host = nullptr;     // pinned mirror
   int nstates = 0;
   int nblk = 0;
+  // cached graph of a sync-free backward sweep and what it was captured for
+  hipGraphExec_t graph = nullptr;
+  double* g_entry[12] = {};           // U and S arrays at capture
+  double* g_exit[12] = {};            // and after the sweep
+  int g_nsteps = 0, g_every = 0;
+  double g_dt = 0.0, g_alpha = 0.0;
+  const void* g_xs = nullptr;
 };
 
 void destroy_adjoint(mhd_ctx* c) {
@@ -325,6 +332,7 @@ void destroy_adjoint(mhd_ctx* c) {
     if (p) (void)hipFree(p);
   if (c->adj->states) (void)hipFree(c->adj->states);
   if (c->adj->fwd_states) (void)hipFree(c->adj->fwd_states);
+  if (c->adj->graph) (void)hipGraphExecDestroy(c->adj->graph);
   if (c->adj->states_host) (void)hipHostFree(c->adj->states_host);
   delete c->adj;
   c->adj = nullptr;
@@ -573,8 +581,8 @@ int backward(mhd_ctx* c, bool spec) {
   const double* rho_next = A.final_rho;
   for (int sg = nseg - 1; sg >= 0; --sg) {
     const int s0 = sg * A.every, s1 = std::min(A.nsteps, s0 + A.every);
-    for (int q = 0; q < 6; ++q)
-      HIP_OR_FAIL(c, hipMemcpyAsync(c->U[q], A.ckpt[6 * sg + q], fb, hipMemcpyDeviceToDevice, c->stream));
+    int rc0 = copy_fields(c, c->U, &A.ckpt[6 * sg], 6);
+    if (rc0) return rc0;
     for (int s = s0; s < s1; ++s) {
       // the recompute takes the implicit solutions the forward pass stored
       StepRecord& r = A.rec[s - s0];
@@ -625,11 +633,11 @@ int mhd_adjoint_forward(mhd_ctx* c, double dt, int nsteps, int checkpoint_every)
   A.fwd_spmv.assign(2 * nsteps, 1);
   rc = mhd_build_operator(c, (dt * c->eta) * c->tab.A_im4);   // constant dt: one operator
   if (rc) return rc;
-  const size_t fb = (size_t)c->n * sizeof(double);
   for (int s = 0; s < nsteps; ++s) {
-    if (s % checkpoint_every == 0)
-      for (int q = 0; q < 6; ++q)
-        HIP_OR_FAIL(c, hipMemcpyAsync(A.ckpt[6 * (s / checkpoint_every) + q], c->U[q], fb, hipMemcpyDeviceToDevice, c->stream));
+    if (s % checkpoint_every == 0) {
+      rc = copy_fields(c, &A.ckpt[6 * (s / checkpoint_every)], c->U, 6);
+      if (rc) return rc;
+    }
     mhd_step_stats st;
     StepRecord r{};
     r.x1 = A.xs[2 * s];
@@ -645,6 +653,22 @@ int mhd_adjoint_forward(mhd_ctx* c, double dt, int nsteps, int checkpoint_every)
   return MHD_OK;
 }
 
+// The sweep of mhd_adjoint_backward between the upload of lam_final and the
+// download of the results: keep the final state, zero eta_bar, run the
+// backward pass, restore the final state.
+int sweep(mhd_ctx* c, bool spec) {
+  AdjointStore& A = *c->adj;
+  int rc = copy_fields(c, A.keep, c->U, 6);
+  if (rc) return rc;
+  rc = copy_fields(c, &A.final_rho, c->U, 1);
+  if (rc) return rc;
+  rc = launch(c, 8, [&] { hipLaunchKernelGGL(ka_zero, dim3(1), dim3(64), 0, c->stream, 1, A.eta_bar); });
+  if (rc) return rc;
+  rc = backward(c, spec);
+  if (rc) return rc;
+  return copy_fields(c, c->U, A.keep, 6);
+}
+
 int mhd_adjoint_backward(mhd_ctx* c, const double* const* lam_final, double* const* lam0, double* eta_bar) {
   if (check(c)) return MHD_EINVAL;
   if (!c->adj || c->adj->nsteps < 1) return fail(c, MHD_EINVAL, "mhd_adjoint_backward before mhd_adjoint_forward");
@@ -652,38 +676,79 @@ int mhd_adjoint_backward(mhd_ctx* c, const double* const* lam_final, double* con
   for (int q = 0; q < 6; ++q)
     if (!lam_final[q]) return fail(c, MHD_EINVAL, "lam_final has a NULL field");
   AdjointStore& A = *c->adj;
-  const size_t fb = (size_t)c->n * sizeof(double);
   int rc;
-  // keep the final state (the recompute overwrites U); its density is the
-  // floor reference of the last step
-  for (int q = 0; q < 6; ++q)
-    HIP_OR_FAIL(c, hipMemcpyAsync(A.keep[q], c->U[q], fb, hipMemcpyDeviceToDevice, c->stream));
-  HIP_OR_FAIL(c, hipMemcpyAsync(A.final_rho, c->U[0], fb, hipMemcpyDeviceToDevice, c->stream));
-  // first pass: every solve speculative (no host round trip), verified
-  // afterwards; a solve that did not finish sends the pass round again with
-  // host-checked solves
-  for (int attempt = 0; attempt < 2; ++attempt) {
-    const bool spec = (attempt == 0);
+  auto upload_lam = [&]() -> int {
     for (int q = 0; q < 6; ++q) {
-      rc = from_host_padded(c, lam_final[q], A.lam[q]);
-      if (rc) return rc;
+      int r = from_host_padded(c, lam_final[q], A.lam[q]);
+      if (r) return r;
     }
-    const double zero = 0.0;
-    HIP_OR_FAIL(c, hipMemcpyAsync(A.eta_bar, &zero, sizeof(double), hipMemcpyHostToDevice, c->stream));
-    rc = backward(c, spec);
+    return MHD_OK;
+  };
+  if (jacobi_sweeps(c) > 0 && c->prof_mask == 0) {
+    // no host round trip anywhere in the sweep: replay it as one graph
+    // (captured on the first call of a configuration, reused after)
+    rc = upload_lam();
     if (rc) return rc;
-    HIP_OR_FAIL(c, hipMemcpyAsync(A.states_host, A.states, sizeof(CgState) * 4 * A.nsteps, hipMemcpyDeviceToHost,
-                                  c->stream));
-    HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
-    bool finished = true;
-    if (jacobi_sweeps(c) == 0)
+    bool same = A.graph && A.g_nsteps == A.nsteps && A.g_every == A.every && A.g_dt == A.dt && A.g_alpha == c->alpha &&
+                A.g_xs == (const void*)A.xs.data();
+    for (int q = 0; q < 6 && same; ++q) same = A.g_entry[q] == c->U[q] && A.g_entry[6 + q] == c->S[q];
+    if (!same) {
+      if (A.graph) (void)hipGraphExecDestroy(A.graph);
+      A.graph = nullptr;
+      for (int q = 0; q < 6; ++q) {
+        A.g_entry[q] = c->U[q];
+        A.g_entry[6 + q] = c->S[q];
+      }
+      HIP_OR_FAIL(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      rc = sweep(c, false);
+      hipGraph_t g = nullptr;
+      hipError_t e = hipStreamEndCapture(c->stream, &g);
+      if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+      }
+      if (e != hipSuccess) return fail(c, MHD_EHIP, std::string("graph capture: ") + hipGetErrorString(e));
+      e = hipGraphInstantiate(&A.graph, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (e != hipSuccess) {
+        A.graph = nullptr;
+        return fail(c, MHD_EHIP, std::string("graph instantiate: ") + hipGetErrorString(e));
+      }
+      for (int q = 0; q < 6; ++q) {
+        A.g_exit[q] = c->U[q];
+        A.g_exit[6 + q] = c->S[q];
+      }
+      A.g_nsteps = A.nsteps;
+      A.g_every = A.every;
+      A.g_dt = A.dt;
+      A.g_alpha = c->alpha;
+      A.g_xs = (const void*)A.xs.data();
+    }
+    HIP_OR_FAIL(c, hipGraphLaunch(A.graph, c->stream));
+    for (int q = 0; q < 6; ++q) {
+      c->U[q] = A.g_exit[q];
+      c->S[q] = A.g_exit[6 + q];
+    }
+  } else {
+    // first pass: every adjoint CG solve speculative (no host round trip),
+    // verified afterwards; a solve that did not finish sends the sweep round
+    // again with host-checked solves
+    for (int attempt = 0; attempt < 2; ++attempt) {
+      const bool spec = (attempt == 0);
+      rc = upload_lam();
+      if (rc) return rc;
+      rc = sweep(c, spec);
+      if (rc) return rc;
+      HIP_OR_FAIL(c, hipMemcpyAsync(A.states_host, A.states, sizeof(CgState) * 4 * A.nsteps, hipMemcpyDeviceToHost,
+                                    c->stream));
+      HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
+      bool finished = true;
       for (int s = 0; s < A.nsteps; ++s)
         finished = finished && A.states_host[4 * s + 2].done != 0 && A.states_host[4 * s + 3].done != 0;
-    if (finished) break;
-    if (!spec) return fail(c, MHD_EHIP, "adjoint: a CG solve did not finish");
+      if (finished) break;
+      if (!spec) return fail(c, MHD_EHIP, "adjoint: a CG solve did not finish");
+    }
   }
-  for (int q = 0; q < 6; ++q)
-    HIP_OR_FAIL(c, hipMemcpyAsync(c->U[q], A.keep[q], fb, hipMemcpyDeviceToDevice, c->stream));
   if (lam0)
     for (int q = 0; q < 6; ++q)
       if (lam0[q]) {

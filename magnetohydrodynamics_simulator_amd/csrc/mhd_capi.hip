@@ -275,10 +275,15 @@ int from_host_padded(mhd_ctx* c, const double* host, double* dev) {
 int check_ctx(mhd_ctx* c) { return c ? MHD_OK : MHD_EINVAL; }
 
 int copy_fields(mhd_ctx* c, double* const* dst, double* const* src, int n) {
-  const size_t fb = (size_t)c->n * sizeof(double);
-  for (int q = 0; q < n; ++q)
-    HIP_OR_FAIL(c, hipMemcpyAsync(dst[q], src[q], fb, hipMemcpyDeviceToDevice, c->stream));
-  return MHD_OK;
+  CopyArgs a{};
+  for (int q = 0; q < n; ++q) {
+    a.src[q] = src[q];
+    a.dst[q] = dst[q];
+  }
+  a.nf = n;
+  return launch(c, 8, [&] {
+    hipLaunchKernelGGL(k_copy_fields, dim3(grid_1d(c->n)), dim3(256), 0, c->stream, a, (long long)c->n);
+  });
 }
 
 int imex_step(mhd_ctx* c, double dt, mhd_step_stats* stats, StepRecord* rec) {

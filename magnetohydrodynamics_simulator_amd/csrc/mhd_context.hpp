@@ -125,6 +125,8 @@ int grid_1d(long long n);
 int cg_run(mhd_ctx* c, const double* bx, const double* by, const double* bin, mhd::CgState* out, int slot,
            int spec = 0, mhd::CgState* spec_out = nullptr);
 int to_host_padded(mhd_ctx* c, const double* dev, double* host, bool ghosts);
+// dst[q] = src[q] for q < n (n <= 6 device fields of the context's size), one launch.
+int copy_fields(mhd_ctx* c, double* const* dst, double* const* src, int n);
 int from_host_padded(mhd_ctx* c, const double* host, double* dev);
 // One IMEX step (mhd_imex_step); rec, if not null, receives the step's data.
 int imex_step(mhd_ctx* c, double dt, mhd_step_stats* stats, StepRecord* rec);

@@ -882,6 +882,15 @@ __global__ __launch_bounds__(256) void k_prims(DiagArgs a, double* vx, double* v
   }
 }
 
+// dst[q] = src[q] for q < nf (nf <= 6), n doubles each.
+__global__ __launch_bounds__(256) void k_copy_fields(CopyArgs a, long long n) {
+  for (long long o = blockIdx.x * (long long)blockDim.x + threadIdx.x; o < n; o += (long long)gridDim.x * blockDim.x) {
+#pragma unroll
+    for (int q = 0; q < 6; ++q)
+      if (q < a.nf) a.dst[q][o] = a.src[q][o];
+  }
+}
+
 // Fresh CG state (done = 0, counters zero) with the iteration limit.
 __global__ void k_cg_reset(CgState* st, long long max_iters) {
   if (blockIdx.x == 0 && threadIdx.x == 0) {

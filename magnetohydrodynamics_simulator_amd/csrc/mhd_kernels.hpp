@@ -167,6 +167,12 @@ __global__ void k_divb(DiagArgs a);
 __global__ void k_prims(DiagArgs a, double* vx, double* vy, double* p, double* bxc, double* byc);
 __global__ void k_floor_rho(double* rho, int n);
 __global__ void k_cg_reset(CgState* st, long long max_iters);
+struct CopyArgs {
+  const double* src[6];
+  double* dst[6];
+  int nf;
+};
+__global__ void k_copy_fields(CopyArgs a, long long n);
 __global__ void k_copy(const double* src, double* dst, long long n);
 
 }  // namespace mhd

@@ -192,3 +192,28 @@ def test_kink_column_gradient_matches_central_differences():
     fde = (_kink_loss(n, amps, phases, eta + he, dt, steps) - _kink_loss(n, amps, phases, eta - he, dt, steps)) / (2 * he)
     print(f"eta: adjoint {eta_bar:.10e} differences {fde:.10e}", flush=True)
     assert abs(eta_bar - fde) <= 1e-5 * abs(fde), (eta_bar, fde)
+
+
+def test_adjoint_repeat_replays_the_same_sweep():
+    """A second forward/backward of the same configuration on one context
+    (the backward sweep is then replayed from its captured graph) gives the
+    same adjoint bit for bit, and leaves the final state in place."""
+    _say("adjoint repeated on one context, 32x32, 6 steps")
+    n, eta, dt, steps = 32, 2e-2, 4e-3, 6
+    ic = _state(n, 5)
+    lamN = dict(zip(CONSERVED, _smooth_fields(n, 6)))
+    res = []
+    with _ctx(ic, n, eta) as c:
+        for _ in range(3):
+            c.upload(*[ic[k] for k in CONSERVED])
+            c.adjoint_forward(dt, steps, 4)
+            fin = c.download()
+            lam0, eb = c.adjoint_backward(lamN)
+            after = c.download()
+            for k in CONSERVED:
+                assert np.array_equal(interior(after[k]), interior(fin[k])), k
+            res.append((lam0, eb))
+    for lam0, eb in res[1:]:
+        assert eb == res[0][1]
+        for k in CONSERVED:
+            assert np.array_equal(interior(lam0[k]), interior(res[0][0][k])), k
