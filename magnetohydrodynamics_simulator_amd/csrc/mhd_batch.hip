@@ -78,6 +78,7 @@ struct mhd_batch {
   bool any_eta = false;
   long long steps_taken = 0;
   std::vector<int> cg_iters_last;   // 2 per run: Eigen iteration counts of the last step
+  double* stage = nullptr;          // pinned staging buffer of the largest run (host transfers)
   std::string err;
 };
 
@@ -309,8 +310,11 @@ int mhd_batch_create(int nruns, const int* nx, const int* ny, const double* dx, 
       hipMalloc(&b->stage_jobs, sizeof(int2) * b->cap_stage_jobs) != hipSuccess ||
       hipMalloc(&b->tile_jobs, sizeof(int2) * b->cap_tiles) != hipSuccess)
     return bail("device allocation failed", MHD_ENOMEM);
+  long long largest = 0;
+  for (int r = 0; r < nruns; ++r) largest = std::max(largest, (long long)nx[r] * ny[r]);
   if (hipHostMalloc(&b->rs_host, sizeof(RunState) * nruns) != hipSuccess ||
-      hipHostMalloc(&b->st_host, sizeof(CgState) * nruns) != hipSuccess)
+      hipHostMalloc(&b->st_host, sizeof(CgState) * nruns) != hipSuccess ||
+      hipHostMalloc(&b->stage, sizeof(double) * largest) != hipSuccess)
     return bail("pinned allocation failed", MHD_ENOMEM);
   b->n_stage_jobs = (int)sj.size();
   b->n_tile_jobs = (int)tj.size();
@@ -341,6 +345,7 @@ int mhd_batch_destroy(mhd_batch* b) {
     if (p) (void)hipFree(p);
   if (b->rs_host) (void)hipHostFree(b->rs_host);
   if (b->st_host) (void)hipHostFree(b->st_host);
+  if (b->stage) (void)hipHostFree(b->stage);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
   return MHD_OK;
@@ -375,10 +380,13 @@ int mhd_batch_upload(mhd_batch* b, int run, const double* rho, const double* mx,
   for (int q = 0; q < 6; ++q)
     if (!src[q]) return bfail(b, MHD_EINVAL, "mhd_batch_upload: NULL field");
   const RunGeom& g = b->geom_h[run];
-  const size_t nxt = (size_t)g.nx + 4;
-  for (int q = 0; q < 6; ++q)
-    BHIP(b, hipMemcpy2DAsync(b->U[q] + g.base, (size_t)g.nx * sizeof(double), src[q] + 2 * nxt + 2, nxt * sizeof(double),
-                             (size_t)g.nx * sizeof(double), (size_t)g.ny, hipMemcpyHostToDevice, b->stream));
+  const size_t nx = (size_t)g.nx, nxt = nx + 4;
+  for (int q = 0; q < 6; ++q) {
+    // through the pinned staging buffer: a strided copy from pageable memory is slow
+    BHIP(b, hipStreamSynchronize(b->stream));
+    for (int j = 0; j < g.ny; ++j) std::memcpy(b->stage + j * nx, src[q] + (j + 2) * nxt + 2, nx * sizeof(double));
+    BHIP(b, hipMemcpyAsync(b->U[q] + g.base, b->stage, nx * g.ny * sizeof(double), hipMemcpyHostToDevice, b->stream));
+  }
   const long long n = (long long)g.nx * g.ny;
   const int blocks = (int)std::min<long long>((n + 255) / 256, 4096);
   BLAUNCH(b, k_floor_rho, dim3(blocks), dim3(256), 0, b->stream, b->U[0] + g.base, (int)n);
@@ -397,10 +405,12 @@ int mhd_batch_download(mhd_batch* b, int run, double* rho, double* mx, double* m
   double* dst[6] = {rho, mx, my, E, bxf, byf};
   for (int q = 0; q < 6; ++q) {
     if (!dst[q]) continue;
-    BHIP(b, hipMemcpy2DAsync(dst[q] + 2 * nxt + 2, nxt * sizeof(double), b->U[q] + g.base, (size_t)g.nx * sizeof(double),
-                             (size_t)g.nx * sizeof(double), (size_t)g.ny, hipMemcpyDeviceToHost, b->stream));
+    BHIP(b, hipMemcpyAsync(b->stage, b->U[q] + g.base, (size_t)g.nx * g.ny * sizeof(double), hipMemcpyDeviceToHost,
+                           b->stream));
+    BHIP(b, hipStreamSynchronize(b->stream));
+    for (int j = 0; j < g.ny; ++j)
+      std::memcpy(dst[q] + (j + 2) * nxt + 2, b->stage + (size_t)j * g.nx, (size_t)g.nx * sizeof(double));
   }
-  BHIP(b, hipStreamSynchronize(b->stream));
   // periodic ghosts (Grid::fillOneField, Grid.hpp:143-161)
   const int nx = g.nx, ny = g.ny, nyt = ny + 4;
   for (int q = 0; q < 6; ++q) {

@@ -256,19 +256,23 @@ void fill_ghosts(double* f, int nx, int ny) {
     }
 }
 
+// Host transfers go through the context's pinned staging buffer: one
+// contiguous DMA copy plus a row copy on the host (a strided copy from
+// pageable memory runs at a small fraction of the link rate).
 int to_host_padded(mhd_ctx* c, const double* dev, double* host, bool ghosts) {
-  const size_t nxt = (size_t)c->nx + 4;
-  HIP_OR_FAIL(c, hipMemcpy2DAsync(host + 2 * nxt + 2, nxt * sizeof(double), dev, (size_t)c->nx * sizeof(double),
-                                  (size_t)c->nx * sizeof(double), (size_t)c->ny, hipMemcpyDeviceToHost, c->stream));
+  const size_t nx = (size_t)c->nx, nxt = nx + 4;
+  HIP_OR_FAIL(c, hipMemcpyAsync(c->stage, dev, (size_t)c->n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
+  for (int j = 0; j < c->ny; ++j) std::memcpy(host + (j + 2) * nxt + 2, c->stage + j * nx, nx * sizeof(double));
   if (ghosts) fill_ghosts(host, c->nx, c->ny);
   return MHD_OK;
 }
 
 int from_host_padded(mhd_ctx* c, const double* host, double* dev) {
-  const size_t nxt = (size_t)c->nx + 4;
-  HIP_OR_FAIL(c, hipMemcpy2DAsync(dev, (size_t)c->nx * sizeof(double), host + 2 * nxt + 2, nxt * sizeof(double),
-                                  (size_t)c->nx * sizeof(double), (size_t)c->ny, hipMemcpyHostToDevice, c->stream));
+  const size_t nx = (size_t)c->nx, nxt = nx + 4;
+  HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));     // the staging buffer is free again
+  for (int j = 0; j < c->ny; ++j) std::memcpy(c->stage + j * nx, host + (j + 2) * nxt + 2, nx * sizeof(double));
+  HIP_OR_FAIL(c, hipMemcpyAsync(dev, c->stage, (size_t)c->n * sizeof(double), hipMemcpyHostToDevice, c->stream));
   return MHD_OK;
 }
 
@@ -457,7 +461,8 @@ int mhd_create(int nx, int ny, double dx, double dy, double eta, const mhd_opts*
       hipMalloc(&c->part_b, pb) != hipSuccess ||
       hipMalloc(&c->st, sizeof(CgState)) != hipSuccess)
     return bail("device allocation failed", MHD_ENOMEM);
-  if (hipHostMalloc(&c->st_host, sizeof(CgState)) != hipSuccess || hipHostMalloc(&c->part_host, pb) != hipSuccess)
+  if (hipHostMalloc(&c->st_host, sizeof(CgState)) != hipSuccess || hipHostMalloc(&c->part_host, pb) != hipSuccess ||
+      hipHostMalloc(&c->stage, fb) != hipSuccess)
     return bail("pinned allocation failed", MHD_ENOMEM);
   for (int q = 0; q < 6; ++q) {
     if (hipMemsetAsync(c->U[q], 0, fb, c->stream) != hipSuccess) return bail("hipMemset failed", MHD_EHIP);
@@ -519,6 +524,7 @@ int mhd_destroy(mhd_ctx* c) {
   if (c->st) (void)hipFree(c->st);
   if (c->st_host) (void)hipHostFree(c->st_host);
   if (c->part_host) (void)hipHostFree(c->part_host);
+  if (c->stage) (void)hipHostFree(c->stage);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return MHD_OK;
@@ -530,10 +536,10 @@ int mhd_upload(mhd_ctx* c, const double* rho, const double* mx, const double* my
   const double* src[6] = {rho, mx, my, E, bxf, byf};
   for (int q = 0; q < 6; ++q)
     if (!src[q]) return fail(c, MHD_EINVAL, "mhd_upload: NULL field");
-  const size_t nxt = (size_t)c->nx + 4;
-  for (int q = 0; q < 6; ++q)
-    HIP_OR_FAIL(c, hipMemcpy2DAsync(c->U[q], (size_t)c->nx * sizeof(double), src[q] + 2 * nxt + 2, nxt * sizeof(double),
-                                    (size_t)c->nx * sizeof(double), (size_t)c->ny, hipMemcpyHostToDevice, c->stream));
+  for (int q = 0; q < 6; ++q) {
+    int rc0 = from_host_padded(c, src[q], c->U[q]);
+    if (rc0) return rc0;
+  }
   int rc = launch(c, 8, [&] {
     hipLaunchKernelGGL(k_floor_rho, dim3(grid_1d(c->n)), dim3(256), 0, c->stream, c->U[0], (int)c->n);
   });

@@ -70,6 +70,7 @@ struct mhd_ctx {
   mhd::CgState* st = nullptr;
   mhd::CgState* st_host = nullptr;    // pinned
   double* part_host = nullptr;   // pinned, two partials per tile
+  double* stage = nullptr;       // pinned staging buffer of one field (host transfers)
   double alpha = 0.0;
   bool have_op = false;
   int last_spmv[4] = {1, 1, 1, 1};   // products of the previous solve per slot (0, 1: the step's
