@@ -302,10 +302,19 @@ def bench_adjoint(a) -> dict:
         t3 = time.perf_counter()
     t_ckpt, t_bwd = t1 - t0, t3 - t2
     grad = setups.kink_column_gradient(lam0, setups.kink_column_dstate(n, amps, phases))
-    # central differences of the forward path at three steps: the upwind E_z
-    # selection of the CT step switches discontinuously where a corner
-    # velocity changes sign, so a large step crosses jumps over a long run,
-    # while a small one carries the differences' rounding (about 1e-9 / h)
+    # Central differences need a forward map without jumps.  The CG stopping
+    # test is one: near the tolerance a perturbation adds or drops an
+    # iteration and moves x by the last residual.  The check therefore runs
+    # forward and adjoint with a fixed CG iteration count (tolerance 0,
+    # 16 iterations, far past convergence for this operator); the timings
+    # above use the reference's tolerance-based solves.
+    fixed = dict(cg_tol=0.0, cg_max_iter=16)
+    with Context(n, n, dx, dx, eta, **fixed) as c:
+        c.upload(*[st[k] for k in names])
+        c.adjoint_forward(dt, steps, 10)
+        _, lam_f = setups.window_magnetic_energy(c.download(), n, dx)
+        lam0_f, eta_bar = c.adjoint_backward(lam_f)
+    grad = setups.kink_column_gradient(lam0_f, setups.kink_column_dstate(n, amps, phases))
     nfd = min(a.fd_modes, len(amps))
     fd_report, fds = {}, {}
 
@@ -315,7 +324,7 @@ def bench_adjoint(a) -> dict:
                 "rel_err_median": float(np.median(rel)), "modes_within_1e-6": int(np.sum(rel <= 1e-6)),
                 "modes_within_1e-5": int(np.sum(rel <= 1e-5))}
 
-    with Context(n, n, dx, dx, eta) as c_:
+    with Context(n, n, dx, dx, eta, **fixed) as c_:
         for h in (1e-6, 1e-7, 3e-8):
             fd = np.zeros(nfd)
             for k in range(nfd):
@@ -330,7 +339,7 @@ def bench_adjoint(a) -> dict:
     cons = np.where(agree, 0.5 * (fds[1e-7] + fds[3e-8]), fds[3e-8])
     fd_report["consensus_1e-7_3e-8"] = dict(summary(cons), modes_with_jump=int(np.sum(~agree)))
     he = 1e-4 * eta
-    with Context(n, n, dx, dx, eta + he) as cp, Context(n, n, dx, dx, eta - he) as cm:
+    with Context(n, n, dx, dx, eta + he, **fixed) as cp, Context(n, n, dx, dx, eta - he, **fixed) as cm:
         fde = (loss_of(cp, amps, dt) - loss_of(cm, amps, dt)) / (2 * he)
     return {"workload": f"kink column {n}x{n} on [0,1]^2, {len(amps)} displacement modes (amplitude 1e-3), eta {eta}, "
                         f"{steps} steps of frozen dt {dt:.4e}, loss = window magnetic energy, checkpoints every 10",
@@ -340,6 +349,7 @@ def bench_adjoint(a) -> dict:
             "backward_over_forward": round(t_bwd / t_fwd, 3),
             "gradient_cost_over_forward": round((t_ckpt + t_bwd) / t_fwd, 3),
             "fd_modes_checked": int(nfd), "gradient_norm": float(np.linalg.norm(grad)),
+            "fd_check_cg": "16 iterations, no tolerance stop (forward and adjoint alike)",
             "central_differences": fd_report,
             "eta_bar": eta_bar, "eta_fd_h": he, "eta_fd": fde, "eta_rel_err": abs(eta_bar - fde) / abs(fde)}
 

@@ -133,9 +133,15 @@ def test_adjoint_run_dot_product(steps, every):
     assert abs(eta_bar - fd) <= 1e-5 * abs(fd), (eta_bar, fd)
 
 
+# Central differences need a forward map without jumps; the CG stopping test
+# is one (near the tolerance a perturbation adds or drops an iteration), so
+# the gradient checks run forward and adjoint with a fixed iteration count.
+FIXED_CG = dict(cg_tol=0.0, cg_max_iter=16)
+
+
 def _kink_loss(n, amps, phases, eta, dt, steps):
     st = setups.kink_column(n, amps, phases)
-    with Context(n, n, 1.0 / n, 1.0 / n, eta) as c:
+    with Context(n, n, 1.0 / n, 1.0 / n, eta, **FIXED_CG) as c:
         c.upload(*[st[k] for k in CONSERVED])
         c.run(1e9, dt_override=dt, max_steps=steps)
         fin = c.download()
@@ -157,7 +163,7 @@ def test_kink_column_gradient_matches_central_differences():
     _say(f"kink column {n}^2 gradient, 8 modes, {steps} steps")
     amps, phases = setups.kink_column_modes(8, 1e-3, seed=0)
     st = setups.kink_column(n, amps, phases)
-    with Context(n, n, 1.0 / n, 1.0 / n, eta, cfl=0.4) as c:
+    with Context(n, n, 1.0 / n, 1.0 / n, eta, cfl=0.4, **FIXED_CG) as c:
         c.upload(*[st[k] for k in CONSERVED])
         dt = 0.8 * c.CFL_condition_checker()
         c.adjoint_forward(dt, steps, 5)
