@@ -341,6 +341,34 @@ def bench_adjoint(a) -> dict:
     he = 1e-4 * eta
     with Context(n, n, dx, dx, eta + he, **fixed) as cp, Context(n, n, dx, dx, eta - he, **fixed) as cm:
         fde = (loss_of(cp, amps, dt) - loss_of(cm, amps, dt)) / (2 * he)
+    # dot-product test of the whole adjoint at full size: <lam0, v> against
+    # <lamN, (F(U0 + e v) - F(U0 - e v)) / 2e> for smooth (low-mode) lamN and
+    # v, with a state step of 1e-8 (below the limiter kinks' scale)
+    rng = np.random.default_rng(7)
+    X, Y = np.meshgrid((np.arange(n + 4) - 2) * dx, (np.arange(n + 4) - 2) * dx)
+
+    def smooth(scale):
+        f = np.zeros_like(X)
+        for _ in range(4):
+            kx, ky = rng.integers(-3, 4, size=2)
+            f += rng.standard_normal() * np.sin(2 * np.pi * (kx * X + ky * Y) + rng.uniform(0, 2 * np.pi))
+        return scale * f
+
+    lamN = {k: smooth(1.0) for k in names}
+    vdir = {k: smooth(1e-3) for k in names}
+    with Context(n, n, dx, dx, eta) as c:
+        c.upload(*[st[k] for k in names])
+        c.adjoint_forward(dt, steps, 10)
+        lam0_d, _ = c.adjoint_backward(lamN)
+        eps = 1e-5
+        fin = []
+        for sg in (1, -1):
+            c.upload(*[st[k] + sg * eps * vdir[k] for k in names])
+            c.run(1e9, dt_override=dt, max_steps=steps)
+            fin.append(c.download())
+    inner = lambda a_, b_: sum(float(np.sum(a_[k][2:-2, 2:-2] * b_[k][2:-2, 2:-2])) for k in names)
+    dot_adj = inner(lam0_d, vdir)
+    dot_fd = inner(lamN, {k: (fin[0][k] - fin[1][k]) / (2 * eps) for k in names})
     return {"workload": f"kink column {n}x{n} on [0,1]^2, {len(amps)} displacement modes (amplitude 1e-3), eta {eta}, "
                         f"{steps} steps of frozen dt {dt:.4e}, loss = window magnetic energy, checkpoints every 10",
             "loss": loss, "forward_ms": round(t_fwd * 1e3, 3),
@@ -351,7 +379,9 @@ def bench_adjoint(a) -> dict:
             "fd_modes_checked": int(nfd), "gradient_norm": float(np.linalg.norm(grad)),
             "fd_check_cg": "16 iterations, no tolerance stop (forward and adjoint alike)",
             "central_differences": fd_report,
-            "eta_bar": eta_bar, "eta_fd_h": he, "eta_fd": fde, "eta_rel_err": abs(eta_bar - fde) / abs(fde)}
+            "eta_bar": eta_bar, "eta_fd_h": he, "eta_fd": fde, "eta_rel_err": abs(eta_bar - fde) / abs(fde),
+            "dot_product_test": {"adjoint": dot_adj, "central_difference": dot_fd, "state_step": 1e-8,
+                                 "rel_err": abs(dot_adj - dot_fd) / abs(dot_fd)}}
 
 
 def main():

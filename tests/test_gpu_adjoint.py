@@ -223,3 +223,36 @@ def test_adjoint_repeat_replays_the_same_sweep():
         assert eb == res[0][1]
         for k in CONSERVED:
             assert np.array_equal(interior(lam0[k]), interior(res[0][0][k])), k
+
+
+def test_kink_column_dot_product_small_step():
+    """The drifting kink column over 12 steps: with a state step of 1e-8
+    (eps 1e-5 on a 1e-3 direction) the central differences stay on one side
+    of every limiter kink and the dot product closes to round-off; larger
+    steps show the kinks (about 4e-6 at eps 1e-4)."""
+    n, eta, steps = 64, 1e-3, 12
+    _say(f"kink column {n}^2 dot product, {steps} steps, eps 1e-5")
+    amps, phases = setups.kink_column_modes(8, 1e-3, seed=0)
+    st = setups.kink_column(n, amps, phases)
+    with Context(n, n, 1.0 / n, 1.0 / n, eta, cfl=0.4) as c:
+        c.upload(*[st[k] for k in CONSERVED])
+        dt = 0.8 * c.CFL_condition_checker()
+    lamN = dict(zip(CONSERVED, _smooth_fields(n, 3)))
+    v = _smooth_fields(n, 4, 1e-3)
+    with Context(n, n, 1.0 / n, 1.0 / n, eta) as c:
+        c.upload(*[st[k] for k in CONSERVED])
+        c.adjoint_forward(dt, steps, 5)
+        lam0, _ = c.adjoint_backward(lamN)
+
+    def run(s):
+        with Context(n, n, 1.0 / n, 1.0 / n, eta) as c:
+            c.upload(*[s[k] for k in CONSERVED])
+            c.run(1e9, dt_override=dt, max_steps=steps)
+            return c.download()
+
+    eps = 1e-5
+    outs = [run({k: st[k] + sg * eps * v[q] for q, k in enumerate(CONSERVED)}) for sg in (1, -1)]
+    jv = [(outs[0][k] - outs[1][k]) / (2 * eps) for k in CONSERVED]
+    lhs = _dot([lam0[k] for k in CONSERVED], v)
+    rhs = _dot([lamN[k] for k in CONSERVED], jv)
+    assert abs(lhs - rhs) <= 1e-7 * abs(rhs), (lhs, rhs)
