@@ -86,6 +86,8 @@ def parse():
     ap.add_argument("--adjoint-steps", type=int, default=50)
     ap.add_argument("--adjoint-modes", type=int, default=64)
     ap.add_argument("--fd-modes", type=int, default=64, help="modes checked against central differences")
+    ap.add_argument("--only", choices=("batch", "adjoint"), default=None,
+                    help="run only the batched or the differentiation leg and print its record")
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     return ap.parse_args()
 
@@ -393,6 +395,12 @@ def main():
         print(f"bench.py measures one GPU only (got --gpus {a.gpus}); multi-GPU runs are not part of this build",
               file=sys.stderr)
         sys.exit(2)
+    if a.only == "batch":
+        print(json.dumps({"batched": bench_batch(a)}), flush=True)
+        return
+    if a.only == "adjoint":
+        print(json.dumps({"adjoint": bench_adjoint(a)}), flush=True)
+        return
     from magnetohydrodynamics_simulator_amd import Context, setups
 
     n = a.n
