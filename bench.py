@@ -321,6 +321,8 @@ def bench_adjoint(a) -> dict:
     fd_report, fds = {}, {}
 
     def summary(fd):
+        if nfd == 0:
+            return {}
         rel = np.abs(grad[:nfd] - fd) / np.maximum(np.abs(fd), 1e-300)
         return {"rel_err_norm": float(np.linalg.norm(grad[:nfd] - fd) / np.linalg.norm(fd)),
                 "rel_err_median": float(np.median(rel)), "modes_within_1e-6": int(np.sum(rel <= 1e-6)),
@@ -337,9 +339,10 @@ def bench_adjoint(a) -> dict:
             fd_report[f"h={h:g}"] = summary(fd)
     # consensus of the two small steps (independent of the adjoint): where
     # they disagree by more than 1e-5 a jump was crossed; keep the smaller
-    agree = np.abs(fds[1e-7] - fds[3e-8]) <= 1e-5 * np.abs(fds[3e-8])
-    cons = np.where(agree, 0.5 * (fds[1e-7] + fds[3e-8]), fds[3e-8])
-    fd_report["consensus_1e-7_3e-8"] = dict(summary(cons), modes_with_jump=int(np.sum(~agree)))
+    if nfd > 0:
+        agree = np.abs(fds[1e-7] - fds[3e-8]) <= 1e-5 * np.abs(fds[3e-8])
+        cons = np.where(agree, 0.5 * (fds[1e-7] + fds[3e-8]), fds[3e-8])
+        fd_report["consensus_1e-7_3e-8"] = dict(summary(cons), modes_with_jump=int(np.sum(~agree)))
     he = 1e-4 * eta
     with Context(n, n, dx, dx, eta + he, **fixed) as cp, Context(n, n, dx, dx, eta - he, **fixed) as cm:
         fde = (loss_of(cp, amps, dt) - loss_of(cm, amps, dt)) / (2 * he)
