@@ -201,6 +201,32 @@ int mhd_batch_autotune(mhd_batch* b, int trial_steps, int* stage_rows, int* tile
 int mhd_batch_run_info(mhd_batch* b, int run, int* steps, double* t, double* last_dt, int* cg_iters);
 int mhd_batch_synchronize(mhd_batch* b);
 
+/* ------------------------------------------------------------------------
+ * Batched adjoint (no reference counterpart): mhd_adjoint_* for every run of
+ * a batch at once.  Every run needs eta > 0 and a fixed time step
+ * (mhd_batch_set_schedule dt_fixed > 0); each run's gradient is that of a
+ * single-run context on the same inputs (Jacobi or CG adjoint solves by the
+ * same rule; sums of the eta derivative in another order).
+ * ------------------------------------------------------------------------ */
+
+/* Runs nsteps steps of every run from the uploaded states, recording each
+ * step's stage data and keeping a copy of the state every checkpoint_every
+ * steps (the stage data of one segment of checkpoint_every steps is held on
+ * the device; checkpoint_every >= nsteps keeps all of it and the backward pass
+ * recomputes nothing).  Fails if a run reaches its t_end or step limit on the
+ * way.  The batch then holds the final states.  Sets every run's final-state
+ * adjoint to zero. */
+int mhd_batch_adjoint_forward(mhd_batch* b, int nsteps, int checkpoint_every);
+/* dJ/dU(final) of one run: six padded arrays rho, mx, my, E, bxf, byf. */
+int mhd_batch_adjoint_set_lambda(mhd_batch* b, int run, const double* const* lam_final);
+/* The backward pass of every run through the steps of the last
+ * mhd_batch_adjoint_forward.  The final-state adjoints are consumed: set them
+ * again before another backward pass. */
+int mhd_batch_adjoint_backward(mhd_batch* b);
+/* dJ/dU(initial) of one run (six padded arrays, ghosts filled; the array
+ * pointer or any entry may be NULL) and dJ/d eta (may be NULL). */
+int mhd_batch_adjoint_result(mhd_batch* b, int run, double* const* lam0, double* eta_bar);
+
 #ifdef __cplusplus
 }
 #endif

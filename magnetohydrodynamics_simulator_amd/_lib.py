@@ -65,6 +65,10 @@ SIGNATURES = {
     "mhd_batch_run_info": (C.c_int, [_vp, C.c_int, _ip, _dp, _dp, _ip]),
     "mhd_batch_autotune": (C.c_int, [_vp, C.c_int, _ip, _ip]),
     "mhd_batch_synchronize": (C.c_int, [_vp]),
+    "mhd_batch_adjoint_forward": (C.c_int, [_vp, C.c_int, C.c_int]),
+    "mhd_batch_adjoint_set_lambda": (C.c_int, [_vp, C.c_int, C.POINTER(_dp)]),
+    "mhd_batch_adjoint_backward": (C.c_int, [_vp]),
+    "mhd_batch_adjoint_result": (C.c_int, [_vp, C.c_int, C.POINTER(_dp), _dp]),
 }
 
 _LIB: C.CDLL | None = None

@@ -34,6 +34,7 @@
 
 #include "../../include/mhd_mi355x.h"
 #include "mhd_context.hpp"
+#include "mhd_adjoint_lin.hpp"
 #include "mhd_adjoint_math.hpp"
 #include "mhd_device.hpp"
 #include "mhd_kernels.hpp"
@@ -43,117 +44,80 @@ using namespace mhd_host;
 
 namespace mhd {
 
-// ---------------------------------------------------------------------------
-// Linear stencils of the resistive terms and their transposes (periodic).
-//   J(B)(i,j)  = (by(i,j) - by(i-1,j))/dx - (bx(i,j) - bx(i,j-1))/dy   (corner)
-//   R(x)       = (-(x(i,j+1) - x(i,j))/dy, (x(i+1,j) - x(i,j))/dx)    (faces)
-//   Lap(x)     = (x(i-1,j) - 2x + x(i+1,j))/dx^2 + (x(i,j-1) - 2x + x(i,j+1))/dy^2
-// A CG solution x whose solve did no product (spmv = 0, x = 0 in Eigen) is
-// read as zero: kernels that take a CgState pointer check it.
-// ---------------------------------------------------------------------------
+// Single-run kernels: grid-stride loops over the cells of one grid running the
+// bodies of mhd_adjoint_lin.hpp.
+#define FOR_CELLS(g)                                                                        \
+  for (int o_ = blockIdx.x * blockDim.x + threadIdx.x, n_ = (g).nx * (g).ny; o_ < n_;       \
+       o_ += gridDim.x * blockDim.x)                                                        \
+    if (const int j_ = o_ / (g).nx, i_ = o_ - j_ * (g).nx; true)
 
-struct LinArgs {
-  int nx, ny;
-  double idx, idy;
-};
-
-__device__ __forceinline__ bool x_is_zero(const CgState* st) { return st != nullptr && st->spmv == 0; }
-
-__device__ __forceinline__ double corner_J(const LinArgs& g, const double* bx, const double* by, int i, int j) {
-  const int o = j * g.nx + i;
-  return (by[o] - by[j * g.nx + wp(i - 1, g.nx)]) * g.idx - (bx[o] - bx[wp(j - 1, g.ny) * g.nx + i]) * g.idy;
-}
-
-// (fx, fy) += c J^T(q)
-__global__ __launch_bounds__(256) void ka_Jt(LinArgs g, const double* q, double c, double* fx, double* fy) {
-  const int n = g.nx * g.ny;
+// Adjoint of the explicit right-hand side, one kernel per body of
+// mhd_adjoint_math.hpp.
+__global__ __launch_bounds__(256) void ka_prims(AdjArgs a) {
+  const int n = a.nx * a.ny;
   for (int o = blockIdx.x * blockDim.x + threadIdx.x; o < n; o += gridDim.x * blockDim.x) {
-    const int j = o / g.nx, i = o - j * g.nx;
-    fx[o] += c * (q[wp(j + 1, g.ny) * g.nx + i] - q[o]) * g.idy;
-    fy[o] += c * (q[o] - q[j * g.nx + wp(i + 1, g.nx)]) * g.idx;
+    const int j = o / a.nx;
+    ka_prims_cell(a, o - j * a.nx, j);
   }
 }
 
-// t = c J^T(q) and out = base + t
+__global__ __launch_bounds__(256) void ka_faces(AdjArgs a) {
+  const int n = a.nx * a.ny;
+  for (int o = blockIdx.x * blockDim.x + threadIdx.x; o < n; o += gridDim.x * blockDim.x) {
+    const int j = o / a.nx;
+    ka_faces_cell(a, o - j * a.nx, j);
+  }
+}
+
+__global__ __launch_bounds__(256) void ka_cells(AdjArgs a) {
+  const int n = a.nx * a.ny;
+  for (int o = blockIdx.x * blockDim.x + threadIdx.x; o < n; o += gridDim.x * blockDim.x) {
+    const int j = o / a.nx;
+    ka_cells_cell(a, o - j * a.nx, j);
+  }
+}
+
+__global__ __launch_bounds__(256) void ka_bfaces(AdjArgs a) {
+  const int n = a.nx * a.ny;
+  for (int o = blockIdx.x * blockDim.x + threadIdx.x; o < n; o += gridDim.x * blockDim.x) {
+    const int j = o / a.nx;
+    ka_bfaces_cell(a, o - j * a.nx, j);
+  }
+}
+
+__global__ __launch_bounds__(256) void ka_Jt(LinArgs g, const double* q, double c, double* fx, double* fy) {
+  FOR_CELLS(g) Jt_cell(g, q, c, fx, fy, i_, j_);
+}
+
 __global__ __launch_bounds__(256) void ka_Jt2(LinArgs g, const double* q, double c, const double* bx, const double* by,
                                               double* tx, double* ty, double* ox, double* oy) {
-  const int n = g.nx * g.ny;
-  for (int o = blockIdx.x * blockDim.x + threadIdx.x; o < n; o += gridDim.x * blockDim.x) {
-    const int j = o / g.nx, i = o - j * g.nx;
-    const double ax = c * (q[wp(j + 1, g.ny) * g.nx + i] - q[o]) * g.idy;
-    const double ay = c * (q[o] - q[j * g.nx + wp(i + 1, g.nx)]) * g.idx;
-    tx[o] = ax;
-    ty[o] = ay;
-    ox[o] = bx[o] + ax;
-    oy[o] = by[o] + ay;
-  }
+  FOR_CELLS(g) Jt2_cell(g, q, c, bx, by, tx, ty, ox, oy, i_, j_);
 }
 
-// (fx, fy) = (bx, by) + c R(x)
 __global__ __launch_bounds__(256) void ka_R(LinArgs g, const double* bx, const double* by, const double* x,
                                             const CgState* st, double c, double* fx, double* fy) {
-  const int n = g.nx * g.ny;
   const bool zx = x_is_zero(st);
-  for (int o = blockIdx.x * blockDim.x + threadIdx.x; o < n; o += gridDim.x * blockDim.x) {
-    const int j = o / g.nx, i = o - j * g.nx;
-    const double xc = zx ? 0.0 : x[o];
-    const double xn = zx ? 0.0 : x[wp(j + 1, g.ny) * g.nx + i];
-    const double xe = zx ? 0.0 : x[j * g.nx + wp(i + 1, g.nx)];
-    fx[o] = bx[o] + c * (-(xn - xc) * g.idy);
-    fy[o] = by[o] + c * ((xe - xc) * g.idx);
-  }
+  FOR_CELLS(g) R_cell(g, bx, by, x, zx, c, fx, fy, i_, j_);
 }
 
-// q = c R^T(fx, fy)
 __global__ __launch_bounds__(256) void ka_Rt(LinArgs g, const double* fx, const double* fy, double c, double* q) {
-  const int n = g.nx * g.ny;
-  for (int o = blockIdx.x * blockDim.x + threadIdx.x; o < n; o += gridDim.x * blockDim.x) {
-    const int j = o / g.nx, i = o - j * g.nx;
-    q[o] = c * ((fx[o] - fx[wp(j - 1, g.ny) * g.nx + i]) * g.idy + (fy[j * g.nx + wp(i - 1, g.nx)] - fy[o]) * g.idx);
-  }
-}
-
-__device__ __forceinline__ void block_partial(double acc, double* partial) {
-  __shared__ double lds[8];
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
-  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-  if (l == 0) lds[w] = acc;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double t = lds[0];
-    for (int ww = 1; ww < (int)(blockDim.x >> 6); ++ww) t += lds[ww];
-    partial[blockIdx.x] = t;
-  }
+  FOR_CELLS(g) Rt_cell(g, fx, fy, c, q, i_, j_);
 }
 
 // partial[b] = block sum of u . Lap(v)  (0 if v is a solution of a solve with no product)
 __global__ __launch_bounds__(256) void ka_dot_lap(LinArgs g, const double* u, const double* v, const CgState* st,
                                                   double* partial) {
-  const int n = g.nx * g.ny;
-  const bool zv = x_is_zero(st);
   double acc = 0.0;
-  if (!zv) {
-    for (int o = blockIdx.x * blockDim.x + threadIdx.x; o < n; o += gridDim.x * blockDim.x) {
-      const int j = o / g.nx, i = o - j * g.nx;
-      const double lv = (v[j * g.nx + wp(i - 1, g.nx)] - 2.0 * v[o] + v[j * g.nx + wp(i + 1, g.nx)]) * (g.idx * g.idx) +
-                        (v[wp(j - 1, g.ny) * g.nx + i] - 2.0 * v[o] + v[wp(j + 1, g.ny) * g.nx + i]) * (g.idy * g.idy);
-      acc += u[o] * lv;
-    }
-  }
-  block_partial(acc, partial);
+  if (!x_is_zero(st)) FOR_CELLS(g) acc += lap_dot_cell(g, u, v, i_, j_);
+  block_partial(acc, partial + blockIdx.x);
 }
 
 // partial[b] = block sum of q . J(bx, by)
 __global__ __launch_bounds__(256) void ka_dot_J(LinArgs g, const double* q, const double* bx, const double* by,
                                                 double* partial) {
-  const int n = g.nx * g.ny;
   double acc = 0.0;
-  for (int o = blockIdx.x * blockDim.x + threadIdx.x; o < n; o += gridDim.x * blockDim.x) {
-    const int j = o / g.nx, i = o - j * g.nx;
-    acc += q[o] * corner_J(g, bx, by, i, j);
-  }
-  block_partial(acc, partial);
+  FOR_CELLS(g) acc += q[o_] * corner_J(g, bx, by, i_, j_);
+  block_partial(acc, partial + blockIdx.x);
 }
 
 struct EtaScales {
@@ -164,27 +128,13 @@ struct EtaScales {
 __global__ __launch_bounds__(256) void ka_accumulate(const double* partial, int nslots, int nb, EtaScales sc,
                                                      double* eta_bar) {
   __shared__ double lds[8];
-  __shared__ double tot[6];
+  double e = 0.0;
   for (int k = 0; k < nslots; ++k) {
     double acc = 0.0;
     for (int b = threadIdx.x; b < nb; b += blockDim.x) acc += partial[k * nb + b];
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
-    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-    if (l == 0) lds[w] = acc;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      double t = lds[0];
-      for (int ww = 1; ww < (int)(blockDim.x >> 6); ++ww) t += lds[ww];
-      tot[k] = t;
-    }
-    __syncthreads();
+    e += sc.s[k] * block_total(acc, lds);
   }
-  if (threadIdx.x == 0) {
-    double e = eta_bar[0];
-    for (int k = 0; k < nslots; ++k) e += sc.s[k] * tot[k];
-    eta_bar[0] = e;
-  }
+  if (threadIdx.x == 0) eta_bar[0] += e;
 }
 
 // x = 0 where the solve did no product
@@ -193,86 +143,30 @@ __global__ __launch_bounds__(256) void ka_zero_if_no_product(const CgState* st, 
   for (int o = blockIdx.x * blockDim.x + threadIdx.x; o < n; o += gridDim.x * blockDim.x) x[o] = 0.0;
 }
 
-struct Fields6 {
-  double* f[6];
-};
-
-// Entry of stage 2's adjoint: Wbar = U3*bar with U3*.hbar = lam.h, the density
-// floor of U3* as a mask (U3*.rho = the next state's density), fbar = ca Wbar,
-// Sbar = 0.
 __global__ __launch_bounds__(256) void ka_adj_stage2_in(int n, Fields6 lam, Fields6 wb, Fields6 fb, Fields6 sb,
                                                         const double* rho_next, double ca) {
-  for (int o = blockIdx.x * blockDim.x + threadIdx.x; o < n; o += gridDim.x * blockDim.x) {
-#pragma unroll
-    for (int q = 0; q < 6; ++q) {
-      double w = (q < 4) ? lam.f[q][o] : wb.f[q][o];
-      if (q == 0 && !(rho_next[o] > kFloor)) w = 0.0;
-      if (q < 4) wb.f[q][o] = w;
-      fb.f[q][o] = ca * w;
-      sb.f[q][o] = 0.0;
-    }
-  }
+  for (int o = blockIdx.x * blockDim.x + threadIdx.x; o < n; o += gridDim.x * blockDim.x)
+    stage2_in_cell(o, lam, wb, fb, sb, rho_next, ca);
 }
 
-// Exit of stage 1's adjoint: density floor of S* as a mask (S.rho = S*.rho),
-// fbar = cs S*bar + cw Wbar, lam = S*bar + Wbar.
 __global__ __launch_bounds__(256) void ka_adj_stage1_out(int n, Fields6 sb, Fields6 wb, Fields6 fb, Fields6 lam,
                                                          const double* rho_s, double cs, double cw) {
-  for (int o = blockIdx.x * blockDim.x + threadIdx.x; o < n; o += gridDim.x * blockDim.x) {
-#pragma unroll
-    for (int q = 0; q < 6; ++q) {
-      double sv = sb.f[q][o];
-      if (q == 0 && !(rho_s[o] > kFloor)) sv = 0.0;
-      const double wv = wb.f[q][o];
-      fb.f[q][o] = cs * sv + cw * wv;
-      lam.f[q][o] = sv + wv;
-    }
-  }
+  for (int o = blockIdx.x * blockDim.x + threadIdx.x; o < n; o += gridDim.x * blockDim.x)
+    stage1_out_cell(o, sb, wb, fb, lam, rho_s, cs, cw);
 }
 
-// S Jacobi sweeps x <- (b + cx (x_w + x_e) + cy (x_s + x_n)) / d of
-// A x = b, A = d I - cx (E + W) - cy (N + S) (periodic), on T x T tiles
-// staged in LDS with a halo of S corners: each sweep shrinks the region
-// that is still exact by one, so the tile's T x T interior is exact after S
-// sweeps.  xin == nullptr starts from x = 0.
-template <int T, int S>
+// one block per Jacobi tile, tiles numbered x fastest
 __global__ __launch_bounds__(256) void ka_jacobi(LinArgs g, double cx, double cy, double invd, const double* b,
                                                  const double* xin, double* xout) {
-  constexpr int W = T + 2 * S;
-  __shared__ double xs[2][W * W];
-  __shared__ double bs[W * W];
-  const int tiles_x = (g.nx + T - 1) / T;
-  const int tx = blockIdx.x % tiles_x, ty = blockIdx.x / tiles_x;
-  const int x0 = tx * T - S, y0 = ty * T - S;
-  for (int k = threadIdx.x; k < W * W; k += blockDim.x) {
-    const int i = k % W, j = k / W;
-    const int gi = pmod(x0 + i, g.nx), gj = pmod(y0 + j, g.ny);
-    bs[k] = b[gj * g.nx + gi];
-    xs[0][k] = xin ? xin[gj * g.nx + gi] : 0.0;
-  }
-  __syncthreads();
-  for (int sw = 0; sw < S; ++sw) {
-    const double* cur = xs[sw & 1];
-    double* nxt = xs[(sw & 1) ^ 1];
-    for (int k = threadIdx.x; k < W * W; k += blockDim.x) {
-      const int i = k % W, j = k / W;
-      if (i > sw && i < W - 1 - sw && j > sw && j < W - 1 - sw)
-        nxt[k] = (bs[k] + cx * (cur[k - 1] + cur[k + 1]) + cy * (cur[k - W] + cur[k + W])) * invd;
-    }
-    __syncthreads();
-  }
-  const double* res = xs[S & 1];
-  for (int k = threadIdx.x; k < T * T; k += blockDim.x) {
-    const int i = k % T, j = k / T;
-    const int gi = tx * T + i, gj = ty * T + j;
-    if (gi < g.nx && gj < g.ny) xout[gj * g.nx + gi] = res[(j + S) * W + (i + S)];
-  }
+  const int tiles_x = (g.nx + kJacTile - 1) / kJacTile;
+  jacobi_tile<kJacTile, kJacSweeps>(g, cx, cy, invd, b, xin, xout, blockIdx.x % tiles_x, blockIdx.x / tiles_x);
 }
-template __global__ void ka_jacobi<32, 8>(LinArgs, double, double, double, const double*, const double*, double*);
 
 __global__ __launch_bounds__(256) void ka_zero(int n, double* y) {
   for (int o = blockIdx.x * blockDim.x + threadIdx.x; o < n; o += gridDim.x * blockDim.x) y[o] = 0.0;
 }
+
+#undef FOR_CELLS
 
 }  // namespace mhd
 
@@ -481,15 +375,15 @@ int adjoint_solve(mhd_ctx* c, const double* xbar, int spec, CgState* state) {
     const double cx = c->alpha / (c->dx * c->dx), cy = c->alpha / (c->dy * c->dy);
     const double invd = 1.0 / (1 + 2 * (cx + cy));
     LinArgs g = lin_args(c);
-    const int tiles = ((c->nx + 31) / 32) * ((c->ny + 31) / 32);
-    const int launches = (sweeps + 7) / 8;
+    const int tiles = ((c->nx + kJacTile - 1) / kJacTile) * ((c->ny + kJacTile - 1) / kJacTile);
+    const int launches = (sweeps + kJacSweeps - 1) / kJacSweeps;
     // ping-pong so that the last launch writes c->x
     double* bufs[2] = {(launches & 1) ? c->x : A.jac, (launches & 1) ? A.jac : c->x};
     const double* in = nullptr;
     for (int k = 0; k < launches; ++k) {
       double* out = bufs[k & 1];
       int rc = launch(c, 8, [&] {
-        hipLaunchKernelGGL((ka_jacobi<32, 8>), dim3(tiles), dim3(256), 0, c->stream, g, cx, cy, invd, xbar, in, out);
+        hipLaunchKernelGGL(ka_jacobi, dim3(tiles), dim3(256), 0, c->stream, g, cx, cy, invd, xbar, in, out);
       });
       if (rc) return rc;
       in = out;

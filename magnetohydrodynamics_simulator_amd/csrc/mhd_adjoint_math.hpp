@@ -1,7 +1,8 @@
 // mhd_adjoint_math.hpp -- reverse-mode (vector-Jacobian) products of the
 // per-face device math of mhd_device.hpp (physical flux, fast wave speed, HLL
-// x/y, MUSCL-minmod) and the kernels of the explicit right-hand side's
-// adjoint built from them.  Included by mhd_adjoint.hip only.
+// x/y, MUSCL-minmod) and the per-cell bodies of the explicit right-hand
+// side's adjoint built from them (run by the single-run kernels of
+// mhd_adjoint.hip and the batched ones of mhd_batch_adjoint.hip).
 #pragma once
 
 #include "mhd_device.hpp"
@@ -160,7 +161,7 @@ __device__ __forceinline__ int wp(int i, int n) { return i >= n ? i - n : (i < 0
 
 // ---------------------------------------------------------------------------
 // Adjoint of the explicit right-hand side L at state s:  out += L'(s)^T lam.
-// Kernels (one thread per cell, periodic):
+// Per-cell bodies, run in this order over all cells (periodic):
 //   ka_prims   primitives P and the pressure-floor mask
 //   ka_faces   flux adjoints of the right x-face and top y-face of the cell
 //              (hydro from lam, induction from the corner E_z adjoint through
@@ -187,22 +188,20 @@ struct AdjArgs {
   double* out[6];        // += L'^T lam
 };
 
-__global__ __launch_bounds__(256) void ka_prims(AdjArgs a) {
-  const int n = a.nx * a.ny;
-  for (int o = blockIdx.x * blockDim.x + threadIdx.x; o < n; o += gridDim.x * blockDim.x) {
-    const int j = o / a.nx, i = o - j * a.nx;
-    const double bxc = (a.s[4][o] + a.s[4][j * a.nx + wp(i + 1, a.nx)]) / 2.0;
-    const double byc = (a.s[5][o] + a.s[5][wp(j + 1, a.ny) * a.nx + i]) / 2.0;
-    double rf, vx, vy, p;
-    primitives(a.s[0][o], a.s[1][o], a.s[2][o], a.s[3][o], bxc, byc, rf, vx, vy, p);
-    // pressure-floor mask: recompute the unfloored pressure
-    const double kin = 0.5 * rf * (vx * vx + vy * vy);
-    const double mag = (bxc * bxc + byc * byc) / 2;
-    const double praw = kGammaM1 * (a.s[3][o] - kin - mag);
-    a.P[0][o] = rf; a.P[1][o] = vx; a.P[2][o] = vy; a.P[3][o] = p; a.P[4][o] = bxc; a.P[5][o] = byc;
-    a.P[6][o] = (praw < kFloor) ? 0.0 : 1.0;
-  }
+__device__ __forceinline__ void ka_prims_cell(const AdjArgs& a, int i, int j) {
+  const int o = j * a.nx + i;
+  const double bxc = (a.s[4][o] + a.s[4][j * a.nx + wp(i + 1, a.nx)]) / 2.0;
+  const double byc = (a.s[5][o] + a.s[5][wp(j + 1, a.ny) * a.nx + i]) / 2.0;
+  double rf, vx, vy, p;
+  primitives(a.s[0][o], a.s[1][o], a.s[2][o], a.s[3][o], bxc, byc, rf, vx, vy, p);
+  // pressure-floor mask: recompute the unfloored pressure
+  const double kin = 0.5 * rf * (vx * vx + vy * vy);
+  const double mag = (bxc * bxc + byc * byc) / 2;
+  const double praw = kGammaM1 * (a.s[3][o] - kin - mag);
+  a.P[0][o] = rf; a.P[1][o] = vx; a.P[2][o] = vy; a.P[3][o] = p; a.P[4][o] = bxc; a.P[5][o] = byc;
+  a.P[6][o] = (praw < kFloor) ? 0.0 : 1.0;
 }
+
 
 __device__ __forceinline__ double ez_adj(const AdjArgs& a, int i, int j) {
   // E_z adjoint at corner (i, j) from the CT update
@@ -244,124 +243,121 @@ __device__ __forceinline__ void recon(const AdjArgs& a, int dir, int i, int j, d
   }
 }
 
-__global__ __launch_bounds__(256) void ka_faces(AdjArgs a) {
-  const int nx = a.nx, ny = a.ny, n = nx * ny;
-  for (int o = blockIdx.x * blockDim.x + threadIdx.x; o < n; o += gridDim.x * blockDim.x) {
-    const int j = o / nx, i = o - j * nx;
-    const int ip = wp(i + 1, nx), jp = wp(j + 1, ny);
-    // ---- right x-face of (i, j): between (i, j) and (i+1, j)
-    {
-      double Fb[5];
+__device__ __forceinline__ void ka_faces_cell(const AdjArgs& a, int i, int j) {
+  const int nx = a.nx, ny = a.ny;
+  const int o = j * a.nx + i;
+  const int ip = wp(i + 1, nx), jp = wp(j + 1, ny);
+  // ---- right x-face of (i, j): between (i, j) and (i+1, j)
+  {
+    double Fb[5];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) Fb[q] = (a.lam[q][j * nx + ip] - a.lam[q][o]) * a.idx;
-      // F.b enters E_z at corners (i+1, j) as -eN and (i+1, j+1) as -eS
-      double wS, wN, wW, wE;
-      ez_weights(a, ip, j, wS, wN, wW, wE);
-      double fbb = -0.5 * wN * ez_adj(a, ip, j);
-      ez_weights(a, ip, jp, wS, wN, wW, wE);
-      fbb += -0.5 * wS * ez_adj(a, ip, jp);
-      Fb[4] = fbb;
-      double lo[5], hi[5], lo2[5], hi2[5];
-      recon(a, 0, i, j, lo, hi);
-      recon(a, 0, ip, j, lo2, hi2);
-      const double bn = a.s[4][j * nx + ip];
-      const Prim L{hi[0], hi[1], hi[2], hi[3], bn, hi[4]};
-      const Prim R{lo2[0], lo2[1], lo2[2], lo2[3], bn, lo2[4]};
-      double Lb[6] = {0, 0, 0, 0, 0, 0}, Rb[6] = {0, 0, 0, 0, 0, 0};
-      hll_x_vjp(L, R, Fb, Lb, Rb);
-      a.xrb[0][o] = Lb[0]; a.xrb[1][o] = Lb[1]; a.xrb[2][o] = Lb[2]; a.xrb[3][o] = Lb[3]; a.xrb[4][o] = Lb[5];
-      const int o2 = j * nx + ip;
-      a.xlb[0][o2] = Rb[0]; a.xlb[1][o2] = Rb[1]; a.xlb[2][o2] = Rb[2]; a.xlb[3][o2] = Rb[3]; a.xlb[4][o2] = Rb[5];
-      a.bnx[o2] = Lb[4] + Rb[4];
-    }
-    // ---- top y-face of (i, j): between (i, j) and (i, j+1)
-    {
-      double Gb[5];
+    for (int q = 0; q < 4; ++q) Fb[q] = (a.lam[q][j * nx + ip] - a.lam[q][o]) * a.idx;
+    // F.b enters E_z at corners (i+1, j) as -eN and (i+1, j+1) as -eS
+    double wS, wN, wW, wE;
+    ez_weights(a, ip, j, wS, wN, wW, wE);
+    double fbb = -0.5 * wN * ez_adj(a, ip, j);
+    ez_weights(a, ip, jp, wS, wN, wW, wE);
+    fbb += -0.5 * wS * ez_adj(a, ip, jp);
+    Fb[4] = fbb;
+    double lo[5], hi[5], lo2[5], hi2[5];
+    recon(a, 0, i, j, lo, hi);
+    recon(a, 0, ip, j, lo2, hi2);
+    const double bn = a.s[4][j * nx + ip];
+    const Prim L{hi[0], hi[1], hi[2], hi[3], bn, hi[4]};
+    const Prim R{lo2[0], lo2[1], lo2[2], lo2[3], bn, lo2[4]};
+    double Lb[6] = {0, 0, 0, 0, 0, 0}, Rb[6] = {0, 0, 0, 0, 0, 0};
+    hll_x_vjp(L, R, Fb, Lb, Rb);
+    a.xrb[0][o] = Lb[0]; a.xrb[1][o] = Lb[1]; a.xrb[2][o] = Lb[2]; a.xrb[3][o] = Lb[3]; a.xrb[4][o] = Lb[5];
+    const int o2 = j * nx + ip;
+    a.xlb[0][o2] = Rb[0]; a.xlb[1][o2] = Rb[1]; a.xlb[2][o2] = Rb[2]; a.xlb[3][o2] = Rb[3]; a.xlb[4][o2] = Rb[5];
+    a.bnx[o2] = Lb[4] + Rb[4];
+  }
+  // ---- top y-face of (i, j): between (i, j) and (i, j+1)
+  {
+    double Gb[5];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) Gb[q] = (a.lam[q][jp * nx + i] - a.lam[q][o]) * a.idy;
-      // G.b enters E_z at corners (i, j+1) as eE and (i+1, j+1) as eW
-      double wS, wN, wW, wE;
-      ez_weights(a, i, jp, wS, wN, wW, wE);
-      double gbb = 0.5 * wE * ez_adj(a, i, jp);
-      ez_weights(a, ip, jp, wS, wN, wW, wE);
-      gbb += 0.5 * wW * ez_adj(a, ip, jp);
-      Gb[4] = gbb;
-      double lo[5], hi[5], lo2[5], hi2[5];
-      recon(a, 1, i, j, lo, hi);
-      recon(a, 1, i, jp, lo2, hi2);
-      const double bn = a.s[5][jp * nx + i];
-      const Prim D{hi[0], hi[1], hi[2], hi[3], hi[4], bn};
-      const Prim U{lo2[0], lo2[1], lo2[2], lo2[3], lo2[4], bn};
-      double Db[6] = {0, 0, 0, 0, 0, 0}, Ub[6] = {0, 0, 0, 0, 0, 0};
-      hll_y_vjp(D, U, Gb, Db, Ub);
-      a.yrb[0][o] = Db[0]; a.yrb[1][o] = Db[1]; a.yrb[2][o] = Db[2]; a.yrb[3][o] = Db[3]; a.yrb[4][o] = Db[4];
-      const int o2 = jp * nx + i;
-      a.ylb[0][o2] = Ub[0]; a.ylb[1][o2] = Ub[1]; a.ylb[2][o2] = Ub[2]; a.ylb[3][o2] = Ub[3]; a.ylb[4][o2] = Ub[4];
-      a.bny[o2] = Db[5] + Ub[5];
-    }
+    for (int q = 0; q < 4; ++q) Gb[q] = (a.lam[q][jp * nx + i] - a.lam[q][o]) * a.idy;
+    // G.b enters E_z at corners (i, j+1) as eE and (i+1, j+1) as eW
+    double wS, wN, wW, wE;
+    ez_weights(a, i, jp, wS, wN, wW, wE);
+    double gbb = 0.5 * wE * ez_adj(a, i, jp);
+    ez_weights(a, ip, jp, wS, wN, wW, wE);
+    gbb += 0.5 * wW * ez_adj(a, ip, jp);
+    Gb[4] = gbb;
+    double lo[5], hi[5], lo2[5], hi2[5];
+    recon(a, 1, i, j, lo, hi);
+    recon(a, 1, i, jp, lo2, hi2);
+    const double bn = a.s[5][jp * nx + i];
+    const Prim D{hi[0], hi[1], hi[2], hi[3], hi[4], bn};
+    const Prim U{lo2[0], lo2[1], lo2[2], lo2[3], lo2[4], bn};
+    double Db[6] = {0, 0, 0, 0, 0, 0}, Ub[6] = {0, 0, 0, 0, 0, 0};
+    hll_y_vjp(D, U, Gb, Db, Ub);
+    a.yrb[0][o] = Db[0]; a.yrb[1][o] = Db[1]; a.yrb[2][o] = Db[2]; a.yrb[3][o] = Db[3]; a.yrb[4][o] = Db[4];
+    const int o2 = jp * nx + i;
+    a.ylb[0][o2] = Ub[0]; a.ylb[1][o2] = Ub[1]; a.ylb[2][o2] = Ub[2]; a.ylb[3][o2] = Ub[3]; a.ylb[4][o2] = Ub[4];
+    a.bny[o2] = Db[5] + Ub[5];
   }
 }
 
-__global__ __launch_bounds__(256) void ka_cells(AdjArgs a) {
-  const int nx = a.nx, ny = a.ny, n = nx * ny;
-  for (int o = blockIdx.x * blockDim.x + threadIdx.x; o < n; o += gridDim.x * blockDim.x) {
-    const int j = o / nx, i = o - j * nx;
-    // primitive adjoint: rho, vx, vy, p, bxc, byc
-    double pb[6] = {0, 0, 0, 0, 0, 0};
-    for (int dir = 0; dir < 2; ++dir) {
-      const int tb = (dir == 0) ? 5 : 4;
-      const int vars[5] = {0, 1, 2, 3, tb};
-      double* const* lob = (dir == 0) ? a.xlb : a.ylb;
-      double* const* hib = (dir == 0) ? a.xrb : a.yrb;
-      // cells m = -1, 0, +1 along dir whose reconstruction reads this cell
-      for (int m = -1; m <= 1; ++m) {
-        const int ci = (dir == 0) ? wp(i + m, nx) : i;
-        const int cj = (dir == 0) ? j : wp(j + m, ny);
-        const int oc = cj * nx + ci;
-        const int om = (dir == 0) ? cj * nx + wp(ci - 1, nx) : wp(cj - 1, ny) * nx + ci;
-        const int op = (dir == 0) ? cj * nx + wp(ci + 1, nx) : wp(cj + 1, ny) * nx + ci;
+
+__device__ __forceinline__ void ka_cells_cell(const AdjArgs& a, int i, int j) {
+  const int nx = a.nx, ny = a.ny;
+  const int o = j * a.nx + i;
+  // primitive adjoint: rho, vx, vy, p, bxc, byc
+  double pb[6] = {0, 0, 0, 0, 0, 0};
+  for (int dir = 0; dir < 2; ++dir) {
+    const int tb = (dir == 0) ? 5 : 4;
+    const int vars[5] = {0, 1, 2, 3, tb};
+    double* const* lob = (dir == 0) ? a.xlb : a.ylb;
+    double* const* hib = (dir == 0) ? a.xrb : a.yrb;
+    // cells m = -1, 0, +1 along dir whose reconstruction reads this cell
+    for (int m = -1; m <= 1; ++m) {
+      const int ci = (dir == 0) ? wp(i + m, nx) : i;
+      const int cj = (dir == 0) ? j : wp(j + m, ny);
+      const int oc = cj * nx + ci;
+      const int om = (dir == 0) ? cj * nx + wp(ci - 1, nx) : wp(cj - 1, ny) * nx + ci;
+      const int op = (dir == 0) ? cj * nx + wp(ci + 1, nx) : wp(cj + 1, ny) * nx + ci;
 #pragma unroll
-        for (int q = 0; q < 5; ++q) {
-          const double* f = a.P[vars[q]];
-          double fmb = 0.0, f0b = 0.0, fpb = 0.0;
-          muscl_vjp(f[om], f[oc], f[op], lob[q][oc], hib[q][oc], fmb, f0b, fpb);
-          // this cell is cell m's right neighbour (m = -1), itself (0), left neighbour (+1)
-          pb[vars[q]] += (m == -1) ? fpb : (m == 0 ? f0b : fmb);
-        }
+      for (int q = 0; q < 5; ++q) {
+        const double* f = a.P[vars[q]];
+        double fmb = 0.0, f0b = 0.0, fpb = 0.0;
+        muscl_vjp(f[om], f[oc], f[op], lob[q][oc], hib[q][oc], fmb, f0b, fpb);
+        // this cell is cell m's right neighbour (m = -1), itself (0), left neighbour (+1)
+        pb[vars[q]] += (m == -1) ? fpb : (m == 0 ? f0b : fmb);
       }
     }
-    // primitives (Grid::computePrimitives, Grid.hpp:111-126) -> conserved
-    const double rho = a.s[0][o];
-    const double rf = a.P[0][o], vx = a.P[1][o], vy = a.P[2][o], bxc = a.P[4][o], byc = a.P[5][o];
-    const double pr = pb[3] * a.P[6][o];          // floor mask of p
-    const double kinb = -kGammaM1 * pr, magb = -kGammaM1 * pr;
-    double Eb = kGammaM1 * pr;
-    double rfb = pb[0] + kinb * 0.5 * (vx * vx + vy * vy);
-    double vxb = pb[1] + kinb * rf * vx;
-    double vyb = pb[2] + kinb * rf * vy;
-    const double bxcb = pb[4] + magb * bxc;
-    const double bycb = pb[5] + magb * byc;
-    const double ir = 1.0 / rho;
-    double rhob = (rho < kFloor) ? 0.0 : rfb;
-    const double mxb = vxb * ir, myb = vyb * ir;
-    rhob += -(vxb * vx + vyb * vy) * ir;
-    a.out[0][o] += rhob;
-    a.out[1][o] += mxb;
-    a.out[2][o] += myb;
-    a.out[3][o] += Eb;
-    a.bcb[0][o] = bxcb;
-    a.bcb[1][o] = bycb;
   }
+  // primitives (Grid::computePrimitives, Grid.hpp:111-126) -> conserved
+  const double rho = a.s[0][o];
+  const double rf = a.P[0][o], vx = a.P[1][o], vy = a.P[2][o], bxc = a.P[4][o], byc = a.P[5][o];
+  const double pr = pb[3] * a.P[6][o];          // floor mask of p
+  const double kinb = -kGammaM1 * pr, magb = -kGammaM1 * pr;
+  double Eb = kGammaM1 * pr;
+  double rfb = pb[0] + kinb * 0.5 * (vx * vx + vy * vy);
+  double vxb = pb[1] + kinb * rf * vx;
+  double vyb = pb[2] + kinb * rf * vy;
+  const double bxcb = pb[4] + magb * bxc;
+  const double bycb = pb[5] + magb * byc;
+  const double ir = 1.0 / rho;
+  double rhob = (rho < kFloor) ? 0.0 : rfb;
+  const double mxb = vxb * ir, myb = vyb * ir;
+  rhob += -(vxb * vx + vyb * vy) * ir;
+  a.out[0][o] += rhob;
+  a.out[1][o] += mxb;
+  a.out[2][o] += myb;
+  a.out[3][o] += Eb;
+  a.bcb[0][o] = bxcb;
+  a.bcb[1][o] = bycb;
 }
 
-__global__ __launch_bounds__(256) void ka_bfaces(AdjArgs a) {
-  const int nx = a.nx, ny = a.ny, n = nx * ny;
-  for (int o = blockIdx.x * blockDim.x + threadIdx.x; o < n; o += gridDim.x * blockDim.x) {
-    const int j = o / nx, i = o - j * nx;
-    // bxf(i) feeds bxc(i - 1) and bxc(i) with weight 1/2 and is the normal field of its face
-    a.out[4][o] += 0.5 * (a.bcb[0][j * nx + wp(i - 1, nx)] + a.bcb[0][o]) + a.bnx[o];
-    a.out[5][o] += 0.5 * (a.bcb[1][wp(j - 1, ny) * nx + i] + a.bcb[1][o]) + a.bny[o];
-  }
+
+__device__ __forceinline__ void ka_bfaces_cell(const AdjArgs& a, int i, int j) {
+  const int nx = a.nx, ny = a.ny;
+  const int o = j * a.nx + i;
+  // bxf(i) feeds bxc(i - 1) and bxc(i) with weight 1/2 and is the normal field of its face
+  a.out[4][o] += 0.5 * (a.bcb[0][j * nx + wp(i - 1, nx)] + a.bcb[0][o]) + a.bnx[o];
+  a.out[5][o] += 0.5 * (a.bcb[1][wp(j - 1, ny) * nx + i] + a.bcb[1][o]) + a.bny[o];
 }
+
 
 }  // namespace mhd

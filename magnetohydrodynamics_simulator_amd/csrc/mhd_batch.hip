@@ -23,66 +23,14 @@
 #include <vector>
 
 #include "../../include/mhd_mi355x.h"
+#include "mhd_batch.hpp"
 #include "mhd_kernels.hpp"
 
 using namespace mhd;
 
-namespace {
+namespace mhd_batch_host {
 
 thread_local std::string g_batch_create_error;
-
-// Candidate launch shapes of the autotuner: rows per stage-kernel segment and
-// rows per 2D tile.  Buffers are sized for the smallest of each.
-constexpr int kStageRowChoices[] = {16, 32, 64};
-constexpr int kTileRowChoices[] = {16, 32, 64};
-
-struct BatchTableau {
-  double gamma = (2.0 - std::sqrt(2.0)) / 2.0;
-  double delta = 1.0 - (1.0 / (2.0 * gamma));
-  double A_ex3 = gamma, A_ex6 = delta, A_ex7 = 1.0 - delta;
-  double A_im4 = gamma, A_im7 = 1.0 - gamma;
-};
-
-}  // namespace
-
-struct mhd_batch {
-  int nruns = 0;
-  long long total = 0;              // cells of all runs
-  int total_tiles = 0;
-  mhd_opts o{};
-  BatchTableau tab;
-  hipStream_t stream = nullptr;
-  std::vector<RunGeom> geom_h;
-  RunGeom* geom = nullptr;
-  RunState* rs = nullptr;
-  RunState* rs_host = nullptr;      // pinned
-  double* U[6] = {};
-  double* S[6] = {};
-  double* W[6] = {};
-  double* x = nullptr;
-  double* r = nullptr;
-  double* p[2] = {};
-  double* part = nullptr;           // CFL: two partials per tile
-  double* part_s = nullptr;
-  double* part_a = nullptr;
-  double* part_b = nullptr;
-  CgState* st[2] = {};
-  CgState* st_host = nullptr;       // pinned, one solve slot (nruns states)
-  int2* stage_jobs = nullptr;
-  int n_stage_jobs = 0;
-  int2* tile_jobs = nullptr;
-  int n_tile_jobs = 0;
-  size_t cap_stage_jobs = 0, cap_tiles = 0;   // capacity of the job tables and partial arrays
-  int stage_rows = 32, tile_rows = 32;
-  int last_spmv[2] = {1, 1};
-  bool any_eta = false;
-  long long steps_taken = 0;
-  std::vector<int> cg_iters_last;   // 2 per run: Eigen iteration counts of the last step
-  double* stage = nullptr;          // pinned staging buffer of the largest run (host transfers)
-  std::string err;
-};
-
-namespace {
 
 int bfail(mhd_batch* b, int code, const std::string& msg) {
   if (b) b->err = msg;
@@ -90,19 +38,16 @@ int bfail(mhd_batch* b, int code, const std::string& msg) {
   return code;
 }
 
-#define BHIP(b, call)                                                                           \
-  do {                                                                                          \
-    hipError_t e_ = (call);                                                                     \
-    if (e_ != hipSuccess)                                                                       \
-      return bfail((b), MHD_EHIP, std::string(#call) + ": " + hipGetErrorString(e_));           \
-  } while (0)
+}  // namespace mhd_batch_host
 
-#define BLAUNCH(b, ...)                                                                         \
-  do {                                                                                          \
-    hipLaunchKernelGGL(__VA_ARGS__);                                                            \
-    hipError_t e_ = hipGetLastError();                                                          \
-    if (e_ != hipSuccess) return bfail((b), MHD_EHIP, std::string("kernel launch: ") + hipGetErrorString(e_)); \
-  } while (0)
+using namespace mhd_batch_host;
+
+namespace {
+
+// Candidate launch shapes of the autotuner: rows per stage-kernel segment and
+// rows per 2D tile.  Buffers are sized for the smallest of each.
+constexpr int kStageRowChoices[] = {16, 32, 64};
+constexpr int kTileRowChoices[] = {16, 32, 64};
 
 // Lays out the runs: pooled offsets, stage segments and 2D tiles, job tables.
 void plan(mhd_batch* b, std::vector<int2>& stage_jobs, std::vector<int2>& tile_jobs) {
@@ -148,38 +93,52 @@ int replan(mhd_batch* b, int stage_rows, int tile_rows) {
   return MHD_OK;
 }
 
-BatchCgArgs cg_args(mhd_batch* b, int slot, const double* bx, const double* by) {
+BatchCgArgs cg_args(mhd_batch* b, const CgCall& c) {
   BatchCgArgs a{};
   a.geom = b->geom;
-  a.rs = b->rs;
+  a.rs = c.rs;
   a.jobs = b->tile_jobs;
-  a.bx = bx;
-  a.by = by;
+  a.bx = c.bx;
+  a.by = c.by;
   a.r = b->r;
   a.p0 = b->p[0];
-  a.x = b->x;
+  a.x = c.x;
   a.part_s = b->part_s;
   a.part_a = b->part_a;
   a.part_b = b->part_b;
-  a.st = b->st[slot];
+  a.st = c.st;
   a.tol = b->o.cg_tol;
   return a;
 }
 
-// One CG solve of every active run with eta > 0 (Eigen's algorithm per run,
-// the single-run kernel sequence; see cg_run in mhd_capi.hip).  The host
-// checks all runs' states after each chunk; the first chunk is as long as the
-// longest solve of this slot in the previous step.
-int cg_batch(mhd_batch* b, int slot, const double* bx, const double* by) {
-  BatchCgArgs a = cg_args(b, slot, bx, by);
+}  // namespace
+
+namespace mhd_batch_host {
+
+// The single-run kernel sequence of a solve (cg_run in mhd_capi.hip) over
+// all runs at once.
+int cg_batch(mhd_batch* b, const CgCall& c, int* most_spmv) {
+  BatchCgArgs a = cg_args(b, c);
   const int nb = b->n_tile_jobs;
-  BLAUNCH(b, kb_cg_start, dim3(nb), dim3(kTileX), 0, b->stream, a);
+  if (c.bin) BLAUNCH(b, kb_cg_start_rhs, dim3(nb), dim3(kTileX), 0, b->stream, a, c.bin);
+  else BLAUNCH(b, kb_cg_start, dim3(nb), dim3(kTileX), 0, b->stream, a);
   BLAUNCH(b, kb_cg_pass_b<true>, dim3(nb), dim3(kTileX), 0, b->stream, a, (const double*)b->p[0], 0);
   long long it = 1;
+  if (c.spec > 0) {
+    // A(1) B(1) ... A(spec) B(spec) A(spec + 1): the last A takes the decision
+    for (; it <= c.spec + 1; ++it) {
+      const double* pold = b->p[(it - 1) & 1];
+      double* pnew = b->p[it & 1];
+      BLAUNCH(b, kb_cg_pass_a, dim3(nb), dim3(kTileX), 0, b->stream, a, pold, pnew, (int)it);
+      if (it == c.spec + 1) break;
+      BLAUNCH(b, kb_cg_pass_b<false>, dim3(nb), dim3(kTileX), 0, b->stream, a, (const double*)pnew, (int)it);
+    }
+    return MHD_OK;
+  }
   long long maxit = 0;
   for (const RunGeom& g : b->geom_h) maxit = std::max(maxit, g.max_iters);
   for (;;) {
-    const int chunk = (it == 1) ? std::max(1, b->last_spmv[slot]) : 1;
+    const int chunk = (it == 1) ? std::max(1, b->last_spmv[c.slot]) : 1;
     for (int k = 0; k < chunk; ++k, ++it) {
       const double* pold = b->p[(it - 1) & 1];
       double* pnew = b->p[it & 1];
@@ -187,7 +146,7 @@ int cg_batch(mhd_batch* b, int slot, const double* bx, const double* by) {
       if (k + 1 == chunk) break;
       BLAUNCH(b, kb_cg_pass_b<false>, dim3(nb), dim3(kTileX), 0, b->stream, a, (const double*)pnew, (int)it);
     }
-    BHIP(b, hipMemcpyAsync(b->st_host, b->st[slot], sizeof(CgState) * b->nruns, hipMemcpyDeviceToHost, b->stream));
+    BHIP(b, hipMemcpyAsync(b->st_host, c.st, sizeof(CgState) * b->nruns, hipMemcpyDeviceToHost, b->stream));
     BHIP(b, hipStreamSynchronize(b->stream));
     bool all = true;
     for (int r = 0; r < b->nruns; ++r) all = all && b->st_host[r].done;
@@ -201,28 +160,73 @@ int cg_batch(mhd_batch* b, int slot, const double* bx, const double* by) {
   for (int r = 0; r < b->nruns; ++r) {
     const CgState& s = b->st_host[r];
     if (s.done != 3) most = std::max(most, s.spmv);
-    if (b->rs_host[r].active) b->cg_iters_last[2 * r + slot] = s.iters;   // runs that stepped (rs_host arrived before the first state copy)
+    // runs that stepped (rs_host arrived before the first state copy)
+    if (c.slot < 2 && b->rs_host[r].active) b->cg_iters_last[2 * r + c.slot] = s.iters;
     if (s.done == 2 && b->o.cg_fail_is_error)
       return bfail(b, MHD_ECG, "run " + std::to_string(r) + ": CG reached the iteration limit");
   }
-  if (most > 0) b->last_spmv[slot] = most;
+  if (most > 0) b->last_spmv[c.slot] = most;
+  if (most_spmv) *most_spmv = most;
   return MHD_OK;
 }
 
-BatchFinArgs fin_args(mhd_batch* b, int slot, const double* bx, const double* by, double* bxo, double* byo) {
+// Periodic ghosts of a padded (ny+4) x (nx+4) field (Grid::fillOneField,
+// Grid.hpp:143-161).
+void fill_ghosts_padded(double* f, int nx, int ny) {
+  const size_t nxt = (size_t)nx + 4;
+  for (int j = 2; j < ny + 2; ++j)
+    for (int k = 0; k < 2; ++k) {
+      f[j * nxt + k] = f[j * nxt + nx + k];
+      f[j * nxt + nx + 2 + k] = f[j * nxt + 2 + k];
+    }
+  for (size_t i = 0; i < nxt; ++i)
+    for (int k = 0; k < 2; ++k) {
+      f[k * nxt + i] = f[(ny + k) * nxt + i];
+      f[(ny + 2 + k) * nxt + i] = f[(2 + k) * nxt + i];
+    }
+}
+
+// Interior of a padded host field of run `run` into the pooled device array,
+// through the pinned staging buffer (a strided copy from pageable memory is slow).
+int stage_in(mhd_batch* b, int run, const double* host, double* pooled) {
+  const RunGeom& g = b->geom_h[run];
+  const size_t nx = (size_t)g.nx, nxt = nx + 4;
+  BHIP(b, hipStreamSynchronize(b->stream));   // the staging buffer is free again
+  for (int j = 0; j < g.ny; ++j) std::memcpy(b->stage + j * nx, host + (j + 2) * nxt + 2, nx * sizeof(double));
+  BHIP(b, hipMemcpyAsync(pooled + g.base, b->stage, nx * g.ny * sizeof(double), hipMemcpyHostToDevice, b->stream));
+  return MHD_OK;
+}
+
+// Run `run` of a pooled device array into a padded host field, ghosts filled.
+int stage_out(mhd_batch* b, int run, const double* pooled, double* host) {
+  const RunGeom& g = b->geom_h[run];
+  const size_t nx = (size_t)g.nx, nxt = nx + 4;
+  BHIP(b, hipMemcpyAsync(b->stage, pooled + g.base, nx * g.ny * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+  BHIP(b, hipStreamSynchronize(b->stream));
+  for (int j = 0; j < g.ny; ++j) std::memcpy(host + (j + 2) * nxt + 2, b->stage + (size_t)j * nx, nx * sizeof(double));
+  fill_ghosts_padded(host, g.nx, g.ny);
+  return MHD_OK;
+}
+
+BatchFinArgs fin_args(mhd_batch* b, const RunState* rs, const double* x, const CgState* st, const double* bx,
+                      const double* by, double* bxo, double* byo) {
   BatchFinArgs f{};
   f.geom = b->geom;
-  f.rs = b->rs;
+  f.rs = rs;
   f.jobs = b->tile_jobs;
-  f.x = b->x;
+  f.x = x;
   f.bx = bx;
   f.by = by;
   f.bx_out = bxo;
   f.by_out = byo;
-  f.st = b->st[slot];
+  f.st = st;
   f.fc = b->tab.A_im4;
   return f;
 }
+
+}  // namespace mhd_batch_host
+
+namespace {
 
 int check_run(mhd_batch* b, int run) {
   if (!b) return MHD_EINVAL;
@@ -334,6 +338,7 @@ int mhd_batch_create(int nruns, const int* nx, const int* ny, const double* dx, 
 int mhd_batch_destroy(mhd_batch* b) {
   if (!b) return MHD_EINVAL;
   if (b->stream) (void)hipStreamSynchronize(b->stream);
+  destroy_batch_adjoint(b);
   for (int q = 0; q < 6; ++q) {
     if (b->U[q]) (void)hipFree(b->U[q]);
     if (b->S[q]) (void)hipFree(b->S[q]);
@@ -380,13 +385,8 @@ int mhd_batch_upload(mhd_batch* b, int run, const double* rho, const double* mx,
   for (int q = 0; q < 6; ++q)
     if (!src[q]) return bfail(b, MHD_EINVAL, "mhd_batch_upload: NULL field");
   const RunGeom& g = b->geom_h[run];
-  const size_t nx = (size_t)g.nx, nxt = nx + 4;
-  for (int q = 0; q < 6; ++q) {
-    // through the pinned staging buffer: a strided copy from pageable memory is slow
-    BHIP(b, hipStreamSynchronize(b->stream));
-    for (int j = 0; j < g.ny; ++j) std::memcpy(b->stage + j * nx, src[q] + (j + 2) * nxt + 2, nx * sizeof(double));
-    BHIP(b, hipMemcpyAsync(b->U[q] + g.base, b->stage, nx * g.ny * sizeof(double), hipMemcpyHostToDevice, b->stream));
-  }
+  for (int q = 0; q < 6; ++q)
+    if ((rc = stage_in(b, run, src[q], b->U[q]))) return rc;
   const long long n = (long long)g.nx * g.ny;
   const int blocks = (int)std::min<long long>((n + 255) / 256, 4096);
   BLAUNCH(b, k_floor_rho, dim3(blocks), dim3(256), 0, b->stream, b->U[0] + g.base, (int)n);
@@ -400,33 +400,9 @@ int mhd_batch_upload(mhd_batch* b, int run, const double* rho, const double* mx,
 int mhd_batch_download(mhd_batch* b, int run, double* rho, double* mx, double* my, double* E, double* bxf, double* byf) {
   int rc = check_run(b, run);
   if (rc) return rc;
-  const RunGeom& g = b->geom_h[run];
-  const size_t nxt = (size_t)g.nx + 4;
   double* dst[6] = {rho, mx, my, E, bxf, byf};
-  for (int q = 0; q < 6; ++q) {
-    if (!dst[q]) continue;
-    BHIP(b, hipMemcpyAsync(b->stage, b->U[q] + g.base, (size_t)g.nx * g.ny * sizeof(double), hipMemcpyDeviceToHost,
-                           b->stream));
-    BHIP(b, hipStreamSynchronize(b->stream));
-    for (int j = 0; j < g.ny; ++j)
-      std::memcpy(dst[q] + (j + 2) * nxt + 2, b->stage + (size_t)j * g.nx, (size_t)g.nx * sizeof(double));
-  }
-  // periodic ghosts (Grid::fillOneField, Grid.hpp:143-161)
-  const int nx = g.nx, ny = g.ny, nyt = ny + 4;
-  for (int q = 0; q < 6; ++q) {
-    double* f = dst[q];
-    if (!f) continue;
-    for (int j = 2; j < nyt - 2; ++j)
-      for (int k = 0; k < 2; ++k) {
-        f[j * nxt + k] = f[j * nxt + nx + k];
-        f[j * nxt + nx + 2 + k] = f[j * nxt + 2 + k];
-      }
-    for (size_t i = 0; i < nxt; ++i)
-      for (int k = 0; k < 2; ++k) {
-        f[k * nxt + i] = f[(ny + k) * nxt + i];
-        f[(ny + 2 + k) * nxt + i] = f[(2 + k) * nxt + i];
-      }
-  }
+  for (int q = 0; q < 6; ++q)
+    if (dst[q] && (rc = stage_out(b, run, b->U[q], dst[q]))) return rc;
   return MHD_OK;
 }
 
@@ -459,9 +435,16 @@ int mhd_batch_step(mhd_batch* b, int* n_active) {
   s1.fb = T.A_ex6;
   BLAUNCH(b, kb_stage<1>, dim3(b->n_stage_jobs), dim3(64), 0, b->stream, s1);
   if (b->any_eta) {
-    int rc = cg_batch(b, 0, b->S[4], b->S[5]);
+    CgCall cc;
+    cc.slot = 0;
+    cc.bx = b->S[4];
+    cc.by = b->S[5];
+    cc.x = b->x;
+    cc.st = b->st[0];
+    cc.rs = b->rs;
+    int rc = cg_batch(b, cc);
     if (rc) return rc;
-    BatchFinArgs f = fin_args(b, 0, b->S[4], b->S[5], b->S[4], b->S[5]);
+    BatchFinArgs f = fin_args(b, b->rs, b->x, b->st[0], b->S[4], b->S[5], b->S[4], b->S[5]);
     BLAUNCH(b, kb_finalize1, dim3(nt), dim3(kTileX), 0, b->stream, f);
   }
   // stage 2: U3* = W + dt (1 - delta) L(S) [+ dt (1 - gamma) g2] into U
@@ -478,12 +461,19 @@ int mhd_batch_step(mhd_batch* b, int* n_active) {
   s2.fg = T.A_im7;
   BLAUNCH(b, kb_stage<2>, dim3(b->n_stage_jobs), dim3(64), 0, b->stream, s2);
   if (b->any_eta) {
-    int rc = cg_batch(b, 1, b->U[4], b->U[5]);
+    CgCall cc;
+    cc.slot = 1;
+    cc.bx = b->U[4];
+    cc.by = b->U[5];
+    cc.x = b->x;
+    cc.st = b->st[1];
+    cc.rs = b->rs;
+    int rc = cg_batch(b, cc);
     if (rc) return rc;
     // B^{n+1} = B3* + dt gamma g3(B3) into S.B, then S.B becomes the state's B.
     // Runs with eta = 0 have no implicit step: their B is copied by the
     // finalizer as well (x = 0 there, so g3 = 0 and c g3 = 0).
-    BatchFinArgs f = fin_args(b, 1, b->U[4], b->U[5], b->S[4], b->S[5]);
+    BatchFinArgs f = fin_args(b, b->rs, b->x, b->st[1], b->U[4], b->U[5], b->S[4], b->S[5]);
     BLAUNCH(b, kb_finalize2, dim3(nt), dim3(kTileX), 0, b->stream, f);
     std::swap(b->U[4], b->S[4]);
     std::swap(b->U[5], b->S[5]);
@@ -537,7 +527,7 @@ int mhd_batch_autotune(mhd_batch* b, int trial_steps, int* stage_rows, int* tile
   for (int q = 0; q < 6; ++q)
     (void)hipMemcpyAsync(saved[q], b->U[q], fb, hipMemcpyDeviceToDevice, b->stream);
   (void)hipMemcpyAsync(rs_saved, b->rs, sizeof(RunState) * b->nruns, hipMemcpyDeviceToDevice, b->stream);
-  const int spmv_saved[2] = {b->last_spmv[0], b->last_spmv[1]};
+  const int spmv_saved[3] = {b->last_spmv[0], b->last_spmv[1], b->last_spmv[2]};
   const std::vector<int> its_saved = b->cg_iters_last;
   const long long steps_saved = b->steps_taken;
   std::vector<RunGeom> geom_saved = b->geom_h;
@@ -564,8 +554,7 @@ int mhd_batch_autotune(mhd_batch* b, int trial_steps, int* stage_rows, int* tile
       for (int q = 0; q < 6; ++q)
         (void)hipMemcpyAsync(b->U[q], saved[q], fb, hipMemcpyDeviceToDevice, b->stream);
       (void)hipMemcpyAsync(b->rs, rs_saved, sizeof(RunState) * b->nruns, hipMemcpyDeviceToDevice, b->stream);
-      b->last_spmv[0] = spmv_saved[0];
-      b->last_spmv[1] = spmv_saved[1];
+      for (int k = 0; k < 3; ++k) b->last_spmv[k] = spmv_saved[k];
       if (rc) break;
       if (total < best) {
         best = total;

@@ -199,6 +199,13 @@ __device__ __forceinline__ double corner_ez(double eN, double eS, double eE, dou
   return 0.5 * (ezx + ezy);
 }
 
+// Virtual block index: consecutive workgroup ids are dealt to the 8 XCDs in
+// turn; remapping gives each XCD (and its L2) a contiguous run of tiles.
+__device__ __forceinline__ int xcd_block(int b, int nb) {
+  if ((nb & 7) != 0) return b;
+  return (b & 7) * (nb >> 3) + (b >> 3);
+}
+
 // Periodic index helpers.
 __device__ __forceinline__ int pmod(int a, int n) {
   int r = a % n;

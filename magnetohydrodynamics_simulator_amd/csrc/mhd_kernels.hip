@@ -303,13 +303,6 @@ __device__ void sum_partials(const double* __restrict__ part, int nb, double (&o
   __syncthreads();
 }
 
-// Virtual block index: consecutive workgroup ids are dealt to the 8 XCDs in
-// turn; remapping gives each XCD (and its L2) a contiguous run of tiles.
-__device__ __forceinline__ int xcd_block(int b, int nb) {
-  if ((nb & 7) != 0) return b;
-  return (b & 7) * (nb >> 3) + (b >> 3);
-}
-
 // Tile of a 2D sweep: 256 consecutive columns x `ty` rows; tiles are numbered
 // with y fastest so that vertically adjacent tiles run on the same XCD.
 struct Tile {
@@ -658,8 +651,18 @@ template __global__ void k_cg_pass_b<false>(CgArgs, const double*, int);
 // B += rate(x) * c (solvers.hpp:528-529).  x is taken as 0 when CG did no
 // product (Eigen returns x = 0), which leaves B unchanged.
 __device__ __forceinline__ void finalize1_body(const FinArgs& a, int vb) {
-  if (a.st->spmv == 0) return;
   const int nx = a.nx, ny = a.ny;
+  if (a.st->spmv == 0) {            // x = 0: B unchanged (copied when the output is another array)
+    if (a.bx_out == a.bx) return;
+    const Tile t = tile_of(nx, ny, a.ty, a.tiles_y, vb);
+    if (!t.valid) return;
+    for (int j = t.y0; j < t.y1; ++j) {
+      const int o = j * nx + t.ic;
+      a.bx_out[o] = a.bx[o];
+      a.by_out[o] = a.by[o];
+    }
+    return;
+  }
   const Tile t = tile_of(nx, ny, a.ty, a.tiles_y, vb);
   if (!t.valid) return;
   auto at = [&](int col, int j) -> int { return wrap_row(j, ny) * nx + col; };
@@ -983,6 +986,14 @@ __global__ __launch_bounds__(kTileX) void kb_cg_start(BatchCgArgs b) {
   if (b.st[job.x].done) return;            // stopped run, or eta = 0: no solve
   const CgArgs a = batch_cg_args(b, job.x);
   cg_start_body<true>(a, nullptr, job.y);
+}
+
+// Start of a solve with the given right-hand side bin (pooled).
+__global__ __launch_bounds__(kTileX) void kb_cg_start_rhs(BatchCgArgs b, const double* __restrict__ bin) {
+  const int2 job = b.jobs[xcd_block(blockIdx.x, gridDim.x)];
+  if (b.st[job.x].done) return;
+  const CgArgs a = batch_cg_args(b, job.x);
+  cg_start_body<false>(a, bin + b.geom[job.x].base, job.y);
 }
 
 __global__ __launch_bounds__(kTileX) void kb_cg_pass_a(BatchCgArgs b, const double* __restrict__ pold,

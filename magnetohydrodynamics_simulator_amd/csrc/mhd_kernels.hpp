@@ -146,6 +146,7 @@ struct BatchDiagArgs {
 
 template <int MODE> __global__ void kb_stage(BatchStageArgs b);
 __global__ void kb_cg_start(BatchCgArgs b);
+__global__ void kb_cg_start_rhs(BatchCgArgs b, const double* bin);
 __global__ void kb_cg_pass_a(BatchCgArgs b, const double* pold, double* pnew, int it);
 template <bool FIRST> __global__ void kb_cg_pass_b(BatchCgArgs b, const double* pv, int it);
 __global__ void kb_finalize1(BatchFinArgs b);

@@ -240,46 +240,101 @@ def _kink_background(n):
     return vx, vy, mod_rho, mod_p, bx, by
 
 
-def _kink_fields(n, amps, phases):
+def _column_fields(n, amps, phases, sausage):
+    """Fields of the displaced column.  The core profile is evaluated at
+    s = x - 1/2 + xi(y) w(x): the kink shifts the core (w = -1), the sausage
+    (m = 0) widens and narrows it symmetrically about x = 1/2 with the
+    periodic odd weight w = -sin(2 pi (x - 1/2)) / (2 pi) ~ -(x - 1/2)."""
     dx = 1.0 / n
     xi, sines = _kink_xi(n, np.asarray(amps, dtype=np.float64), phases)
     xc = (np.arange(n + 4) - NG + 0.5) * dx
     X = xc[None, :]
-    g, dg = _kink_profile(X - 0.5 - xi[:, None])
+    if sausage:
+        w = -np.sin(2 * np.pi * (X - 0.5)) / (2 * np.pi)
+        g, dg = _kink_profile(X - 0.5 + xi[:, None] * w)
+    else:
+        w = -np.ones_like(X)
+        g, dg = _kink_profile(X - 0.5 - xi[:, None])
     gb, _ = _kink_profile(X - 0.5 + 0.0 * xi[:, None])
     vx, vy, mr, mp, bxb, byb = _kink_background(n)
     rho = (KINK_RHO_BG + g) * mr
     p = (KINK_P0 + 0.5 * KINK_B0 ** 2 * (1.0 - g * g)) * mp
     bx = bxb
     by = KINK_B0 * gb + byb
-    return rho, vx, vy, p, bx, by, g, dg, mr, mp, sines
+    return rho, vx, vy, p, bx, by, g, dg * w, mr, mp, sines
 
 
-def kink_column(n: int, amps, phases):
-    """Padded state of the kink column on [0, 1]^2 (dx = dy = 1/n)."""
-    rho, vx, vy, p, bx, by, *_ = _kink_fields(n, amps, phases)
-    return _from_primitive(rho, vx, vy, p, bx, by)
-
-
-def kink_column_dstate(n: int, amps, phases):
-    """d(state)/d(a_k) for every mode: dict of arrays (modes, ny+4, nx+4) for
-    rho, mx, my and E (the face field does not depend on the amplitudes)."""
-    rho, vx, vy, p, bx, by, g, dg, mr, mp, sines = _kink_fields(n, amps, phases)
-    drho_dxi = -dg * mr                                       # s = x - 1/2 - xi
-    dp_dxi = 0.5 * KINK_B0 ** 2 * 2.0 * g * dg * mp
+def _column_dstate(n, amps, phases, sausage):
+    rho, vx, vy, p, bx, by, g, dg_dxi, mr, mp, sines = _column_fields(n, amps, phases, sausage)
+    drho_dxi = dg_dxi * mr
+    dp_dxi = -0.5 * KINK_B0 ** 2 * 2.0 * g * dg_dxi * mp
     dE_dxi = dp_dxi / (GAMMA - 1.0) + 0.5 * (vx * vx + vy * vy) * drho_dxi
     S = sines.T[:, :, None]
     return {"rho": S * drho_dxi[None], "mx": S * (vx * drho_dxi)[None], "my": S * (vy * drho_dxi)[None],
             "E": S * dE_dxi[None]}
 
 
-def kink_column_gradient(lam0: dict, dstate: dict) -> np.ndarray:
+def kink_column(n: int, amps, phases):
+    """Padded state of the kink column on [0, 1]^2 (dx = dy = 1/n)."""
+    rho, vx, vy, p, bx, by, *_ = _column_fields(n, amps, phases, False)
+    return _from_primitive(rho, vx, vy, p, bx, by)
+
+
+def kink_column_dstate(n: int, amps, phases):
+    """d(state)/d(a_k) for every mode: dict of arrays (modes, ny+4, nx+4) for
+    rho, mx, my and E (the face field does not depend on the amplitudes)."""
+    return _column_dstate(n, amps, phases, False)
+
+
+def column_gradient(lam0: dict, dstate: dict) -> np.ndarray:
     """dJ/d(a_k) = sum over cells of lam0 . d(state)/d(a_k) (physical cells)."""
     m = dstate["rho"].shape[0]
     g = np.zeros(m)
     for k in ("rho", "mx", "my", "E"):
         g += np.einsum("kij,ij->k", dstate[k][:, NG:-NG, NG:-NG], lam0[k][NG:-NG, NG:-NG])
     return g
+
+
+kink_column_gradient = column_gradient
+
+
+# ---------------------------------------------------------------------------
+# Sausage column on the unit square (the batched differentiation workload):
+# the kink column's core, field, background and drift, with the core widened
+# and narrowed symmetrically about x = 1/2 (m = 0) by xi(y), seeded modes of
+# total amplitude 1e-2.  Ensembles draw the side from {256, 512, 768, 1024}
+# and eta log-uniformly from [1e-4, 1e-2].
+# ---------------------------------------------------------------------------
+
+SAUSAGE_SIDES = (256, 512, 768, 1024)
+
+
+def sausage_column_modes(modes: int = 8, amp: float = 1e-2, seed: int = 0):
+    """Amplitudes amp / modes each (|xi| <= amp) and seeded phases."""
+    rng = np.random.default_rng(seed)
+    return np.full(modes, amp / modes), rng.uniform(0, 2 * np.pi, size=modes)
+
+
+def sausage_column(n: int, amps, phases):
+    """Padded state of the sausage column on [0, 1]^2 (dx = dy = 1/n)."""
+    rho, vx, vy, p, bx, by, *_ = _column_fields(n, amps, phases, True)
+    return _from_primitive(rho, vx, vy, p, bx, by)
+
+
+def sausage_column_dstate(n: int, amps, phases):
+    """d(state)/d(a_k) of the sausage column (as kink_column_dstate)."""
+    return _column_dstate(n, amps, phases, True)
+
+
+def sausage_ensemble(nruns: int = 64, seed: int = 0, sides=SAUSAGE_SIDES, eta_range=(1e-4, 1e-2)):
+    """Seeded ensemble: side from `sides`, eta log-uniform in eta_range and a
+    phase seed per run.  Returns a list of (n, eta, phase_seed); the state is
+    sausage_column(n, *sausage_column_modes(seed=phase_seed))."""
+    rng = np.random.default_rng(seed)
+    ns = rng.choice(np.asarray(sides), size=nruns)
+    etas = 10.0 ** rng.uniform(np.log10(eta_range[0]), np.log10(eta_range[1]), size=nruns)
+    seeds = rng.integers(0, 2**31 - 1, size=nruns)
+    return [(int(n), float(e), int(s)) for n, e, s in zip(ns, etas, seeds)]
 
 
 def window_magnetic_energy(state: dict, n: int, dx: float, lo: float = 0.25, hi: float = 0.75):

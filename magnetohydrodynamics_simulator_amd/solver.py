@@ -354,6 +354,36 @@ class Batch:
     def synchronize(self):
         self._check(self.lib.mhd_batch_synchronize(self.h))
 
+    # ---- adjoint ----
+    def adjoint_forward(self, nsteps: int, checkpoint_every: int):
+        """Record nsteps steps of every run (each with its fixed dt_fixed and
+        eta > 0) for the backward pass; checkpoints every checkpoint_every
+        steps.  The batch then holds the final states."""
+        self._check(self.lib.mhd_batch_adjoint_forward(self.h, nsteps, checkpoint_every))
+
+    def adjoint_set_lambda(self, run: int, lam_final: dict):
+        """dJ/dU(final) of one run (dict of six padded arrays)."""
+        shape = (int(self.ny[run]) + 4, int(self.nx[run]) + 4)
+        arrs = [np.ascontiguousarray(lam_final[k], dtype=np.float64) for k in self.FIELDS]
+        for a in arrs:
+            if a.shape != shape:
+                raise ValueError(f"run {run}: expected padded shape {shape}, got {a.shape}")
+        lp = (_dp * 6)(*[_ptr(a) for a in arrs])
+        self._check(self.lib.mhd_batch_adjoint_set_lambda(self.h, run, lp))
+
+    def adjoint_backward(self):
+        """Backward pass of every run (consumes the final-state adjoints)."""
+        self._check(self.lib.mhd_batch_adjoint_backward(self.h))
+
+    def adjoint_result(self, run: int):
+        """(dJ/dU(initial) as a dict of padded arrays, dJ/d eta) of one run."""
+        shape = (int(self.ny[run]) + 4, int(self.nx[run]) + 4)
+        outs = {k: np.zeros(shape) for k in self.FIELDS}
+        op = (_dp * 6)(*[_ptr(outs[k]) for k in self.FIELDS])
+        eb = C.c_double()
+        self._check(self.lib.mhd_batch_adjoint_result(self.h, run, op, C.byref(eb)))
+        return outs, eb.value
+
 
 def operator_alpha(dt: float, eta: float) -> float:
     """alpha of generate_A_matrix as the driver forms it:

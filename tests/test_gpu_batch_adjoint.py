@@ -1,0 +1,183 @@
+"""GPU checks of the batched adjoint (mhd_batch_adjoint_* of the C ABI).
+
+* Parity: every run's dJ/dU(initial) and dJ/d eta from one batched backward
+  pass against a single-run context's adjoint (mhd_adjoint_*) of the same run
+  -- the per-run arithmetic is the same; the CG dot products and the eta sums
+  run in another order, so the bar is 1e-8 relative (max norm per field).
+  Runs of uneven size mix the two adjoint solvers: Jacobi sweeps where the
+  operator is close to the identity, CG for the run whose fixed step is far
+  above the resistive limit.
+* The recorded forward pass is the plain batched run (bitwise), and a second
+  backward pass, which recomputes the last segment from its checkpoint, gives
+  bitwise the same gradient as the first, which used the forward's stage data.
+* Dot-product test of the batched gradient against central differences of the
+  batched forward path: <lam0, v> vs <lamN, (F(U + e v) - F(U - e v)) / 2e>
+  per run, with the 2R perturbed runs advanced as one batch (fixed CG
+  iteration count, as in the single-run adjoint tests, so that the stopping
+  test does not jump between the two sides).
+"""
+import numpy as np
+import pytest
+
+from helpers import CONSERVED, interior
+from magnetohydrodynamics_simulator_amd import Batch, Context, setups
+
+pytestmark = pytest.mark.gpu
+
+FIXED_CG = dict(cg_tol=0.0, cg_max_iter=16)
+TOL_PARITY = 1e-8
+TOL_DOT = 1e-6
+
+# (side, eta, dt or None for 0.8 x the CFL step at Courant number 0.4)
+RUNS = [
+    (32, 1e-2, None),
+    (48, 3e-3, None),
+    (32, 0.3, 2e-3),     # fixed step far above the resistive limit: CG adjoint solves
+    (40, 1e-3, None),
+]
+
+
+def _say(msg):
+    print(f"\n[case] {msg}", flush=True)
+
+
+def _cases():
+    out = []
+    for k, (n, eta, dt) in enumerate(RUNS):
+        amps, phases = setups.sausage_column_modes(8, 1e-2, seed=20 + k)
+        st = setups.sausage_column(n, amps, phases)
+        if dt is None:
+            with Context(n, n, 1.0 / n, 1.0 / n, eta, cfl=0.4) as c:
+                c.upload(*[st[f] for f in CONSERVED])
+                dt = 0.8 * c.CFL_condition_checker()
+        out.append((n, eta, dt, st))
+    return out
+
+
+def _smooth(n, seed, scale=1.0):
+    rng = np.random.default_rng(seed)
+    X, Y = np.meshgrid((np.arange(n + 4) - 2) / n, (np.arange(n + 4) - 2) / n)
+    out = {}
+    for f in CONSERVED:
+        g = np.zeros_like(X)
+        for _ in range(3):
+            kx, ky = rng.integers(-2, 3, size=2)
+            g += rng.standard_normal() * np.sin(2 * np.pi * (kx * X + ky * Y) + rng.uniform(0, 2 * np.pi))
+        out[f] = scale * g
+    return out
+
+
+def _batch(cases, states=None, **opts):
+    b = Batch([c[0] for c in cases], [c[0] for c in cases], [1.0 / c[0] for c in cases],
+              [1.0 / c[0] for c in cases], [c[1] for c in cases], **opts)
+    b.set_schedule(dt_fixed=[c[2] for c in cases])
+    for r, c in enumerate(cases):
+        b.upload(r, c[3] if states is None else states[r])
+    return b
+
+
+def _dot(a, b):
+    return float(sum(np.sum(interior(a[k]) * interior(b[k])) for k in CONSERVED))
+
+
+@pytest.mark.parametrize("every,opts", [(2, FIXED_CG), (10, dict(cg_tol=1e-12))],
+                         ids=["every2-fixedcg", "every10-tol1e-12"])
+def test_batch_adjoint_matches_single_run(every, opts):
+    steps = 5
+    cases = _cases()
+    _say(f"batched adjoint of {len(cases)} sausage runs, {steps} steps, checkpoints every {every}, {opts}")
+    lams = [_smooth(c[0], 100 + r) for r, c in enumerate(cases)]
+    with _batch(cases, **opts) as b:
+        b.adjoint_forward(steps, every)
+        fin_b = [b.download(r) for r in range(len(cases))]
+        for r in range(len(cases)):
+            b.adjoint_set_lambda(r, lams[r])
+        b.adjoint_backward()
+        res_b = [b.adjoint_result(r) for r in range(len(cases))]
+    with _batch(cases, **opts) as p:     # the plain batched run of the same schedule
+        p.set_schedule(max_steps=steps)
+        p.run()
+        fin_p = [p.download(r) for r in range(len(cases))]
+    for r, (n, eta, dt, st) in enumerate(cases):
+        for f in CONSERVED:
+            assert np.array_equal(interior(fin_b[r][f]), interior(fin_p[r][f])), (r, f)
+        with Context(n, n, 1.0 / n, 1.0 / n, eta, **opts) as c:
+            c.upload(*[st[f] for f in CONSERVED])
+            c.adjoint_forward(dt, steps, every)
+            fin_s = c.download()
+            lam0_s, eb_s = c.adjoint_backward(lams[r])
+        lam0_b, eb_b = res_b[r]
+        for f in CONSERVED:
+            ref = np.abs(interior(fin_s[f])).max()
+            assert np.abs(interior(fin_b[r][f]) - interior(fin_s[f])).max() <= 1e-11 * ref, (r, f)
+            scale = np.abs(interior(lam0_s[f])).max()
+            err = np.abs(interior(lam0_b[f]) - interior(lam0_s[f])).max()
+            assert err <= TOL_PARITY * scale, (r, f, err, scale)
+        assert abs(eb_b - eb_s) <= TOL_PARITY * abs(eb_s), (r, eb_b, eb_s)
+
+
+def test_batch_adjoint_repeat_recomputes_identically():
+    steps, every = 5, 2
+    cases = _cases()
+    _say(f"batched adjoint twice, {steps} steps, checkpoints every {every}")
+    lams = [_smooth(c[0], 200 + r) for r, c in enumerate(cases)]
+    with _batch(cases, **FIXED_CG) as b:
+        b.adjoint_forward(steps, every)
+        out = []
+        for _ in range(2):
+            for r in range(len(cases)):
+                b.adjoint_set_lambda(r, lams[r])
+            b.adjoint_backward()
+            out.append([b.adjoint_result(r) for r in range(len(cases))])
+    for r in range(len(cases)):
+        (l1, e1), (l2, e2) = out[0][r], out[1][r]
+        assert e1 == e2, r
+        for f in CONSERVED:
+            assert np.array_equal(l1[f], l2[f]), (r, f)
+
+
+def test_batch_adjoint_dot_product():
+    steps, every, eps = 3, 2, 1e-6
+    cases = _cases()
+    _say(f"batched adjoint dot-product test, {steps} steps, eps {eps}")
+    lams = [_smooth(c[0], 300 + r) for r, c in enumerate(cases)]
+    vs = [_smooth(c[0], 400 + r, 1e-2) for r, c in enumerate(cases)]
+    with _batch(cases, **FIXED_CG) as b:
+        b.adjoint_forward(steps, every)
+        for r in range(len(cases)):
+            b.adjoint_set_lambda(r, lams[r])
+        b.adjoint_backward()
+        lam0 = [b.adjoint_result(r)[0] for r in range(len(cases))]
+    pert_cases, pert_states = [], []
+    for r, c in enumerate(cases):
+        for sgn in (1, -1):
+            pert_cases.append(c)
+            pert_states.append({f: c[3][f] + sgn * eps * vs[r][f] for f in CONSERVED})
+    with _batch(pert_cases, pert_states, **FIXED_CG) as p:
+        p.set_schedule(max_steps=steps)
+        p.run()
+        outs = [p.download(r) for r in range(len(pert_cases))]
+    for r in range(len(cases)):
+        jv = {f: (outs[2 * r][f] - outs[2 * r + 1][f]) / (2 * eps) for f in CONSERVED}
+        lhs, rhs = _dot(lam0[r], vs[r]), _dot(lams[r], jv)
+        assert abs(lhs - rhs) <= TOL_DOT * abs(rhs), (r, lhs, rhs)
+
+
+def test_batch_adjoint_argument_errors():
+    cases = _cases()[:2]
+    _say("batched adjoint argument errors")
+    with _batch(cases) as b:
+        with pytest.raises(RuntimeError):
+            b.adjoint_backward()                       # nothing recorded
+        b.set_schedule(dt_fixed=[cases[0][2], 0.0])    # run 1 on the CFL step
+        with pytest.raises(RuntimeError):
+            b.adjoint_forward(2, 1)
+        b.set_schedule(dt_fixed=[c[2] for c in cases], max_steps=[1, 5])
+        with pytest.raises(RuntimeError):              # run 0 stops after one step
+            b.adjoint_forward(2, 1)
+        with pytest.raises(RuntimeError):
+            b.adjoint_result(0)
+    eta0 = [(cases[0][0], 0.0, cases[0][2], cases[0][3])]
+    with _batch(eta0) as b:
+        with pytest.raises(RuntimeError):
+            b.adjoint_forward(2, 1)

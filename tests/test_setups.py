@@ -66,3 +66,35 @@ def test_kink_column_state_derivative_and_divergence():
     # interior divergence (the potential is periodic, so the wrap holds too)
     div = (np.roll(bx, -1, axis=1) - bx) / dx + (np.roll(by, -1, axis=0) - by) / dx
     assert np.abs(div).max() <= 1e-10
+
+
+def test_sausage_column_derivative_symmetry_and_ensemble():
+    """The sausage column's d(state)/d(a_k) matches central differences of the
+    setup; the perturbation is symmetric about the column axis x = 1/2 (m = 0)
+    where the kink's is not; the ensemble draw is seeded."""
+    n = 32
+    amps, phases = setups.sausage_column_modes(8, 1e-2, seed=2)
+    assert np.isclose(amps.sum(), 1e-2)
+    d = setups.sausage_column_dstate(n, amps, phases)
+    h = 1e-6
+    for k in (0, 5):
+        e = np.zeros_like(amps)
+        e[k] = h
+        sp, sm = setups.sausage_column(n, amps + e, phases), setups.sausage_column(n, amps - e, phases)
+        for f in ("rho", "mx", "my", "E"):
+            assert np.abs((sp[f] - sm[f]) / (2 * h) - d[f][k]).max() <= 1e-7 * np.abs(d[f][k]).max(), f
+    # first-order density response at xi = 0: even in x - 1/2 for the sausage
+    # (cell centres x_i and 1 - x_i are mirror cells i and n - 1 - i), odd for
+    # the kink; the 5 % background modulation is not mirror-symmetric, so it is
+    # divided out
+    _, _, mr, _, _, _ = setups._kink_background(n)
+    mr = mr[2:-2, 2:-2]
+    ds = setups.sausage_column_dstate(n, 0 * amps, phases)["rho"][:, 2:-2, 2:-2] / mr
+    dk = setups.kink_column_dstate(n, 0 * amps, phases)["rho"][:, 2:-2, 2:-2] / mr
+    assert np.abs(ds - ds[:, :, ::-1]).max() <= 1e-12 * np.abs(ds).max()
+    assert np.abs(dk + dk[:, :, ::-1]).max() <= 1e-12 * np.abs(dk).max()
+    assert np.abs(ds + ds[:, :, ::-1]).max() > 0.5 * np.abs(ds).max()
+    e1 = setups.sausage_ensemble(64, seed=0)
+    assert e1 == setups.sausage_ensemble(64, seed=0)
+    assert {n_ for n_, _, _ in e1} <= set(setups.SAUSAGE_SIDES)
+    assert all(1e-4 <= eta <= 1e-2 for _, eta, _ in e1)
