@@ -86,10 +86,136 @@ def parse():
     ap.add_argument("--adjoint-steps", type=int, default=50)
     ap.add_argument("--adjoint-modes", type=int, default=64)
     ap.add_argument("--fd-modes", type=int, default=64, help="modes checked against central differences")
-    ap.add_argument("--only", choices=("batch", "adjoint"), default=None,
-                    help="run only the batched or the differentiation leg and print its record")
+    ap.add_argument("--ba-runs", type=int, default=64, help="runs of the batched differentiation workload (0: skip)")
+    ap.add_argument("--ba-steps", type=int, default=40)
+    ap.add_argument("--ba-every", type=int, default=10, help="checkpoint interval of the batched adjoint")
+    ap.add_argument("--ba-parity-runs", type=int, default=2,
+                    help="smallest runs of the batched differentiation workload checked against single-run contexts")
+    ap.add_argument("--ba-plain-runs", type=int, default=16,
+                    help="runs of the batched differentiation workload timed through single-run contexts")
+    ap.add_argument("--only", choices=("batch", "adjoint", "badjoint"), default=None,
+                    help="run only the batched, the differentiation or the batched differentiation leg")
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     return ap.parse_args()
+
+
+def bench_batch_adjoint(a) -> dict:
+    """Seeded sausage-column ensemble on [0, 1]^2 (sides from {256, 512, 768,
+    1024}, eta log-uniform in [1e-4, 1e-2], p0 = 0.1, B0 = 5, m = 0
+    perturbation of total amplitude 1e-2), each run at its own frozen step
+    (0.8 x its CFL step at Courant number 0.4) for ba_steps steps; loss per
+    run = magnetic energy of the central window.  Recorded forward and
+    backward of all runs batched (checkpoints every ba_every steps), against
+    the plain batched forward of the same runs and against the single-run
+    adjoint (one context per run) on the first ba_plain_runs runs; the
+    gradients of the two smallest runs are compared with the single-run
+    adjoint's."""
+    from magnetohydrodynamics_simulator_amd import Batch, Context, setups
+
+    names = ("rho", "mx", "my", "E", "bxf", "byf")
+    steps, every = a.ba_steps, a.ba_every
+    members = setups.sausage_ensemble(a.ba_runs, seed=0)
+    ns = [m[0] for m in members]
+    etas = [m[1] for m in members]
+    dxs = [1.0 / n for n in ns]
+    cells = int(sum(n * n for n in ns))
+    modes = [setups.sausage_column_modes(8, 1e-2, seed=s) for _, _, s in members]
+    states = [setups.sausage_column(n, *md) for n, md in zip(ns, modes)]
+    b = Batch(ns, ns, dxs, dxs, etas, cfl=0.4, cg_tol=a.cg_tol)
+
+    def upload_all():
+        for r, st in enumerate(states):
+            b.upload(r, st)
+
+    upload_all()
+    b.set_schedule(max_steps=1)                  # one CFL step gives each run's step of the initial state
+    b.run()
+    dts = [0.8 * b.run_info(r)[2] for r in range(len(members))]
+    b.set_schedule(max_steps=steps, dt_fixed=dts)
+    upload_all()
+    shape = b.autotune(trial_steps=2)
+    # plain batched forward (warm-up of 2 steps, then timed)
+    b.set_schedule(max_steps=2)
+    b.run()
+    b.set_schedule(max_steps=steps)
+    upload_all()
+    b.synchronize()
+    t0 = time.perf_counter()
+    b.run()
+    b.synchronize()
+    t_fwd = time.perf_counter() - t0
+    # adjoint: recorded forward (timed), losses and final adjoints on the host, backward (timed)
+    upload_all()
+    b.adjoint_forward(steps, every)              # first call allocates the store
+    upload_all()
+    b.synchronize()
+    t0 = time.perf_counter()
+    b.adjoint_forward(steps, every)
+    b.synchronize()
+    t_rec = time.perf_counter() - t0
+    losses = []
+    for r, n in enumerate(ns):
+        loss, lam = setups.window_magnetic_energy(b.download(r), n, 1.0 / n)
+        losses.append(loss)
+        b.adjoint_set_lambda(r, lam)
+    b.synchronize()
+    t0 = time.perf_counter()
+    b.adjoint_backward()
+    b.synchronize()
+    t_bwd = time.perf_counter() - t0
+    order = sorted(range(len(members)), key=lambda r: (ns[r], r))
+    check = order[:a.ba_parity_runs]
+    res = {r: b.adjoint_result(r) for r in check}
+    b.close()
+    # single-run adjoint: parity on the two smallest runs, cost on the first ba_plain_runs runs
+    parity = []
+    for r in check:
+        n = ns[r]
+        with Context(n, n, dxs[r], dxs[r], etas[r], cfl=0.4, cg_tol=a.cg_tol) as c:
+            c.upload(*[states[r][k] for k in names])
+            c.adjoint_forward(dts[r], steps, every)
+            _, lam = setups.window_magnetic_energy(c.download(), n, 1.0 / n)
+            lam0_s, eb_s = c.adjoint_backward(lam)
+        lam0_b, eb_b = res[r]
+        g_s = setups.column_gradient(lam0_s, setups.sausage_column_dstate(n, *modes[r]))
+        g_b = setups.column_gradient(lam0_b, setups.sausage_column_dstate(n, *modes[r]))
+        parity.append({"run": r, "side": n, "eta": etas[r],
+                       "lam0_max_rel_diff": float(max(np.abs(lam0_b[k] - lam0_s[k]).max() / np.abs(lam0_s[k]).max()
+                                                      for k in names)),
+                       "amp_grad_max_rel_diff": float(np.abs(g_b - g_s).max() / np.abs(g_s).max()),
+                       "eta_bar_rel_diff": float(abs(eb_b - eb_s) / abs(eb_s))})
+    t_plain, plain_cells = 0.0, 0
+    n_plain = min(a.ba_plain_runs, len(members))
+    for r in range(n_plain):
+        n = ns[r]
+        with Context(n, n, dxs[r], dxs[r], etas[r], cfl=0.4, cg_tol=a.cg_tol) as c:
+            for k in range(2):                   # the first pass allocates and captures; the second is timed
+                c.upload(*[states[r][q] for q in names])
+                c.synchronize()
+                t0 = time.perf_counter()
+                c.adjoint_forward(dts[r], steps, every)
+                _, lam = setups.window_magnetic_energy(c.download(), n, 1.0 / n)
+                c.adjoint_backward(lam)
+                c.synchronize()
+                if k == 1:
+                    t_plain += time.perf_counter() - t0
+        plain_cells += n * n
+    t_grad = t_rec + t_bwd
+    return {"workload": f"{len(members)} sausage-column runs, sides {min(ns)}..{max(ns)} (seeded), eta 1e-4..1e-2, "
+                        f"frozen dt = 0.8 x CFL(0.4), {steps} steps, checkpoints every {every}, CG tol {a.cg_tol}",
+            "runs": len(members), "cells": cells, "steps": steps,
+            "value": round(cells * steps / t_grad / 1e6, 2), "unit": "Mcell-steps/s of gradient (forward + backward)",
+            "gradients_per_s": round(len(members) / t_grad, 3),
+            "ms_recorded_forward": round(t_rec * 1e3, 2), "ms_backward": round(t_bwd * 1e3, 2),
+            "ms_plain_batched_forward": round(t_fwd * 1e3, 2),
+            "adjoint_over_forward": round(t_grad / t_fwd, 3),
+            "plain_single_run_adjoint": {"runs": n_plain,
+                                         "value": round(plain_cells * steps / t_plain / 1e6, 2) if n_plain else None,
+                                         "note": "forward + host loss + backward per context, second call timed"},
+            "speedup_vs_single_run_adjoint": (round((cells * steps / t_grad) / (plain_cells * steps / t_plain), 2)
+                                              if n_plain else None),
+            "autotuned_shape": {"stage_rows": shape[0], "tile_rows": shape[1]},
+            "parity_vs_single_run": parity}
 
 
 def pmc_child(a) -> None:
@@ -404,6 +530,9 @@ def main():
     if a.only == "adjoint":
         print(json.dumps({"adjoint": bench_adjoint(a)}), flush=True)
         return
+    if a.only == "badjoint":
+        print(json.dumps({"batched_adjoint": bench_batch_adjoint(a)}), flush=True)
+        return
     from magnetohydrodynamics_simulator_amd import Context, setups
 
     n = a.n
@@ -492,6 +621,8 @@ def main():
         line["batched"] = bench_batch(a)
     if a.adjoint_n > 0:
         line["adjoint"] = bench_adjoint(a)
+    if a.ba_runs > 0:
+        line["batched_adjoint"] = bench_batch_adjoint(a)
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baseline(n, a.eta, a.cg_tol, a.cfl, a.cpu_steps)
     print(json.dumps(line), flush=True)
