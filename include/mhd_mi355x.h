@@ -81,6 +81,14 @@ int mhd_download(mhd_ctx* ctx, double* rho, double* mx, double* my, double* E,
                  double* bxf, double* byf, double* vx, double* vy, double* p,
                  double* bxc, double* byc);
 
+/* Device-resident counterparts of mhd_upload / mhd_download (no reference
+ * counterpart): six compact nx*ny device arrays (HIP device pointers on the
+ * context's device, row-major j * nx + i) rho, mx, my, E, bxf, byf.  The
+ * upload applies the density floor; NULL entries are skipped on download.
+ * Both return after the copies have completed. */
+int mhd_upload_device(mhd_ctx* ctx, const double* const* fields);
+int mhd_download_device(mhd_ctx* ctx, double* const* fields);
+
 /* CFL_condition_checker (include/diagnostics.hpp:44-88) on the current state,
  * with the Courant number of opts.cfl. */
 int mhd_cfl_dt(mhd_ctx* ctx, double* dt);
@@ -153,10 +161,19 @@ int mhd_adjoint_forward(mhd_ctx* ctx, double dt, int nsteps, int checkpoint_ever
  * step of the last mhd_adjoint_forward.  Each checkpoint segment is
  * recomputed once; the context's state is left at the final state. */
 int mhd_adjoint_backward(mhd_ctx* ctx, const double* const* lam_final, double* const* lam0, double* eta_bar);
+/* mhd_adjoint_backward on compact nx*ny device arrays (HIP device pointers on
+ * the context's device) instead of padded host arrays; for callers that keep
+ * fields on the GPU (the LibTorch autograd Function of this package). */
+int mhd_adjoint_backward_device(mhd_ctx* ctx, const double* const* lam_final, double* const* lam0,
+                                double* eta_bar);
 /* Unit entry point: out = L'(U)^T lam for the explicit right-hand side L of
  * explicit_rhs (solvers.hpp:393-402) at the current state U (padded arrays;
  * out's physical cells are written). */
 int mhd_explicit_rhs_vjp(mhd_ctx* ctx, const double* const* lam, double* const* out);
+/* The same product by four unfused gather kernels (primitives, face
+ * adjoints, MUSCL / primitive adjoint, face-field terms); the reference the
+ * fused kernel of mhd_explicit_rhs_vjp is tested against. */
+int mhd_explicit_rhs_vjp_unfused(mhd_ctx* ctx, const double* const* lam, double* const* out);
 
 /* ------------------------------------------------------------------------
  * Batched ensembles (no reference counterpart: the reference advances one

@@ -30,7 +30,7 @@ def hipcc() -> str:
 
 def flags() -> list[str]:
     return [
-        "-O3", "-std=c++17", f"--offload-arch={ARCH}", "-fPIC", "-shared",
+        "-O3", "-std=c++17", f"--offload-arch={ARCH}", "-fPIC", "-shared", "-Wl,-soname,libmhd_mi355x.so",
         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
         "-munsafe-fp-atomics",
         "-I", os.path.join(ROOT, "include"), "-I", CSRC,
@@ -54,5 +54,36 @@ def build(verbose: bool = False, extra: list[str] | None = None) -> str:
     return LIB
 
 
+TORCH_SRC = os.path.join(CSRC, "torch_mhd.cpp")
+TORCH_BUILD = os.path.join(PKG, "_torch_build")
+TORCH_NAME = "_torch_mhd"
+
+
+def build_torch(verbose: bool = False) -> str:
+    """Compile the LibTorch autograd Function (csrc/torch_mhd.cpp) against the
+    torch wheel's headers into _torch_build/_torch_mhd*.so, linked to LIB.
+    Host C++ only (the GPU work is the library's); always recompiles."""
+    import ctypes
+
+    from torch.utils import cpp_extension
+
+    # the extension's dependency on libmhd_mi355x.so resolves to this copy
+    ctypes.CDLL(LIB, mode=ctypes.RTLD_GLOBAL)
+    os.makedirs(TORCH_BUILD, exist_ok=True)
+    for f in os.listdir(TORCH_BUILD):          # force a fresh build
+        if f.endswith((".o", ".so", ".ninja_deps", ".ninja_log")) or f == "build.ninja":
+            os.remove(os.path.join(TORCH_BUILD, f))
+    cpp_extension.load(
+        name=TORCH_NAME, sources=[TORCH_SRC], build_directory=TORCH_BUILD, verbose=verbose,
+        extra_include_paths=[os.path.join(ROOT, "include")],
+        extra_cflags=["-O2"], extra_ldflags=[f"-L{PKG}", "-lmhd_mi355x"], with_cuda=False)
+    so = [f for f in os.listdir(TORCH_BUILD) if f.startswith(TORCH_NAME) and f.endswith(".so")]
+    if not so:
+        raise RuntimeError("the LibTorch extension was not produced")
+    return os.path.join(TORCH_BUILD, so[0])
+
+
 if __name__ == "__main__":
     print(build(verbose=True))
+    if "--torch" in sys.argv:
+        print(build_torch(verbose=True))

@@ -53,7 +53,11 @@ SIGNATURES = {
     "mhd_synchronize": (C.c_int, [_vp]),
     "mhd_adjoint_forward": (C.c_int, [_vp, C.c_double, C.c_int, C.c_int]),
     "mhd_adjoint_backward": (C.c_int, [_vp, C.POINTER(_dp), C.POINTER(_dp), _dp]),
+    "mhd_adjoint_backward_device": (C.c_int, [_vp, C.POINTER(_dp), C.POINTER(_dp), _dp]),
+    "mhd_upload_device": (C.c_int, [_vp, C.POINTER(_dp)]),
+    "mhd_download_device": (C.c_int, [_vp, C.POINTER(_dp)]),
     "mhd_explicit_rhs_vjp": (C.c_int, [_vp, C.POINTER(_dp), C.POINTER(_dp)]),
+    "mhd_explicit_rhs_vjp_unfused": (C.c_int, [_vp, C.POINTER(_dp), C.POINTER(_dp)]),
     "mhd_batch_create": (C.c_int, [C.c_int, _ip, _ip, _dp, _dp, _dp, C.POINTER(Opts), C.POINTER(_vp)]),
     "mhd_batch_destroy": (C.c_int, [_vp]),
     "mhd_batch_last_error": (C.c_char_p, [_vp]),

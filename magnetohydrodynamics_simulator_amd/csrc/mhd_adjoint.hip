@@ -35,6 +35,7 @@
 #include "../../include/mhd_mi355x.h"
 #include "mhd_context.hpp"
 #include "mhd_adjoint_lin.hpp"
+#include "mhd_adjoint_march.hpp"
 #include "mhd_adjoint_math.hpp"
 #include "mhd_device.hpp"
 #include "mhd_kernels.hpp"
@@ -84,6 +85,10 @@ __global__ __launch_bounds__(256) void ka_bfaces(AdjArgs a) {
     ka_bfaces_cell(a, o - j * a.nx, j);
   }
 }
+
+// The fused adjoint of the explicit right-hand side (mhd_adjoint_march.hpp):
+// one wavefront per (strip, segment).
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MHD_MARCH_WAVES))) void ka_march(MarchArgs a) { adj_march(a, blockIdx.x, blockIdx.y); }
 
 __global__ __launch_bounds__(256) void ka_Jt(LinArgs g, const double* q, double c, double* fx, double* fy) {
   FOR_CELLS(g) Jt_cell(g, q, c, fx, fy, i_, j_);
@@ -315,8 +320,9 @@ Fields6 f6(double* const* p) {
   return f;
 }
 
-// out += L'(s)^T lam for the state s (6 device fields).
-int explicit_rhs_vjp(mhd_ctx* c, double* const* s, double* const* lam, double* const* out) {
+// out += L'(s)^T lam for the state s (6 device fields), by the four unfused
+// gather kernels (the reference of the fused kernel).
+int explicit_rhs_vjp_unfused(mhd_ctx* c, double* const* s, double* const* lam, double* const* out) {
   AdjointStore& A = *c->adj;
   AdjArgs a;
   a.nx = c->nx;
@@ -349,6 +355,32 @@ int explicit_rhs_vjp(mhd_ctx* c, double* const* s, double* const* lam, double* c
   rc = launch(c, 8, [&] { hipLaunchKernelGGL(ka_cells, dim3(g), dim3(256), 0, c->stream, a); });
   if (rc) return rc;
   return launch(c, 8, [&] { hipLaunchKernelGGL(ka_bfaces, dim3(g), dim3(256), 0, c->stream, a); });
+}
+
+// Rows per segment of the fused adjoint march: 64, fewer when that leaves
+// fewer than about 2048 wavefronts (8 per CU), at least 8.
+int march_rows(int nx, int ny) {
+  const long long strips = (nx + kAdjOut - 1) / kAdjOut;
+  long long rows = 64;
+  while (rows > 8 && strips * ((ny + rows - 1) / rows) < 2048) rows /= 2;
+  return (int)rows;
+}
+
+// out += L'(s)^T lam for the state s (6 device fields), fused.
+int explicit_rhs_vjp(mhd_ctx* c, double* const* s, double* const* lam, double* const* out) {
+  MarchArgs a;
+  for (int q = 0; q < 6; ++q) {
+    a.s[q] = s[q];
+    a.lam[q] = lam[q];
+    a.out[q] = out[q];
+  }
+  a.nx = c->nx;
+  a.ny = c->ny;
+  a.rows = march_rows(c->nx, c->ny);
+  a.idx = 1.0 / c->dx;
+  a.idy = 1.0 / c->dy;
+  const dim3 grid((c->nx + kAdjOut - 1) / kAdjOut, (c->ny + a.rows - 1) / a.rows);
+  return launch(c, 8, [&] { hipLaunchKernelGGL(ka_march, grid, dim3(64), 0, c->stream, a); });
 }
 
 // Jacobi sweeps that bring the error of A^-1 below 2^-60 of the solution
@@ -505,6 +537,34 @@ int backward(mhd_ctx* c, bool spec) {
 
 }  // namespace
 
+namespace {
+
+int rhs_vjp_entry(mhd_ctx* c, const double* const* lam, double* const* out, bool fused) {
+  if (check(c)) return MHD_EINVAL;
+  if (!lam || !out) return fail(c, MHD_EINVAL, "mhd_explicit_rhs_vjp: NULL argument");
+  int rc = ensure_store(c, 0, 0);
+  if (rc) return rc;
+  AdjointStore& A = *c->adj;
+  const int gr = grid_1d(c->n);
+  for (int q = 0; q < 6; ++q) {
+    if (!lam[q]) return fail(c, MHD_EINVAL, "mhd_explicit_rhs_vjp: NULL field");
+    rc = from_host_padded(c, lam[q], A.fb[q]);
+    if (rc) return rc;
+    rc = launch(c, 8, [&] { hipLaunchKernelGGL(ka_zero, dim3(gr), dim3(256), 0, c->stream, (int)c->n, A.sb[q]); });
+    if (rc) return rc;
+  }
+  rc = fused ? explicit_rhs_vjp(c, c->U, A.fb, A.sb) : explicit_rhs_vjp_unfused(c, c->U, A.fb, A.sb);
+  if (rc) return rc;
+  for (int q = 0; q < 6; ++q)
+    if (out[q]) {
+      rc = to_host_padded(c, A.sb[q], out[q], false);
+      if (rc) return rc;
+    }
+  return MHD_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 int mhd_adjoint_forward(mhd_ctx* c, double dt, int nsteps, int checkpoint_every) {
@@ -547,6 +607,10 @@ int mhd_adjoint_forward(mhd_ctx* c, double dt, int nsteps, int checkpoint_every)
   return MHD_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
 // The sweep of mhd_adjoint_backward between the upload of lam_final and the
 // download of the results: keep the final state, zero eta_bar, run the
 // backward pass, restore the final state.
@@ -563,7 +627,9 @@ int sweep(mhd_ctx* c, bool spec) {
   return copy_fields(c, c->U, A.keep, 6);
 }
 
-int mhd_adjoint_backward(mhd_ctx* c, const double* const* lam_final, double* const* lam0, double* eta_bar) {
+// mhd_adjoint_backward with the final adjoint read from, and the initial one
+// written to, padded host arrays (device == false) or compact device arrays.
+int backward_entry(mhd_ctx* c, const double* const* lam_final, double* const* lam0, double* eta_bar, bool device) {
   if (check(c)) return MHD_EINVAL;
   if (!c->adj || c->adj->nsteps < 1) return fail(c, MHD_EINVAL, "mhd_adjoint_backward before mhd_adjoint_forward");
   if (!lam_final) return fail(c, MHD_EINVAL, "lam_final is NULL");
@@ -572,6 +638,11 @@ int mhd_adjoint_backward(mhd_ctx* c, const double* const* lam_final, double* con
   AdjointStore& A = *c->adj;
   int rc;
   auto upload_lam = [&]() -> int {
+    if (device) {
+      double* src[6];
+      for (int q = 0; q < 6; ++q) src[q] = const_cast<double*>(lam_final[q]);
+      return copy_fields(c, A.lam, src, 6);
+    }
     for (int q = 0; q < 6; ++q) {
       int r = from_host_padded(c, lam_final[q], A.lam[q]);
       if (r) return r;
@@ -646,7 +717,12 @@ int mhd_adjoint_backward(mhd_ctx* c, const double* const* lam_final, double* con
   if (lam0)
     for (int q = 0; q < 6; ++q)
       if (lam0[q]) {
-        rc = to_host_padded(c, A.lam[q], lam0[q], true);
+        if (device) {
+          double* dst[1] = {lam0[q]};
+          rc = copy_fields(c, dst, &A.lam[q], 1);
+        } else {
+          rc = to_host_padded(c, A.lam[q], lam0[q], true);
+        }
         if (rc) return rc;
       }
   double eb = 0.0;
@@ -656,28 +732,24 @@ int mhd_adjoint_backward(mhd_ctx* c, const double* const* lam_final, double* con
   return MHD_OK;
 }
 
+}  // namespace
+
+extern "C" {
+
+int mhd_adjoint_backward(mhd_ctx* c, const double* const* lam_final, double* const* lam0, double* eta_bar) {
+  return backward_entry(c, lam_final, lam0, eta_bar, false);
+}
+
+int mhd_adjoint_backward_device(mhd_ctx* c, const double* const* lam_final, double* const* lam0, double* eta_bar) {
+  return backward_entry(c, lam_final, lam0, eta_bar, true);
+}
+
 int mhd_explicit_rhs_vjp(mhd_ctx* c, const double* const* lam, double* const* out) {
-  if (check(c)) return MHD_EINVAL;
-  if (!lam || !out) return fail(c, MHD_EINVAL, "mhd_explicit_rhs_vjp: NULL argument");
-  int rc = ensure_store(c, 0, 0);
-  if (rc) return rc;
-  AdjointStore& A = *c->adj;
-  const int gr = grid_1d(c->n);
-  for (int q = 0; q < 6; ++q) {
-    if (!lam[q]) return fail(c, MHD_EINVAL, "mhd_explicit_rhs_vjp: NULL field");
-    rc = from_host_padded(c, lam[q], A.fb[q]);
-    if (rc) return rc;
-    rc = launch(c, 8, [&] { hipLaunchKernelGGL(ka_zero, dim3(gr), dim3(256), 0, c->stream, (int)c->n, A.sb[q]); });
-    if (rc) return rc;
-  }
-  rc = explicit_rhs_vjp(c, c->U, A.fb, A.sb);
-  if (rc) return rc;
-  for (int q = 0; q < 6; ++q)
-    if (out[q]) {
-      rc = to_host_padded(c, A.sb[q], out[q], false);
-      if (rc) return rc;
-    }
-  return MHD_OK;
+  return rhs_vjp_entry(c, lam, out, true);
+}
+
+int mhd_explicit_rhs_vjp_unfused(mhd_ctx* c, const double* const* lam, double* const* out) {
+  return rhs_vjp_entry(c, lam, out, false);
 }
 
 }  // extern "C"

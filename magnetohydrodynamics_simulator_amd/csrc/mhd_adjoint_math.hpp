@@ -195,9 +195,7 @@ __device__ __forceinline__ void ka_prims_cell(const AdjArgs& a, int i, int j) {
   double rf, vx, vy, p;
   primitives(a.s[0][o], a.s[1][o], a.s[2][o], a.s[3][o], bxc, byc, rf, vx, vy, p);
   // pressure-floor mask: recompute the unfloored pressure
-  const double kin = 0.5 * rf * (vx * vx + vy * vy);
-  const double mag = (bxc * bxc + byc * byc) / 2;
-  const double praw = kGammaM1 * (a.s[3][o] - kin - mag);
+  const double praw = pressure_raw(a.s[3][o], rf, vx, vy, bxc, byc);
   a.P[0][o] = rf; a.P[1][o] = vx; a.P[2][o] = vy; a.P[3][o] = p; a.P[4][o] = bxc; a.P[5][o] = byc;
   a.P[6][o] = (praw < kFloor) ? 0.0 : 1.0;
 }

@@ -35,6 +35,7 @@
 
 #include "../../include/mhd_mi355x.h"
 #include "mhd_adjoint_lin.hpp"
+#include "mhd_adjoint_march.hpp"
 #include "mhd_adjoint_math.hpp"
 #include "mhd_batch.hpp"
 #include "mhd_kernels.hpp"
@@ -45,6 +46,7 @@ using namespace mhd_batch_host;
 namespace mhd {
 
 constexpr int kCellBlock = 256;   // cells per job of the per-cell kernels
+constexpr int kMarchRows = 32;    // rows per segment of the fused adjoint of L
 
 struct BArgs {
   const RunGeom* geom;
@@ -150,49 +152,33 @@ __global__ __launch_bounds__(256) void kba_stage1_out(BArgs b, Fields6 sb, Field
                   rho_s + c.base, dt * fs, dt * fw);
 }
 
-// The run's view of pooled adjoint arguments.
-__device__ __forceinline__ AdjArgs adj_at(const AdjArgs& p, const RunGeom& g) {
-  AdjArgs a = p;
-  const long long o = g.base;
-  a.nx = g.nx;
-  a.ny = g.ny;
-  a.dx = g.dx;
-  a.dy = g.dy;
-  a.idx = g.inv_dx;
-  a.idy = g.inv_dy;
+// The fused adjoint of the explicit right-hand side (mhd_adjoint_march.hpp)
+// of every run: jobs (run, strip + segment x strips), pooled arrays.
+struct BMarchArgs {
+  const RunGeom* geom;
+  const int2* jobs;
+  MarchArgs p;          // pooled pointers
+  int rows;             // rows per segment
+};
+
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MHD_MARCH_WAVES))) void kba_march(BMarchArgs b) {
+  const int2 job = b.jobs[blockIdx.x];
+  const RunGeom& g = b.geom[job.x];
+  MarchArgs a = b.p;
 #pragma unroll
   for (int q = 0; q < 6; ++q) {
-    a.s[q] += o;
-    a.lam[q] += o;
-    a.out[q] += o;
+    a.s[q] += g.base;
+    a.lam[q] += g.base;
+    a.out[q] += g.base;
   }
-#pragma unroll
-  for (int q = 0; q < 7; ++q) a.P[q] += o;
-#pragma unroll
-  for (int q = 0; q < 5; ++q) {
-    a.xrb[q] += o;
-    a.xlb[q] += o;
-    a.yrb[q] += o;
-    a.ylb[q] += o;
-  }
-  a.bnx += o;
-  a.bny += o;
-  a.bcb[0] += o;
-  a.bcb[1] += o;
-  return a;
+  a.nx = g.nx;
+  a.ny = g.ny;
+  a.rows = b.rows;
+  a.idx = g.inv_dx;
+  a.idy = g.inv_dy;
+  const int strips = (g.nx + kAdjOut - 1) / kAdjOut;
+  adj_march(a, job.y % strips, job.y / strips);
 }
-
-#define KBA_ADJ(name, body)                                                 \
-  __global__ __launch_bounds__(256) void name(BArgs b, AdjArgs p) {        \
-    const BCell c = bcell(b);                                               \
-    if (!c.in) return;                                                      \
-    body(adj_at(p, b.geom[c.run]), c.i, c.j);                               \
-  }
-KBA_ADJ(kba_prims, ka_prims_cell)
-KBA_ADJ(kba_faces, ka_faces_cell)
-KBA_ADJ(kba_cells, ka_cells_cell)
-KBA_ADJ(kba_bfaces, ka_bfaces_cell)
-#undef KBA_ADJ
 
 // Launch l of the Jacobi sweeps of every run with more than l launches
 // (launches[run]; 0 = the run solves by CG).  Per run the launches ping-pong
@@ -293,14 +279,8 @@ struct BatchAdjoint {
   int2* crange = nullptr;           // [first, end) of each run's jobs
   int2* jjobs = nullptr;            // Jacobi tiles of the Jacobi runs
   int njj = 0;
-  double* P[7] = {};
-  double* xrb[5] = {};
-  double* xlb[5] = {};
-  double* yrb[5] = {};
-  double* ylb[5] = {};
-  double* bnx = nullptr;
-  double* bny = nullptr;
-  double* bcb[2] = {};
+  int2* mjobs = nullptr;            // (run, strip + segment x strips) of the fused adjoint of L
+  int nmj = 0;
   double* lam[6] = {};              // adjoint of the state after the step (in), before it (out)
   double* lam_in[6] = {};           // the final-state adjoint, kept for a repeated sweep
   double* wb[6] = {};
@@ -324,7 +304,7 @@ void destroy_batch_adjoint(mhd_batch* b) {
   BatchAdjoint* A = b->adj;
   for (double* p : A->owned)
     if (p) (void)hipFree(p);
-  void* bufs[] = {A->fst, A->ast, A->rs, A->jl, A->cjobs, A->crange, A->jjobs, A->partial, A->eta_bar};
+  void* bufs[] = {A->fst, A->ast, A->rs, A->jl, A->cjobs, A->crange, A->jjobs, A->mjobs, A->partial, A->eta_bar};
   for (void* p : bufs)
     if (p) (void)hipFree(p);
   if (A->ast_host) (void)hipHostFree(A->ast_host);
@@ -422,12 +402,11 @@ int build_store(mhd_batch* b, int nsteps, int every) {
   A.xs.resize(2 * nsteps);
   for (double*& p : A.xs)
     if ((rc = field(b, &p))) return rc;
-  double** groups[] = {A.P, A.xrb, A.xlb, A.yrb, A.ylb, A.bcb, A.lam, A.lam_in, A.wb, A.sb, A.fb};
-  const int counts[] = {7, 5, 5, 5, 5, 2, 6, 6, 6, 6, 6};
-  for (int gi = 0; gi < 11; ++gi)
-    for (int q = 0; q < counts[gi]; ++q)
-      if ((rc = field(b, &groups[gi][q]))) return rc;
-  double** singles[] = {&A.bnx, &A.bny, &A.q1, &A.fx, &A.fy, &A.gx, &A.gy, &A.jac};
+  double** groups[] = {A.lam, A.lam_in, A.wb, A.sb, A.fb};
+  for (double** gp : groups)
+    for (int q = 0; q < 6; ++q)
+      if ((rc = field(b, &gp[q]))) return rc;
+  double** singles[] = {&A.q1, &A.fx, &A.fy, &A.gx, &A.gy, &A.jac};
   for (double** p : singles)
     if ((rc = field(b, p))) return rc;
   const int nr = b->nruns;
@@ -449,7 +428,17 @@ int build_store(mhd_batch* b, int nsteps, int every) {
     cr[r].y = (int)cj.size();
   }
   A.ncj = (int)cj.size();
-  if ((rc = dev_alloc(b, &A.cjobs, cj.size())) || (rc = dev_alloc(b, &A.partial, (size_t)6 * A.ncj))) return rc;
+  std::vector<int2> mj;
+  for (int r = 0; r < nr; ++r) {
+    const RunGeom& g = b->geom_h[r];
+    const int w = ((g.nx + kAdjOut - 1) / kAdjOut) * ((g.ny + kMarchRows - 1) / kMarchRows);
+    for (int k = 0; k < w; ++k) mj.push_back(make_int2(r, k));
+  }
+  A.nmj = (int)mj.size();
+  if ((rc = dev_alloc(b, &A.cjobs, cj.size())) || (rc = dev_alloc(b, &A.partial, (size_t)6 * A.ncj)) ||
+      (rc = dev_alloc(b, &A.mjobs, mj.size())))
+    return rc;
+  BHIP(b, hipMemcpyAsync(A.mjobs, mj.data(), sizeof(int2) * mj.size(), hipMemcpyHostToDevice, b->stream));
   BHIP(b, hipMemcpyAsync(A.cjobs, cj.data(), sizeof(int2) * cj.size(), hipMemcpyHostToDevice, b->stream));
   BHIP(b, hipMemcpyAsync(A.crange, cr.data(), sizeof(int2) * nr, hipMemcpyHostToDevice, b->stream));
   BHIP(b, hipStreamSynchronize(b->stream));
@@ -607,29 +596,16 @@ Fields6 six(double* const* p) {
 // out += L'(s)^T lam for the runs' states s.
 int rhs_vjp(mhd_batch* b, double* const* s, double* const* lam, double* const* out) {
   BatchAdjoint& A = *b->adj;
-  AdjArgs a{};
+  BMarchArgs m{};
+  m.geom = b->geom;
+  m.jobs = A.mjobs;
   for (int q = 0; q < 6; ++q) {
-    a.s[q] = s[q];
-    a.lam[q] = lam[q];
-    a.out[q] = out[q];
+    m.p.s[q] = s[q];
+    m.p.lam[q] = lam[q];
+    m.p.out[q] = out[q];
   }
-  for (int q = 0; q < 7; ++q) a.P[q] = A.P[q];
-  for (int q = 0; q < 5; ++q) {
-    a.xrb[q] = A.xrb[q];
-    a.xlb[q] = A.xlb[q];
-    a.yrb[q] = A.yrb[q];
-    a.ylb[q] = A.ylb[q];
-  }
-  a.bnx = A.bnx;
-  a.bny = A.bny;
-  a.bcb[0] = A.bcb[0];
-  a.bcb[1] = A.bcb[1];
-  const BArgs ba = bargs(b);
-  const dim3 G(A.ncj), B(256);
-  BLAUNCH(b, kba_prims, G, B, 0, b->stream, ba, a);
-  BLAUNCH(b, kba_faces, G, B, 0, b->stream, ba, a);
-  BLAUNCH(b, kba_cells, G, B, 0, b->stream, ba, a);
-  BLAUNCH(b, kba_bfaces, G, B, 0, b->stream, ba, a);
+  m.rows = kMarchRows;
+  BLAUNCH(b, kba_march, dim3(A.nmj), dim3(64), 0, b->stream, m);
   return MHD_OK;
 }
 

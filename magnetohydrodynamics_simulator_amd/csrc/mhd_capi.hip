@@ -576,6 +576,37 @@ int mhd_download(mhd_ctx* c, double* rho, double* mx, double* my, double* E, dou
   return MHD_OK;
 }
 
+int mhd_upload_device(mhd_ctx* c, const double* const* fields) {
+  if (check_ctx(c)) return MHD_EINVAL;
+  if (!fields) return fail(c, MHD_EINVAL, "mhd_upload_device: NULL field array");
+  double* src[6];
+  for (int q = 0; q < 6; ++q) {
+    if (!fields[q]) return fail(c, MHD_EINVAL, "mhd_upload_device: NULL field");
+    src[q] = const_cast<double*>(fields[q]);
+  }
+  int rc = copy_fields(c, c->U, src, 6);
+  if (rc) return rc;
+  rc = launch(c, 8, [&] {
+    hipLaunchKernelGGL(k_floor_rho, dim3(grid_1d(c->n)), dim3(256), 0, c->stream, c->U[0], (int)c->n);
+  });
+  if (rc) return rc;
+  HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
+  return MHD_OK;
+}
+
+int mhd_download_device(mhd_ctx* c, double* const* fields) {
+  if (check_ctx(c)) return MHD_EINVAL;
+  if (!fields) return fail(c, MHD_EINVAL, "mhd_download_device: NULL field array");
+  for (int q = 0; q < 6; ++q)
+    if (fields[q]) {
+      double* dst[1] = {fields[q]};
+      int rc = copy_fields(c, dst, &c->U[q], 1);
+      if (rc) return rc;
+    }
+  HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
+  return MHD_OK;
+}
+
 static int max_speed(mhd_ctx* c, double* maxs, double* nbad) {
   int rc = diag_partials(c, k_cfl, 7);
   if (rc) return rc;

@@ -78,6 +78,21 @@ __device__ __forceinline__ double sqrt_nr(double x) {
   return fma(d, h, g);
 }
 
+// Unfloored pressure of Grid::computePrimitives (Grid.hpp:121-126).  FMA
+// contraction is off here, so every kernel that evaluates it -- the explicit
+// stage and the adjoint kernels that recompute the forward's branch choices
+// -- rounds alike whatever code it is inlined into (a uniform pressure is
+// recovered with rounding noise, and the minmod branch of its slopes follows
+// that noise).
+__device__ __forceinline__ double pressure_raw(double E, double rho_f, double vx, double vy, double bxc,
+                                               double byc) {
+#pragma clang fp contract(off)
+  const double kinE = 0.5 * rho_f * (vx * vx + vy * vy);
+  const double magE = (bxc * bxc + byc * byc) / 2;
+  const double intE = E - kinE - magE;
+  return kGammaM1 * intE;
+}
+
 // Grid::computePrimitives for one cell, Grid.hpp:111-126.  rho is the stored
 // density; the velocities use it before the floor is applied.
 __device__ __forceinline__ void primitives(double rho, double mx, double my, double E, double bxc, double byc,
@@ -86,10 +101,7 @@ __device__ __forceinline__ void primitives(double rho, double mx, double my, dou
   vx = mx * inv;
   vy = my * inv;
   rho_f = (rho < kFloor) ? kFloor : rho;
-  double kinE = 0.5 * rho_f * (vx * vx + vy * vy);
-  double magE = (bxc * bxc + byc * byc) / 2;
-  double intE = E - kinE - magE;
-  p = kGammaM1 * intE;
+  p = pressure_raw(E, rho_f, vx, vy, bxc, byc);
   if (p < kFloor) p = kFloor;
 }
 

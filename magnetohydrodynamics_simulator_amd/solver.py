@@ -199,13 +199,15 @@ class Context:
                 raise ValueError(f"expected padded shape {(self.nyt, self.nxt)}, got {a.shape}")
         return keep, (_dp * 6)(*[_ptr(a) for a in keep])
 
-    def explicit_rhs_vjp(self, lam):
+    def explicit_rhs_vjp(self, lam, fused: bool = True):
         """L'(U)^T lam for the explicit right-hand side at the current state;
-        lam and the result are six padded arrays (rho, mx, my, E, bxf, byf)."""
+        lam and the result are six padded arrays (rho, mx, my, E, bxf, byf).
+        fused=False runs the four unfused kernels (the fused kernel's reference)."""
         keep, lp = self._field_ptrs(lam)
         outs = [self._padded() for _ in range(6)]
         op = (_dp * 6)(*[_ptr(o) for o in outs])
-        self._check(self.lib.mhd_explicit_rhs_vjp(self.h, lp, op))
+        fn = self.lib.mhd_explicit_rhs_vjp if fused else self.lib.mhd_explicit_rhs_vjp_unfused
+        self._check(fn(self.h, lp, op))
         return outs
 
     def adjoint_forward(self, dt: float, nsteps: int, checkpoint_every: int = 10):

@@ -96,6 +96,28 @@ def test_explicit_rhs_vjp_dot_product():
     assert abs(lhs - rhs) <= TOL * abs(rhs), (lhs, rhs)
 
 
+@pytest.mark.parametrize("nx,ny,seed", [(40, 40, 7), (70, 44, 8), (130, 9, 9), (5, 6, 10)])
+def test_fused_rhs_vjp_matches_unfused(nx, ny, seed):
+    """The fused adjoint march against the four unfused gather kernels it
+    replaces (the same per-face formulas and summation order): equal to
+    round-off, on grids narrower than one strip, wider than two, and with
+    segments of a few rows."""
+    _say(f"fused vs unfused L'^T lam, {nx}x{ny}")
+    n = max(nx, ny)
+    ic = _state(n, seed)
+    st = {k: np.ascontiguousarray(ic[k][: ny + 4, : nx + 4]) for k in CONSERVED}
+    rng = np.random.default_rng(seed)
+    lam = [rng.standard_normal((ny + 4, nx + 4)) for _ in range(6)]
+    dx = setups.grid_spacing(n)
+    with Context(nx, ny, dx, dx, 0.0) as c:
+        c.upload(*[st[k] for k in CONSERVED])
+        fused = c.explicit_rhs_vjp(lam)
+        ref = c.explicit_rhs_vjp(lam, fused=False)
+    for q, k in enumerate(CONSERVED):
+        a, b = interior(fused[q]), interior(ref[q])
+        assert np.abs(a - b).max() <= 1e-13 * np.abs(b).max(), (k, np.abs(a - b).max(), np.abs(b).max())
+
+
 def _run(ic, n, eta, dt, steps):
     with _ctx(ic, n, eta) as c:
         c.run(1e9, dt_override=dt, max_steps=steps)
