@@ -86,9 +86,12 @@ def parse():
     ap.add_argument("--adjoint-steps", type=int, default=50)
     ap.add_argument("--adjoint-modes", type=int, default=64)
     ap.add_argument("--fd-modes", type=int, default=64, help="modes checked against central differences")
+    ap.add_argument("--adjoint-no-graph", action="store_true",
+                    help="run the adjoint backward without graph replay (for kernel traces)")
     ap.add_argument("--ba-runs", type=int, default=64, help="runs of the batched differentiation workload (0: skip)")
     ap.add_argument("--ba-steps", type=int, default=40)
-    ap.add_argument("--ba-every", type=int, default=10, help="checkpoint interval of the batched adjoint")
+    ap.add_argument("--ba-every", type=int, default=40,
+                    help="checkpoint interval of the batched adjoint (40: the whole forward stays on the device)")
     ap.add_argument("--ba-parity-runs", type=int, default=2,
                     help="smallest runs of the batched differentiation workload checked against single-run contexts")
     ap.add_argument("--ba-plain-runs", type=int, default=16,
@@ -399,6 +402,11 @@ def bench_adjoint(a) -> dict:
         return setups.window_magnetic_energy(c_.download(), n, dx)[0]
 
     with Context(n, n, dx, dx, eta, cfl=0.4) as c:
+        if a.adjoint_no_graph:
+            # event timing on a kernel the adjoint does not launch: the
+            # backward then runs its kernels directly instead of replaying
+            # them as a graph (the kernels and their order are the same)
+            c.profile(["cfl"])
         c.upload(*[st[k] for k in names])
         dt = 0.8 * c.CFL_condition_checker()
         # plain forward (warm, then timed)

@@ -168,7 +168,8 @@ int mhd_adjoint_backward_device(mhd_ctx* ctx, const double* const* lam_final, do
                                 double* eta_bar);
 /* Unit entry point: out = L'(U)^T lam for the explicit right-hand side L of
  * explicit_rhs (solvers.hpp:393-402) at the current state U (padded arrays;
- * out's physical cells are written). */
+ * out's physical cells are written), by the fused single-pass kernel (the
+ * backward pass uses it from 2^19 cells on and the unfused kernels below). */
 int mhd_explicit_rhs_vjp(mhd_ctx* ctx, const double* const* lam, double* const* out);
 /* The same product by four unfused gather kernels (primitives, face
  * adjoints, MUSCL / primitive adjoint, face-field terms); the reference the

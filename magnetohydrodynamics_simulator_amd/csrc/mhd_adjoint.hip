@@ -366,8 +366,15 @@ int march_rows(int nx, int ny) {
   return (int)rows;
 }
 
-// out += L'(s)^T lam for the state s (6 device fields), fused.
-int explicit_rhs_vjp(mhd_ctx* c, double* const* s, double* const* lam, double* const* out) {
+// Grids below this many cells take the unfused kernels: the fused march is
+// a chain of (rows + 7) dependent row steps per wavefront, about 75 us
+// whatever the size, while the four unfused kernels are fully parallel
+// (measured on MI355X: unfused faster up to 512^2, fused from 768^2 on,
+// 1.4x at 1024^2 and 1.9x at 2048^2).
+constexpr long long kFusedMinCells = 1LL << 19;
+
+// out += L'(s)^T lam for the state s (6 device fields), by the fused march.
+int explicit_rhs_vjp_fused(mhd_ctx* c, double* const* s, double* const* lam, double* const* out) {
   MarchArgs a;
   for (int q = 0; q < 6; ++q) {
     a.s[q] = s[q];
@@ -383,16 +390,15 @@ int explicit_rhs_vjp(mhd_ctx* c, double* const* s, double* const* lam, double* c
   return launch(c, 8, [&] { hipLaunchKernelGGL(ka_march, grid, dim3(64), 0, c->stream, a); });
 }
 
-// Jacobi sweeps that bring the error of A^-1 below 2^-60 of the solution
-// when the iteration's contraction factor 2 (cx + cy) / d is at most 1/4
-// (the operator of small alpha / h^2), else 0.
+// out += L'(s)^T lam, by whichever of the two is faster at this size.
+int explicit_rhs_vjp(mhd_ctx* c, double* const* s, double* const* lam, double* const* out) {
+  if (c->n < kFusedMinCells) return explicit_rhs_vjp_unfused(c, s, lam, out);
+  return explicit_rhs_vjp_fused(c, s, lam, out);
+}
+
+// Jacobi sweeps of the adjoint solves with this context's operator (0: CG).
 int jacobi_sweeps(mhd_ctx* c) {
-  const double cx = c->alpha / (c->dx * c->dx), cy = c->alpha / (c->dy * c->dy);
-  const double d = 1 + 2 * (cx + cy);
-  const double rho = 2 * (cx + cy) / d;
-  if (!(rho <= 0.25)) return 0;
-  if (rho < 1e-300) return 1;
-  return std::max(1, (int)std::ceil(-60.0 * std::log(2.0) / std::log(rho)));
+  return jacobi_sweep_count(c->alpha / (c->dx * c->dx), c->alpha / (c->dy * c->dy));
 }
 
 // bbar = A^-1 xbar in c->x.  A is symmetric, so the adjoint of a solve is a
@@ -553,7 +559,7 @@ int rhs_vjp_entry(mhd_ctx* c, const double* const* lam, double* const* out, bool
     rc = launch(c, 8, [&] { hipLaunchKernelGGL(ka_zero, dim3(gr), dim3(256), 0, c->stream, (int)c->n, A.sb[q]); });
     if (rc) return rc;
   }
-  rc = fused ? explicit_rhs_vjp(c, c->U, A.fb, A.sb) : explicit_rhs_vjp_unfused(c, c->U, A.fb, A.sb);
+  rc = fused ? explicit_rhs_vjp_fused(c, c->U, A.fb, A.sb) : explicit_rhs_vjp_unfused(c, c->U, A.fb, A.sb);
   if (rc) return rc;
   for (int q = 0; q < 6; ++q)
     if (out[q]) {

@@ -355,17 +355,12 @@ int fill_pooled(mhd_batch* b, double* y, long long n, double v) {
   return MHD_OK;
 }
 
-// Jacobi launches of a run: sweeps that bring the error of A^-1 below 2^-60
-// of the solution when the contraction factor 2 (cx + cy) / d is at most 1/4,
-// 8 sweeps per launch; 0 (CG) otherwise.  The operator is that of kb_dt for
-// the run's fixed step: alpha = dt eta gamma.
+// Jacobi launches of a run's adjoint solves (0: CG): the sweep count of
+// jacobi_sweep_count for the operator kb_dt builds for the run's fixed step
+// (alpha = dt eta gamma), 8 sweeps per launch.
 int jacobi_launches(const RunGeom& g, double a_im4) {
   const double alpha = (g.dt_fixed * g.eta) * a_im4;
-  const double cx = alpha / (g.dx * g.dx), cy = alpha / (g.dy * g.dy);
-  const double d = 1 + 2 * (cx + cy);
-  const double rho = 2 * (cx + cy) / d;
-  if (!(rho <= 0.25)) return 0;
-  const int sweeps = (rho < 1e-300) ? 1 : std::max(1, (int)std::ceil(-60.0 * std::log(2.0) / std::log(rho)));
+  const int sweeps = jacobi_sweep_count(alpha / (g.dx * g.dx), alpha / (g.dy * g.dy));
   return (sweeps + kJacSweeps - 1) / kJacSweeps;
 }
 
