@@ -713,9 +713,11 @@ int backward_entry(mhd_ctx* c, const double* const* lam_final, double* const* la
       HIP_OR_FAIL(c, hipMemcpyAsync(A.states_host, A.states, sizeof(CgState) * 4 * A.nsteps, hipMemcpyDeviceToHost,
                                     c->stream));
       HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
+      // only CG solves leave a state to check (Jacobi sweeps always finish)
       bool finished = true;
-      for (int s = 0; s < A.nsteps; ++s)
-        finished = finished && A.states_host[4 * s + 2].done != 0 && A.states_host[4 * s + 3].done != 0;
+      if (jacobi_sweeps(c) == 0)
+        for (int s = 0; s < A.nsteps; ++s)
+          finished = finished && A.states_host[4 * s + 2].done != 0 && A.states_host[4 * s + 3].done != 0;
       if (finished) break;
       if (!spec) return fail(c, MHD_EHIP, "adjoint: a CG solve did not finish");
     }

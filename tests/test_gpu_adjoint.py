@@ -278,3 +278,25 @@ def test_kink_column_dot_product_small_step():
     lhs = _dot([lam0[k] for k in CONSERVED], v)
     rhs = _dot([lamN[k] for k in CONSERVED], jv)
     assert abs(lhs - rhs) <= 1e-7 * abs(rhs), (lhs, rhs)
+
+
+def test_adjoint_direct_launches_match_graph_replay():
+    """With kernel event timing on, the backward pass launches its kernels
+    directly (speculative solves checked afterwards) instead of replaying a
+    graph; with Jacobi adjoint solves there is no CG state to check, and the
+    result is the graph replay's bit for bit."""
+    n, eta, steps, every = 32, 2e-2, 5, 2
+    _say(f"adjoint with direct launches vs graph replay, {n}x{n}, {steps} steps")
+    ic = _state(n, 21)
+    lamN = dict(zip(CONSERVED, _smooth_fields(n, 22)))
+    res = []
+    for timed in (False, True):
+        with _ctx(ic, n, eta) as c:
+            if timed:
+                c.profile(["cfl"])
+            c.adjoint_forward(4e-3, steps, every)
+            res.append(c.adjoint_backward(lamN))
+    (l1, e1), (l2, e2) = res
+    assert e1 == e2
+    for k in CONSERVED:
+        assert np.array_equal(l1[k], l2[k]), k
