@@ -126,6 +126,20 @@ int mhd_divb(mhd_ctx* ctx, double* rms, double* maxabs);
 int mhd_run(mhd_ctx* ctx, double t_end, double dt_override, int max_steps,
             int* steps_out, double* t_out, double* step_dt, int* cg_iters);
 
+/* write_data_to_binary_file (include/diagnostics.hpp:90-131): the current
+ * state as a snapshot file in the reference's format -- 9 doubles (nx, ny,
+ * ng = 2, snapshot_rate, dx, dy, time_step, rms div B, max |div B|), then the
+ * padded (ny+4) x (nx+4) arrays rho, E, p, vx, vy, bxc, byc with ghosts
+ * filled (read by scripts/display_functions.py:3-22).  The reference names
+ * the file output/snapshot_%06d.dat (the C++ adapter does the same). */
+int mhd_write_snapshot(mhd_ctx* ctx, const char* path, int snapshot_rate, double time_step);
+/* Exact-restart files (no reference counterpart: the reference cannot
+ * resume): "MHDRST01", 7 doubles (nx, ny, ng, dx, dy, eta, t) and the six
+ * padded conserved arrays.  Reading checks the grid and uploads the state;
+ * *t (may be NULL) receives the stored time. */
+int mhd_write_restart(mhd_ctx* ctx, const char* path, double t);
+int mhd_read_restart(mhd_ctx* ctx, const char* path, double* t);
+
 /* Instrumentation (no reference counterpart).  mask selects kernel ids
  * (bit k = id k, 0 = off, 0x1ff = all); each launch of a selected kernel is
  * bracketed by HIP events on the context's stream, and mhd_profile_get

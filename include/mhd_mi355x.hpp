@@ -15,6 +15,7 @@
 //   generate_A_matrix(g, mem, alpha); solver.compute    mhd_mi355x::generate_A_matrix(ctx, alpha)
 //   CFL_condition_checker(g)                            mhd_mi355x::CFL_condition_checker(g, ctx)
 //   calculate_current_div_B(g)                          mhd_mi355x::calculate_current_div_B(g, ctx)
+//   write_data_to_binary_file(g, frame, rate, dt)       mhd_mi355x::write_data_to_binary_file(g, ctx, frame, rate, dt)
 //
 // The Grid-taking functions copy the state to the device, run, and copy it
 // back (PCIe both ways).  Loops that keep the state on the GPU call
@@ -22,6 +23,7 @@
 // Context::download when they need host data.
 #pragma once
 
+#include <cstdio>
 #include <stdexcept>
 #include <string>
 
@@ -124,6 +126,17 @@ divergence_checker_values calculate_current_div_B(const GridT& g, Context& ctx) 
   divergence_checker_values v{};
   check(mhd_divb(ctx.get(), &v.root_mean_square_div, &v.max_norm_divergence), ctx.get());
   return v;
+}
+
+// write_data_to_binary_file (diagnostics.hpp:90-131): the reference's file
+// name output/snapshot_<frame, 6 digits>.dat and format.
+template <class GridT>
+void write_data_to_binary_file(const GridT& g, Context& ctx, int current_frame_index, int snapshot_rate,
+                               double time_step) {
+  ctx.upload(g);
+  char name[64];
+  std::snprintf(name, sizeof(name), "output/snapshot_%06d.dat", current_frame_index);
+  check(mhd_write_snapshot(ctx.get(), name, snapshot_rate, time_step), ctx.get());
 }
 
 }  // namespace mhd_mi355x

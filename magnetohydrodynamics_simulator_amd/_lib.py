@@ -55,6 +55,9 @@ SIGNATURES = {
     "mhd_adjoint_backward": (C.c_int, [_vp, C.POINTER(_dp), C.POINTER(_dp), _dp]),
     "mhd_adjoint_backward_device": (C.c_int, [_vp, C.POINTER(_dp), C.POINTER(_dp), _dp]),
     "mhd_upload_device": (C.c_int, [_vp, C.POINTER(_dp)]),
+    "mhd_write_snapshot": (C.c_int, [_vp, C.c_char_p, C.c_int, C.c_double]),
+    "mhd_write_restart": (C.c_int, [_vp, C.c_char_p, C.c_double]),
+    "mhd_read_restart": (C.c_int, [_vp, C.c_char_p, _dp]),
     "mhd_download_device": (C.c_int, [_vp, C.POINTER(_dp)]),
     "mhd_explicit_rhs_vjp": (C.c_int, [_vp, C.POINTER(_dp), C.POINTER(_dp)]),
     "mhd_explicit_rhs_vjp_unfused": (C.c_int, [_vp, C.POINTER(_dp), C.POINTER(_dp)]),

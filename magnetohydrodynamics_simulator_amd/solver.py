@@ -14,6 +14,7 @@ float64, exactly the reference Grid's storage.
 from __future__ import annotations
 
 import ctypes as C
+import os
 import math
 
 import numpy as np
@@ -198,6 +199,19 @@ class Context:
             if a.shape != (self.nyt, self.nxt):
                 raise ValueError(f"expected padded shape {(self.nyt, self.nxt)}, got {a.shape}")
         return keep, (_dp * 6)(*[_ptr(a) for a in keep])
+
+    # ---- files ----
+    def write_snapshot(self, path: str, snapshot_rate: int, time_step: float):
+        """write_data_to_binary_file (diagnostics.hpp:90-131) of the current state."""
+        self._check(self.lib.mhd_write_snapshot(self.h, os.fsencode(path), snapshot_rate, time_step))
+
+    def write_restart(self, path: str, t: float):
+        self._check(self.lib.mhd_write_restart(self.h, os.fsencode(path), t))
+
+    def read_restart(self, path: str) -> float:
+        t = C.c_double()
+        self._check(self.lib.mhd_read_restart(self.h, os.fsencode(path), C.byref(t)))
+        return t.value
 
     def explicit_rhs_vjp(self, lam, fused: bool = True):
         """L'(U)^T lam for the explicit right-hand side at the current state;

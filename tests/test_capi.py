@@ -95,6 +95,7 @@ int main() {
     mhd_mi355x::resistive_B_term(grid, ctx, d[4], d[5], true);
     auto dv = mhd_mi355x::calculate_current_div_B(grid, ctx);
     (void)dv;
+    mhd_mi355x::write_data_to_binary_file(grid, ctx, 0, 20, dt);
   } catch (const mhd_mi355x::Error& e) {
     return e.code == MHD_EHIP ? 3 : 1;
   }
