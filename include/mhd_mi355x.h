@@ -28,6 +28,13 @@
 extern "C" {
 #endif
 
+/* Entry points are exported; everything else in the library is internal. */
+#if defined(__GNUC__) || defined(__clang__)
+#define MHD_API __attribute__((visibility("default")))
+#else
+#define MHD_API
+#endif
+
 #define MHD_OK 0
 #define MHD_EINVAL 1       /* bad argument (size, pointer, option) */
 #define MHD_EHIP 2         /* HIP runtime error or no device */
@@ -58,26 +65,26 @@ typedef struct mhd_step_stats {
 } mhd_step_stats;
 
 /* Fills *opts with the defaults above. */
-int mhd_default_opts(mhd_opts* opts);
+MHD_API int mhd_default_opts(mhd_opts* opts);
 
 /* Creates a context for an nx x ny periodic grid with cell sizes dx, dy and
  * resistivity eta (Grid ctor, Grid.hpp:29-54; cell_width/cell_height/
  * resistivity as set by src/integrator_main.cpp:37-39).  opts may be NULL. */
-int mhd_create(int nx, int ny, double dx, double dy, double eta, const mhd_opts* opts, mhd_ctx** out);
-int mhd_destroy(mhd_ctx* ctx);
-const char* mhd_last_error(const mhd_ctx* ctx);
+MHD_API int mhd_create(int nx, int ny, double dx, double dy, double eta, const mhd_opts* opts, mhd_ctx** out);
+MHD_API int mhd_destroy(mhd_ctx* ctx);
+MHD_API const char* mhd_last_error(const mhd_ctx* ctx);
 
 /* Copies the conserved state from padded host arrays (ghosts ignored) and
  * applies the density floor, as the driver's first fillGhostPeriodic +
  * computePrimitives does (src/integrator_main.cpp:183-184, Grid.hpp:115). */
-int mhd_upload(mhd_ctx* ctx, const double* rho, const double* mx, const double* my,
+MHD_API int mhd_upload(mhd_ctx* ctx, const double* rho, const double* mx, const double* my,
                const double* E, const double* bxf, const double* byf);
 
 /* Copies the state back into padded host arrays with ghosts filled
  * periodically (Grid::fillGhostPeriodic, Grid.hpp:64-80).  Any pointer may be
  * NULL.  vx, vy, p, bxc, byc are the primitives of Grid::computePrimitives
  * (Grid.hpp:100-136), computed on the device. */
-int mhd_download(mhd_ctx* ctx, double* rho, double* mx, double* my, double* E,
+MHD_API int mhd_download(mhd_ctx* ctx, double* rho, double* mx, double* my, double* E,
                  double* bxf, double* byf, double* vx, double* vy, double* p,
                  double* bxc, double* byc);
 
@@ -86,36 +93,36 @@ int mhd_download(mhd_ctx* ctx, double* rho, double* mx, double* my, double* E,
  * context's device, row-major j * nx + i) rho, mx, my, E, bxf, byf.  The
  * upload applies the density floor; NULL entries are skipped on download.
  * Both return after the copies have completed. */
-int mhd_upload_device(mhd_ctx* ctx, const double* const* fields);
-int mhd_download_device(mhd_ctx* ctx, double* const* fields);
+MHD_API int mhd_upload_device(mhd_ctx* ctx, const double* const* fields);
+MHD_API int mhd_download_device(mhd_ctx* ctx, double* const* fields);
 
 /* CFL_condition_checker (include/diagnostics.hpp:44-88) on the current state,
  * with the Courant number of opts.cfl. */
-int mhd_cfl_dt(mhd_ctx* ctx, double* dt);
+MHD_API int mhd_cfl_dt(mhd_ctx* ctx, double* dt);
 
 /* generate_A_matrix + solver.compute (include/solvers.hpp:404-443,
  * src/integrator_main.cpp:189-191): sets alpha of the operator I - alpha L.
  * The driver passes alpha = dt * eta * gamma_ARS. */
-int mhd_build_operator(mhd_ctx* ctx, double alpha);
+MHD_API int mhd_build_operator(mhd_ctx* ctx, double alpha);
 
 /* IMEX (include/solvers.hpp:491-613): one ARS(2,2,2) step of size dt.  The
  * implicit solves use the operator of the last mhd_build_operator call, as
  * the reference's solver does.  stats may be NULL. */
-int mhd_imex_step(mhd_ctx* ctx, double dt, mhd_step_stats* stats);
+MHD_API int mhd_imex_step(mhd_ctx* ctx, double dt, mhd_step_stats* stats);
 
 /* Unit entry points for parity tests. */
 /* explicit_rhs (solvers.hpp:393-402): dU/dt of the current state into padded
  * arrays (physical cells written, ghosts left untouched). */
-int mhd_explicit_rhs(mhd_ctx* ctx, double* drho, double* dmx, double* dmy, double* dE,
+MHD_API int mhd_explicit_rhs(mhd_ctx* ctx, double* drho, double* dmx, double* dmy, double* dE,
                      double* dbxf, double* dbyf);
 /* resistive_B_term (solvers.hpp:445-489) of the current state's face B.
  * implicit != 0 solves with the operator of mhd_build_operator.  iters may be NULL. */
-int mhd_resistive_term(mhd_ctx* ctx, int implicit, double* bx_rate, double* by_rate, int* iters);
+MHD_API int mhd_resistive_term(mhd_ctx* ctx, int implicit, double* bx_rate, double* by_rate, int* iters);
 /* The CG solve alone (Eigen ConjugateGradient::solve, solvers.hpp:469) on a
  * compact nx*ny right-hand side. */
-int mhd_cg_solve(mhd_ctx* ctx, const double* b, double* x, int* iters, double* error);
+MHD_API int mhd_cg_solve(mhd_ctx* ctx, const double* b, double* x, int* iters, double* error);
 /* calculate_current_div_B (include/diagnostics.hpp:15-42). */
-int mhd_divb(mhd_ctx* ctx, double* rms, double* maxabs);
+MHD_API int mhd_divb(mhd_ctx* ctx, double* rms, double* maxabs);
 
 /* Driver loop (src/integrator_main.cpp:183-243): CFL time step (or the
  * constant dt_override when it is non-zero), the clamp of the last step to
@@ -123,7 +130,7 @@ int mhd_divb(mhd_ctx* ctx, double* rms, double* maxabs);
  * then IMEX; stops at t_end or after max_steps steps (max_steps < 0: no
  * limit).  step_dt / cg_iters (2 per step) may be NULL, else hold max_steps
  * entries.  *t_out receives the final time. */
-int mhd_run(mhd_ctx* ctx, double t_end, double dt_override, int max_steps,
+MHD_API int mhd_run(mhd_ctx* ctx, double t_end, double dt_override, int max_steps,
             int* steps_out, double* t_out, double* step_dt, int* cg_iters);
 
 /* write_data_to_binary_file (include/diagnostics.hpp:90-131): the current
@@ -132,13 +139,13 @@ int mhd_run(mhd_ctx* ctx, double t_end, double dt_override, int max_steps,
  * padded (ny+4) x (nx+4) arrays rho, E, p, vx, vy, bxc, byc with ghosts
  * filled (read by scripts/display_functions.py:3-22).  The reference names
  * the file output/snapshot_%06d.dat (the C++ adapter does the same). */
-int mhd_write_snapshot(mhd_ctx* ctx, const char* path, int snapshot_rate, double time_step);
+MHD_API int mhd_write_snapshot(mhd_ctx* ctx, const char* path, int snapshot_rate, double time_step);
 /* Exact-restart files (no reference counterpart: the reference cannot
  * resume): "MHDRST01", 7 doubles (nx, ny, ng, dx, dy, eta, t) and the six
  * padded conserved arrays.  Reading checks the grid and uploads the state;
  * *t (may be NULL) receives the stored time. */
-int mhd_write_restart(mhd_ctx* ctx, const char* path, double t);
-int mhd_read_restart(mhd_ctx* ctx, const char* path, double* t);
+MHD_API int mhd_write_restart(mhd_ctx* ctx, const char* path, double t);
+MHD_API int mhd_read_restart(mhd_ctx* ctx, const char* path, double* t);
 
 /* Instrumentation (no reference counterpart).  mask selects kernel ids
  * (bit k = id k, 0 = off, 0x1ff = all); each launch of a selected kernel is
@@ -147,12 +154,12 @@ int mhd_read_restart(mhd_ctx* ctx, const char* path, double* t);
  * mhd_profile_enable.  Kernel ids: 0 stage1, 1 stage2, 2 cg_start,
  * 3 cg_pass_a, 4 cg_pass_b, 5 implicit_finalize1, 6 implicit_finalize2,
  * 7 cfl, 8 other. */
-int mhd_profile_enable(mhd_ctx* ctx, int mask);
-int mhd_profile_get(mhd_ctx* ctx, int kernel_id, long long* launches, double* total_ms);
+MHD_API int mhd_profile_enable(mhd_ctx* ctx, int mask);
+MHD_API int mhd_profile_get(mhd_ctx* ctx, int kernel_id, long long* launches, double* total_ms);
 /* Bytes-calibration helper for hardware counters: one plain device copy
  * kernel of nbytes (rounded down to doubles), run synchronously. */
-int mhd_calibration_copy(mhd_ctx* ctx, long long nbytes);
-int mhd_synchronize(mhd_ctx* ctx);
+MHD_API int mhd_calibration_copy(mhd_ctx* ctx, long long nbytes);
+MHD_API int mhd_synchronize(mhd_ctx* ctx);
 
 /* ------------------------------------------------------------------------
  * Discrete adjoint (no reference counterpart: the reference has no
@@ -168,27 +175,27 @@ int mhd_synchronize(mhd_ctx* ctx);
  * alpha = dt eta gamma, as the driver builds it for a fixed dt,
  * src/integrator_main.cpp:56-60), keeping a copy of the state every
  * checkpoint_every steps.  The context then holds the final state. */
-int mhd_adjoint_forward(mhd_ctx* ctx, double dt, int nsteps, int checkpoint_every);
+MHD_API int mhd_adjoint_forward(mhd_ctx* ctx, double dt, int nsteps, int checkpoint_every);
 /* Given lam_final = dJ/dU(final) (six padded arrays rho, mx, my, E, bxf, byf),
  * returns lam0 = dJ/dU(initial) (six padded arrays, ghosts filled; the array
  * pointer or any entry may be NULL) and *eta_bar = dJ/d eta through every
  * step of the last mhd_adjoint_forward.  Each checkpoint segment is
  * recomputed once; the context's state is left at the final state. */
-int mhd_adjoint_backward(mhd_ctx* ctx, const double* const* lam_final, double* const* lam0, double* eta_bar);
+MHD_API int mhd_adjoint_backward(mhd_ctx* ctx, const double* const* lam_final, double* const* lam0, double* eta_bar);
 /* mhd_adjoint_backward on compact nx*ny device arrays (HIP device pointers on
  * the context's device) instead of padded host arrays; for callers that keep
  * fields on the GPU (the LibTorch autograd Function of this package). */
-int mhd_adjoint_backward_device(mhd_ctx* ctx, const double* const* lam_final, double* const* lam0,
+MHD_API int mhd_adjoint_backward_device(mhd_ctx* ctx, const double* const* lam_final, double* const* lam0,
                                 double* eta_bar);
 /* Unit entry point: out = L'(U)^T lam for the explicit right-hand side L of
  * explicit_rhs (solvers.hpp:393-402) at the current state U (padded arrays;
  * out's physical cells are written), by the fused single-pass kernel (the
  * backward pass uses it from 2^19 cells on and the unfused kernels below). */
-int mhd_explicit_rhs_vjp(mhd_ctx* ctx, const double* const* lam, double* const* out);
+MHD_API int mhd_explicit_rhs_vjp(mhd_ctx* ctx, const double* const* lam, double* const* out);
 /* The same product by four unfused gather kernels (primitives, face
  * adjoints, MUSCL / primitive adjoint, face-field terms); the reference the
  * fused kernel of mhd_explicit_rhs_vjp is tested against. */
-int mhd_explicit_rhs_vjp_unfused(mhd_ctx* ctx, const double* const* lam, double* const* out);
+MHD_API int mhd_explicit_rhs_vjp_unfused(mhd_ctx* ctx, const double* const* lam, double* const* out);
 
 /* ------------------------------------------------------------------------
  * Batched ensembles (no reference counterpart: the reference advances one
@@ -204,34 +211,34 @@ typedef struct mhd_batch mhd_batch;
 
 /* nx, ny, dx, dy, eta: arrays of nruns entries.  Every run starts at t = 0
  * with no end time and no step limit (see mhd_batch_set_schedule). */
-int mhd_batch_create(int nruns, const int* nx, const int* ny, const double* dx, const double* dy,
+MHD_API int mhd_batch_create(int nruns, const int* nx, const int* ny, const double* dx, const double* dy,
                      const double* eta, const mhd_opts* opts, mhd_batch** out);
-int mhd_batch_destroy(mhd_batch* b);
-const char* mhd_batch_last_error(const mhd_batch* b);
+MHD_API int mhd_batch_destroy(mhd_batch* b);
+MHD_API const char* mhd_batch_last_error(const mhd_batch* b);
 /* Per-run end time, step limit (< 0: none) and constant time step (0: CFL);
  * any array may be NULL to keep the current values. */
-int mhd_batch_set_schedule(mhd_batch* b, const double* t_end, const int* max_steps, const double* dt_fixed);
+MHD_API int mhd_batch_set_schedule(mhd_batch* b, const double* t_end, const int* max_steps, const double* dt_fixed);
 /* State of one run from / to padded host arrays (as mhd_upload / mhd_download);
  * an upload restarts that run's clock and step count. */
-int mhd_batch_upload(mhd_batch* b, int run, const double* rho, const double* mx, const double* my,
+MHD_API int mhd_batch_upload(mhd_batch* b, int run, const double* rho, const double* mx, const double* my,
                      const double* E, const double* bxf, const double* byf);
-int mhd_batch_download(mhd_batch* b, int run, double* rho, double* mx, double* my, double* E,
+MHD_API int mhd_batch_download(mhd_batch* b, int run, double* rho, double* mx, double* my, double* E,
                        double* bxf, double* byf);
 /* One driver step of every run that has not reached its end time or step
  * limit; *n_active (may be NULL) receives the number of runs that stepped. */
-int mhd_batch_step(mhd_batch* b, int* n_active);
+MHD_API int mhd_batch_step(mhd_batch* b, int* n_active);
 /* mhd_batch_step until no run is active or max_batch_steps (< 0: no limit). */
-int mhd_batch_run(mhd_batch* b, int max_batch_steps, int* steps_done);
+MHD_API int mhd_batch_run(mhd_batch* b, int max_batch_steps, int* steps_done);
 /* Autotuner: times trial_steps steps of the uploaded batch for each candidate
  * launch shape (rows per explicit-stage segment x rows per CG / finalize
  * tile, each in {16, 32, 64}), restores the state, and keeps the fastest
  * shape; the chosen rows are returned (either pointer may be NULL).  Results
  * do not depend on the shape beyond the summation order of the CG dot
  * products. */
-int mhd_batch_autotune(mhd_batch* b, int trial_steps, int* stage_rows, int* tile_rows);
+MHD_API int mhd_batch_autotune(mhd_batch* b, int trial_steps, int* stage_rows, int* tile_rows);
 /* Steps taken, time, last dt and the two CG iteration counts of the last step of one run. */
-int mhd_batch_run_info(mhd_batch* b, int run, int* steps, double* t, double* last_dt, int* cg_iters);
-int mhd_batch_synchronize(mhd_batch* b);
+MHD_API int mhd_batch_run_info(mhd_batch* b, int run, int* steps, double* t, double* last_dt, int* cg_iters);
+MHD_API int mhd_batch_synchronize(mhd_batch* b);
 
 /* ------------------------------------------------------------------------
  * Batched adjoint (no reference counterpart): mhd_adjoint_* for every run of
@@ -248,16 +255,16 @@ int mhd_batch_synchronize(mhd_batch* b);
  * recomputes nothing).  Fails if a run reaches its t_end or step limit on the
  * way.  The batch then holds the final states.  Sets every run's final-state
  * adjoint to zero. */
-int mhd_batch_adjoint_forward(mhd_batch* b, int nsteps, int checkpoint_every);
+MHD_API int mhd_batch_adjoint_forward(mhd_batch* b, int nsteps, int checkpoint_every);
 /* dJ/dU(final) of one run: six padded arrays rho, mx, my, E, bxf, byf. */
-int mhd_batch_adjoint_set_lambda(mhd_batch* b, int run, const double* const* lam_final);
+MHD_API int mhd_batch_adjoint_set_lambda(mhd_batch* b, int run, const double* const* lam_final);
 /* The backward pass of every run through the steps of the last
  * mhd_batch_adjoint_forward.  The final-state adjoints are consumed: set them
  * again before another backward pass. */
-int mhd_batch_adjoint_backward(mhd_batch* b);
+MHD_API int mhd_batch_adjoint_backward(mhd_batch* b);
 /* dJ/dU(initial) of one run (six padded arrays, ghosts filled; the array
  * pointer or any entry may be NULL) and dJ/d eta (may be NULL). */
-int mhd_batch_adjoint_result(mhd_batch* b, int run, double* const* lam0, double* eta_bar);
+MHD_API int mhd_batch_adjoint_result(mhd_batch* b, int run, double* const* lam0, double* eta_bar);
 
 #ifdef __cplusplus
 }

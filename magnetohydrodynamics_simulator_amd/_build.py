@@ -33,6 +33,7 @@ def flags() -> list[str]:
         "-O3", "-std=c++17", f"--offload-arch={ARCH}", "-fPIC", "-shared", "-Wl,-soname,libmhd_mi355x.so",
         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
         "-munsafe-fp-atomics",
+        "-Xarch_host", "-fvisibility=hidden",     # export only the C ABI (MHD_API)
         "-I", os.path.join(ROOT, "include"), "-I", CSRC,
     ]
 

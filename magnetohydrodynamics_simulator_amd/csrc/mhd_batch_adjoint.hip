@@ -46,7 +46,7 @@ using namespace mhd_batch_host;
 namespace mhd {
 
 constexpr int kCellBlock = 256;   // cells per job of the per-cell kernels
-constexpr int kMarchRows = 32;    // rows per segment of the fused adjoint of L
+constexpr int kMarchRows = 64;    // rows per segment of the fused adjoint of L
 
 struct BArgs {
   const RunGeom* geom;

@@ -221,8 +221,12 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, int strip, int se
   }
 }
 
+#ifndef MHD_STAGE_WAVES
+#define MHD_STAGE_WAVES 3      // waves per SIMD the single-run stage kernel is compiled for (tuning parameter)
+#endif
+
 template <int MODE>
-__global__ __launch_bounds__(64) void k_stage(StageArgs a) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MHD_STAGE_WAVES))) void k_stage(StageArgs a) {
   stage_body<MODE>(a, blockIdx.x, blockIdx.y);
 }
 
