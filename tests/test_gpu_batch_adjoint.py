@@ -181,3 +181,43 @@ def test_batch_adjoint_argument_errors():
     with _batch(eta0) as b:
         with pytest.raises(RuntimeError):
             b.adjoint_forward(2, 1)
+
+
+def test_batch_adjoint_non_square_runs_match_single_run():
+    """Non-square runs (the fused adjoint march's strips and segments in
+    both directions) against the single-run adjoint."""
+    steps, every = 3, 2
+    shapes = [(70, 24), (24, 70), (130, 16)]
+    _say(f"batched adjoint of non-square runs {shapes}, {steps} steps")
+    cases = []
+    for r, (nx, ny) in enumerate(shapes):
+        n = max(nx, ny)
+        amps, phases = setups.sausage_column_modes(8, 1e-2, seed=40 + r)
+        full = setups.sausage_column(n, amps, phases)
+        st = {k: np.ascontiguousarray(full[k][: ny + 4, : nx + 4]) for k in CONSERVED}
+        cases.append((nx, ny, 1.0 / n, 1e-2, st))
+    lams = [_smooth(max(nx, ny), 500 + r) for r, (nx, ny, *_rest) in enumerate(cases)]
+    lams = [{k: np.ascontiguousarray(l[k][: ny + 4, : nx + 4]) for k in CONSERVED}
+            for l, (nx, ny, *_rest) in zip(lams, cases)]
+    dt = 2e-4
+    b = Batch([c[0] for c in cases], [c[1] for c in cases], [c[2] for c in cases], [c[2] for c in cases],
+              [c[3] for c in cases], **FIXED_CG)
+    with b:
+        b.set_schedule(dt_fixed=[dt] * len(cases))
+        for r, c in enumerate(cases):
+            b.upload(r, c[4])
+        b.adjoint_forward(steps, every)
+        for r in range(len(cases)):
+            b.adjoint_set_lambda(r, lams[r])
+        b.adjoint_backward()
+        res = [b.adjoint_result(r) for r in range(len(cases))]
+    for r, (nx, ny, h, eta, st) in enumerate(cases):
+        with Context(nx, ny, h, h, eta, **FIXED_CG) as c:
+            c.upload(*[st[k] for k in CONSERVED])
+            c.adjoint_forward(dt, steps, every)
+            lam0, eb = c.adjoint_backward(lams[r])
+        for k in CONSERVED:
+            scale = np.abs(interior(lam0[k])).max()
+            err = np.abs(interior(res[r][0][k]) - interior(lam0[k])).max()
+            assert err <= TOL_PARITY * scale, (r, k, err, scale)
+        assert abs(res[r][1] - eb) <= TOL_PARITY * abs(eb), (r, res[r][1], eb)
