@@ -23,6 +23,7 @@ import ctypes as C
 import json
 import os
 import subprocess
+import tempfile
 import sys
 import time
 
@@ -296,7 +297,7 @@ def cpu_baseline(n: int, eta: float, tol: float, cfl: float, steps: int) -> dict
     from oracle import pyoracle as po
 
     lib_path, flags = None, "-O3 -ffast-math -march=x86-64-v3 (prebuilt)"
-    out_dir = os.path.join(ROOT, "gpurun_out") if os.path.isdir(os.path.join(ROOT, "gpurun_out")) else "/tmp"
+    out_dir = tempfile.mkdtemp(prefix="mhd_oracle_")    # a native build of the oracle for this host
     try:
         po.build(fast_native=True, out_dir=out_dir)
         lib_path, flags = os.path.join(out_dir, "libmhd_oracle_native.so"), "-O3 -ffast-math -march=native"
