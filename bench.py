@@ -78,6 +78,8 @@ def parse():
     ap.add_argument("--cg-tol", type=float, default=1e-10)
     ap.add_argument("--cfl", type=float, default=0.4)
     ap.add_argument("--cpu-steps", type=int, default=2, help="CPU-baseline sample (steps at --n)")
+    ap.add_argument("--pcie-calls", type=int, default=3,
+                    help="Grid-taking calls (upload + step + download) timed for the PCIe-inclusive rate")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 counter passes")
     ap.add_argument("--batch-runs", type=int, default=256, help="ensemble size of the batched workload (0: skip)")
@@ -584,6 +586,22 @@ def main():
     ncopy, ms_copy = ctx.profile_get("other")
     copy_gbs = 2 * (1 << 30) / (ms_copy / ncopy * 1e-3) / 1e9
     ctx.profile(False)
+    # the PCIe-inclusive rate of a Grid-taking call, IMEX(grid, ctx, dt) of the
+    # C++ adapter: upload of the 6 conserved padded arrays, one step, download
+    # of the 11 padded arrays (state and primitives) -- not the metric
+    pcie = None
+    if a.pcie_calls > 0:
+        host = ctx.download()
+        dt_last = float(dts[-1])
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(a.pcie_calls):
+            ctx.upload(*[host[k] for k in ("rho", "mx", "my", "E", "bxf", "byf")])
+            ctx.IMEX(dt_last)
+            host = ctx.download(primitives=True)
+        tp = (time.perf_counter() - t0) / a.pcie_calls
+        pcie = {"value": round(N / tp / 1e6, 2), "unit": "Mcell-updates/s", "ms_per_call": round(tp * 1e3, 2),
+                "note": "upload 6 + step + download 11 padded host arrays per call (pageable host memory)"}
     ctx.close()
 
     solves = 2 * steps
@@ -617,6 +635,7 @@ def main():
         "copy_kernel_GBs": round(copy_gbs, 1),
         "divb_max": divmax,
         "kernels": kb,
+        "pcie_inclusive": pcie,
     }
     if not a.no_traffic:
         tr = hbm_traffic(a, dom)

@@ -79,11 +79,9 @@ StageArgs stage_args(mhd_ctx* c) {
 
 int run_stage(mhd_ctx* c, int mode, const StageArgs& a, int id) {
   dim3 grid((c->nx + kStripOut - 1) / kStripOut, (c->ny + a.rows - 1) / a.rows);
-  return launch(c, id, [&] {
-    if (mode == 0) hipLaunchKernelGGL(k_stage<0>, grid, dim3(64), 0, c->stream, a);
-    else if (mode == 1) hipLaunchKernelGGL(k_stage<1>, grid, dim3(64), 0, c->stream, a);
-    else hipLaunchKernelGGL(k_stage<2>, grid, dim3(64), 0, c->stream, a);
-  });
+  if (mode == 0) return launch_k(c, id, k_stage<0>, grid, dim3(64), a);
+  if (mode == 1) return launch_k(c, id, k_stage<1>, grid, dim3(64), a);
+  return launch_k(c, id, k_stage<2>, grid, dim3(64), a);
 }
 
 CgArgs cg_args(mhd_ctx* c, const double* bx, const double* by) {
@@ -128,14 +126,10 @@ int cg_run(mhd_ctx* c, const double* bx, const double* by, const double* bin, Cg
   int rc0 = launch(c, 8, [&] { hipLaunchKernelGGL(k_cg_reset, dim3(1), dim3(1), 0, c->stream, c->st, maxit); });
   if (rc0) return rc0;
   const int nb = c->tiles;
-  int rc = launch(c, 2, [&] {
-    if (bin) hipLaunchKernelGGL(k_cg_start<false>, dim3(nb), dim3(kTileX), 0, c->stream, a, bin);
-    else hipLaunchKernelGGL(k_cg_start<true>, dim3(nb), dim3(kTileX), 0, c->stream, a, bin);
-  });
+  int rc = bin ? launch_k(c, 2, k_cg_start<false>, dim3(nb), dim3(kTileX), a, bin)
+               : launch_k(c, 2, k_cg_start<true>, dim3(nb), dim3(kTileX), a, bin);
   if (rc) return rc;
-  rc = launch(c, 4, [&] {
-    hipLaunchKernelGGL(k_cg_pass_b<true>, dim3(nb), dim3(kTileX), 0, c->stream, a, (const double*)c->p[0], 0);
-  });
+  rc = launch_k(c, 4, k_cg_pass_b<true>, dim3(nb), dim3(kTileX), a, (const double*)c->p[0], 0);
   if (rc) return rc;
   // the direction of iteration it lives in p[it & 1] (p[0] from the start kernel)
   long long it = 1;
@@ -144,14 +138,10 @@ int cg_run(mhd_ctx* c, const double* bx, const double* by, const double* bin, Cg
     for (; it <= spec + 1; ++it) {
       const double* pold = c->p[(it - 1) & 1];
       double* pnew = c->p[it & 1];
-      rc = launch(c, 3, [&] {
-        hipLaunchKernelGGL(k_cg_pass_a, dim3(nb), dim3(kTileX), 0, c->stream, a, pold, pnew, (int)it);
-      });
+      rc = launch_k(c, 3, k_cg_pass_a, dim3(nb), dim3(kTileX), a, pold, pnew, (int)it);
       if (rc) return rc;
       if (it == spec + 1) break;
-      rc = launch(c, 4, [&] {
-        hipLaunchKernelGGL(k_cg_pass_b<false>, dim3(nb), dim3(kTileX), 0, c->stream, a, (const double*)pnew, (int)it);
-      });
+      rc = launch_k(c, 4, k_cg_pass_b<false>, dim3(nb), dim3(kTileX), a, (const double*)pnew, (int)it);
       if (rc) return rc;
     }
     if (spec_out) HIP_OR_FAIL(c, hipMemcpyAsync(spec_out, c->st, sizeof(CgState), hipMemcpyDeviceToDevice, c->stream));
@@ -165,14 +155,10 @@ int cg_run(mhd_ctx* c, const double* bx, const double* by, const double* bin, Cg
     for (int k = 0; k < chunk; ++k, ++it) {
       const double* pold = c->p[(it - 1) & 1];
       double* pnew = c->p[it & 1];
-      rc = launch(c, 3, [&] {
-        hipLaunchKernelGGL(k_cg_pass_a, dim3(nb), dim3(kTileX), 0, c->stream, a, pold, pnew, (int)it);
-      });
+      rc = launch_k(c, 3, k_cg_pass_a, dim3(nb), dim3(kTileX), a, pold, pnew, (int)it);
       if (rc) return rc;
       if (k + 1 == chunk) break;   // the chunk ends with an A: it holds the decision
-      rc = launch(c, 4, [&] {
-        hipLaunchKernelGGL(k_cg_pass_b<false>, dim3(nb), dim3(kTileX), 0, c->stream, a, (const double*)pnew, (int)it);
-      });
+      rc = launch_k(c, 4, k_cg_pass_b<false>, dim3(nb), dim3(kTileX), a, (const double*)pnew, (int)it);
       if (rc) return rc;
     }
     HIP_OR_FAIL(c, hipMemcpyAsync(c->st_host, c->st, sizeof(CgState), hipMemcpyDeviceToHost, c->stream));
@@ -180,9 +166,7 @@ int cg_run(mhd_ctx* c, const double* bx, const double* by, const double* bin, Cg
     if (c->st_host->done) break;
     // not converged: run B(it) for the A just launched, then continue
     const double* pv = c->p[it & 1];
-    rc = launch(c, 4, [&] {
-      hipLaunchKernelGGL(k_cg_pass_b<false>, dim3(nb), dim3(kTileX), 0, c->stream, a, pv, (int)it);
-    });
+    rc = launch_k(c, 4, k_cg_pass_b<false>, dim3(nb), dim3(kTileX), a, pv, (int)it);
     if (rc) return rc;
     ++it;
     if (it > maxit + 4) return fail(c, MHD_EHIP, "CG state machine did not terminate");
@@ -234,7 +218,7 @@ DiagArgs diag_args(mhd_ctx* c) {
 template <typename K>
 int diag_partials(mhd_ctx* c, K kernel, int id) {
   DiagArgs d = diag_args(c);
-  int rc = launch(c, id, [&] { hipLaunchKernelGGL(kernel, dim3(c->tiles), dim3(kTileX), 0, c->stream, d); });
+  int rc = launch_k(c, id, kernel, dim3(c->tiles), dim3(kTileX), d);
   if (rc) return rc;
   HIP_OR_FAIL(c, hipMemcpyAsync(c->part_host, c->part, 2 * sizeof(double) * c->tiles, hipMemcpyDeviceToHost, c->stream));
   HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
@@ -338,7 +322,7 @@ int imex_step(mhd_ctx* c, double dt, mhd_step_stats* stats, StepRecord* rec) {
         if (rec->spec[0] == 0 && rec->st[0]) HIP_OR_FAIL(c, hipMemcpyAsync(rec->st[0], c->st, sizeof(CgState), hipMemcpyDeviceToDevice, c->stream));
       }
     }
-    rc = launch(c, 5, [&] { hipLaunchKernelGGL(k_finalize1, dim3(c->tiles), dim3(kTileX), 0, c->stream, f); });
+    rc = launch_k(c, 5, k_finalize1, dim3(c->tiles), dim3(kTileX), f);
     if (rc) return rc;
   }
   if (rec && rec->S[0]) {
@@ -379,7 +363,7 @@ int imex_step(mhd_ctx* c, double dt, mhd_step_stats* stats, StepRecord* rec) {
         if (rec->spec[1] == 0 && rec->st[1]) HIP_OR_FAIL(c, hipMemcpyAsync(rec->st[1], c->st, sizeof(CgState), hipMemcpyDeviceToDevice, c->stream));
       }
     }
-    rc = launch(c, 6, [&] { hipLaunchKernelGGL(k_finalize2, dim3(c->tiles), dim3(kTileX), 0, c->stream, f); });
+    rc = launch_k(c, 6, k_finalize2, dim3(c->tiles), dim3(kTileX), f);
     if (rc) return rc;
     std::swap(c->U[4], c->S[4]);
     std::swap(c->U[5], c->S[5]);
