@@ -133,6 +133,35 @@ MHD_API int mhd_divb(mhd_ctx* ctx, double* rms, double* maxabs);
 MHD_API int mhd_run(mhd_ctx* ctx, double t_end, double dt_override, int max_steps,
             int* steps_out, double* t_out, double* step_dt, int* cg_iters);
 
+/* Snapshot callback of mhd_run_resident: frame = steps taken (0 before the
+ * first step), t = the time after it, dt = that step's dt (frame 0: the
+ * initial CFL dt or the override, as the reference's first snapshot records).
+ * The stream is idle, so the callback may call mhd_download /
+ * mhd_write_snapshot or change the state (the loop continues from the state
+ * the callback leaves); it must not call mhd_run_resident on the same
+ * context.  A non-zero return stops the loop and is returned. */
+typedef int (*mhd_snapshot_fn)(mhd_ctx* ctx, int frame, double t, double dt, void* user);
+
+/* The driver loop of mhd_run (src/integrator_main.cpp:183-243, same
+ * arguments and results, identical states, time steps and CG counts) with
+ * the per-step decisions on the device: dt, the t_end clamp and the operator
+ * rebuild rule are computed by a kernel, and CG solves run a launched
+ * iteration budget whose unused passes return at once, so the host reads the
+ * driver state once per group of steps instead of three times per step.
+ * With snapshot_every > 0 and on_snapshot non-NULL, on_snapshot runs before
+ * the first step and after every snapshot_every-th step, like the
+ * reference's snapshot cadence (integrator_main.cpp:197-203, 236-241).
+ * No reference counterpart beyond the loop itself. */
+MHD_API int mhd_run_resident(mhd_ctx* ctx, double t_end, double dt_override, int max_steps, int snapshot_every,
+                             mhd_snapshot_fn on_snapshot, void* user, int* steps_out, double* t_out,
+                             double* step_dt, int* cg_iters);
+/* Tuning of mhd_run_resident: steps enqueued between two read-backs (1..256,
+ * default 16), extra CG iterations launched per solve beyond the most of the
+ * previous group (default 1), and fixed_budget >= 0: that many launched
+ * iterations per solve, not adapted (a solve that needs more halts the loop
+ * and the host finishes its step; -1 restores the adaptive budget). */
+MHD_API int mhd_resident_tune(mhd_ctx* ctx, int group_steps, int spec_margin, int fixed_budget);
+
 /* write_data_to_binary_file (include/diagnostics.hpp:90-131): the current
  * state as a snapshot file in the reference's format -- 9 doubles (nx, ny,
  * ng = 2, snapshot_rate, dx, dy, time_step, rms div B, max |div B|), then the

@@ -16,6 +16,7 @@
 //   CFL_condition_checker(g)                            mhd_mi355x::CFL_condition_checker(g, ctx)
 //   calculate_current_div_B(g)                          mhd_mi355x::calculate_current_div_B(g, ctx)
 //   write_data_to_binary_file(g, frame, rate, dt)       mhd_mi355x::write_data_to_binary_file(g, ctx, frame, rate, dt)
+//   main's time loop (integrator_main.cpp:183-243)      mhd_mi355x::integrate(g, ctx, t_end, dt_override, rate)
 //
 // The Grid-taking functions copy the state to the device, run, and copy it
 // back (PCIe both ways).  Loops that keep the state on the GPU call
@@ -137,6 +138,32 @@ void write_data_to_binary_file(const GridT& g, Context& ctx, int current_frame_i
   char name[64];
   std::snprintf(name, sizeof(name), "output/snapshot_%06d.dat", current_frame_index);
   check(mhd_write_snapshot(ctx.get(), name, snapshot_rate, time_step), ctx.get());
+}
+
+// The time loop of the reference's main (src/integrator_main.cpp:183-243) on
+// the device: g's state is uploaded once, advanced to t_end (constant dt when
+// dt_override != 0) by mhd_run_resident, with a snapshot
+// output/snapshot_<frame>.dat before the first step and every
+// snapshot_frequency steps (0: none), and downloaded into g at the end.
+// Returns the number of steps taken.
+template <class GridT>
+int integrate(GridT& g, Context& ctx, double t_end, double dt_override, int snapshot_frequency) {
+  ctx.upload(g);
+  struct Cadence {
+    int rate;
+  } cad{snapshot_frequency};
+  auto write = [](mhd_ctx* c, int frame, double, double dt, void* user) -> int {
+    char name[64];
+    std::snprintf(name, sizeof(name), "output/snapshot_%06d.dat", frame);
+    return mhd_write_snapshot(c, name, static_cast<Cadence*>(user)->rate, dt);
+  };
+  int steps = 0;
+  double t = 0.0;
+  check(mhd_run_resident(ctx.get(), t_end, dt_override, -1, snapshot_frequency,
+                         snapshot_frequency > 0 ? +write : nullptr, &cad, &steps, &t, nullptr, nullptr),
+        ctx.get());
+  ctx.download(g);
+  return steps;
 }
 
 }  // namespace mhd_mi355x

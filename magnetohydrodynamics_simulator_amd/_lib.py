@@ -32,6 +32,8 @@ class StepStats(C.Structure):
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
 _vp = C.c_void_p
+# mhd_snapshot_fn: int (*)(mhd_ctx*, int frame, double t, double dt, void* user)
+SNAPSHOT_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_void_p)
 SIGNATURES = {
     "mhd_default_opts": (C.c_int, [C.POINTER(Opts)]),
     "mhd_create": (C.c_int, [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.POINTER(Opts), C.POINTER(_vp)]),
@@ -47,6 +49,8 @@ SIGNATURES = {
     "mhd_cg_solve": (C.c_int, [_vp, _dp, _dp, _ip, _dp]),
     "mhd_divb": (C.c_int, [_vp, _dp, _dp]),
     "mhd_run": (C.c_int, [_vp, C.c_double, C.c_double, C.c_int, _ip, _dp, _dp, _ip]),
+    "mhd_run_resident": (C.c_int, [_vp, C.c_double, C.c_double, C.c_int, C.c_int, _vp, _vp, _ip, _dp, _dp, _ip]),
+    "mhd_resident_tune": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
     "mhd_profile_enable": (C.c_int, [_vp, C.c_int]),
     "mhd_profile_get": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_longlong), _dp]),
     "mhd_calibration_copy": (C.c_int, [_vp, C.c_longlong]),

@@ -220,7 +220,6 @@ BatchFinArgs fin_args(mhd_batch* b, const RunState* rs, const double* x, const C
   f.bx_out = bxo;
   f.by_out = byo;
   f.st = st;
-  f.fc = b->tab.A_im4;
   return f;
 }
 
@@ -419,7 +418,7 @@ int mhd_batch_step(mhd_batch* b, int* n_active) {
   d.part = b->part;
   BLAUNCH(b, kb_cfl, dim3(nt), dim3(kTileX), 0, b->stream, d);
   BLAUNCH(b, kb_dt, dim3(b->nruns), dim3(256), 0, b->stream, (const RunGeom*)b->geom, b->rs, (const double*)b->part,
-          b->st[0], b->st[1], b->nruns, T.A_im4);
+          b->st[0], b->st[1], b->nruns, step_factors(T), 0);
   BHIP(b, hipMemcpyAsync(b->rs_host, b->rs, sizeof(RunState) * b->nruns, hipMemcpyDeviceToHost, b->stream));
   // stage 1: W = U + dt delta f1 (into W), U2* = U + dt gamma f1 (into S)
   BatchStageArgs s1{};
@@ -431,8 +430,6 @@ int mhd_batch_step(mhd_batch* b, int* n_active) {
     s1.o1[q] = b->W[q];
     s1.o2[q] = b->S[q];
   }
-  s1.fa = T.A_ex3;
-  s1.fb = T.A_ex6;
   BLAUNCH(b, kb_stage<1>, dim3(b->n_stage_jobs), dim3(64), 0, b->stream, s1);
   if (b->any_eta) {
     CgCall cc;
@@ -457,8 +454,6 @@ int mhd_batch_step(mhd_batch* b, int* n_active) {
     s2.u[q] = b->W[q];
     s2.o1[q] = b->U[q];
   }
-  s2.fb = T.A_ex7;
-  s2.fg = T.A_im7;
   BLAUNCH(b, kb_stage<2>, dim3(b->n_stage_jobs), dim3(64), 0, b->stream, s2);
   if (b->any_eta) {
     CgCall cc;

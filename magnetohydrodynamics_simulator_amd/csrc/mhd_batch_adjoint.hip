@@ -507,7 +507,7 @@ int record_step(mhd_batch* b, int s, int k, bool given) {
     d.part = b->part;
     BLAUNCH(b, kb_cfl, dim3(nt), dim3(kTileX), 0, b->stream, d);
     BLAUNCH(b, kb_dt, dim3(nr), dim3(256), 0, b->stream, (const RunGeom*)b->geom, b->rs, (const double*)b->part, st0,
-            st1, nr, T.A_im4);
+            st1, nr, step_factors(T), 0);
     BHIP(b, hipMemcpyAsync(b->rs_host, b->rs, sizeof(RunState) * nr, hipMemcpyDeviceToHost, b->stream));
   }
   // stage 1: W = U + dt delta f1, S* = U + dt gamma f1 (S*.B into SsB)
@@ -520,8 +520,6 @@ int record_step(mhd_batch* b, int s, int k, bool given) {
     s1.o1[q] = b->W[q];
     s1.o2[q] = (q < 4) ? A.S[k][q] : A.SsB[k][q - 4];
   }
-  s1.fa = T.A_ex3;
-  s1.fb = T.A_ex6;
   BLAUNCH(b, kb_stage<1>, dim3(b->n_stage_jobs), dim3(64), 0, b->stream, s1);
   if (!given) {
     CgCall cc;
@@ -547,8 +545,6 @@ int record_step(mhd_batch* b, int s, int k, bool given) {
     s2.u[q] = b->W[q];
     s2.o1[q] = (q < 4) ? out[q] : A.U3sB[k][q - 4];
   }
-  s2.fb = T.A_ex7;
-  s2.fg = T.A_im7;
   BLAUNCH(b, kb_stage<2>, dim3(b->n_stage_jobs), dim3(64), 0, b->stream, s2);
   if (!given) {
     CgCall cc;

@@ -491,6 +491,7 @@ int mhd_destroy(mhd_ctx* c) {
   if (!c) return MHD_EINVAL;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   destroy_adjoint(c);
+  destroy_resident(c);
   for (int k = 0; k < kNumKernelIds; ++k) {
     for (auto& pr : c->pending[k]) {
       (void)hipEventDestroy(pr.first);

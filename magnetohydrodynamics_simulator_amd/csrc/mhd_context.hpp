@@ -28,6 +28,7 @@ struct Tableau {
 };
 
 struct AdjointStore;   // checkpoints and scratch of the adjoint (mhd_adjoint.hip)
+struct Resident;       // driver state and step log of the device-resident loop (mhd_resident.hip)
 
 // Device buffers a step writes besides the state, for the adjoint's recompute:
 // the state at the start of the step, the stage state after implicit 1, the
@@ -84,6 +85,7 @@ struct mhd_ctx {
   double prof_ms[mhd_host::kNumKernelIds] = {};
   std::vector<std::pair<hipEvent_t, hipEvent_t>> pending[mhd_host::kNumKernelIds];
   mhd_host::AdjointStore* adj = nullptr;
+  mhd_host::Resident* res = nullptr;
 };
 
 namespace mhd_host {
@@ -140,6 +142,13 @@ int launch_k(mhd_ctx* c, int id, K kernel, dim3 grid, dim3 block, Args... args) 
 }
 
 int grid_1d(long long n);
+// Kernel argument blocks of the context's current buffers (coefficients from
+// the host-side dt / operator; the resident loop's kernels replace them with
+// the device driver state's).
+mhd::StageArgs stage_args(mhd_ctx* c);
+mhd::CgArgs cg_args(mhd_ctx* c, const double* bx, const double* by);
+mhd::FinArgs fin_args(mhd_ctx* c, const double* bx, const double* by, double* bxo, double* byo, double coef);
+mhd::DiagArgs diag_args(mhd_ctx* c);
 // One CG solve (Eigen's algorithm) of A x = eta J(bx, by) or A x = bin into
 // c->x.  spec == 0: chunks predicted from the slot's previous solve, the host
 // reads the state after each chunk until the solve is done, *out receives it.
@@ -155,5 +164,6 @@ int from_host_padded(mhd_ctx* c, const double* host, double* dev);
 // One IMEX step (mhd_imex_step); rec, if not null, receives the step's data.
 int imex_step(mhd_ctx* c, double dt, mhd_step_stats* stats, StepRecord* rec);
 void destroy_adjoint(mhd_ctx* c);
+void destroy_resident(mhd_ctx* c);
 
 }  // namespace mhd_host

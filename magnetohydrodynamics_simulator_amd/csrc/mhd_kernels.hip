@@ -949,9 +949,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void kb
   a.inv_dx = g.inv_dx;
   a.inv_dy = g.inv_dy;
   a.eta = g.eta;
-  a.c_a = rs.dt * b.fa;
-  a.c_b = rs.dt * b.fb;
-  a.c_g = rs.dt * b.fg;
+  a.c_a = (MODE == 1) ? rs.coef[0] : 0.0;
+  a.c_b = (MODE == 1) ? rs.coef[1] : rs.coef[2];
+  a.c_g = (MODE == 2) ? rs.coef[3] : 0.0;
   stage_body<MODE>(a, job.y % g.strips, job.y / g.strips);
 }
 template __global__ void kb_stage<1>(BatchStageArgs);
@@ -1029,7 +1029,7 @@ __device__ __forceinline__ FinArgs batch_fin_args(const BatchFinArgs& b, int run
   f.dx = g.dx;
   f.dy = g.dy;
   f.eta = g.eta;
-  f.c = b.rs[run].dt * b.fc;
+  f.c = b.rs[run].coef[4];
   f.x = b.x + g.base;
   f.bx = b.bx + g.base;
   f.by = b.by + g.base;
@@ -1091,12 +1091,15 @@ __global__ __launch_bounds__(kTileX) void kb_cfl(BatchDiagArgs b) {
 // Also resets both CG states of the run.  One block per run.
 __global__ __launch_bounds__(256) void kb_dt(const RunGeom* __restrict__ geom, RunState* __restrict__ rs,
                                              const double* __restrict__ part, CgState* __restrict__ st0,
-                                             CgState* __restrict__ st1, int nruns, double a_im4) {
+                                             CgState* __restrict__ st1, int nruns, StepFactors f,
+                                             int halt_on_nonfinite) {
+  const double a_im4 = f.im4;
   __shared__ double lds[16];
   const int run = blockIdx.x;
   if (run >= nruns) return;
   const RunGeom& g = geom[run];
   RunState s = rs[run];
+  if (s.halt) return;                        // halted resident loop: nothing moves until the host resumes
   const bool active = (s.t < g.t_end) && (g.max_steps < 0 || s.steps < g.max_steps);
   double m = 1e-8;
   double bad[1] = {0.0};
@@ -1115,6 +1118,12 @@ __global__ __launch_bounds__(256) void kb_dt(const RunGeom* __restrict__ geom, R
     s.active = 0;
     c.done = 3;
   } else {
+    if (halt_on_nonfinite && g.dt_fixed <= 0.0 && bad[0] > 0.0) {
+      s.nonfinite = 1;                       // the step is not taken (mhd_cfl_dt fails before it)
+      s.halt = kHaltNonfinite;
+      rs[run] = s;
+      return;
+    }
     const double h = (g.dy < g.dx) ? g.dy : g.dx;
     const double dt_res = (g.eta != 0.0) ? h * h / (4.0 * g.eta) : INFINITY;
     const double dt_hyp = (g.cfl / 0.9) * h / m;
@@ -1129,6 +1138,11 @@ __global__ __launch_bounds__(256) void kb_dt(const RunGeom* __restrict__ geom, R
     if (g.eta > 0.0 && fabs(dt - s.prev_dt) > 1e-10 * dt) s.alpha = (dt * g.eta) * a_im4;
     s.prev_dt = dt;
     s.dt = dt;
+    s.coef[0] = dt * f.ex3;
+    s.coef[1] = dt * f.ex6;
+    s.coef[2] = dt * f.ex7;
+    s.coef[3] = dt * f.im7;
+    s.coef[4] = dt * f.im4;
     s.cx = s.alpha / (g.dx * g.dx);
     s.cy = s.alpha / (g.dy * g.dy);
     s.diag = 1 + 2 * (s.cx + s.cy);
@@ -1142,6 +1156,111 @@ __global__ __launch_bounds__(256) void kb_dt(const RunGeom* __restrict__ geom, R
   rs[run] = s;
   st0[run] = c;
   st1[run] = c;
+}
+
+// ---------------------------------------------------------------------------
+// Single-run device-resident driver loop (mhd_run_resident).  kb_dt (one run)
+// writes the step's dt, the operator coefficients and the run's activity into
+// a RunState; the kernels below are the plain kernels with those values read
+// from it, so the host enqueues many steps without reading anything back.
+// Each returns at once when the run has stopped (t >= t_end or the step
+// limit) or the loop has halted.  The host launches a fixed number of CG
+// iterations per solve (the passes of a finished solve return at once); a
+// finaliser that finds its solve unfinished halts the loop, and the host
+// finishes that step itself.
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ bool res_stopped(const RunState* rs) { return !rs->active || rs->halt != 0; }
+
+template <int MODE>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MHD_STAGE_WAVES))) void k_stage_res(
+    StageArgs a, const RunState* __restrict__ rs) {
+  if (res_stopped(rs)) return;
+  StageArgs b = a;
+  b.c_a = (MODE == 1) ? rs->coef[0] : 0.0;
+  b.c_b = (MODE == 1) ? rs->coef[1] : rs->coef[2];
+  b.c_g = (MODE == 2) ? rs->coef[3] : 0.0;
+  stage_body<MODE>(b, blockIdx.x, blockIdx.y);
+}
+template __global__ void k_stage_res<1>(StageArgs, const RunState*);
+template __global__ void k_stage_res<2>(StageArgs, const RunState*);
+
+__device__ __forceinline__ CgArgs res_cg_args(const CgArgs& a, const RunState* rs) {
+  CgArgs b = a;
+  b.cx = rs->cx;
+  b.cy = rs->cy;
+  b.diag = rs->diag;
+  b.invd = rs->invd;
+  return b;
+}
+
+__global__ __launch_bounds__(kTileX) void k_cg_start_res(CgArgs a, const RunState* __restrict__ rs) {
+  if (res_stopped(rs)) return;
+  cg_start_body<true>(res_cg_args(a, rs), nullptr, xcd_block(blockIdx.x, gridDim.x));
+}
+
+__global__ __launch_bounds__(kTileX) void k_cg_pass_a_res(CgArgs a, const RunState* __restrict__ rs,
+                                                          const double* __restrict__ pold, double* __restrict__ pnew,
+                                                          int it) {
+  if (res_stopped(rs)) return;
+  cg_pass_a_body(res_cg_args(a, rs), pold, pnew, it, xcd_block(blockIdx.x, gridDim.x), gridDim.x);
+}
+
+template <bool FIRST>
+__global__ __launch_bounds__(kTileX) void k_cg_pass_b_res(CgArgs a, const RunState* __restrict__ rs,
+                                                          const double* __restrict__ pv, int it) {
+  if (res_stopped(rs)) return;
+  cg_pass_b_body<FIRST>(res_cg_args(a, rs), pv, it, xcd_block(blockIdx.x, gridDim.x), gridDim.x);
+}
+template __global__ void k_cg_pass_b_res<true>(CgArgs, const RunState*, const double*, int);
+template __global__ void k_cg_pass_b_res<false>(CgArgs, const RunState*, const double*, int);
+
+// Finaliser of solve SLOT (0: B2 = B2* + c rate(x); 1: the final
+// recombination).  Every block takes the same decision from the solve's
+// state: unfinished (or at the iteration limit with fail_is_error) halts the
+// loop before any field changes.
+template <int SLOT>
+__global__ __launch_bounds__(kTileX) void k_finalize_res(FinArgs a, RunState* __restrict__ rs, int fail_is_error) {
+  if (res_stopped(rs)) return;
+  const int done = a.st->done;
+  if (done == 0 || (done == 2 && fail_is_error)) {
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+      rs->halt = (done == 0) ? (SLOT == 0 ? kHaltSolve1 : kHaltSolve2) : (SLOT == 0 ? kHaltCgLimit1 : kHaltCgLimit2);
+    return;
+  }
+  FinArgs f = a;
+  f.c = rs->coef[4];
+  const int vb = xcd_block(blockIdx.x, gridDim.x);
+  if (SLOT == 0) finalize1_body(f, vb);
+  else finalize2_body(f, vb);
+}
+template __global__ void k_finalize_res<0>(FinArgs, RunState*, int);
+template __global__ void k_finalize_res<1>(FinArgs, RunState*, int);
+
+// End of a resident step: the step's log entry (block 0) and, with
+// `partials`, the CFL partials of the new state for the next kb_dt.
+__global__ __launch_bounds__(kTileX) void k_cfl_res(DiagArgs a, const RunState* __restrict__ rs,
+                                                    const CgState* __restrict__ st, StepLog* __restrict__ log,
+                                                    int log_cap, int partials) {
+  if (res_stopped(rs)) return;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    StepLog e;
+    e.dt = rs->dt;
+    e.t = rs->t;
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      const CgState& s = st[k];
+      e.iters[k] = s.iters;
+      e.spmv[k] = s.spmv;
+      e.err[k] = (s.done == 1 || s.done == 2) ? sqrt(s.rn2 / s.rhs_norm2) : 0.0;
+    }
+    log[(rs->steps - 1) % log_cap] = e;
+  }
+  if (partials) cfl_body(a, xcd_block(blockIdx.x, gridDim.x));
+}
+
+__global__ void k_clear_halt(RunState* rs) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) rs->halt = 0;
 }
 
 }  // namespace mhd

@@ -91,9 +91,37 @@ struct RunGeom {
 };
 
 // Driver state of one run, device resident; kb_dt writes it once per step.
+// halt (single-run resident loop only): a kHalt* code once the loop has
+// stopped inside a step; every kernel of the loop then returns at once.
+// coef: the step's stage coefficients dt * (A_ex[3], A_ex[6], A_ex[7],
+// A_im[7], A_im[4]), read by the stage kernels and finalisers (loaded from
+// memory they live in scalar registers; formed in a stage kernel they would
+// cost it vector registers it does not have).
 struct RunState {
   double t, dt, prev_dt, alpha, cx, cy, diag, invd;
-  int steps, active, nonfinite, pad_;
+  double coef[5];
+  int steps, active, nonfinite, halt;
+};
+
+// The tableau weights kb_dt multiplies by dt (IMEX_Butcher_Tableus, solvers.hpp:39-66).
+struct StepFactors {
+  double ex3, ex6, ex7, im7, im4;
+};
+template <typename Tab>
+inline StepFactors step_factors(const Tab& T) {
+  return StepFactors{T.A_ex3, T.A_ex6, T.A_ex7, T.A_im7, T.A_im4};
+}
+
+constexpr int kHaltSolve1 = 1;      // implicit solve 1 not finished within the launched iterations
+constexpr int kHaltSolve2 = 2;      // implicit solve 2 likewise
+constexpr int kHaltNonfinite = 3;   // non-finite wave speed: the step was not taken
+constexpr int kHaltCgLimit1 = 4;    // solve 1 reached the iteration limit (opts.cg_fail_is_error)
+constexpr int kHaltCgLimit2 = 5;    // solve 2 likewise
+
+// One entry of the resident loop's step log (ring buffer on the device).
+struct StepLog {
+  double dt, t, err[2];
+  int iters[2], spmv[2];
 };
 
 struct BatchStageArgs {
@@ -104,7 +132,6 @@ struct BatchStageArgs {
   const double* u[6];
   double* o1[6];
   double* o2[6];
-  double fa, fb, fg;       // stage coefficients per unit of dt (c = dt * f)
 };
 
 struct BatchCgArgs {
@@ -133,7 +160,6 @@ struct BatchFinArgs {
   double* bx_out;
   double* by_out;
   const CgState* st;
-  double fc;               // c = dt * fc
 };
 
 struct BatchDiagArgs {
@@ -153,7 +179,19 @@ __global__ void kb_finalize1(BatchFinArgs b);
 __global__ void kb_finalize2(BatchFinArgs b);
 __global__ void kb_cfl(BatchDiagArgs b);
 __global__ void kb_dt(const RunGeom* geom, RunState* rs, const double* part, CgState* st0, CgState* st1,
-                      int nruns, double a_im4);
+                      int nruns, StepFactors f, int halt_on_nonfinite);
+
+// Single-run resident loop: the plain kernels with their step coefficients
+// read from the device driver state (c = dt * f), returning at once once the
+// run has stopped or the loop has halted.
+template <int MODE> __global__ void k_stage_res(StageArgs a, const RunState* rs);
+__global__ void k_cg_start_res(CgArgs a, const RunState* rs);
+__global__ void k_cg_pass_a_res(CgArgs a, const RunState* rs, const double* pold, double* pnew, int it);
+template <bool FIRST> __global__ void k_cg_pass_b_res(CgArgs a, const RunState* rs, const double* pv, int it);
+template <int SLOT> __global__ void k_finalize_res(FinArgs a, RunState* rs, int fail_is_error);
+__global__ void k_cfl_res(DiagArgs a, const RunState* rs, const CgState* st, StepLog* log, int log_cap,
+                          int partials);
+__global__ void k_clear_halt(RunState* rs);
 
 template <int MODE> __global__ void k_stage(StageArgs a);
 template <bool FROM_B> __global__ void k_cg_start(CgArgs a, const double* bin);

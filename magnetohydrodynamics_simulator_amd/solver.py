@@ -192,6 +192,42 @@ class Context:
         n = steps.value
         return n, dts[:n].copy(), its[: 2 * n].reshape(n, 2).copy(), t.value
 
+    def run_resident(self, t_end: float, dt_override: float = 0.0, max_steps: int = -1, snapshot_every: int = 0,
+                     on_snapshot=None):
+        """The same driver loop with its per-step decisions on the device
+        (mhd_run_resident): identical results to run(), one host read-back
+        per group of steps.  on_snapshot(ctx, frame, t, dt) runs before the
+        first step and after every snapshot_every-th step; it may download
+        or write the state."""
+        cap = max_steps if max_steps > 0 else 1_000_000
+        dts = np.zeros(cap)
+        its = np.zeros(2 * cap, dtype=np.int32)
+        steps, t = C.c_int(), C.c_double()
+        raised = []
+
+        def trampoline(_h, frame, tt, dt, _user):
+            try:
+                on_snapshot(self, frame, tt, dt)
+                return 0
+            except BaseException as e:   # re-raised below, after the C loop has returned
+                raised.append(e)
+                return _lib.MHD_EINVAL
+
+        cb = _lib.SNAPSHOT_FN(trampoline) if on_snapshot is not None else None
+        rc = self.lib.mhd_run_resident(self.h, t_end, dt_override, max_steps, snapshot_every,
+                                       C.cast(cb, C.c_void_p) if cb is not None else None, None,
+                                       C.byref(steps), C.byref(t), _ptr(dts), its.ctypes.data_as(C.POINTER(C.c_int)))
+        if raised:
+            raise raised[0]
+        self._check(rc)
+        n = steps.value
+        return n, dts[:n].copy(), its[: 2 * n].reshape(n, 2).copy(), t.value
+
+    def resident_tune(self, group_steps: int = 16, spec_margin: int = 1, fixed_budget: int = -1):
+        """Group size, CG iteration margin and an optional fixed iteration
+        budget of run_resident (mhd_resident_tune)."""
+        self._check(self.lib.mhd_resident_tune(self.h, group_steps, spec_margin, fixed_budget))
+
     # ---- adjoint ----
     def _field_ptrs(self, arrs):
         keep = [np.ascontiguousarray(a, dtype=np.float64) for a in arrs]

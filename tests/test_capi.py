@@ -96,6 +96,8 @@ int main() {
     auto dv = mhd_mi355x::calculate_current_div_B(grid, ctx);
     (void)dv;
     mhd_mi355x::write_data_to_binary_file(grid, ctx, 0, 20, dt);
+    int steps = mhd_mi355x::integrate(grid, ctx, 0.1, 0.0, 20);
+    (void)steps;
   } catch (const mhd_mi355x::Error& e) {
     return e.code == MHD_EHIP ? 3 : 1;
   }

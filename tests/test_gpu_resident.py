@@ -1,0 +1,139 @@
+"""GPU checks of the device-resident driver loop (mhd_run_resident).
+
+The resident loop takes the driver's per-step decisions on the device (CFL
+dt, the t_end clamp, the operator rebuild rule, CG termination) and must give
+exactly what mhd_run gives -- the same kernels on the same operands, so the
+bar is bitwise: every conserved field, every dt, every CG iteration count and
+the final time.  mhd_run itself is held to the oracle by test_gpu_parity.py.
+
+Cases: the adaptive CG budget; fixed budgets of 0 and 1 launched iterations,
+which halt the loop in (nearly) every solve and send those steps through the
+host's resume path, with group sizes 1, 3 and 16; the t_end clamp; a constant
+dt; eta = 0 (no implicit solve); a non-square grid; the snapshot cadence
+(frame numbers, the state at each frame, files in the reference's format);
+and the error paths (non-finite wave speed, the CG iteration limit with
+cg_fail_is_error, an exception in the callback).
+"""
+import os
+
+import numpy as np
+import pytest
+
+from helpers import CONSERVED, ic_arrays, make_gpu
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(f"\n[case] {msg}", flush=True)
+
+
+def _plain(ic, nx, ny, dx, dy, eta, t_end, max_steps, dt=0.0, **kw):
+    with make_gpu(ic, nx, ny, dx, dy, eta, **kw) as ctx:
+        n, dts, its, t = ctx.run(t_end, dt_override=dt, max_steps=max_steps)
+        return n, dts, its, t, ctx.download()
+
+
+def _resident(ic, nx, ny, dx, dy, eta, t_end, max_steps, dt=0.0, tune=None, **kw):
+    with make_gpu(ic, nx, ny, dx, dy, eta, **kw) as ctx:
+        if tune:
+            ctx.resident_tune(*tune)
+        n, dts, its, t = ctx.run_resident(t_end, dt_override=dt, max_steps=max_steps)
+        return n, dts, its, t, ctx.download()
+
+
+def _same(a, b):
+    na, dta, ita, ta, fa = a
+    nb, dtb, itb, tb, fb = b
+    assert na == nb, (na, nb)
+    assert np.array_equal(dta, dtb)
+    assert np.array_equal(ita, itb), (ita, itb)
+    assert ta == tb, (ta, tb)
+    for k in CONSERVED:
+        assert np.array_equal(fa[k], fb[k]), k
+
+
+@pytest.mark.parametrize("tune", [None, (16, 1, 0), (3, 1, 1), (1, 0, -1), (5, 0, 0)],
+                         ids=["adaptive", "fixed0-g16", "fixed1-g3", "margin0-g1", "fixed0-g5"])
+def test_resident_matches_run(tune):
+    n, eta = 96, 1e-2
+    _say(f"resident vs mhd_run, OT {n}^2 eta={eta}, 25 steps, tune={tune}")
+    ic, dx, dy = ic_arrays("orszag_tang", n, n)
+    ref = _plain(ic, n, n, dx, dy, eta, 10.0, 25, tol=1e-10, cfl=0.4)
+    got = _resident(ic, n, n, dx, dy, eta, 10.0, 25, tune=tune, tol=1e-10, cfl=0.4)
+    assert ref[2].min() > 0          # the solves iterate
+    _same(got, ref)
+
+
+def test_resident_t_end_clamp_and_eta0():
+    n = 64
+    ic, dx, dy = ic_arrays("orszag_tang", n, n)
+    for eta in (0.0, 3e-3):
+        _say(f"resident to t_end=0.05 (clamped last step), OT {n}^2 eta={eta}")
+        ref = _plain(ic, n, n, dx, dy, eta, 0.05, -1, cfl=0.4, tol=1e-12)
+        got = _resident(ic, n, n, dx, dy, eta, 0.05, -1, cfl=0.4, tol=1e-12)
+        assert ref[3] == 0.05 and ref[0] > 2
+        assert ref[1][-1] < ref[1][-2]   # the clamp shortened the last step
+        _same(got, ref)
+
+
+def test_resident_fixed_dt_non_square():
+    nx, ny, eta = 90, 40, 5e-3
+    _say(f"resident with dt_override, {nx}x{ny} eta={eta}")
+    ic, dx, dy = ic_arrays("orszag_tang", nx, ny, seed=3, amp=1e-3)
+    ref = _plain(ic, nx, ny, dx, dy, eta, 1.0, 12, dt=2e-3, cfl=0.4, tol=1e-11)
+    got = _resident(ic, nx, ny, dx, dy, eta, 1.0, 12, dt=2e-3, tune=(4, 1, -1), cfl=0.4, tol=1e-11)
+    assert np.all(ref[1] == 2e-3)
+    _same(got, ref)
+
+
+def test_resident_snapshot_cadence(tmp_path):
+    n, eta, every, steps = 64, 1e-3, 5, 12
+    _say(f"resident snapshots every {every}, {steps} steps")
+    ic, dx, dy = ic_arrays("orszag_tang", n, n)
+    frames, states = [], {}
+
+    def on_snap(ctx, frame, t, dt):
+        frames.append((frame, t, dt))
+        states[frame] = ctx.download()
+        ctx.write_snapshot(os.fsdecode(tmp_path / f"snapshot_{frame:06d}.dat"), every, dt)
+
+    with make_gpu(ic, n, n, dx, dy, eta, cfl=0.4, tol=1e-10) as ctx:
+        dt0 = ctx.CFL_condition_checker()
+        ns, dts, _, t = ctx.run_resident(10.0, max_steps=steps, snapshot_every=every, on_snapshot=on_snap)
+    assert ns == steps
+    assert [f for f, _, _ in frames] == [0, 5, 10]
+    assert frames[0][1] == 0.0 and frames[0][2] == dt0
+    for f, tt, dt in frames[1:]:
+        assert dt == dts[f - 1] and tt == pytest.approx(float(np.sum(dts[:f])), rel=1e-15)
+    for f in (0, 5, 10):
+        ref = _plain(ic, n, n, dx, dy, eta, 10.0, f, cfl=0.4, tol=1e-10)
+        for k in CONSERVED:
+            assert np.array_equal(states[f][k], ref[4][k]), (f, k)
+        head = np.fromfile(tmp_path / f"snapshot_{f:06d}.dat", dtype=np.float64, count=9)
+        assert list(head[:4]) == [n, n, 2, every]
+
+
+def test_resident_errors():
+    n = 32
+    ic, dx, dy = ic_arrays("orszag_tang", n, n)
+    _say("resident error paths")
+    bad = {k: v.copy() for k, v in ic.items()}
+    bad["E"][10, 10] = np.nan
+    with make_gpu(bad, n, n, dx, dy, 1e-3, cfl=0.4) as ctx:
+        with pytest.raises(RuntimeError, match="non-finite"):
+            ctx.run_resident(1.0, max_steps=3)
+    with make_gpu(ic, n, n, dx, dy, 1e-1, cfl=0.4, cg_max_iter=1, cg_fail_is_error=True) as ctx:
+        with pytest.raises(RuntimeError, match="iteration limit"):
+            ctx.run_resident(1.0, max_steps=3)
+
+    def boom(ctx, frame, t, dt):
+        if frame == 2:
+            raise KeyError("stop here")
+
+    with make_gpu(ic, n, n, dx, dy, 1e-3, cfl=0.4) as ctx:
+        with pytest.raises(KeyError):
+            ctx.run_resident(1.0, max_steps=6, snapshot_every=2, on_snapshot=boom)
+    with make_gpu(ic, n, n, dx, dy, 1e-3, cfl=0.4) as ctx:
+        with pytest.raises(RuntimeError):
+            ctx.resident_tune(0, 1, -1)
