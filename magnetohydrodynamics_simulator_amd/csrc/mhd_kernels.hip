@@ -733,8 +733,33 @@ __device__ __forceinline__ Fin2Chain fin2_begin(const FinArgs& a, bool xv, int c
   return ch;
 }
 
-__device__ __forceinline__ void finalize2_body(const FinArgs& a, int vb) {
+// CFL_condition_checker's wave speed of one cell (diagnostics.hpp:58-80) from
+// its conserved values and cell-centred B: max into m, non-finite count into bad.
+__device__ __forceinline__ void cfl_cell(double rho, double mx, double my, double E, double bxc, double byc,
+                                         double& m, double& bad) {
+  double rf, vx, vy, p;
+  primitives(rho, mx, my, E, bxc, byc, rf, vx, vy, p);
+  rf = dmax(rf, 1e-8);
+  p = dmax(p, 1e-8);
+  const double ss = kGamma * p / rf;
+  const double as = (bxc * bxc + byc * byc) / rf;
+  const double cur = sqrt(ss + as) + sqrt(vx * vx + vy * vy);
+  if (cur > m) m = cur;
+  if (!(fabs(cur) <= 1.7976931348623157e308)) bad += 1.0;
+}
+
+// CFL == true: the tile also produces the CFL partials of the new state (the
+// per-tile max speed and non-finite count of cfl_body, from the B it has just
+// formed and d.s[0..3] = rho, mx, my, E of the new state) into d.part.  The
+// march then runs one row further (the new by of row y1 closes the top row's
+// cell-centred by) and the CFL of row j - 1 is formed at row j; the new bx of
+// the column to the right comes through LDS, that of the halo column from the
+// edge thread's own chain.
+template <bool CFL>
+__device__ __forceinline__ void finalize2_body(const FinArgs& a, int vb, const DiagArgs& d) {
   __shared__ double jrow[2][kTileX + 2];
+  __shared__ double bxrow[CFL ? 2 : 1][CFL ? kTileX + 2 : 1];
+  __shared__ double lds[16];
   const bool xv = a.st->spmv != 0;
   const int nx = a.nx, ny = a.ny;
   const Tile t = tile_of(nx, ny, a.ty, a.tiles_y, vb);
@@ -746,31 +771,80 @@ __device__ __forceinline__ void finalize2_body(const FinArgs& a, int vb) {
   Fin2Chain own = fin2_begin(a, xv, t.ic, t.y0);
   Fin2Chain hal = fin2_begin(a, xv, xr, t.y0);
   double jc = fin2_j3(a, xv, t.ic, t.y0, own);
+  double jhc = 0.0;                              // J3(xr, j) of the halo chain (CFL only)
   if (t.valid) jrow[0][tid + 1] = jc;
-  if (edge) jrow[0][w + 1] = fin2_j3(a, xv, xr, t.y0, hal);
+  if (edge) {
+    const double jh = fin2_j3(a, xv, xr, t.y0, hal);
+    jrow[0][w + 1] = jh;
+    jhc = jh;
+  }
   __syncthreads();
-  for (int j = t.y0; j < t.y1; ++j) {
+  double m = 1e-8, bad = 0.0;
+  double bxo_prev = 0.0, byo_prev = 0.0;
+  double u_prev[4] = {0.0, 0.0, 0.0, 0.0};     // rho, mx, my, E of row j - 1 (loaded one row ahead)
+  const int y_end = CFL ? t.y1 + 1 : t.y1;
+  for (int j = t.y0; j < y_end; ++j) {
+    double u_row[4] = {0.0, 0.0, 0.0, 0.0};
+    if (CFL && j < t.y1) {
+      const int o = j * nx + t.ic;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) u_row[q] = d.s[q][o];
+    }
     const int buf = (j - t.y0) & 1;
-    const int jp = (j == ny - 1) ? 0 : j + 1;
+    const int jw = (j >= ny) ? j - ny : j;      // row j wrapped (the extra row of a top tile)
+    const int jp = (jw == ny - 1) ? 0 : jw + 1;
     const double bxs = own.bxs, bys = own.bys;   // B3* faces of row j
+    const double hbxs = hal.bxs;                 // B3* bx of row j, halo column (edge thread)
     const double jn = fin2_j3(a, xv, t.ic, jp, own);
     if (t.valid) jrow[buf ^ 1][tid + 1] = jn;
-    if (edge) jrow[buf ^ 1][w + 1] = fin2_j3(a, xv, xr, jp, hal);
+    double jhn = 0.0;
+    if (edge) {
+      jhn = fin2_j3(a, xv, xr, jp, hal);
+      jrow[buf ^ 1][w + 1] = jhn;
+    }
     const double jr = jrow[buf][tid + 2];        // J3(i + 1, j)
-    if (t.valid) {
+    const double g3x = -(jn - jc) / a.dy;
+    const double g3y = (jr - jc) / a.dx;
+    const double bxo = bxs + g3x * a.c;
+    const double byo = bys + g3y * a.c;
+    if (t.valid && j < t.y1) {
       const int o = j * nx + t.ic;
-      const double g3x = -(jn - jc) / a.dy;
-      const double g3y = (jr - jc) / a.dx;
-      a.bx_out[o] = bxs + g3x * a.c;
-      a.by_out[o] = bys + g3y * a.c;
+      a.bx_out[o] = bxo;
+      a.by_out[o] = byo;
+    }
+    if (CFL) {
+      if (t.valid) bxrow[buf][tid + 1] = bxo;
+      if (edge) bxrow[buf][w + 1] = hbxs + (-(jhn - jhc) / a.dy) * a.c;   // new bx(xr, j)
+      if (j > t.y0 && t.valid) {
+        // cell (i, j - 1): bx faces (i, j - 1), (i + 1, j - 1); by faces (i, j - 1), (i, j)
+        const double bxc = (bxo_prev + bxrow[buf ^ 1][tid + 2]) / 2.0;
+        const double byc = (byo_prev + byo) / 2.0;
+        cfl_cell(u_prev[0], u_prev[1], u_prev[2], u_prev[3], bxc, byc, m, bad);
+      }
+      bxo_prev = bxo;
+      byo_prev = byo;
+      jhc = jhn;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) u_prev[q] = u_row[q];
     }
     __syncthreads();
     jc = jn;
   }
+  if (CFL) {
+    double b1[1] = {bad};
+    block_max(m, lds);
+    block_sum<1>(b1, lds + 8);
+    if (threadIdx.x == 0) {
+      d.part[t.vb * 2 + 0] = m;
+      d.part[t.vb * 2 + 1] = b1[0];
+    }
+  }
 }
 
 __global__ __launch_bounds__(kTileX) void k_finalize1(FinArgs a) { finalize1_body(a, xcd_block(blockIdx.x, gridDim.x)); }
-__global__ __launch_bounds__(kTileX) void k_finalize2(FinArgs a) { finalize2_body(a, xcd_block(blockIdx.x, gridDim.x)); }
+__global__ __launch_bounds__(kTileX) void k_finalize2(FinArgs a) {
+  finalize2_body<false>(a, xcd_block(blockIdx.x, gridDim.x), DiagArgs{});
+}
 
 // rate arrays of resistive_B_term for the unit entry point.
 __global__ __launch_bounds__(256) void k_rate(FinArgs a, int zero) {
@@ -819,15 +893,7 @@ __device__ __forceinline__ void cfl_body(const DiagArgs& a, int vb) {
       const double by_hi_n = more ? a.s[5][at(t.ic, j + 2)] : 0.0;
       const double bxc = (bxl + bxr) / 2.0;
       const double byc = (by_lo + by_hi) / 2.0;
-      double rf, vx, vy, p;
-      primitives(rho, mx, my, E, bxc, byc, rf, vx, vy, p);
-      rf = dmax(rf, 1e-8);
-      p = dmax(p, 1e-8);
-      const double ss = kGamma * p / rf;
-      const double as = (bxc * bxc + byc * byc) / rf;
-      const double cur = sqrt(ss + as) + sqrt(vx * vx + vy * vy);
-      if (cur > m) m = cur;
-      if (!(fabs(cur) <= 1.7976931348623157e308)) bad[0] += 1.0;
+      cfl_cell(rho, mx, my, E, bxc, byc, m, bad[0]);
       by_lo = by_hi; by_hi = by_hi_n;
       rho = rho_n; mx = mx_n; my = my_n; E = E_n; bxl = bxl_n; bxr = bxr_n;
       o = on;
@@ -1051,7 +1117,7 @@ __global__ __launch_bounds__(kTileX) void kb_finalize2(BatchFinArgs b) {
   const int2 job = b.jobs[xcd_block(blockIdx.x, gridDim.x)];
   const FinArgs f = batch_fin_args(b, job.x);
   if (b.rs[job.x].active) {
-    finalize2_body(f, job.y);
+    finalize2_body<false>(f, job.y, DiagArgs{});
     return;
   }
   const Tile t = tile_of(f.nx, f.ny, f.ty, f.tiles_y, job.y);
@@ -1215,47 +1281,67 @@ __global__ __launch_bounds__(kTileX) void k_cg_pass_b_res(CgArgs a, const RunSta
 template __global__ void k_cg_pass_b_res<true>(CgArgs, const RunState*, const double*, int);
 template __global__ void k_cg_pass_b_res<false>(CgArgs, const RunState*, const double*, int);
 
-// Finaliser of solve SLOT (0: B2 = B2* + c rate(x); 1: the final
-// recombination).  Every block takes the same decision from the solve's
-// state: unfinished (or at the iteration limit with fail_is_error) halts the
-// loop before any field changes.
-template <int SLOT>
-__global__ __launch_bounds__(kTileX) void k_finalize_res(FinArgs a, RunState* __restrict__ rs, int fail_is_error) {
-  if (res_stopped(rs)) return;
-  const int done = a.st->done;
+// Log entry of the step just finished (block 0, thread 0 of the caller).
+__device__ __forceinline__ void write_step_log(const RunState* rs, const CgState* st, StepLog* log, int log_cap) {
+  StepLog e;
+  e.dt = rs->dt;
+  e.t = rs->t;
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    const CgState& s = st[k];
+    e.iters[k] = s.iters;
+    e.spmv[k] = s.spmv;
+    e.err[k] = (s.done == 1 || s.done == 2) ? sqrt(s.rn2 / s.rhs_norm2) : 0.0;
+  }
+  log[(rs->steps - 1) % log_cap] = e;
+}
+
+// Every block of a finaliser takes the same decision from its solve's state:
+// unfinished (or at the iteration limit with fail_is_error) halts the loop
+// before any field changes.
+__device__ __forceinline__ bool halt_unfinished(const CgState* st, RunState* rs, int slot, int fail_is_error) {
+  const int done = st->done;
   if (done == 0 || (done == 2 && fail_is_error)) {
     if (blockIdx.x == 0 && threadIdx.x == 0)
-      rs->halt = (done == 0) ? (SLOT == 0 ? kHaltSolve1 : kHaltSolve2) : (SLOT == 0 ? kHaltCgLimit1 : kHaltCgLimit2);
-    return;
+      rs->halt = (done == 0) ? (slot == 0 ? kHaltSolve1 : kHaltSolve2) : (slot == 0 ? kHaltCgLimit1 : kHaltCgLimit2);
+    return true;
   }
+  return false;
+}
+
+// Finaliser of solve 1: B2 = B2* + c rate(x).
+__global__ __launch_bounds__(kTileX) void k_finalize1_res(FinArgs a, RunState* __restrict__ rs, int fail_is_error) {
+  if (res_stopped(rs)) return;
+  if (halt_unfinished(a.st, rs, 0, fail_is_error)) return;
   FinArgs f = a;
   f.c = rs->coef[4];
-  const int vb = xcd_block(blockIdx.x, gridDim.x);
-  if (SLOT == 0) finalize1_body(f, vb);
-  else finalize2_body(f, vb);
+  finalize1_body(f, xcd_block(blockIdx.x, gridDim.x));
 }
-template __global__ void k_finalize_res<0>(FinArgs, RunState*, int);
-template __global__ void k_finalize_res<1>(FinArgs, RunState*, int);
 
-// End of a resident step: the step's log entry (block 0) and, with
-// `partials`, the CFL partials of the new state for the next kb_dt.
+// Finaliser of solve 2 (the final recombination), the end of a resistive
+// step: the step's log entry and, with CFL, the CFL partials of the new
+// state (d: its rho, mx, my, E and the partials buffer) for the next kb_dt.
+template <bool CFL>
+__global__ __launch_bounds__(kTileX) void k_finalize2_res(FinArgs a, DiagArgs d, RunState* __restrict__ rs,
+                                                          const CgState* __restrict__ st, StepLog* __restrict__ log,
+                                                          int log_cap, int fail_is_error) {
+  if (res_stopped(rs)) return;
+  if (halt_unfinished(a.st, rs, 1, fail_is_error)) return;
+  if (blockIdx.x == 0 && threadIdx.x == 0) write_step_log(rs, st, log, log_cap);
+  FinArgs f = a;
+  f.c = rs->coef[4];
+  finalize2_body<CFL>(f, xcd_block(blockIdx.x, gridDim.x), d);
+}
+template __global__ void k_finalize2_res<false>(FinArgs, DiagArgs, RunState*, const CgState*, StepLog*, int, int);
+template __global__ void k_finalize2_res<true>(FinArgs, DiagArgs, RunState*, const CgState*, StepLog*, int, int);
+
+// End of a step without a resistive solve: the step's log entry (block 0)
+// and, with `partials`, the CFL partials of the new state for the next kb_dt.
 __global__ __launch_bounds__(kTileX) void k_cfl_res(DiagArgs a, const RunState* __restrict__ rs,
                                                     const CgState* __restrict__ st, StepLog* __restrict__ log,
                                                     int log_cap, int partials) {
   if (res_stopped(rs)) return;
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    StepLog e;
-    e.dt = rs->dt;
-    e.t = rs->t;
-#pragma unroll
-    for (int k = 0; k < 2; ++k) {
-      const CgState& s = st[k];
-      e.iters[k] = s.iters;
-      e.spmv[k] = s.spmv;
-      e.err[k] = (s.done == 1 || s.done == 2) ? sqrt(s.rn2 / s.rhs_norm2) : 0.0;
-    }
-    log[(rs->steps - 1) % log_cap] = e;
-  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) write_step_log(rs, st, log, log_cap);
   if (partials) cfl_body(a, xcd_block(blockIdx.x, gridDim.x));
 }
 

@@ -188,7 +188,10 @@ template <int MODE> __global__ void k_stage_res(StageArgs a, const RunState* rs)
 __global__ void k_cg_start_res(CgArgs a, const RunState* rs);
 __global__ void k_cg_pass_a_res(CgArgs a, const RunState* rs, const double* pold, double* pnew, int it);
 template <bool FIRST> __global__ void k_cg_pass_b_res(CgArgs a, const RunState* rs, const double* pv, int it);
-template <int SLOT> __global__ void k_finalize_res(FinArgs a, RunState* rs, int fail_is_error);
+__global__ void k_finalize1_res(FinArgs a, RunState* rs, int fail_is_error);
+template <bool CFL>
+__global__ void k_finalize2_res(FinArgs a, DiagArgs d, RunState* rs, const CgState* st, StepLog* log, int log_cap,
+                                int fail_is_error);
 __global__ void k_cfl_res(DiagArgs a, const RunState* rs, const CgState* st, StepLog* log, int log_cap,
                           int partials);
 __global__ void k_clear_halt(RunState* rs);

@@ -6,8 +6,10 @@
 // to t_end, the operator rebuild when dt moves by more than 1e-10 relative,
 // IMEX), with the per-step decisions taken on the device:
 //   kb_dt (one run)  dt, clamp, rebuild rule, t, step count, activity
-//   k_stage_res<1>, CG 1, k_finalize_res<0>, k_stage_res<2>, CG 2,
-//   k_finalize_res<1>, k_cfl_res (step log + CFL partials of the new state)
+//   k_stage_res<1>, CG 1, k_finalize1_res, k_stage_res<2>, CG 2,
+//   k_finalize2_res (final recombination + step log + the CFL partials of
+//   the new state, fused: the separate CFL pass of mhd_run re-reads the
+//   whole state); eta = 0: k_cfl_res (step log + CFL partials)
 // Every kernel reads its coefficients from the device driver state, so the
 // host enqueues a group of steps and reads the driver state back once per
 // group instead of three times per step (the CFL value and the two CG
@@ -137,7 +139,8 @@ int enqueue_cg(mhd_ctx* c, const CgArgs& a, const RunState* rs, int spec) {
 }
 
 // Phases of a step: 0 kb_dt, 1 stage 1, 2 CG 1, 3 finaliser 1, 4 stage 2,
-// 5 CG 2, 6 finaliser 2 (+ the B swap), 7 step log and CFL partials.
+// 5 CG 2, 6 finaliser 2 (+ the B swap; it also writes the step log and the
+// CFL partials); without a resistive solve, 7 step log and CFL partials.
 int enqueue_step(mhd_ctx* c, Loop& L, int from) {
   Resident* R = L.R;
   const Tableau& T = c->tab;
@@ -166,7 +169,7 @@ int enqueue_step(mhd_ctx* c, Loop& L, int from) {
   if (res && from <= 3) {
     FinArgs f = fin_args(c, c->S[4], c->S[5], c->S[4], c->S[5], 0.0);
     f.st = R->st;
-    rc = launch_k(c, 5, k_finalize_res<0>, dim3(c->tiles), dim3(kTileX), f, R->rs, c->o.cg_fail_is_error);
+    rc = launch_k(c, 5, k_finalize1_res, dim3(c->tiles), dim3(kTileX), f, R->rs, c->o.cg_fail_is_error);
     if (rc) return rc;
   }
   if (from <= 4) {
@@ -185,12 +188,20 @@ int enqueue_step(mhd_ctx* c, Loop& L, int from) {
     if (rc) return rc;
   }
   if (res && from <= 6) {
+    // the final recombination ends the step: step log and, with a CFL step,
+    // the CFL partials of the new state (its rho, mx, my, E are final after
+    // stage 2; the kernel forms the new B itself)
     FinArgs f = fin_args(c, c->U[4], c->U[5], c->S[4], c->S[5], 0.0);
     f.st = R->st + 1;
-    rc = launch_k(c, 6, k_finalize_res<1>, dim3(c->tiles), dim3(kTileX), f, R->rs, c->o.cg_fail_is_error);
+    DiagArgs d = diag_args(c);
+    rc = L.fixed_dt ? launch_k(c, 6, k_finalize2_res<false>, dim3(c->tiles), dim3(kTileX), f, d, R->rs,
+                               (const CgState*)R->st, R->log, kLogCap, c->o.cg_fail_is_error)
+                    : launch_k(c, 6, k_finalize2_res<true>, dim3(c->tiles), dim3(kTileX), f, d, R->rs,
+                               (const CgState*)R->st, R->log, kLogCap, c->o.cg_fail_is_error);
     if (rc) return rc;
     std::swap(c->U[4], c->S[4]);
     std::swap(c->U[5], c->S[5]);
+    return MHD_OK;
   }
   DiagArgs d = diag_args(c);
   return launch_k(c, 7, k_cfl_res, dim3(L.fixed_dt ? 1 : c->tiles), dim3(kTileX), d, (const RunState*)R->rs,

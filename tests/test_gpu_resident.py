@@ -87,6 +87,19 @@ def test_resident_fixed_dt_non_square():
     _same(got, ref)
 
 
+@pytest.mark.parametrize("shape", [(300, 48), (256, 70), (40, 300)], ids=["300x48", "256x70", "40x300"])
+def test_resident_tile_edges(shape):
+    """The final recombination forms the next step's CFL partials itself: grids
+    with a partial last column of tiles, an exact multiple of the tile width,
+    and tall grids (many tile rows, the wrap of the top tile)."""
+    nx, ny = shape
+    _say(f"resident tile edges {nx}x{ny}, eta=2e-3, 8 CFL steps")
+    ic, dx, dy = ic_arrays("orszag_tang", nx, ny, seed=5, amp=1e-3)
+    ref = _plain(ic, nx, ny, dx, dy, 2e-3, 10.0, 8, cfl=0.4, tol=1e-10)
+    got = _resident(ic, nx, ny, dx, dy, 2e-3, 10.0, 8, cfl=0.4, tol=1e-10)
+    _same(got, ref)
+
+
 def test_resident_snapshot_cadence(tmp_path):
     n, eta, every, steps = 64, 1e-3, 5, 12
     _say(f"resident snapshots every {every}, {steps} steps")
