@@ -55,14 +55,30 @@ KERNELS = ("stage1", "stage2", "cg_start", "cg_pass_a", "cg_pass_b", "implicit_f
 KERNEL_SYMBOLS = {"stage1": "k_stage<1>", "stage2": "k_stage<2>", "cg_start": "k_cg_start",
                   "cg_pass_a": "k_cg_pass_a", "cg_pass_b": "k_cg_pass_b", "implicit_finalize1": "k_finalize1",
                   "implicit_finalize2": "k_finalize2", "cfl": "k_cfl"}
+# the same kernels of the device-resident loop (mhd_run_resident)
+KERNEL_SYMBOLS_RESIDENT = {"stage1": "k_stage_res<1>", "stage2": "k_stage_res<2>", "cg_start": "k_cg_start_res",
+                           "cg_pass_a": "k_cg_pass_a_res", "cg_pass_b": "k_cg_pass_b_res",
+                           "implicit_finalize1": "k_finalize1_res", "implicit_finalize2": "k_finalize2_res",
+                           "cfl": "k_cfl"}
 CALIB_BYTES = 1 << 30          # the copy kernel of the counter calibration reads and writes 1 GiB
 
 
-def kernel_bytes(k: str, N: int, launches: int, solves: int, spmv: int) -> float:
-    """Algorithmic bytes moved by all launches of kernel k in a run with
-    `solves` CG solves and `spmv` operator products in total."""
-    if k in STAGE_BYTES:
+def kernel_bytes(k: str, N: int, launches: int, steps: int, solves: int, spmv: int, fused_cfl: bool = False) -> float:
+    """Algorithmic bytes moved by kernel k in a run of `steps` steps with
+    `solves` CG solves and `spmv` operator products in total.  Work is
+    counted, not launches: the resident loop's launches that find their run
+    stopped, halted or their solve finished do no work.  fused_cfl: the final
+    recombination also reads rho, mx, my, E for the next step's CFL partials
+    (the resident loop); `launches` counts the separate CFL kernel's launches,
+    all of which work."""
+    if k == "implicit_finalize2" and fused_cfl:
+        return (STAGE_BYTES[k] + 32) * N * steps
+    if k == "cfl":
         return STAGE_BYTES[k] * N * launches
+    if k == "cg_start":
+        return STAGE_BYTES[k] * N * solves
+    if k in STAGE_BYTES:
+        return STAGE_BYTES[k] * N * steps
     if k == "cg_pass_a":             # one direction update per product after the first of a solve
         return 24.0 * N * (spmv - solves)
     return N * (32.0 * solves + 40.0 * (spmv - solves))   # cg_pass_b
@@ -101,6 +117,11 @@ def parse():
                     help="runs of the batched differentiation workload timed through single-run contexts")
     ap.add_argument("--only", choices=("batch", "adjoint", "badjoint"), default=None,
                     help="run only the batched, the differentiation or the batched differentiation leg")
+    ap.add_argument("--driver", choices=("resident", "host"), default="resident",
+                    help="time loop of the main workload: mhd_run_resident (per-step decisions on the device) "
+                         "or mhd_run (host reads the CFL value and the CG state each step)")
+    ap.add_argument("--compare-steps", type=int, default=20,
+                    help="steps of the un-instrumented host-loop / resident-loop comparison (0: skip)")
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     return ap.parse_args()
 
@@ -234,7 +255,10 @@ def pmc_child(a) -> None:
     ic = setups.orszag_tang(n)
     ctx = Context(n, n, dx, dx, a.eta, cfl=a.cfl, cg_tol=a.cg_tol)
     ctx.upload(*[ic[k] for k in ("rho", "mx", "my", "E", "bxf", "byf")])
-    ctx.run(1e9, max_steps=2)
+    if a.driver == "resident":
+        ctx.run_resident(1e9, max_steps=2)
+    else:
+        ctx.run(1e9, max_steps=2)
     ctx.calibration_copy(CALIB_BYTES)
     ctx.synchronize()
     ctx.close()
@@ -261,13 +285,14 @@ def hbm_traffic(a, kernel: str) -> dict | None:
     exe = shutil.which("rocprofv3")
     if exe is None:
         return None
-    sym = KERNEL_SYMBOLS[kernel]
+    sym = (KERNEL_SYMBOLS_RESIDENT if a.driver == "resident" else KERNEL_SYMBOLS)[kernel]
     res = {}
     for counter in ("FETCH_SIZE", "WRITE_SIZE"):
         out_dir = tempfile.mkdtemp(prefix="mhd_pmc_")
         cmd = [exe, "--pmc", counter, "--kernel-include-regex", f"{sym}|k_copy", "-d", out_dir, "-o", "pmc",
                "--output-format", "csv", "--", sys.executable, os.path.abspath(__file__), "--pmc-child",
-               "--n", str(a.n), "--eta", str(a.eta), "--cg-tol", str(a.cg_tol), "--cfl", str(a.cfl)]
+               "--n", str(a.n), "--eta", str(a.eta), "--cg-tol", str(a.cg_tol), "--cfl", str(a.cfl),
+               "--driver", a.driver]
         try:
             r = subprocess.run(cmd, capture_output=True, text=True, timeout=120, cwd="/tmp")
         except subprocess.TimeoutExpired:
@@ -287,6 +312,9 @@ def hbm_traffic(a, kernel: str) -> dict | None:
         shutil.rmtree(out_dir, ignore_errors=True)
         if not vals or not copy:
             return None
+        # launches that found their solve finished read nothing: average the working ones
+        top = max(vals)
+        vals = [v for v in vals if v > 0.05 * top]
         scale = CALIB_BYTES / (sum(copy) / len(copy) * 1024.0)     # true bytes per counted byte
         res[counter] = sum(vals) / len(vals) * 1024.0 * scale
         res[counter + "_scale"] = scale
@@ -552,27 +580,51 @@ def main():
     ctx = Context(n, n, dx, dx, a.eta, cfl=a.cfl, cg_tol=a.cg_tol)
     ctx.upload(*[ic[k] for k in ("rho", "mx", "my", "E", "bxf", "byf")])
     del ic
+    resident = a.driver == "resident"
+    fused = resident and a.eta > 0
+    drive = ctx.run_resident if resident else ctx.run
     # warm-up: W driver steps
     if a.warmup > 0:
-        ctx.run(1e9, max_steps=a.warmup)
+        drive(1e9, max_steps=a.warmup)
     ctx.synchronize()
     N = n * n
     # untimed pass: every kernel bracketed by events -> per-kernel breakdown
     ctx.profile(True)
-    psteps, _, pits, _ = ctx.run(1e9, max_steps=max(3, min(a.steps, 10)))
+    psteps, _, pits, _ = drive(1e9, max_steps=max(3, min(a.steps, 10)))
     prof = {k: ctx.profile_get(k) for k in KERNELS}
-    psolves, pspmv = 2 * psteps, int(pits.sum()) + 2 * psteps
+    psolves = 2 * psteps if a.eta > 0 else 0
+    pspmv = int(pits.sum()) + psolves
     kb = {}
     for k, (cnt, ms) in prof.items():
-        byt = kernel_bytes(k, N, cnt, psolves, pspmv)
+        if cnt == 0:
+            continue
+        byt = kernel_bytes(k, N, cnt, psteps, psolves, pspmv, fused)
         kb[k] = {"launches_per_step": round(cnt / psteps, 2), "ms_per_step": round(ms / psteps, 4),
                  "GBs": round(byt / (ms * 1e-3) / 1e9, 1) if ms > 0 else None}
     dom = max(prof, key=lambda k: prof[k][1])
+    # un-instrumented comparison of the two time loops over the same steps from the same state
+    compare = None
+    if a.compare_steps > 0:
+        ctx.profile(False)
+        st0 = ctx.download()
+        cmp_ms = {}
+        for name, fn in (("host_loop", ctx.run), ("resident_loop", ctx.run_resident)):
+            ctx.upload(*[st0[k] for k in ("rho", "mx", "my", "E", "bxf", "byf")])
+            ctx.synchronize()
+            tc = time.perf_counter()
+            nc, _, _, _ = fn(1e9, max_steps=a.compare_steps)
+            ctx.synchronize()
+            cmp_ms[name] = (time.perf_counter() - tc) / nc * 1e3
+        ctx.upload(*[st0[k] for k in ("rho", "mx", "my", "E", "bxf", "byf")])
+        del st0
+        compare = {"steps": a.compare_steps, "host_loop_ms_per_step": round(cmp_ms["host_loop"], 4),
+                   "resident_loop_ms_per_step": round(cmp_ms["resident_loop"], 4),
+                   "note": "no events; the same steps from the same state (mhd_run vs mhd_run_resident)"}
     # timed region: only the dominant kernel's launches carry events
     ctx.profile([dom])
     ctx.synchronize()
     t0 = time.perf_counter()
-    steps, dts, its, _ = ctx.run(1e9, max_steps=a.steps)
+    steps, dts, its, _ = drive(1e9, max_steps=a.steps)
     ctx.synchronize()
     t1 = time.perf_counter()
     elapsed = t1 - t0
@@ -604,9 +656,9 @@ def main():
                 "note": "upload 6 + step + download 11 padded host arrays per call (pageable host memory)"}
     ctx.close()
 
-    solves = 2 * steps
+    solves = 2 * steps if a.eta > 0 else 0
     spmv = int(its.sum()) + solves          # converged solves: products = iterations + 1
-    bytes_per_launch = kernel_bytes(dom, N, dom_cnt, solves, spmv) / dom_cnt
+    bytes_per_launch = kernel_bytes(dom, N, dom_cnt, steps, solves, spmv, fused) / dom_cnt
     achieved = bytes_per_launch / (dom_ms * 1e-3 / dom_cnt) / 1e9
     step_bytes = N * (336 + (144 if a.eta > 0 else 0) + 64 * (spmv / steps))
 
@@ -625,16 +677,22 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"resistive Orszag-Tang {n}x{n}, eta={a.eta}, Jacobi-PCG tol {a.cg_tol}, CFL {a.cfl}",
                    "nx": n, "ny": n, "eta": a.eta, "cg_tol": a.cg_tol, "cfl": a.cfl,
-                   "parallelism": "single-gpu"},
+                   "driver": "mhd_run_resident" if resident else "mhd_run", "parallelism": "single-gpu"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "bytes_per_launch": int(bytes_per_launch), "ms_per_launch": round(dom_ms / dom_cnt, 4)},
+                     "bytes_per_launch": int(bytes_per_launch), "ms_per_launch": round(dom_ms / dom_cnt, 4),
+                     "launches": dom_cnt,
+                     "working_launches": spmv if dom == "cg_pass_b" else None},
         "step_roofline": {"algorithmic_bytes_per_step": int(step_bytes),
                           "frac": round(step_bytes / (elapsed / steps) / 1e9 / HBM_PEAK_GBS, 4)},
         "cg_spmv_per_step": round(spmv / steps, 2),
         "copy_kernel_GBs": round(copy_gbs, 1),
         "divb_max": divmax,
         "kernels": kb,
+        "kernels_note": "launches include the resident loop's launches that do no work (steps enqueued after a "
+                        "solve halted the loop, passes after convergence); GBs = algorithmic bytes of the work "
+                        "done / the kernel's total time",
+        "driver_comparison": compare,
         "pcie_inclusive": pcie,
     }
     if not a.no_traffic:

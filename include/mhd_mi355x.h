@@ -157,7 +157,7 @@ MHD_API int mhd_run_resident(mhd_ctx* ctx, double t_end, double dt_override, int
                              double* step_dt, int* cg_iters);
 /* Tuning of mhd_run_resident: steps enqueued between two read-backs (1..256,
  * default 16), extra CG iterations launched per solve beyond the most of the
- * previous group (default 1), and fixed_budget >= 0: that many launched
+ * previous group (default 0), and fixed_budget >= 0: that many launched
  * iterations per solve, not adapted (a solve that needs more halts the loop
  * and the host finishes its step; -1 restores the adaptive budget). */
 MHD_API int mhd_resident_tune(mhd_ctx* ctx, int group_steps, int spec_margin, int fixed_budget);

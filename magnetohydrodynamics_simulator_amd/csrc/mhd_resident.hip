@@ -16,8 +16,10 @@
 // convergence checks of mhd_run).
 //
 // CG: each solve gets a budget of launched iterations -- the most products
-// of the slot's solves in the previous group, plus a margin; the passes of a
-// solve that has converged return at once.  A finaliser that finds its solve
+// of the slot's solves in the previous group, plus a margin (default 0: a
+// smooth run's counts change rarely, and each launched pass after convergence
+// still costs a dispatch); the passes of a solve that has converged return at
+// once.  A finaliser that finds its solve
 // unfinished halts the loop (RunState.halt): every later kernel returns at
 // once, the host finds the halt at the group's read-back, continues that
 // solve with host-checked iterations, finishes the step and re-enters the
@@ -54,7 +56,7 @@ struct Resident {
   CgState* st_host = nullptr;      // pinned, 2
   StepLog* log_host = nullptr;     // pinned, kLogCap
   int group = 16;              // steps enqueued between two read-backs
-  int margin = 1;              // extra launched iterations per solve
+  int margin = 0;              // extra launched iterations per solve
   int fixed = -1;              // >= 0: fixed launched iterations per solve
 };
 
