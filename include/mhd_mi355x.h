@@ -157,10 +157,12 @@ MHD_API int mhd_run_resident(mhd_ctx* ctx, double t_end, double dt_override, int
                              double* step_dt, int* cg_iters);
 /* Tuning of mhd_run_resident: steps enqueued between two read-backs (1..256,
  * default 16), extra CG iterations launched per solve beyond the most of the
- * previous group (default 0), and fixed_budget >= 0: that many launched
+ * previous group (default 0), fixed_budget >= 0: that many launched
  * iterations per solve, not adapted (a solve that needs more halts the loop
- * and the host finishes its step; -1 restores the adaptive budget). */
-MHD_API int mhd_resident_tune(mhd_ctx* ctx, int group_steps, int spec_margin, int fixed_budget);
+ * and the host finishes its step; -1 restores the adaptive budget), and
+ * graph = 1: pairs of steps are captured once per configuration (B pointers,
+ * budgets, dt mode) as a HIP graph and replayed (0: direct launches). */
+MHD_API int mhd_resident_tune(mhd_ctx* ctx, int group_steps, int spec_margin, int fixed_budget, int graph);
 
 /* write_data_to_binary_file (include/diagnostics.hpp:90-131): the current
  * state as a snapshot file in the reference's format -- 9 doubles (nx, ny,

@@ -50,7 +50,7 @@ SIGNATURES = {
     "mhd_divb": (C.c_int, [_vp, _dp, _dp]),
     "mhd_run": (C.c_int, [_vp, C.c_double, C.c_double, C.c_int, _ip, _dp, _dp, _ip]),
     "mhd_run_resident": (C.c_int, [_vp, C.c_double, C.c_double, C.c_int, C.c_int, _vp, _vp, _ip, _dp, _dp, _ip]),
-    "mhd_resident_tune": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
+    "mhd_resident_tune": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int]),
     "mhd_profile_enable": (C.c_int, [_vp, C.c_int]),
     "mhd_profile_get": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_longlong), _dp]),
     "mhd_calibration_copy": (C.c_int, [_vp, C.c_longlong]),

@@ -46,6 +46,18 @@ using namespace mhd;
 namespace mhd_host {
 
 constexpr int kLogCap = 256;   // step-log ring entries (>= the largest group)
+constexpr int kGraphCache = 4; // captured two-step graphs kept per context
+
+// Two steps captured as a graph: valid for the B pointers, CG budgets and
+// dt mode it was captured with (every other argument is fixed for the
+// context's life).
+struct StepGraph {
+  hipGraphExec_t exec = nullptr;
+  double* key[4] = {};         // U[4], U[5], S[4], S[5] at capture
+  int spec[2] = {0, 0};
+  bool fixed_dt = false;
+  unsigned long long last_use = 0;
+};
 
 struct Resident {
   RunGeom* geom = nullptr;     // device, one run
@@ -58,6 +70,9 @@ struct Resident {
   int group = 16;              // steps enqueued between two read-backs
   int margin = 0;              // extra launched iterations per solve
   int fixed = -1;              // >= 0: fixed launched iterations per solve
+  int graph = 0;               // 1: replay pairs of steps as captured graphs
+  StepGraph graphs[kGraphCache];
+  unsigned long long clock = 0;
 };
 
 void destroy_resident(mhd_ctx* c) {
@@ -70,6 +85,8 @@ void destroy_resident(mhd_ctx* c) {
   if (R->rs_host) (void)hipHostFree(R->rs_host);
   if (R->st_host) (void)hipHostFree(R->st_host);
   if (R->log_host) (void)hipHostFree(R->log_host);
+  for (StepGraph& g : R->graphs)
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
   delete R;
   c->res = nullptr;
 }
@@ -210,6 +227,48 @@ int enqueue_step(mhd_ctx* c, Loop& L, int from) {
                   (const CgState*)R->st, R->log, kLogCap, L.fixed_dt ? 0 : 1);
 }
 
+// The two-step graph for the context's current B pointers and budgets,
+// captured on first use (least recently used entry replaced).
+int step_graph(mhd_ctx* c, Loop& L, StepGraph** out) {
+  Resident* R = L.R;
+  double* key[4] = {c->U[4], c->U[5], c->S[4], c->S[5]};
+  StepGraph* victim = &R->graphs[0];
+  for (StepGraph& g : R->graphs) {
+    if (g.exec && std::equal(key, key + 4, g.key) && g.spec[0] == L.spec[0] && g.spec[1] == L.spec[1] &&
+        g.fixed_dt == L.fixed_dt) {
+      g.last_use = ++R->clock;
+      *out = &g;
+      return MHD_OK;
+    }
+    if (!g.exec || (victim->exec && g.last_use < victim->last_use)) victim = &g;
+  }
+  if (victim->exec) (void)hipGraphExecDestroy(victim->exec);
+  victim->exec = nullptr;
+  HIP_OR_FAIL(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  int rc = enqueue_step(c, L, 0);
+  if (!rc) rc = enqueue_step(c, L, 0);     // two swaps: the B pointers are back where they started
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(c->stream, &g);
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) return fail(c, MHD_EHIP, std::string("resident graph capture: ") + hipGetErrorString(e));
+  const hipError_t ei = hipGraphInstantiate(&victim->exec, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (ei != hipSuccess) {
+    victim->exec = nullptr;
+    return fail(c, MHD_EHIP, std::string("resident graph instantiate: ") + hipGetErrorString(ei));
+  }
+  std::copy(key, key + 4, victim->key);
+  victim->spec[0] = L.spec[0];
+  victim->spec[1] = L.spec[1];
+  victim->fixed_dt = L.fixed_dt;
+  victim->last_use = ++R->clock;
+  *out = victim;
+  return MHD_OK;
+}
+
 int read_state(mhd_ctx* c) {
   Resident* R = c->res;
   HIP_OR_FAIL(c, hipMemcpyAsync(R->rs_host, R->rs, sizeof(RunState), hipMemcpyDeviceToHost, c->stream));
@@ -263,15 +322,16 @@ int resume_step(mhd_ctx* c, Loop& L, int slot) {
 
 extern "C" {
 
-int mhd_resident_tune(mhd_ctx* c, int group_steps, int spec_margin, int fixed_budget) {
+int mhd_resident_tune(mhd_ctx* c, int group_steps, int spec_margin, int fixed_budget, int graph) {
   if (!c) return MHD_EINVAL;
-  if (group_steps < 1 || group_steps > kLogCap || spec_margin < 0)
-    return fail(c, MHD_EINVAL, "mhd_resident_tune: group_steps in [1, 256], spec_margin >= 0");
+  if (group_steps < 1 || group_steps > kLogCap || spec_margin < 0 || (graph != 0 && graph != 1))
+    return fail(c, MHD_EINVAL, "mhd_resident_tune: group_steps in [1, 256], spec_margin >= 0, graph 0 or 1");
   int rc = ensure_resident(c);
   if (rc) return rc;
   c->res->group = group_steps;
   c->res->margin = spec_margin;
   c->res->fixed = fixed_budget < 0 ? -1 : fixed_budget;
+  c->res->graph = graph;
   return MHD_OK;
 }
 
@@ -335,7 +395,15 @@ int mhd_run_resident(mhd_ctx* c, double t_end, double dt_override, int max_steps
     long long G = R->group;
     if (max_steps >= 0) G = std::min<long long>(G, max_steps - done);
     if (snapshot_every > 0) G = std::min<long long>(G, snapshot_every - done % snapshot_every);
-    for (long long k = 0; k < G; ++k)
+    long long k = 0;
+    if (R->graph && c->prof_mask == 0) {   // pairs of steps as graphs (no timing events inside a capture)
+      for (; k + 2 <= G; k += 2) {
+        StepGraph* sg = nullptr;
+        if ((rc = step_graph(c, L, &sg))) return rc;
+        HIP_OR_FAIL(c, hipGraphLaunch(sg->exec, c->stream));
+      }
+    }
+    for (; k < G; ++k)
       if ((rc = enqueue_step(c, L, 0))) return rc;
     if ((rc = read_state(c))) return rc;
     // a halted step: finish it on the host, then let the group's remaining steps run again

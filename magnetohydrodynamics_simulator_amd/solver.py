@@ -223,10 +223,11 @@ class Context:
         n = steps.value
         return n, dts[:n].copy(), its[: 2 * n].reshape(n, 2).copy(), t.value
 
-    def resident_tune(self, group_steps: int = 16, spec_margin: int = 0, fixed_budget: int = -1):
-        """Group size, CG iteration margin and an optional fixed iteration
-        budget of run_resident (mhd_resident_tune)."""
-        self._check(self.lib.mhd_resident_tune(self.h, group_steps, spec_margin, fixed_budget))
+    def resident_tune(self, group_steps: int = 16, spec_margin: int = 0, fixed_budget: int = -1,
+                      graph: bool = False):
+        """Group size, CG iteration margin, an optional fixed iteration budget
+        and graph replay of run_resident (mhd_resident_tune)."""
+        self._check(self.lib.mhd_resident_tune(self.h, group_steps, spec_margin, fixed_budget, int(graph)))
 
     # ---- adjoint ----
     def _field_ptrs(self, arrs):

@@ -8,7 +8,8 @@ the final time.  mhd_run itself is held to the oracle by test_gpu_parity.py.
 
 Cases: the adaptive CG budget; fixed budgets of 0 and 1 launched iterations,
 which halt the loop in (nearly) every solve and send those steps through the
-host's resume path, with group sizes 1, 3 and 16; the t_end clamp; a constant
+host's resume path, with group sizes 1, 3 and 16; pairs of steps replayed as
+captured graphs (every replay of a group checked through the final state); the t_end clamp; a constant
 dt; eta = 0 (no implicit solve); a non-square grid; the snapshot cadence
 (frame numbers, the state at each frame, files in the reference's format);
 and the error paths (non-finite wave speed, the CG iteration limit with
@@ -53,8 +54,10 @@ def _same(a, b):
         assert np.array_equal(fa[k], fb[k]), k
 
 
-@pytest.mark.parametrize("tune", [None, (16, 1, 0), (3, 1, 1), (1, 0, -1), (5, 0, 0)],
-                         ids=["adaptive", "fixed0-g16", "fixed1-g3", "margin0-g1", "fixed0-g5"])
+@pytest.mark.parametrize("tune", [None, (16, 1, 0), (3, 1, 1), (1, 0, -1), (5, 0, 0), (16, 0, -1, True),
+                                  (5, 0, 0, True), (4, 1, 1, True)],
+                         ids=["adaptive", "fixed0-g16", "fixed1-g3", "margin0-g1", "fixed0-g5", "graph-g16",
+                              "graph-fixed0-g5", "graph-fixed1-g4"])
 def test_resident_matches_run(tune):
     n, eta = 96, 1e-2
     _say(f"resident vs mhd_run, OT {n}^2 eta={eta}, 25 steps, tune={tune}")
@@ -83,6 +86,8 @@ def test_resident_fixed_dt_non_square():
     ic, dx, dy = ic_arrays("orszag_tang", nx, ny, seed=3, amp=1e-3)
     ref = _plain(ic, nx, ny, dx, dy, eta, 1.0, 12, dt=2e-3, cfl=0.4, tol=1e-11)
     got = _resident(ic, nx, ny, dx, dy, eta, 1.0, 12, dt=2e-3, tune=(4, 1, -1), cfl=0.4, tol=1e-11)
+    _same(got, ref)
+    got = _resident(ic, nx, ny, dx, dy, eta, 1.0, 12, dt=2e-3, tune=(4, 0, -1, True), cfl=0.4, tol=1e-11)
     assert np.all(ref[1] == 2e-3)
     _same(got, ref)
 
