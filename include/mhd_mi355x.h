@@ -171,6 +171,10 @@ MHD_API int mhd_resident_tune(mhd_ctx* ctx, int group_steps, int spec_margin, in
  * filled (read by scripts/display_functions.py:3-22).  The reference names
  * the file output/snapshot_%06d.dat (the C++ adapter does the same). */
 MHD_API int mhd_write_snapshot(mhd_ctx* ctx, const char* path, int snapshot_rate, double time_step);
+/* The device generators of mhd_batch_init_fourier / mhd_batch_init_column
+ * (declared with the batch entry points) into a context's state. */
+MHD_API int mhd_init_fourier(mhd_ctx* ctx, int nmodes, const double* modes);
+MHD_API int mhd_init_column(mhd_ctx* ctx, int sausage, int nmodes, const double* amps, const double* phases);
 /* Exact-restart files (no reference counterpart: the reference cannot
  * resume): "MHDRST01", 7 doubles (nx, ny, ng, dx, dy, eta, t) and the six
  * padded conserved arrays.  Reading checks the grid and uploads the state;
@@ -255,6 +259,25 @@ MHD_API int mhd_batch_upload(mhd_batch* b, int run, const double* rho, const dou
                      const double* E, const double* bxf, const double* byf);
 MHD_API int mhd_batch_download(mhd_batch* b, int run, double* rho, double* mx, double* my, double* E,
                        double* bxf, double* byf);
+/* Initial states of the ensemble workloads generated on the device, straight
+ * into a run (no host arrays, no upload; the run restarts as after an upload).
+ * No reference counterpart: the reference's setups are host loops
+ * (include/benchmarks.hpp:15-417); these evaluate the seeded setups of the
+ * ensemble workloads (the Python package's setups.fourier_ensemble_member and
+ * setups.kink_column / sausage_column, which keep the seeded draws) per cell.
+ * Square runs only (nx == ny, dx == dy).
+ *   fourier: rho = p = 1, B = (1, 0.5) + curl of a corner potential, v and
+ *     the potential from nmodes modes of 8 doubles each: kx, ky (nonzero
+ *     wave vector), phases of vx, vy, Az, amplitudes of vx, vy, Az; on a
+ *     box of side nx * dx.
+ *   column: the kink (sausage = 0) or sausage (sausage = 1) column on the
+ *     unit box (dx = 1 / nx) with displacement sum_k amps[k]
+ *     sin(2 pi (k + 1) y + phases[k]).
+ * The fields agree with the host generators to a few ulps (the same
+ * operations in the same order; the device's sin, cos, exp). */
+MHD_API int mhd_batch_init_fourier(mhd_batch* b, int run, int nmodes, const double* modes);
+MHD_API int mhd_batch_init_column(mhd_batch* b, int run, int sausage, int nmodes, const double* amps,
+                                  const double* phases);
 /* One driver step of every run that has not reached its end time or step
  * limit; *n_active (may be NULL) receives the number of runs that stepped. */
 MHD_API int mhd_batch_step(mhd_batch* b, int* n_active);

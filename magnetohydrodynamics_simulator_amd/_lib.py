@@ -51,6 +51,10 @@ SIGNATURES = {
     "mhd_run": (C.c_int, [_vp, C.c_double, C.c_double, C.c_int, _ip, _dp, _dp, _ip]),
     "mhd_run_resident": (C.c_int, [_vp, C.c_double, C.c_double, C.c_int, C.c_int, _vp, _vp, _ip, _dp, _dp, _ip]),
     "mhd_resident_tune": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "mhd_init_fourier": (C.c_int, [_vp, C.c_int, _dp]),
+    "mhd_init_column": (C.c_int, [_vp, C.c_int, C.c_int, _dp, _dp]),
+    "mhd_batch_init_fourier": (C.c_int, [_vp, C.c_int, C.c_int, _dp]),
+    "mhd_batch_init_column": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _dp, _dp]),
     "mhd_profile_enable": (C.c_int, [_vp, C.c_int]),
     "mhd_profile_get": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_longlong), _dp]),
     "mhd_calibration_copy": (C.c_int, [_vp, C.c_longlong]),

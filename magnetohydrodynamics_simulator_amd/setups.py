@@ -154,31 +154,50 @@ def kink(nx: int, ny: int | None = None, dx: float | None = None, dy: float | No
     return _from_primitive(env + 0.1, 0.05 * np.sin(2.0 * PI_REF * Y) * env, z, env + 0.1, z, env)
 
 
-def fourier_ensemble_member(n: int, rng: np.random.Generator, modes: int = 8, amp: float = 1e-2, kmax: int = 4):
-    """Ensemble member on [0, 2 pi)^2: rho = 1, p = 1, B = (1, 0.5) plus seeded
-    Fourier perturbations of the velocity and of B (B through a corner vector
-    potential, so div B = 0 to round-off)."""
-    dx = grid_spacing(n)
-    X, Y = _centres(n, n, dx, dx)
-    XC, YC = _corners(n, n, dx, dx)
-    L = n * dx
-    vx = np.zeros_like(X); vy = np.zeros_like(X)
-    Az = np.zeros_like(XC)
-    for _ in range(modes):
+def fourier_member_modes(rng: np.random.Generator, modes: int = 8, amp: float = 1e-2, kmax: int = 4) -> np.ndarray:
+    """The seeded draws of a Fourier ensemble member: an (M, 8) table of
+    kx, ky (0 < |k| <= kmax), the phases of vx, vy, Az and the amplitudes
+    of vx, vy, Az (uniform in [-amp, amp]).  The table defines the member:
+    fourier_member_state evaluates it on the host, the library's
+    mhd_batch_init_fourier / mhd_init_fourier on the device."""
+    table = np.zeros((modes, 8))
+    for m in range(modes):
         while True:
             kx, ky = rng.integers(-kmax, kmax + 1, size=2)
             if 0 < kx * kx + ky * ky <= kmax * kmax:
                 break
         ph = rng.uniform(0, 2 * np.pi, size=3)
         a = amp * rng.uniform(-1, 1, size=3)
+        table[m] = (kx, ky, ph[0], ph[1], ph[2], a[0], a[1], a[2])
+    return table
+
+
+def fourier_member_state(n: int, table: np.ndarray):
+    """Padded state of a Fourier ensemble member on [0, 2 pi)^2 from its mode
+    table: rho = 1, p = 1, B = (1, 0.5), plus the modes on v (cell centres)
+    and on B through a corner vector potential (div B = 0 to round-off)."""
+    dx = grid_spacing(n)
+    X, Y = _centres(n, n, dx, dx)
+    XC, YC = _corners(n, n, dx, dx)
+    L = n * dx
+    vx = np.zeros_like(X); vy = np.zeros_like(X)
+    Az = np.zeros_like(XC)
+    for kx, ky, ph0, ph1, ph2, a0, a1, a2 in np.asarray(table, dtype=np.float64):
         arg = 2 * np.pi * (kx * X + ky * Y) / L
-        vx += a[0] * np.sin(arg + ph[0])
-        vy += a[1] * np.sin(arg + ph[1])
+        vx += a0 * np.sin(arg + ph0)
+        vy += a1 * np.sin(arg + ph1)
         k2 = (2 * np.pi / L) * np.sqrt(kx * kx + ky * ky)
-        Az += (a[2] / k2) * np.cos(2 * np.pi * (kx * XC + ky * YC) / L + ph[2])
+        Az += (a2 / k2) * np.cos(2 * np.pi * (kx * XC + ky * YC) / L + ph2)
     Az += 1.0 * YC - 0.5 * XC           # uniform B = (1, 0.5)
     bx, by = _face_b_from_potential(Az, dx, dx)
     return _from_primitive(1.0, vx, vy, 1.0, bx, by)
+
+
+def fourier_ensemble_member(n: int, rng: np.random.Generator, modes: int = 8, amp: float = 1e-2, kmax: int = 4):
+    """Ensemble member on [0, 2 pi)^2: rho = 1, p = 1, B = (1, 0.5) plus seeded
+    Fourier modes (|k| <= kmax, amplitude amp) on v and on B (through a vector
+    potential, so div B = 0 to round-off)."""
+    return fourier_member_state(n, fourier_member_modes(rng, modes, amp, kmax))
 
 
 # ---------------------------------------------------------------------------
@@ -368,6 +387,11 @@ def ensemble(nruns: int = 256, seed: int = 0, sides=ENSEMBLE_SIDES, eta_range=(1
 
 def ensemble_state(n: int, member_seed: int, **kw):
     return fourier_ensemble_member(n, np.random.default_rng(member_seed), **kw)
+
+
+def ensemble_modes(member_seed: int, **kw) -> np.ndarray:
+    """Mode table of ensemble member `member_seed` (for the device generator)."""
+    return fourier_member_modes(np.random.default_rng(member_seed), **kw)
 
 
 SETUPS = {

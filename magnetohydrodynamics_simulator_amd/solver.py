@@ -237,6 +237,20 @@ class Context:
                 raise ValueError(f"expected padded shape {(self.nyt, self.nxt)}, got {a.shape}")
         return keep, (_dp * 6)(*[_ptr(a) for a in keep])
 
+    # ---- device-generated initial states ----
+    def init_fourier(self, modes: np.ndarray):
+        """Fourier ensemble member generated on the device (mhd_init_fourier)."""
+        t = np.ascontiguousarray(modes, dtype=np.float64).reshape(-1, 8)
+        self._check(self.lib.mhd_init_fourier(self.h, t.shape[0], _ptr(t)))
+
+    def init_column(self, amps, phases, sausage: bool):
+        """Kink / sausage column generated on the device (mhd_init_column)."""
+        a = np.ascontiguousarray(amps, dtype=np.float64)
+        p = np.ascontiguousarray(phases, dtype=np.float64)
+        if a.shape != p.shape or a.ndim != 1:
+            raise ValueError("amps and phases must be 1-D arrays of one length")
+        self._check(self.lib.mhd_init_column(self.h, int(sausage), a.size, _ptr(a), _ptr(p)))
+
     # ---- files ----
     def write_snapshot(self, path: str, snapshot_rate: int, time_step: float):
         """write_data_to_binary_file (diagnostics.hpp:90-131) of the current state."""
@@ -374,6 +388,20 @@ class Batch:
             if a.shape != shape:
                 raise ValueError(f"run {run}: expected padded shape {shape}, got {a.shape}")
         self._check(self.lib.mhd_batch_upload(self.h, run, *[_ptr(a) for a in arrs]))
+
+    def init_fourier(self, run: int, modes: np.ndarray):
+        """Device-generated Fourier ensemble member (mhd_batch_init_fourier);
+        modes: the (M, 8) table of setups.fourier_member_modes."""
+        t = np.ascontiguousarray(modes, dtype=np.float64).reshape(-1, 8)
+        self._check(self.lib.mhd_batch_init_fourier(self.h, run, t.shape[0], _ptr(t)))
+
+    def init_column(self, run: int, amps, phases, sausage: bool):
+        """Device-generated kink / sausage column (mhd_batch_init_column)."""
+        a = np.ascontiguousarray(amps, dtype=np.float64)
+        p = np.ascontiguousarray(phases, dtype=np.float64)
+        if a.shape != p.shape or a.ndim != 1:
+            raise ValueError("amps and phases must be 1-D arrays of one length")
+        self._check(self.lib.mhd_batch_init_column(self.h, run, int(sausage), a.size, _ptr(a), _ptr(p)))
 
     def download(self, run: int) -> dict:
         shape = (int(self.ny[run]) + 4, int(self.nx[run]) + 4)

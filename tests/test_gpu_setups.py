@@ -1,0 +1,108 @@
+"""GPU checks of the device-generated initial states (mhd_batch_init_fourier,
+mhd_batch_init_column, mhd_init_fourier, mhd_init_column).
+
+The generators evaluate the host generators' formulas (setups.py) in the same
+order with the device's sin / cos / exp, so the bar is agreement to a few ulps
+of each field's scale -- 1e-13 of the field's maximum -- plus the invariants
+the setups guarantee: div B = 0 to round-off (the face B is the curl of a
+corner potential), rho = p = 1 for the Fourier members.  A run started from
+the device state tracks the run started from the host state.  A batch run and
+a context generate bitwise the same state (one kernel).
+"""
+import numpy as np
+import pytest
+
+from helpers import CONSERVED, interior, rel_l1
+from magnetohydrodynamics_simulator_amd import Batch, Context, setups
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-13
+
+
+def _say(msg):
+    print(f"\n[case] {msg}", flush=True)
+
+
+def _close(dev, host, what):
+    for k in CONSERVED:
+        scale = np.abs(interior(host[k])).max()
+        err = np.abs(interior(dev[k]) - interior(host[k])).max()
+        assert err <= TOL * scale, (what, k, err, scale)
+
+
+def _divb(s, dx):
+    bx, by = interior(s["bxf"]), interior(s["byf"])
+    return np.abs((np.roll(bx, -1, axis=1) - bx) / dx + (np.roll(by, -1, axis=0) - by) / dx).max()
+
+
+def test_fourier_members_match_host():
+    sides, seeds = [64, 96, 128], [11, 22, 33]
+    _say(f"device Fourier members {sides}")
+    dxs = [setups.grid_spacing(n) for n in sides]
+    tables = [setups.ensemble_modes(s) for s in seeds]
+    with Batch(sides, sides, dxs, dxs, [1e-3] * 3) as b:
+        for r, t in enumerate(tables):
+            b.init_fourier(r, t)
+        dev = [b.download(r) for r in range(3)]
+    for n, dx, t, d in zip(sides, dxs, tables, dev):
+        host = setups.fourier_member_state(n, t)
+        _close(d, host, n)
+        assert np.all(interior(d["rho"]) == 1.0)
+        assert _divb(d, dx) <= 1e-11 * np.abs(interior(d["bxf"])).max() / dx
+    n, dx, t = sides[1], dxs[1], tables[1]
+    with Context(n, n, dx, dx, 1e-3) as c:
+        c.init_fourier(t)
+        single = c.download()
+    for k in CONSERVED:
+        assert np.array_equal(interior(single[k]), interior(dev[1][k])), k
+
+
+@pytest.mark.parametrize("sausage,n,modes,amp", [(False, 128, 64, 1e-3), (True, 256, 8, 1e-2)],
+                         ids=["kink128", "sausage256"])
+def test_columns_match_host(sausage, n, modes, amp):
+    _say(f"device {'sausage' if sausage else 'kink'} column {n}^2, {modes} modes")
+    if sausage:
+        amps, phases = setups.sausage_column_modes(modes, amp, seed=4)
+        host = setups.sausage_column(n, amps, phases)
+    else:
+        amps, phases = setups.kink_column_modes(modes, amp, seed=4)
+        host = setups.kink_column(n, amps, phases)
+    with Batch([n], [n], [1.0 / n], [1.0 / n], [1e-3]) as b:
+        b.init_column(0, amps, phases, sausage)
+        dev = b.download(0)
+    _close(dev, host, n)
+    with Context(n, n, 1.0 / n, 1.0 / n, 1e-3) as c:
+        c.init_column(amps, phases, sausage)
+        single = c.download()
+    for k in CONSERVED:
+        assert np.array_equal(interior(single[k]), interior(dev[k])), k
+
+
+def test_run_from_device_state_tracks_host_state():
+    n, eta, steps = 96, 3e-3, 5
+    _say(f"run {steps} steps from device vs host Fourier member {n}^2")
+    dx = setups.grid_spacing(n)
+    t = setups.ensemble_modes(7)
+    with Context(n, n, dx, dx, eta, cfl=0.4, cg_tol=1e-12) as c:
+        c.init_fourier(t)
+        c.run(1e9, max_steps=steps)
+        dev = c.download()
+        host = setups.fourier_member_state(n, t)
+        c.upload(*[host[k] for k in CONSERVED])
+        c.run(1e9, max_steps=steps)
+        ref = c.download()
+    for k in CONSERVED:
+        assert rel_l1(interior(dev[k]), interior(ref[k])) <= 1e-12, k
+
+
+def test_generator_argument_errors():
+    _say("device generator argument errors")
+    with Batch([64, 32], [64, 48], [0.1, 0.1], [0.1, 0.1], [1e-3, 1e-3]) as b:
+        with pytest.raises(RuntimeError, match="square"):
+            b.init_fourier(1, setups.ensemble_modes(1))
+        bad = setups.ensemble_modes(1)
+        bad[2, :2] = 0.0
+        with pytest.raises(RuntimeError, match="zero wave vector"):
+            b.init_fourier(0, bad)
+        with pytest.raises(RuntimeError):
+            b.init_column(5, [1e-3], [0.0], True)

@@ -98,3 +98,18 @@ def test_sausage_column_derivative_symmetry_and_ensemble():
     assert e1 == setups.sausage_ensemble(64, seed=0)
     assert {n_ for n_, _, _ in e1} <= set(setups.SAUSAGE_SIDES)
     assert all(1e-4 <= eta <= 1e-2 for _, eta, _ in e1)
+
+
+def test_fourier_mode_table_defines_the_member():
+    """The seeded draws and the evaluation are separable: the mode table of a
+    member seed evaluates to the member, with wave vectors 0 < |k| <= 4 and
+    amplitudes within [-1e-2, 1e-2]."""
+    t = setups.ensemble_modes(99)
+    assert t.shape == (8, 8)
+    k2 = t[:, 0] ** 2 + t[:, 1] ** 2
+    assert np.all((k2 > 0) & (k2 <= 16)) and np.all(t[:, :2] == np.round(t[:, :2]))
+    assert np.all(np.abs(t[:, 5:]) <= 1e-2)
+    a = setups.fourier_member_state(64, t)
+    b = setups.ensemble_state(64, 99)
+    for k in a:
+        assert np.array_equal(a[k], b[k]), k
