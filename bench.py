@@ -122,6 +122,8 @@ def parse():
                          "or mhd_run (host reads the CFL value and the CG state each step)")
     ap.add_argument("--compare-steps", type=int, default=20,
                     help="steps of the un-instrumented host-loop / resident-loop comparison (0: skip)")
+    ap.add_argument("--outflow-steps", type=int, default=20,
+                    help="steps of the same workload with outflow boundaries on both axes (0: skip)")
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     return ap.parse_args()
 
@@ -637,10 +639,27 @@ def main():
             ctx.synchronize()
             cmp_ms[name] = (time.perf_counter() - tc) / nc * 1e3
         ctx.upload(*[st0[k] for k in ("rho", "mx", "my", "E", "bxf", "byf")])
-        del st0
         compare = {"steps": a.compare_steps, "host_loop_ms_per_step": round(cmp_ms["host_loop"], 4),
                    "resident_loop_ms_per_step": round(cmp_ms["resident_loop"], 4),
                    "note": "no events; the same steps from the same state (mhd_run vs mhd_run_resident)"}
+    # the same workload with outflow (zero-gradient) boundaries on both axes
+    outflow = None
+    if a.outflow_steps > 0:
+        if a.compare_steps <= 0:
+            st0 = ctx.download()
+        with Context(n, n, dx, dx, a.eta, cfl=a.cfl, cg_tol=a.cg_tol, bc=("outflow", "outflow")) as co:
+            co.upload(*[st0[k] for k in ("rho", "mx", "my", "E", "bxf", "byf")])
+            co.run_resident(1e9, max_steps=2)
+            co.synchronize()
+            to = time.perf_counter()
+            no, _, ito, _ = co.run_resident(1e9, max_steps=a.outflow_steps)
+            co.synchronize()
+            to = time.perf_counter() - to
+        outflow = {"steps": no, "value": round(N * no / to / 1e6, 2), "unit": "Mcell-updates/s",
+                   "ms_per_step": round(to / no * 1e3, 4), "cg_spmv_per_step": round((int(ito.sum()) + 2 * no) / no, 2),
+                   "note": "bc = (outflow, outflow), resident loop, no events; the periodic figure is driver_comparison"}
+    if a.compare_steps > 0 or a.outflow_steps > 0:
+        del st0
     # timed region: only the dominant kernel's launches carry events
     ctx.profile([dom])
     ctx.synchronize()
@@ -714,6 +733,7 @@ def main():
                         "solve halted the loop, passes after convergence); GBs = algorithmic bytes of the work "
                         "done / the kernel's total time",
         "driver_comparison": compare,
+        "outflow_boundaries": outflow,
         "pcie_inclusive": pcie,
     }
     if not a.no_traffic:

@@ -53,7 +53,25 @@ typedef struct mhd_opts {
   long long cg_max_iter;   /* <= 0: 2 N, Eigen's default (N = nx * ny corners) */
   int cg_fail_is_error;    /* 0 (reference): a non-converged solve is reported only in the stats */
   int march_rows;          /* rows per work item of the explicit-stage kernel; 0 = auto */
+  int bc_x, bc_y;          /* boundary of each axis: MHD_BC_PERIODIC (reference, Grid.hpp:64-80) or
+                              MHD_BC_OUTFLOW (zero-gradient; see below) */
 } mhd_opts;
+
+/* Boundary conditions.  The reference is periodic only and names the
+ * missing piece: a non-periodic run needs another ghost-fill rule
+ * (Grid.hpp:141-142).  MHD_BC_OUTFLOW is the zero-gradient rule: along that
+ * axis every field is extended beyond the domain by its value at the nearest
+ * cell (face) inside -- the ghost fill of the state before each explicit
+ * stage, of the recombined field before its resistive term -- and the corner
+ * quantities of the resistive term (eta J and the implicit solve's unknown)
+ * by the nearest corner, which makes the implicit operator's Laplacian a
+ * Neumann one (its Jacobi preconditioner takes the boundary rows' own
+ * diagonal).  The CT corner field E_z at the last corner line comes from the
+ * ghost-filled fluxes like every other corner.  Supported by the single-run
+ * entry points (including mhd_run_resident); batches and the adjoint are
+ * periodic. */
+#define MHD_BC_PERIODIC 0
+#define MHD_BC_OUTFLOW 1
 
 /* Per-step statistics (the reference computes none of these; CG failure is
  * silent there, solvers.hpp:469). */

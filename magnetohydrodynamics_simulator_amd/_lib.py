@@ -20,7 +20,12 @@ KERNEL_IDS = {"stage1": 0, "stage2": 1, "cg_start": 2, "cg_pass_a": 3, "cg_pass_
 
 class Opts(C.Structure):
     _fields_ = [("gamma", C.c_double), ("cfl", C.c_double), ("cg_tol", C.c_double),
-                ("cg_max_iter", C.c_longlong), ("cg_fail_is_error", C.c_int), ("march_rows", C.c_int)]
+                ("cg_max_iter", C.c_longlong), ("cg_fail_is_error", C.c_int), ("march_rows", C.c_int),
+                ("bc_x", C.c_int), ("bc_y", C.c_int)]
+
+
+BC_PERIODIC, BC_OUTFLOW = 0, 1
+BC_NAMES = {"periodic": BC_PERIODIC, "outflow": BC_OUTFLOW}
 
 
 class StepStats(C.Structure):

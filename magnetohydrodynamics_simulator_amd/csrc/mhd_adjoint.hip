@@ -504,6 +504,13 @@ int adjoint_step(mhd_ctx* c, const StepRecord& rec, const double* rho_next, int 
 
 int check(mhd_ctx* c) { return c ? MHD_OK : MHD_EINVAL; }
 
+// The adjoint kernels wrap periodically: outflow contexts are refused.
+int check_periodic(mhd_ctx* c, const char* what) {
+  if (c->o.bc_x != MHD_BC_PERIODIC || c->o.bc_y != MHD_BC_PERIODIC)
+    return fail(c, MHD_EINVAL, std::string(what) + ": the adjoint is periodic (bc_x = bc_y = MHD_BC_PERIODIC)");
+  return MHD_OK;
+}
+
 // The backward sweep; spec: run every solve speculatively (verified by the caller).
 int backward(mhd_ctx* c, bool spec) {
   AdjointStore& A = *c->adj;
@@ -548,6 +555,7 @@ namespace {
 int rhs_vjp_entry(mhd_ctx* c, const double* const* lam, double* const* out, bool fused) {
   if (check(c)) return MHD_EINVAL;
   if (!lam || !out) return fail(c, MHD_EINVAL, "mhd_explicit_rhs_vjp: NULL argument");
+  if (int rcp = check_periodic(c, "mhd_explicit_rhs_vjp")) return rcp;
   int rc = ensure_store(c, 0, 0);
   if (rc) return rc;
   AdjointStore& A = *c->adj;
@@ -575,6 +583,7 @@ extern "C" {
 
 int mhd_adjoint_forward(mhd_ctx* c, double dt, int nsteps, int checkpoint_every) {
   if (check(c)) return MHD_EINVAL;
+  if (int rcp = check_periodic(c, "mhd_adjoint_forward")) return rcp;
   if (!(dt > 0) || !std::isfinite(dt) || nsteps < 1 || checkpoint_every < 1)
     return fail(c, MHD_EINVAL, "mhd_adjoint_forward: need dt > 0, nsteps >= 1, checkpoint_every >= 1");
   if (!(c->eta > 0)) return fail(c, MHD_EINVAL, "mhd_adjoint_forward: the adjoint needs eta > 0");

@@ -247,6 +247,8 @@ int mhd_batch_create(int nruns, const int* nx, const int* ny, const double* dx, 
   if (opts) o = *opts;
   if (o.gamma != 5.0 / 3.0) return bfail(nullptr, MHD_EINVAL, "only gamma = 5/3 is compiled in");
   if (!(o.cfl > 0) || !(o.cg_tol >= 0) || o.march_rows < 0) return bfail(nullptr, MHD_EINVAL, "bad options");
+  if (o.bc_x != MHD_BC_PERIODIC || o.bc_y != MHD_BC_PERIODIC)
+    return bfail(nullptr, MHD_EINVAL, "batched runs are periodic (bc_x = bc_y = MHD_BC_PERIODIC)");
   long long total = 0;
   for (int r = 0; r < nruns; ++r) {
     if (nx[r] < 4 || ny[r] < 4) return bfail(nullptr, MHD_EINVAL, "run " + std::to_string(r) + ": nx, ny must be >= 4");

@@ -69,6 +69,8 @@ StageArgs stage_args(mhd_ctx* c) {
   a.nx = c->nx;
   a.ny = c->ny;
   a.rows = c->stage_rows;
+  a.open_x = c->o.bc_x == MHD_BC_OUTFLOW;
+  a.open_y = c->o.bc_y == MHD_BC_OUTFLOW;
   a.dx = c->dx;
   a.dy = c->dy;
   a.inv_dx = 1.0 / c->dx;
@@ -88,6 +90,8 @@ CgArgs cg_args(mhd_ctx* c, const double* bx, const double* by) {
   CgArgs a{};
   a.nx = c->nx;
   a.ny = c->ny;
+  a.open_x = c->o.bc_x == MHD_BC_OUTFLOW;
+  a.open_y = c->o.bc_y == MHD_BC_OUTFLOW;
   a.dx = c->dx;
   a.dy = c->dy;
   a.eta = c->eta;
@@ -126,10 +130,17 @@ int cg_run(mhd_ctx* c, const double* bx, const double* by, const double* bin, Cg
   int rc0 = launch(c, 8, [&] { hipLaunchKernelGGL(k_cg_reset, dim3(1), dim3(1), 0, c->stream, c->st, maxit); });
   if (rc0) return rc0;
   const int nb = c->tiles;
-  int rc = bin ? launch_k(c, 2, k_cg_start<false>, dim3(nb), dim3(kTileX), a, bin)
-               : launch_k(c, 2, k_cg_start<true>, dim3(nb), dim3(kTileX), a, bin);
+  // outflow contexts run the CG kernels built for clamped neighbours
+  const bool open = a.open_x | a.open_y;
+  auto k_start_rhs = open ? k_cg_start_open<false> : k_cg_start<false>;
+  auto k_start_b = open ? k_cg_start_open<true> : k_cg_start<true>;
+  auto k_b0 = open ? k_cg_pass_b_open<true> : k_cg_pass_b<true>;
+  auto k_a = open ? k_cg_pass_a_open : k_cg_pass_a;
+  auto k_b = open ? k_cg_pass_b_open<false> : k_cg_pass_b<false>;
+  int rc = bin ? launch_k(c, 2, k_start_rhs, dim3(nb), dim3(kTileX), a, bin)
+               : launch_k(c, 2, k_start_b, dim3(nb), dim3(kTileX), a, bin);
   if (rc) return rc;
-  rc = launch_k(c, 4, k_cg_pass_b<true>, dim3(nb), dim3(kTileX), a, (const double*)c->p[0], 0);
+  rc = launch_k(c, 4, k_b0, dim3(nb), dim3(kTileX), a, (const double*)c->p[0], 0);
   if (rc) return rc;
   // the direction of iteration it lives in p[it & 1] (p[0] from the start kernel)
   long long it = 1;
@@ -138,10 +149,10 @@ int cg_run(mhd_ctx* c, const double* bx, const double* by, const double* bin, Cg
     for (; it <= spec + 1; ++it) {
       const double* pold = c->p[(it - 1) & 1];
       double* pnew = c->p[it & 1];
-      rc = launch_k(c, 3, k_cg_pass_a, dim3(nb), dim3(kTileX), a, pold, pnew, (int)it);
+      rc = launch_k(c, 3, k_a, dim3(nb), dim3(kTileX), a, pold, pnew, (int)it);
       if (rc) return rc;
       if (it == spec + 1) break;
-      rc = launch_k(c, 4, k_cg_pass_b<false>, dim3(nb), dim3(kTileX), a, (const double*)pnew, (int)it);
+      rc = launch_k(c, 4, k_b, dim3(nb), dim3(kTileX), a, (const double*)pnew, (int)it);
       if (rc) return rc;
     }
     if (spec_out) HIP_OR_FAIL(c, hipMemcpyAsync(spec_out, c->st, sizeof(CgState), hipMemcpyDeviceToDevice, c->stream));
@@ -155,10 +166,10 @@ int cg_run(mhd_ctx* c, const double* bx, const double* by, const double* bin, Cg
     for (int k = 0; k < chunk; ++k, ++it) {
       const double* pold = c->p[(it - 1) & 1];
       double* pnew = c->p[it & 1];
-      rc = launch_k(c, 3, k_cg_pass_a, dim3(nb), dim3(kTileX), a, pold, pnew, (int)it);
+      rc = launch_k(c, 3, k_a, dim3(nb), dim3(kTileX), a, pold, pnew, (int)it);
       if (rc) return rc;
       if (k + 1 == chunk) break;   // the chunk ends with an A: it holds the decision
-      rc = launch_k(c, 4, k_cg_pass_b<false>, dim3(nb), dim3(kTileX), a, (const double*)pnew, (int)it);
+      rc = launch_k(c, 4, k_b, dim3(nb), dim3(kTileX), a, (const double*)pnew, (int)it);
       if (rc) return rc;
     }
     HIP_OR_FAIL(c, hipMemcpyAsync(c->st_host, c->st, sizeof(CgState), hipMemcpyDeviceToHost, c->stream));
@@ -166,7 +177,7 @@ int cg_run(mhd_ctx* c, const double* bx, const double* by, const double* bin, Cg
     if (c->st_host->done) break;
     // not converged: run B(it) for the A just launched, then continue
     const double* pv = c->p[it & 1];
-    rc = launch_k(c, 4, k_cg_pass_b<false>, dim3(nb), dim3(kTileX), a, pv, (int)it);
+    rc = launch_k(c, 4, k_b, dim3(nb), dim3(kTileX), a, pv, (int)it);
     if (rc) return rc;
     ++it;
     if (it > maxit + 4) return fail(c, MHD_EHIP, "CG state machine did not terminate");
@@ -186,6 +197,8 @@ FinArgs fin_args(mhd_ctx* c, const double* bx, const double* by, double* bxo, do
   FinArgs f{};
   f.nx = c->nx;
   f.ny = c->ny;
+  f.open_x = c->o.bc_x == MHD_BC_OUTFLOW;
+  f.open_y = c->o.bc_y == MHD_BC_OUTFLOW;
   f.dx = c->dx;
   f.dy = c->dy;
   f.eta = c->eta;
@@ -205,6 +218,8 @@ DiagArgs diag_args(mhd_ctx* c) {
   DiagArgs d{};
   d.nx = c->nx;
   d.ny = c->ny;
+  d.open_x = c->o.bc_x == MHD_BC_OUTFLOW;
+  d.open_y = c->o.bc_y == MHD_BC_OUTFLOW;
   d.dx = c->dx;
   d.dy = c->dy;
   for (int q = 0; q < 6; ++q) d.s[q] = c->U[q];
@@ -225,18 +240,19 @@ int diag_partials(mhd_ctx* c, K kernel, int id) {
   return MHD_OK;
 }
 
-// Host-side periodic ghost fill of one padded array (Grid::fillOneField, Grid.hpp:143-161).
-void fill_ghosts(double* f, int nx, int ny) {
+// Host-side ghost fill of one padded array: periodic (Grid::fillOneField,
+// Grid.hpp:143-161) or, on an outflow axis, the nearest inside value.
+void fill_ghosts(double* f, int nx, int ny, int open_x, int open_y) {
   const int ng = 2, nxt = nx + 4, nyt = ny + 4;
   for (int j = ng; j < nyt - ng; ++j)
     for (int g = 0; g < ng; ++g) {
-      f[j * nxt + g] = f[j * nxt + nx + g];
-      f[j * nxt + nx + ng + g] = f[j * nxt + ng + g];
+      f[j * nxt + g] = open_x ? f[j * nxt + ng] : f[j * nxt + nx + g];
+      f[j * nxt + nx + ng + g] = open_x ? f[j * nxt + nx + ng - 1] : f[j * nxt + ng + g];
     }
   for (int i = 0; i < nxt; ++i)
     for (int g = 0; g < ng; ++g) {
-      f[g * nxt + i] = f[(ny + g) * nxt + i];
-      f[(ny + ng + g) * nxt + i] = f[(ng + g) * nxt + i];
+      f[g * nxt + i] = open_y ? f[ng * nxt + i] : f[(ny + g) * nxt + i];
+      f[(ny + ng + g) * nxt + i] = open_y ? f[(ny + ng - 1) * nxt + i] : f[(ng + g) * nxt + i];
     }
 }
 
@@ -248,7 +264,7 @@ int to_host_padded(mhd_ctx* c, const double* dev, double* host, bool ghosts) {
   HIP_OR_FAIL(c, hipMemcpyAsync(c->stage, dev, (size_t)c->n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
   for (int j = 0; j < c->ny; ++j) std::memcpy(host + (j + 2) * nxt + 2, c->stage + j * nx, nx * sizeof(double));
-  if (ghosts) fill_ghosts(host, c->nx, c->ny);
+  if (ghosts) fill_ghosts(host, c->nx, c->ny, c->o.bc_x == MHD_BC_OUTFLOW, c->o.bc_y == MHD_BC_OUTFLOW);
   return MHD_OK;
 }
 
@@ -385,6 +401,7 @@ int mhd_default_opts(mhd_opts* o) {
   o->cg_max_iter = 0;
   o->cg_fail_is_error = 0;
   o->march_rows = 0;
+  o->bc_x = o->bc_y = MHD_BC_PERIODIC;
   return MHD_OK;
 }
 
@@ -404,6 +421,8 @@ int mhd_create(int nx, int ny, double dx, double dy, double eta, const mhd_opts*
   if (o.gamma != 5.0 / 3.0) return fail(nullptr, MHD_EINVAL, "only gamma = 5/3 is compiled in");
   if (!(o.cfl > 0) || !(o.cg_tol >= 0)) return fail(nullptr, MHD_EINVAL, "cfl must be > 0 and cg_tol >= 0");
   if (o.march_rows < 0) return fail(nullptr, MHD_EINVAL, "march_rows must be >= 0");
+  if ((o.bc_x != MHD_BC_PERIODIC && o.bc_x != MHD_BC_OUTFLOW) || (o.bc_y != MHD_BC_PERIODIC && o.bc_y != MHD_BC_OUTFLOW))
+    return fail(nullptr, MHD_EINVAL, "bc_x and bc_y must be MHD_BC_PERIODIC or MHD_BC_OUTFLOW");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(nullptr, MHD_EHIP, "no HIP device available");
 

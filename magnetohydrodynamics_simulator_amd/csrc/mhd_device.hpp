@@ -225,4 +225,15 @@ __device__ __forceinline__ int pmod(int a, int n) {
 }
 __device__ __forceinline__ int wrap1(int a, int n) { return a < 0 ? a + n : (a >= n ? a - n : a); }
 
+// Index k on an axis of n points.  A periodic axis wraps (any k for bc_mod,
+// |k| < 2n for bc_wrap); an outflow axis takes the nearest index inside the
+// domain: every field is extended by its boundary value (the zero-gradient
+// ghost rule), so a corner unknown beyond the last corner equals the last.
+__device__ __forceinline__ int bc_wrap(int k, int n, int open) {
+  return open ? (k < 0 ? 0 : (k >= n ? n - 1 : k)) : wrap1(k, n);
+}
+__device__ __forceinline__ int bc_mod(int k, int n, int open) {
+  return open ? (k < 0 ? 0 : (k >= n ? n - 1 : k)) : pmod(k, n);
+}
+
 }  // namespace mhd

@@ -65,20 +65,21 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, int strip, int se
   const int y0 = seg * a.rows;
   const int y1 = min(y0 + a.rows, ny);
   const int col = x0 - 2 + lane;
-  const int c0 = pmod(col, nx);
-  const int c1 = pmod(col + 1, nx);
+  const int ox = a.open_x, oy = a.open_y;
+  const int c0 = bc_mod(col, nx, ox);
+  const int c1 = bc_mod(col + 1, nx, ox);
   const bool out_lane = (lane >= 2) && (lane < 2 + kStripOut) && (col < nx);
   const double idx_ = a.inv_dx, idy_ = a.inv_dy;
 
   Carry s = {};
-  s.byf_c = a.s[5][wrap1(y0 - 2, ny) * nx + c0];
+  s.byf_c = a.s[5][bc_wrap(y0 - 2, ny, oy) * nx + c0];
   s.bxl_b = 0.0;
   // row loads are issued one row ahead of their use so that their latency
   // overlaps the flux arithmetic of the current row
   double n_rho, n_mx, n_my, n_E, n_bxl, n_bxr, n_byn;
   {
-    const int b0 = wrap1(y0 - 2, ny) * nx;
-    const int b1 = wrap1(y0 - 1, ny) * nx;
+    const int b0 = bc_wrap(y0 - 2, ny, oy) * nx;
+    const int b1 = bc_wrap(y0 - 1, ny, oy) * nx;
     n_rho = a.s[0][b0 + c0]; n_mx = a.s[1][b0 + c0]; n_my = a.s[2][b0 + c0]; n_E = a.s[3][b0 + c0];
     n_bxl = a.s[4][b0 + c0]; n_bxr = a.s[4][b0 + c1]; n_byn = a.s[5][b1 + c0];
   }
@@ -89,8 +90,8 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, int strip, int se
     {
       // unconditional (rows past the segment are valid periodic rows; the last
       // iteration's values are unused) so the wait counters stay static
-      const int bn = wrap1(k + 1, ny) * nx;
-      const int bnn = wrap1(k + 2, ny) * nx;
+      const int bn = bc_wrap(k + 1, ny, oy) * nx;
+      const int bnn = bc_wrap(k + 2, ny, oy) * nx;
       n_rho = a.s[0][bn + c0]; n_mx = a.s[1][bn + c0]; n_my = a.s[2][bn + c0]; n_E = a.s[3][bn + c0];
       n_bxl = a.s[4][bn + c0]; n_bxr = a.s[4][bn + c1]; n_byn = a.s[5][bnn + c0];
     }
@@ -158,7 +159,13 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, int strip, int se
     const int j = k - 2;
     const double fl_rho = sh_up(s.f_rho), fl_mx = sh_up(s.f_mx), fl_my = sh_up(s.f_my), fl_E = sh_up(s.f_E);
     const double ez_right = sh_dn(s.ez);
-    const double j_right = (MODE == 2) ? sh_dn(s.j_a) : 0.0;
+    // the resistive source is a corner quantity: beyond the last corner of an
+    // outflow axis it is the last corner's (J(n) = J(n - 1)).  The shuffle
+    // runs in every lane before the select: a lane that skipped it would leave
+    // its neighbour reading an inactive lane.
+    const double j_dn = (MODE == 2) ? sh_dn(s.j_a) : 0.0;
+    const double j_right = (MODE == 2) ? ((ox && c0 == nx - 1) ? s.j_a : j_dn) : 0.0;
+    const double j_above = (MODE == 2) ? ((oy && k - 2 == ny - 1) ? s.j_a : j_b) : 0.0;
     if (j >= y0 && j < y1 && out_lane) {
       double d[6];
       d[0] = -((s.f_rho - fl_rho) * idx_ + (G.rho - s.g_rho) * idy_);
@@ -189,7 +196,7 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, int strip, int se
         for (int q = 0; q < 6; ++q) wv[q] = a.u[q][o];
         double g2x = 0.0, g2y = 0.0;
         if (a.eta != 0.0) {
-          g2x = -(j_b - s.j_a) / a.dy;
+          g2x = -(j_above - s.j_a) / a.dy;
           g2y = (j_right - s.j_a) / a.dx;
         }
 #pragma unroll
@@ -314,15 +321,15 @@ struct Tile {
   bool valid;
 };
 
-__device__ __forceinline__ Tile tile_of(int nx, int ny, int ty, int tiles_y, int vb) {
+__device__ __forceinline__ Tile tile_of(int nx, int ny, int ty, int tiles_y, int vb, int open_x = 0) {
   Tile t;
   t.vb = vb;
   const int bx = t.vb / tiles_y, by = t.vb - bx * tiles_y;
   t.i = bx * kTileX + threadIdx.x;
   t.valid = t.i < nx;
   t.ic = t.valid ? t.i : nx - 1;
-  t.im = (t.ic == 0) ? nx - 1 : t.ic - 1;
-  t.ip = (t.ic == nx - 1) ? 0 : t.ic + 1;
+  t.im = bc_wrap(t.ic - 1, nx, open_x);
+  t.ip = bc_wrap(t.ic + 1, nx, open_x);
   t.y0 = by * ty;
   t.y1 = min(t.y0 + ty, ny);
   return t;
@@ -360,12 +367,13 @@ __device__ __forceinline__ double stencil(double ps, double pw, double pc, doubl
 
 // Columns of a tile: `w` valid columns starting at x0 and the periodic left
 // and right halo columns xl, xr.
-__device__ __forceinline__ void tile_columns(int nx, int tiles_y, int vb, int& x0, int& w, int& xl, int& xr) {
+__device__ __forceinline__ void tile_columns(int nx, int tiles_y, int vb, int& x0, int& w, int& xl, int& xr,
+                                             int open_x = 0) {
   const int bx = vb / tiles_y;
   x0 = bx * kTileX;
   w = min(kTileX, nx - x0);
-  xl = (x0 == 0) ? nx - 1 : x0 - 1;
-  xr = (x0 + w >= nx) ? x0 + w - nx : x0 + w;
+  xl = bc_wrap(x0 - 1, nx, open_x);
+  xr = bc_wrap(x0 + w, nx, open_x);
 }
 
 // eta J(B) at corner (i, j) (resistive_B_term's first loop, solvers.hpp:450-463)
@@ -375,14 +383,39 @@ __device__ __forceinline__ double rhs_from(const CgArgs& a, double by_c, double 
   const double dBx_dy = (bx_c - bx_s) / a.dy;
   return a.eta * (dBy_dx - dBx_dy);
 }
+template <bool OPEN = true>
 __device__ __forceinline__ double rhs_at(const CgArgs& a, int i, int j) {
   const int nx = a.nx, ny = a.ny;
-  const int im = (i == 0) ? nx - 1 : i - 1;
-  const int jm = (j == 0) ? ny - 1 : j - 1;
+  const int im = bc_wrap(i - 1, nx, OPEN ? a.open_x : 0);
+  const int jm = bc_wrap(j - 1, ny, OPEN ? a.open_y : 0);
   return rhs_from(a, a.by[j * nx + i], a.by[j * nx + im], a.bx[j * nx + i], a.bx[jm * nx + i]);
 }
 
-__device__ __forceinline__ int wrap_row(int j, int ny) { return j < 0 ? j + ny : (j >= ny ? j - ny : j); }
+__device__ __forceinline__ int wrap_row(int j, int ny, int open_y = 0) { return bc_wrap(j, ny, open_y); }
+
+// Jacobi weight 1 / A_ii of corner (i, j).  On an outflow axis a boundary
+// corner's clamped neighbour folds into the diagonal (A_ii = d - c), summed in
+// the order the assembled operator's entries of that row appear (south,
+// north, west, east, then the diagonal).
+__device__ __forceinline__ double invd_at(const CgArgs& a, int i, int j) {
+  if (!(a.open_x | a.open_y)) return a.invd;
+  double s = 0.0;
+  bool edge = false;
+  if (a.open_y && j == 0) { s += -a.cy; edge = true; }
+  if (a.open_y && j == a.ny - 1) { s += -a.cy; edge = true; }
+  if (a.open_x && i == 0) { s += -a.cx; edge = true; }
+  if (a.open_x && i == a.nx - 1) { s += -a.cx; edge = true; }
+  return edge ? 1.0 / (s + a.diag) : a.invd;
+}
+
+// The bodies of the CG passes are instantiated per boundary mode: with both
+// axes periodic the Jacobi weight is the constant 1 / d and nothing of the
+// outflow handling is in the loop.
+template <bool OPEN>
+__device__ __forceinline__ double invd_of(const CgArgs& a, int i, int j) {
+  if (!OPEN) return a.invd;
+  return invd_at(a, i, j);
+}
 
 // The 2D sweeps below march a tile upwards one row per iteration.  Global
 // loads are issued two rows ahead of their use, so each wave keeps a row of
@@ -391,30 +424,35 @@ __device__ __forceinline__ int wrap_row(int j, int ny) { return j < 0 ? j + ny :
 
 // Start of a solve: b = eta J(B) (or bin), r = b, p = invd b, and the first
 // product q = A p; partials |b|^2, b . p, p . q.
-template <bool FROM_B>
+template <bool FROM_B, bool OPEN>
 __device__ __forceinline__ void cg_start_body(const CgArgs& a, const double* __restrict__ bin, int vb) {
   __shared__ double prow[2][kTileX + 2];
   __shared__ double lds[16];
-  const int nx = a.nx, ny = a.ny;
-  const Tile t = tile_of(nx, ny, a.ty, a.tiles_y, vb);
+  const int nx = a.nx, ny = a.ny, oy = (OPEN ? a.open_y : 0);
+  const Tile t = tile_of(nx, ny, a.ty, a.tiles_y, vb, (OPEN ? a.open_x : 0));
   int x0, w, xl, xr;
-  tile_columns(nx, a.tiles_y, vb, x0, w, xl, xr);
+  tile_columns(nx, a.tiles_y, vb, x0, w, xl, xr, (OPEN ? a.open_x : 0));
   const int tid = threadIdx.x;
   const bool left = (tid == 0), right = (tid == w - 1);
-  const double invd = a.invd, d = a.diag, cx = a.cx, cy = a.cy;
-  // b of row j for the own column, with bx of the row below carried
-  double bx_carry = FROM_B ? a.bx[wrap_row(t.y0 - 2, ny) * nx + t.ic] : 0.0;
+  const double d = a.diag, cx = a.cx, cy = a.cy;
+  // b of row j for the own column, with bx of the row below carried (rows are
+  // visited in order; on an outflow axis the corner row below the first is the
+  // first, that above the last is the last)
+  double bx_carry = FROM_B ? a.bx[wrap_row(t.y0 - 2, ny, oy) * nx + t.ic] : 0.0;
+  double b_last = 0.0;
   auto own_b = [&](int j) -> double {
-    const int r = wrap_row(j, ny);
+    const int r = wrap_row(j, ny, oy);
     if (!FROM_B) return bin[r * nx + t.ic];
+    if (oy && j >= ny) return b_last;
     const double bxc = a.bx[r * nx + t.ic];
     const double v = rhs_from(a, a.by[r * nx + t.ic], a.by[r * nx + t.im], bxc, bx_carry);
     bx_carry = bxc;
+    b_last = v;
     return v;
   };
   auto halo_b = [&](int col, int j) -> double {
-    const int r = wrap_row(j, ny);
-    return FROM_B ? rhs_at(a, col, r) : bin[r * nx + col];
+    const int r = wrap_row(j, ny, oy);
+    return FROM_B ? rhs_at<OPEN>(a, col, r) : bin[r * nx + col];
   };
   double acc[3] = {0.0, 0.0, 0.0};
   double bm = own_b(t.y0 - 1);
@@ -428,12 +466,14 @@ __device__ __forceinline__ void cg_start_body(const CgArgs& a, const double* __r
     const double hl_n = (left && j + 1 < t.y1) ? halo_b(xl, j + 1) : 0.0;
     const double hr_n = (right && j + 1 < t.y1) ? halo_b(xr, j + 1) : 0.0;
     const int buf = (j - t.y0) & 1;
-    const double pc = invd * bc;
+    const double pc = invd_of<OPEN>(a, t.ic, j) * bc;
     if (t.valid) prow[buf][tid + 1] = pc;
-    if (left) prow[buf][0] = invd * hl;
-    if (right) prow[buf][w + 1] = invd * hr;
+    if (left) prow[buf][0] = invd_of<OPEN>(a, xl, j) * hl;
+    if (right) prow[buf][w + 1] = invd_of<OPEN>(a, xr, j) * hr;
     __syncthreads();
-    const double q = stencil(invd * bm, prow[buf][tid], pc, prow[buf][tid + 2], invd * bn, cx, cy, d);
+    const double pm = invd_of<OPEN>(a, t.ic, wrap_row(j - 1, ny, oy)) * bm;
+    const double pn = invd_of<OPEN>(a, t.ic, wrap_row(j + 1, ny, oy)) * bn;
+    const double q = stencil(pm, prow[buf][tid], pc, prow[buf][tid + 2], pn, cx, cy, d);
     if (t.valid) {
       const int o = j * nx + t.ic;
       a.r[o] = bc - 0.0;
@@ -459,13 +499,14 @@ __device__ __forceinline__ void cg_start_body(const CgArgs& a, const double* __r
 // Right-hand side only (explicit resistive term): r = eta J(B).
 __global__ __launch_bounds__(kTileX) void k_cg_rhs(CgArgs a) {
   const int nx = a.nx, ny = a.ny;
-  const Tile t = tile_of(nx, ny, a.ty, a.tiles_y, xcd_block(blockIdx.x, gridDim.x));
+  const Tile t = tile_of(nx, ny, a.ty, a.tiles_y, xcd_block(blockIdx.x, gridDim.x), a.open_x);
   if (!t.valid) return;
   for (int j = t.y0; j < t.y1; ++j) a.r[j * nx + t.ic] = rhs_at(a, t.ic, j) - 0.0;
 }
 
 // Pass A(it), it >= 1: convergence decision on the partials of B(it - 1),
 // beta, p_new = invd r + beta p_old, q = A p_new; partial p_new . q.
+template <bool OPEN>
 __device__ __forceinline__ void cg_pass_a_body(const CgArgs& a, const double* __restrict__ pold,
                                                double* __restrict__ pnew, int it, int vb, int nb) {
   __shared__ double prow[2][kTileX + 2];
@@ -474,7 +515,7 @@ __device__ __forceinline__ void cg_pass_a_body(const CgArgs& a, const double* __
   if (st->done) return;
   double rr[2];
   sum_partials<2>(a.part_b, nb, rr, lds);
-  const Tile t = tile_of(a.nx, a.ny, a.ty, a.tiles_y, vb);
+  const Tile t = tile_of(a.nx, a.ny, a.ty, a.tiles_y, vb, (OPEN ? a.open_x : 0));
   const bool lead = (t.vb == 0) && (threadIdx.x == 0);
   const double rn2 = rr[0];
   if (rn2 < st->threshold) {                // converged after B(it - 1)
@@ -487,17 +528,19 @@ __device__ __forceinline__ void cg_pass_a_body(const CgArgs& a, const double* __
     if (lead) st->done = 2;
     return;
   }
-  const int nx = a.nx, ny = a.ny;
+  const int nx = a.nx, ny = a.ny, oy = (OPEN ? a.open_y : 0);
   int x0, w, xl, xr;
-  tile_columns(nx, a.tiles_y, vb, x0, w, xl, xr);
+  tile_columns(nx, a.tiles_y, vb, x0, w, xl, xr, (OPEN ? a.open_x : 0));
   const int tid = threadIdx.x;
   const bool left = (tid == 0), right = (tid == w - 1);
-  const double invd = a.invd, d = a.diag, cx = a.cx, cy = a.cy;
-  auto pv = [&](double po, double ro) -> double { return fma(beta, po, invd * ro); };
-  auto at = [&](int col, int j) -> int { return wrap_row(j, ny) * nx + col; };
+  const double d = a.diag, cx = a.cx, cy = a.cy;
+  auto pv = [&](double po, double ro, int col, int j) -> double {
+    return fma(beta, po, invd_of<OPEN>(a, col, wrap_row(j, ny, oy)) * ro);
+  };
+  auto at = [&](int col, int j) -> int { return wrap_row(j, ny, oy) * nx + col; };
   double acc[1] = {0.0};
-  double pm = pv(pold[at(t.ic, t.y0 - 1)], a.r[at(t.ic, t.y0 - 1)]);
-  double pc = pv(pold[at(t.ic, t.y0)], a.r[at(t.ic, t.y0)]);
+  double pm = pv(pold[at(t.ic, t.y0 - 1)], a.r[at(t.ic, t.y0 - 1)], t.ic, t.y0 - 1);
+  double pc = pv(pold[at(t.ic, t.y0)], a.r[at(t.ic, t.y0)], t.ic, t.y0);
   double n_po = pold[at(t.ic, t.y0 + 1)], n_ro = a.r[at(t.ic, t.y0 + 1)];
   double l_po = left ? pold[at(xl, t.y0)] : 0.0, l_ro = left ? a.r[at(xl, t.y0)] : 0.0;
   double r_po = right ? pold[at(xr, t.y0)] : 0.0, r_ro = right ? a.r[at(xr, t.y0)] : 0.0;
@@ -510,11 +553,11 @@ __device__ __forceinline__ void cg_pass_a_body(const CgArgs& a, const double* __
     const double ln_ro = (left && more) ? a.r[at(xl, j + 1)] : 0.0;
     const double rn_po = (right && more) ? pold[at(xr, j + 1)] : 0.0;
     const double rn_ro = (right && more) ? a.r[at(xr, j + 1)] : 0.0;
-    const double pn = pv(n_po, n_ro);
+    const double pn = pv(n_po, n_ro, t.ic, j + 1);
     const int buf = (j - t.y0) & 1;
     if (t.valid) prow[buf][tid + 1] = pc;
-    if (left) prow[buf][0] = pv(l_po, l_ro);
-    if (right) prow[buf][w + 1] = pv(r_po, r_ro);
+    if (left) prow[buf][0] = pv(l_po, l_ro, xl, j);
+    if (right) prow[buf][w + 1] = pv(r_po, r_ro, xr, j);
     __syncthreads();
     const double q = stencil(pm, prow[buf][tid], pc, prow[buf][tid + 2], pn, cx, cy, d);
     if (t.valid) {
@@ -534,13 +577,13 @@ __device__ __forceinline__ void cg_pass_a_body(const CgArgs& a, const double* __
 // Pass B(it): alpha = absNew / (p . Ap), x += alpha p, r -= alpha Ap;
 // partials |r|^2 and r . (invd r).  B(0) also takes the start-of-solve
 // decisions of Eigen's conjugate_gradient from the partials of the start kernel.
-template <bool FIRST>
+template <bool FIRST, bool OPEN>
 __device__ __forceinline__ void cg_pass_b_body(const CgArgs& a, const double* __restrict__ pv, int it, int vb, int nb) {
   __shared__ double prow[2][kTileX + 2];
   __shared__ double lds[16];
   CgState* st = a.st;
   if (st->done) return;
-  const Tile t = tile_of(a.nx, a.ny, a.ty, a.tiles_y, vb);
+  const Tile t = tile_of(a.nx, a.ny, a.ty, a.tiles_y, vb, (OPEN ? a.open_x : 0));
   const bool lead = (t.vb == 0) && (threadIdx.x == 0);
   double alpha, pap;
   if (FIRST) {
@@ -576,11 +619,11 @@ __device__ __forceinline__ void cg_pass_b_body(const CgArgs& a, const double* __
   }
   const int nx = a.nx, ny = a.ny;
   int x0, w, xl, xr;
-  tile_columns(nx, a.tiles_y, vb, x0, w, xl, xr);
+  tile_columns(nx, a.tiles_y, vb, x0, w, xl, xr, (OPEN ? a.open_x : 0));
   const int tid = threadIdx.x;
   const bool left = (tid == 0), right = (tid == w - 1);
-  const double invd = a.invd, d = a.diag, cx = a.cx, cy = a.cy;
-  auto at = [&](int col, int j) -> int { return wrap_row(j, ny) * nx + col; };
+  const double d = a.diag, cx = a.cx, cy = a.cy;
+  auto at = [&](int col, int j) -> int { return wrap_row(j, ny, (OPEN ? a.open_y : 0)) * nx + col; };
   double acc[2] = {0.0, 0.0};
   double pm = pv[at(t.ic, t.y0 - 1)];
   double pc = pv[at(t.ic, t.y0)];
@@ -608,7 +651,7 @@ __device__ __forceinline__ void cg_pass_b_body(const CgArgs& a, const double* __
       a.x[o] = xc + alpha * pc;
       const double rn = rc - alpha * q;
       a.r[o] = rn;
-      const double z = invd * rn;
+      const double z = invd_of<OPEN>(a, t.ic, j) * rn;
       acc[0] += rn * rn;
       acc[1] += rn * z;
     }
@@ -627,18 +670,37 @@ __device__ __forceinline__ void cg_pass_b_body(const CgArgs& a, const double* __
   }
 }
 
+// Periodic kernels, and separate kernels for contexts with an outflow axis
+// (one body per kernel: a kernel holding both would be compiled for the
+// heavier one, which costs the periodic passes a quarter of their speed).
 template <bool FROM_B>
 __global__ __launch_bounds__(kTileX) void k_cg_start(CgArgs a, const double* __restrict__ bin) {
-  cg_start_body<FROM_B>(a, bin, xcd_block(blockIdx.x, gridDim.x));
+  cg_start_body<FROM_B, false>(a, bin, xcd_block(blockIdx.x, gridDim.x));
 }
 __global__ __launch_bounds__(kTileX) void k_cg_pass_a(CgArgs a, const double* __restrict__ pold,
                                                       double* __restrict__ pnew, int it) {
-  cg_pass_a_body(a, pold, pnew, it, xcd_block(blockIdx.x, gridDim.x), gridDim.x);
+  cg_pass_a_body<false>(a, pold, pnew, it, xcd_block(blockIdx.x, gridDim.x), gridDim.x);
 }
 template <bool FIRST>
 __global__ __launch_bounds__(kTileX) void k_cg_pass_b(CgArgs a, const double* __restrict__ pv, int it) {
-  cg_pass_b_body<FIRST>(a, pv, it, xcd_block(blockIdx.x, gridDim.x), gridDim.x);
+  cg_pass_b_body<FIRST, false>(a, pv, it, xcd_block(blockIdx.x, gridDim.x), gridDim.x);
 }
+template <bool FROM_B>
+__global__ __launch_bounds__(kTileX) void k_cg_start_open(CgArgs a, const double* __restrict__ bin) {
+  cg_start_body<FROM_B, true>(a, bin, xcd_block(blockIdx.x, gridDim.x));
+}
+__global__ __launch_bounds__(kTileX) void k_cg_pass_a_open(CgArgs a, const double* __restrict__ pold,
+                                                           double* __restrict__ pnew, int it) {
+  cg_pass_a_body<true>(a, pold, pnew, it, xcd_block(blockIdx.x, gridDim.x), gridDim.x);
+}
+template <bool FIRST>
+__global__ __launch_bounds__(kTileX) void k_cg_pass_b_open(CgArgs a, const double* __restrict__ pv, int it) {
+  cg_pass_b_body<FIRST, true>(a, pv, it, xcd_block(blockIdx.x, gridDim.x), gridDim.x);
+}
+template __global__ void k_cg_start_open<true>(CgArgs, const double*);
+template __global__ void k_cg_start_open<false>(CgArgs, const double*);
+template __global__ void k_cg_pass_b_open<true>(CgArgs, const double*, int);
+template __global__ void k_cg_pass_b_open<false>(CgArgs, const double*, int);
 
 template __global__ void k_cg_start<true>(CgArgs, const double*);
 template __global__ void k_cg_start<false>(CgArgs, const double*);
@@ -667,9 +729,9 @@ __device__ __forceinline__ void finalize1_body(const FinArgs& a, int vb) {
     }
     return;
   }
-  const Tile t = tile_of(nx, ny, a.ty, a.tiles_y, vb);
+  const Tile t = tile_of(nx, ny, a.ty, a.tiles_y, vb, a.open_x);
   if (!t.valid) return;
-  auto at = [&](int col, int j) -> int { return wrap_row(j, ny) * nx + col; };
+  auto at = [&](int col, int j) -> int { return wrap_row(j, ny, a.open_y) * nx + col; };
   double xc = a.x[at(t.ic, t.y0)], xn = a.x[at(t.ic, t.y0 + 1)], xe = a.x[at(t.ip, t.y0)];
   double bxv = a.bx[t.y0 * nx + t.ic], byv = a.by[t.y0 * nx + t.ic];
   for (int j = t.y0; j < t.y1; ++j) {
@@ -699,11 +761,12 @@ struct Fin2Chain {
 };
 
 // J3(col, row) given chain.bx3_below = bx3(col, row - 1); advances the chain.
+template <bool OPEN>
 __device__ __forceinline__ double fin2_j3(const FinArgs& a, bool xv, int col, int row, Fin2Chain& ch) {
   const int nx = a.nx, ny = a.ny;
-  const int cm = (col == 0) ? nx - 1 : col - 1;
-  const int cp = (col == nx - 1) ? 0 : col + 1;
-  const int rp = (row == ny - 1) ? 0 : row + 1;
+  const int cm = bc_wrap(col - 1, nx, (OPEN ? a.open_x : 0));
+  const int cp = bc_wrap(col + 1, nx, (OPEN ? a.open_x : 0));
+  const int rp = bc_wrap(row + 1, ny, (OPEN ? a.open_y : 0));
   const double xc = xv ? a.x[row * nx + col] : 0.0;
   const double xn = xv ? a.x[rp * nx + col] : 0.0;
   const double xe = xv ? a.x[row * nx + cp] : 0.0;
@@ -713,7 +776,9 @@ __device__ __forceinline__ double fin2_j3(const FinArgs& a, bool xv, int col, in
   const double bysw = a.by[row * nx + cm];
   const double bx3 = bxs + (-(xn - xc) / a.dy) * a.c;           // bx3(col, row)
   const double by3 = bys + ((xe - xc) / a.dx) * a.c;            // by3(col, row)
-  const double by3w = bysw + ((xc - xw) / a.dx) * a.c;          // by3(col - 1, row)
+  // by3(col - 1, row); left of the first column of an outflow axis B3 is
+  // extended by its boundary value (the ghost rule on the recombined field)
+  const double by3w = ((OPEN ? a.open_x : 0) && col == 0) ? by3 : bysw + ((xc - xw) / a.dx) * a.c;
   const double dBy_dx = (by3 - by3w) / a.dx;
   const double dBx_dy = (bx3 - ch.bx3_below) / a.dy;
   ch.bx3_below = bx3;
@@ -722,11 +787,15 @@ __device__ __forceinline__ double fin2_j3(const FinArgs& a, bool xv, int col, in
   return a.eta * (dBy_dx - dBx_dy);
 }
 
+template <bool OPEN>
 __device__ __forceinline__ Fin2Chain fin2_begin(const FinArgs& a, bool xv, int col, int row0) {
   const int nx = a.nx, ny = a.ny;
-  const int rm = (row0 == 0) ? ny - 1 : row0 - 1;
+  // bx3 of the row below row0; below the first row of an outflow axis, that
+  // of the first row (the ghost rule on the recombined field)
+  const int rm = ((OPEN ? a.open_y : 0) && row0 == 0) ? 0 : wrap1(row0 - 1, ny);
+  const int rmn = bc_wrap(rm + 1, ny, (OPEN ? a.open_y : 0));
   const double xc = xv ? a.x[rm * nx + col] : 0.0;
-  const double xn = xv ? a.x[row0 * nx + col] : 0.0;
+  const double xn = xv ? a.x[rmn * nx + col] : 0.0;
   Fin2Chain ch;
   ch.bx3_below = a.bx[rm * nx + col] + (-(xn - xc) / a.dy) * a.c;
   ch.bxs = ch.bys = 0.0;
@@ -755,26 +824,26 @@ __device__ __forceinline__ void cfl_cell(double rho, double mx, double my, doubl
 // cell-centred by) and the CFL of row j - 1 is formed at row j; the new bx of
 // the column to the right comes through LDS, that of the halo column from the
 // edge thread's own chain.
-template <bool CFL>
+template <bool CFL, bool OPEN>
 __device__ __forceinline__ void finalize2_body(const FinArgs& a, int vb, const DiagArgs& d) {
   __shared__ double jrow[2][kTileX + 2];
   __shared__ double bxrow[CFL ? 2 : 1][CFL ? kTileX + 2 : 1];
   __shared__ double lds[16];
   const bool xv = a.st->spmv != 0;
   const int nx = a.nx, ny = a.ny;
-  const Tile t = tile_of(nx, ny, a.ty, a.tiles_y, vb);
+  const Tile t = tile_of(nx, ny, a.ty, a.tiles_y, vb, (OPEN ? a.open_x : 0));
   int x0, w, xl, xr;
-  tile_columns(nx, a.tiles_y, vb, x0, w, xl, xr);
+  tile_columns(nx, a.tiles_y, vb, x0, w, xl, xr, (OPEN ? a.open_x : 0));
   (void)xl;
   const int tid = threadIdx.x;
   const bool edge = (tid == w - 1);
-  Fin2Chain own = fin2_begin(a, xv, t.ic, t.y0);
-  Fin2Chain hal = fin2_begin(a, xv, xr, t.y0);
-  double jc = fin2_j3(a, xv, t.ic, t.y0, own);
+  Fin2Chain own = fin2_begin<OPEN>(a, xv, t.ic, t.y0);
+  Fin2Chain hal = fin2_begin<OPEN>(a, xv, xr, t.y0);
+  double jc = fin2_j3<OPEN>(a, xv, t.ic, t.y0, own);
   double jhc = 0.0;                              // J3(xr, j) of the halo chain (CFL only)
   if (t.valid) jrow[0][tid + 1] = jc;
   if (edge) {
-    const double jh = fin2_j3(a, xv, xr, t.y0, hal);
+    const double jh = fin2_j3<OPEN>(a, xv, xr, t.y0, hal);
     jrow[0][w + 1] = jh;
     jhc = jh;
   }
@@ -791,19 +860,22 @@ __device__ __forceinline__ void finalize2_body(const FinArgs& a, int vb, const D
       for (int q = 0; q < 4; ++q) u_row[q] = d.s[q][o];
     }
     const int buf = (j - t.y0) & 1;
-    const int jw = (j >= ny) ? j - ny : j;      // row j wrapped (the extra row of a top tile)
-    const int jp = (jw == ny - 1) ? 0 : jw + 1;
+    // row j wrapped (the extra row of a top tile; the last row on an outflow axis)
+    const int jw = (j >= ny) ? ((OPEN ? a.open_y : 0) ? ny - 1 : j - ny) : j;
+    const int jp = bc_wrap(jw + 1, ny, (OPEN ? a.open_y : 0));
+    // J3 above the last corner row of an outflow axis is the last row's
+    const bool top = (OPEN ? a.open_y : 0) && jw == ny - 1;
     const double bxs = own.bxs, bys = own.bys;   // B3* faces of row j
     const double hbxs = hal.bxs;                 // B3* bx of row j, halo column (edge thread)
-    const double jn = fin2_j3(a, xv, t.ic, jp, own);
+    const double jn = fin2_j3<OPEN>(a, xv, t.ic, jp, own);
     if (t.valid) jrow[buf ^ 1][tid + 1] = jn;
     double jhn = 0.0;
     if (edge) {
-      jhn = fin2_j3(a, xv, xr, jp, hal);
+      jhn = fin2_j3<OPEN>(a, xv, xr, jp, hal);
       jrow[buf ^ 1][w + 1] = jhn;
     }
     const double jr = jrow[buf][tid + 2];        // J3(i + 1, j)
-    const double g3x = -(jn - jc) / a.dy;
+    const double g3x = -((top ? jc : jn) - jc) / a.dy;
     const double g3y = (jr - jc) / a.dx;
     const double bxo = bxs + g3x * a.c;
     const double byo = bys + g3y * a.c;
@@ -814,11 +886,12 @@ __device__ __forceinline__ void finalize2_body(const FinArgs& a, int vb, const D
     }
     if (CFL) {
       if (t.valid) bxrow[buf][tid + 1] = bxo;
-      if (edge) bxrow[buf][w + 1] = hbxs + (-(jhn - jhc) / a.dy) * a.c;   // new bx(xr, j)
+      if (edge) bxrow[buf][w + 1] = hbxs + (-((top ? jhc : jhn) - jhc) / a.dy) * a.c;   // new bx(xr, j)
       if (j > t.y0 && t.valid) {
-        // cell (i, j - 1): bx faces (i, j - 1), (i + 1, j - 1); by faces (i, j - 1), (i, j)
+        // cell (i, j - 1): bx faces (i, j - 1), (i + 1, j - 1); by faces (i, j - 1), (i, j);
+        // above the last row of an outflow axis by is the last row's
         const double bxc = (bxo_prev + bxrow[buf ^ 1][tid + 2]) / 2.0;
-        const double byc = (byo_prev + byo) / 2.0;
+        const double byc = (byo_prev + (((OPEN ? a.open_y : 0) && j >= ny) ? byo_prev : byo)) / 2.0;
         cfl_cell(u_prev[0], u_prev[1], u_prev[2], u_prev[3], bxc, byc, m, bad);
       }
       bxo_prev = bxo;
@@ -843,7 +916,9 @@ __device__ __forceinline__ void finalize2_body(const FinArgs& a, int vb, const D
 
 __global__ __launch_bounds__(kTileX) void k_finalize1(FinArgs a) { finalize1_body(a, xcd_block(blockIdx.x, gridDim.x)); }
 __global__ __launch_bounds__(kTileX) void k_finalize2(FinArgs a) {
-  finalize2_body<false>(a, xcd_block(blockIdx.x, gridDim.x), DiagArgs{});
+  const int vb = xcd_block(blockIdx.x, gridDim.x);
+  if (a.open_x | a.open_y) finalize2_body<false, true>(a, vb, DiagArgs{});
+  else finalize2_body<false, false>(a, vb, DiagArgs{});
 }
 
 // rate arrays of resistive_B_term for the unit entry point.
@@ -852,8 +927,8 @@ __global__ __launch_bounds__(256) void k_rate(FinArgs a, int zero) {
   const int n = nx * ny;
   for (int o = blockIdx.x * blockDim.x + threadIdx.x; o < n; o += gridDim.x * blockDim.x) {
     const int j = o / nx, i = o - j * nx;
-    const int jp = (j == ny - 1) ? 0 : j + 1;
-    const int ip = (i == nx - 1) ? 0 : i + 1;
+    const int jp = bc_wrap(j + 1, ny, a.open_y);
+    const int ip = bc_wrap(i + 1, nx, a.open_x);
     if (zero) {
       a.bx_out[o] = -(0.0 - 0.0) / a.dy;
       a.by_out[o] = (0.0 - 0.0) / a.dx;
@@ -874,11 +949,11 @@ __global__ __launch_bounds__(256) void k_rate(FinArgs a, int zero) {
 __device__ __forceinline__ void cfl_body(const DiagArgs& a, int vb) {
   __shared__ double lds[16];
   const int nx = a.nx, ny = a.ny;
-  const Tile t = tile_of(nx, ny, a.ty, a.tiles_y, vb);
+  const Tile t = tile_of(nx, ny, a.ty, a.tiles_y, vb, a.open_x);
   double m = 1e-8;
   double bad[1] = {0.0};
   if (t.valid) {
-    auto at = [&](int col, int j) -> int { return wrap_row(j, ny) * nx + col; };
+    auto at = [&](int col, int j) -> int { return wrap_row(j, ny, a.open_y) * nx + col; };
     double by_lo = a.s[5][at(t.ic, t.y0)];
     int o = t.y0 * nx + t.ic;
     double rho = a.s[0][o], mx = a.s[1][o], my = a.s[2][o], E = a.s[3][o];
@@ -913,13 +988,13 @@ __global__ __launch_bounds__(kTileX) void k_cfl(DiagArgs a) { cfl_body(a, xcd_bl
 __global__ __launch_bounds__(kTileX) void k_divb(DiagArgs a) {
   __shared__ double lds[16];
   const int nx = a.nx, ny = a.ny;
-  const Tile t = tile_of(nx, ny, a.ty, a.tiles_y, xcd_block(blockIdx.x, gridDim.x));
+  const Tile t = tile_of(nx, ny, a.ty, a.tiles_y, xcd_block(blockIdx.x, gridDim.x), a.open_x);
   double ss[1] = {0.0};
   double m = 0.0;
   if (t.y0 < ny && t.valid) {
     double by_lo = a.s[5][t.y0 * nx + t.ic];
     for (int j = t.y0; j < t.y1; ++j) {
-      const int jp = (j == ny - 1) ? 0 : j + 1;
+      const int jp = bc_wrap(j + 1, ny, a.open_y);
       const int o = j * nx + t.ic;
       const double by_hi = a.s[5][jp * nx + t.ic];
       const double xd = (a.s[4][j * nx + t.ip] - a.s[4][o]) / a.dx;
@@ -945,8 +1020,8 @@ __global__ __launch_bounds__(256) void k_prims(DiagArgs a, double* vx, double* v
   const int n = nx * ny;
   for (int o = blockIdx.x * blockDim.x + threadIdx.x; o < n; o += gridDim.x * blockDim.x) {
     const int j = o / nx, i = o - j * nx;
-    const int ip = (i == nx - 1) ? 0 : i + 1;
-    const int jp = (j == ny - 1) ? 0 : j + 1;
+    const int ip = bc_wrap(i + 1, nx, a.open_x);
+    const int jp = bc_wrap(j + 1, ny, a.open_y);
     const double bx = (a.s[4][o] + a.s[4][j * nx + ip]) / 2.0;
     const double by = (a.s[5][o] + a.s[5][jp * nx + i]) / 2.0;
     double r, u, v, pp;
@@ -1000,6 +1075,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void kb
   if (!rs.active) return;
   const RunGeom& g = b.geom[job.x];
   StageArgs a;
+  a.open_x = a.open_y = 0;     // batched runs are periodic
 #pragma unroll
   for (int q = 0; q < 6; ++q) {
     a.s[q] = b.s[q] + g.base;
@@ -1027,6 +1103,7 @@ __device__ __forceinline__ CgArgs batch_cg_args(const BatchCgArgs& b, int run) {
   const RunGeom& g = b.geom[run];
   const RunState& rs = b.rs[run];
   CgArgs a;
+  a.open_x = a.open_y = 0;
   a.nx = g.nx;
   a.ny = g.ny;
   a.ty = g.ty;
@@ -1055,7 +1132,7 @@ __global__ __launch_bounds__(kTileX) void kb_cg_start(BatchCgArgs b) {
   const int2 job = b.jobs[xcd_block(blockIdx.x, gridDim.x)];
   if (b.st[job.x].done) return;            // stopped run, or eta = 0: no solve
   const CgArgs a = batch_cg_args(b, job.x);
-  cg_start_body<true>(a, nullptr, job.y);
+  cg_start_body<true, false>(a, nullptr, job.y);
 }
 
 // Start of a solve with the given right-hand side bin (pooled).
@@ -1063,7 +1140,7 @@ __global__ __launch_bounds__(kTileX) void kb_cg_start_rhs(BatchCgArgs b, const d
   const int2 job = b.jobs[xcd_block(blockIdx.x, gridDim.x)];
   if (b.st[job.x].done) return;
   const CgArgs a = batch_cg_args(b, job.x);
-  cg_start_body<false>(a, bin + b.geom[job.x].base, job.y);
+  cg_start_body<false, false>(a, bin + b.geom[job.x].base, job.y);
 }
 
 __global__ __launch_bounds__(kTileX) void kb_cg_pass_a(BatchCgArgs b, const double* __restrict__ pold,
@@ -1072,7 +1149,7 @@ __global__ __launch_bounds__(kTileX) void kb_cg_pass_a(BatchCgArgs b, const doub
   if (b.st[job.x].done) return;
   const CgArgs a = batch_cg_args(b, job.x);
   const long long base = b.geom[job.x].base;
-  cg_pass_a_body(a, pold + base, pnew + base, it, job.y, b.geom[job.x].tiles);
+  cg_pass_a_body<false>(a, pold + base, pnew + base, it, job.y, b.geom[job.x].tiles);
 }
 
 template <bool FIRST>
@@ -1080,7 +1157,7 @@ __global__ __launch_bounds__(kTileX) void kb_cg_pass_b(BatchCgArgs b, const doub
   const int2 job = b.jobs[xcd_block(blockIdx.x, gridDim.x)];
   if (b.st[job.x].done) return;
   const CgArgs a = batch_cg_args(b, job.x);
-  cg_pass_b_body<FIRST>(a, pv + b.geom[job.x].base, it, job.y, b.geom[job.x].tiles);
+  cg_pass_b_body<FIRST, false>(a, pv + b.geom[job.x].base, it, job.y, b.geom[job.x].tiles);
 }
 template __global__ void kb_cg_pass_b<true>(BatchCgArgs, const double*, int);
 template __global__ void kb_cg_pass_b<false>(BatchCgArgs, const double*, int);
@@ -1088,6 +1165,7 @@ template __global__ void kb_cg_pass_b<false>(BatchCgArgs, const double*, int);
 __device__ __forceinline__ FinArgs batch_fin_args(const BatchFinArgs& b, int run) {
   const RunGeom& g = b.geom[run];
   FinArgs f;
+  f.open_x = f.open_y = 0;
   f.nx = g.nx;
   f.ny = g.ny;
   f.ty = g.ty;
@@ -1117,7 +1195,7 @@ __global__ __launch_bounds__(kTileX) void kb_finalize2(BatchFinArgs b) {
   const int2 job = b.jobs[xcd_block(blockIdx.x, gridDim.x)];
   const FinArgs f = batch_fin_args(b, job.x);
   if (b.rs[job.x].active) {
-    finalize2_body<false>(f, job.y, DiagArgs{});
+    finalize2_body<false, false>(f, job.y, DiagArgs{});
     return;
   }
   const Tile t = tile_of(f.nx, f.ny, f.ty, f.tiles_y, job.y);
@@ -1137,6 +1215,7 @@ __global__ __launch_bounds__(kTileX) void kb_cfl(BatchDiagArgs b) {
   const RunState& rs = b.rs[job.x];
   if (!(rs.t < g.t_end) || (g.max_steps >= 0 && rs.steps >= g.max_steps)) return;
   DiagArgs d;
+  d.open_x = d.open_y = 0;
   d.nx = g.nx;
   d.ny = g.ny;
   d.ty = g.ty;
@@ -1260,26 +1339,34 @@ __device__ __forceinline__ CgArgs res_cg_args(const CgArgs& a, const RunState* r
   return b;
 }
 
+template <bool OPEN>
 __global__ __launch_bounds__(kTileX) void k_cg_start_res(CgArgs a, const RunState* __restrict__ rs) {
   if (res_stopped(rs)) return;
-  cg_start_body<true>(res_cg_args(a, rs), nullptr, xcd_block(blockIdx.x, gridDim.x));
+  cg_start_body<true, OPEN>(res_cg_args(a, rs), nullptr, xcd_block(blockIdx.x, gridDim.x));
 }
+template __global__ void k_cg_start_res<false>(CgArgs, const RunState*);
+template __global__ void k_cg_start_res<true>(CgArgs, const RunState*);
 
+template <bool OPEN>
 __global__ __launch_bounds__(kTileX) void k_cg_pass_a_res(CgArgs a, const RunState* __restrict__ rs,
                                                           const double* __restrict__ pold, double* __restrict__ pnew,
                                                           int it) {
   if (res_stopped(rs)) return;
-  cg_pass_a_body(res_cg_args(a, rs), pold, pnew, it, xcd_block(blockIdx.x, gridDim.x), gridDim.x);
+  cg_pass_a_body<OPEN>(res_cg_args(a, rs), pold, pnew, it, xcd_block(blockIdx.x, gridDim.x), gridDim.x);
 }
+template __global__ void k_cg_pass_a_res<false>(CgArgs, const RunState*, const double*, double*, int);
+template __global__ void k_cg_pass_a_res<true>(CgArgs, const RunState*, const double*, double*, int);
 
-template <bool FIRST>
+template <bool FIRST, bool OPEN>
 __global__ __launch_bounds__(kTileX) void k_cg_pass_b_res(CgArgs a, const RunState* __restrict__ rs,
                                                           const double* __restrict__ pv, int it) {
   if (res_stopped(rs)) return;
-  cg_pass_b_body<FIRST>(res_cg_args(a, rs), pv, it, xcd_block(blockIdx.x, gridDim.x), gridDim.x);
+  cg_pass_b_body<FIRST, OPEN>(res_cg_args(a, rs), pv, it, xcd_block(blockIdx.x, gridDim.x), gridDim.x);
 }
-template __global__ void k_cg_pass_b_res<true>(CgArgs, const RunState*, const double*, int);
-template __global__ void k_cg_pass_b_res<false>(CgArgs, const RunState*, const double*, int);
+template __global__ void k_cg_pass_b_res<true, false>(CgArgs, const RunState*, const double*, int);
+template __global__ void k_cg_pass_b_res<false, false>(CgArgs, const RunState*, const double*, int);
+template __global__ void k_cg_pass_b_res<true, true>(CgArgs, const RunState*, const double*, int);
+template __global__ void k_cg_pass_b_res<false, true>(CgArgs, const RunState*, const double*, int);
 
 // Log entry of the step just finished (block 0, thread 0 of the caller).
 __device__ __forceinline__ void write_step_log(const RunState* rs, const CgState* st, StepLog* log, int log_cap) {
@@ -1330,7 +1417,9 @@ __global__ __launch_bounds__(kTileX) void k_finalize2_res(FinArgs a, DiagArgs d,
   if (blockIdx.x == 0 && threadIdx.x == 0) write_step_log(rs, st, log, log_cap);
   FinArgs f = a;
   f.c = rs->coef[4];
-  finalize2_body<CFL>(f, xcd_block(blockIdx.x, gridDim.x), d);
+  const int vb = xcd_block(blockIdx.x, gridDim.x);
+  if (a.open_x | a.open_y) finalize2_body<CFL, true>(f, vb, d);
+  else finalize2_body<CFL, false>(f, vb, d);
 }
 template __global__ void k_finalize2_res<false>(FinArgs, DiagArgs, RunState*, const CgState*, StepLog*, int, int);
 template __global__ void k_finalize2_res<true>(FinArgs, DiagArgs, RunState*, const CgState*, StepLog*, int, int);

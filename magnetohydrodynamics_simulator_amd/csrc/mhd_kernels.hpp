@@ -22,6 +22,7 @@ struct StageArgs {
   double* o1[6];
   double* o2[6];
   int nx, ny, rows;
+  int open_x, open_y;   // outflow (zero-gradient) boundaries instead of periodic ones
   double dx, dy, inv_dx, inv_dy, eta;
   double c_a, c_b, c_g;
 };
@@ -36,6 +37,7 @@ struct CgState {
 
 struct CgArgs {
   int nx, ny, ty, tiles_y;
+  int open_x, open_y;
   double dx, dy, eta;
   double invd, diag, cx, cy, tol;
   const double* bx;  // face B for the right-hand side
@@ -51,6 +53,7 @@ struct CgArgs {
 
 struct FinArgs {
   int nx, ny, ty, tiles_y;
+  int open_x, open_y;
   double dx, dy, eta, c;
   const double* x;
   const double* bx;
@@ -62,6 +65,7 @@ struct FinArgs {
 
 struct DiagArgs {
   int nx, ny, ty, tiles_y;
+  int open_x, open_y;
   double dx, dy;
   const double* s[6];
   double* part;    // two partials per tile
@@ -185,9 +189,11 @@ __global__ void kb_dt(const RunGeom* geom, RunState* rs, const double* part, CgS
 // read from the device driver state (c = dt * f), returning at once once the
 // run has stopped or the loop has halted.
 template <int MODE> __global__ void k_stage_res(StageArgs a, const RunState* rs);
-__global__ void k_cg_start_res(CgArgs a, const RunState* rs);
+template <bool OPEN> __global__ void k_cg_start_res(CgArgs a, const RunState* rs);
+template <bool OPEN>
 __global__ void k_cg_pass_a_res(CgArgs a, const RunState* rs, const double* pold, double* pnew, int it);
-template <bool FIRST> __global__ void k_cg_pass_b_res(CgArgs a, const RunState* rs, const double* pv, int it);
+template <bool FIRST, bool OPEN>
+__global__ void k_cg_pass_b_res(CgArgs a, const RunState* rs, const double* pv, int it);
 __global__ void k_finalize1_res(FinArgs a, RunState* rs, int fail_is_error);
 template <bool CFL>
 __global__ void k_finalize2_res(FinArgs a, DiagArgs d, RunState* rs, const CgState* st, StepLog* log, int log_cap,
@@ -198,6 +204,9 @@ __global__ void k_clear_halt(RunState* rs);
 
 template <int MODE> __global__ void k_stage(StageArgs a);
 template <bool FROM_B> __global__ void k_cg_start(CgArgs a, const double* bin);
+template <bool FROM_B> __global__ void k_cg_start_open(CgArgs a, const double* bin);
+__global__ void k_cg_pass_a_open(CgArgs a, const double* pold, double* pnew, int it);
+template <bool FIRST> __global__ void k_cg_pass_b_open(CgArgs a, const double* pv, int it);
 __global__ void k_cg_rhs(CgArgs a);
 __global__ void k_cg_pass_a(CgArgs a, const double* pold, double* pnew, int it);
 template <bool FIRST> __global__ void k_cg_pass_b(CgArgs a, const double* pv, int it);

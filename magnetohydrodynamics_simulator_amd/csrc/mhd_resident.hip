@@ -142,16 +142,20 @@ CgArgs res_cg(mhd_ctx* c, const Loop& L, const double* bx, const double* by, int
 // Launched iterations of a solve: start, B(0), then A(1) B(1) ... A(spec) B(spec) A(spec + 1).
 int enqueue_cg(mhd_ctx* c, const CgArgs& a, const RunState* rs, int spec) {
   const int nb = c->tiles;
-  int rc = launch_k(c, 2, k_cg_start_res, dim3(nb), dim3(kTileX), a, rs);
+  const bool open = a.open_x | a.open_y;
+  auto k_start = open ? k_cg_start_res<true> : k_cg_start_res<false>;
+  auto k_b0 = open ? k_cg_pass_b_res<true, true> : k_cg_pass_b_res<true, false>;
+  auto k_a = open ? k_cg_pass_a_res<true> : k_cg_pass_a_res<false>;
+  auto k_b = open ? k_cg_pass_b_res<false, true> : k_cg_pass_b_res<false, false>;
+  int rc = launch_k(c, 2, k_start, dim3(nb), dim3(kTileX), a, rs);
   if (rc) return rc;
-  rc = launch_k(c, 4, k_cg_pass_b_res<true>, dim3(nb), dim3(kTileX), a, rs, (const double*)c->p[0], 0);
+  rc = launch_k(c, 4, k_b0, dim3(nb), dim3(kTileX), a, rs, (const double*)c->p[0], 0);
   if (rc) return rc;
   for (int it = 1; it <= spec + 1; ++it) {
-    rc = launch_k(c, 3, k_cg_pass_a_res, dim3(nb), dim3(kTileX), a, rs, (const double*)c->p[(it - 1) & 1],
-                  c->p[it & 1], it);
+    rc = launch_k(c, 3, k_a, dim3(nb), dim3(kTileX), a, rs, (const double*)c->p[(it - 1) & 1], c->p[it & 1], it);
     if (rc) return rc;
     if (it == spec + 1) break;
-    rc = launch_k(c, 4, k_cg_pass_b_res<false>, dim3(nb), dim3(kTileX), a, rs, (const double*)c->p[it & 1], it);
+    rc = launch_k(c, 4, k_b, dim3(nb), dim3(kTileX), a, rs, (const double*)c->p[it & 1], it);
     if (rc) return rc;
   }
   return MHD_OK;
@@ -299,14 +303,17 @@ int resume_step(mhd_ctx* c, Loop& L, int slot) {
   if (rc) return rc;
   const CgArgs a = slot == 0 ? res_cg(c, L, c->S[4], c->S[5], 0) : res_cg(c, L, c->U[4], c->U[5], 1);
   const int nb = c->tiles;
+  const bool open = a.open_x | a.open_y;
+  auto k_a = open ? k_cg_pass_a_res<true> : k_cg_pass_a_res<false>;
+  auto k_b = open ? k_cg_pass_b_res<false, true> : k_cg_pass_b_res<false, false>;
   const long long maxit = R->st_host[slot].max_iters;
   long long it = L.spec[slot] + 1;   // the last A launched
   for (;;) {
-    rc = launch_k(c, 4, k_cg_pass_b_res<false>, dim3(nb), dim3(kTileX), a, (const RunState*)R->rs,
+    rc = launch_k(c, 4, k_b, dim3(nb), dim3(kTileX), a, (const RunState*)R->rs,
                   (const double*)c->p[it & 1], (int)it);
     if (rc) return rc;
     ++it;
-    rc = launch_k(c, 3, k_cg_pass_a_res, dim3(nb), dim3(kTileX), a, (const RunState*)R->rs,
+    rc = launch_k(c, 3, k_a, dim3(nb), dim3(kTileX), a, (const RunState*)R->rs,
                   (const double*)c->p[(it - 1) & 1], c->p[it & 1], (int)it);
     if (rc) return rc;
     HIP_OR_FAIL(c, hipMemcpyAsync(R->st_host + slot, R->st + slot, sizeof(CgState), hipMemcpyDeviceToHost, c->stream));

@@ -394,6 +394,25 @@ def ensemble_modes(member_seed: int, **kw) -> np.ndarray:
     return fourier_member_modes(np.random.default_rng(member_seed), **kw)
 
 
+def shock_tube(nx: int, ny: int = 4, dx: float | None = None, dy: float | None = None, x0: float = 0.5):
+    """Brio & Wu's MHD shock tube along x on [0, 1) (dx = dy = 1 / nx unless
+    given): rho = 1, p = 1, by = 1 left of x0; rho = 0.125, p = 0.1, by = -1
+    right of it; bx = 0.75, v = 0.  The solver's gamma is 5/3 (Brio & Wu use
+    2), so the wave pattern is that of the same Riemann problem at 5/3.  Run
+    with outflow boundaries in x (the reference cannot: it is periodic only,
+    Grid.hpp:141-142)."""
+    dx = dx if dx is not None else 1.0 / nx
+    dy = dy if dy is not None else dx
+    X, _ = _centres(nx, ny, dx, dy)
+    left = X < x0
+    rho = np.where(left, 1.0, 0.125)
+    p = np.where(left, 1.0, 0.1)
+    by = np.where(left, 1.0, -1.0)
+    bx = np.full_like(X, 0.75)
+    zero = np.zeros_like(X)
+    return _from_primitive(rho, zero, zero, p, bx, by)
+
+
 SETUPS = {
     "orszag_tang": orszag_tang,
     "decaying_sine": decaying_sine,

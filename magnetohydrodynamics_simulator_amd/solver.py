@@ -72,7 +72,9 @@ class Context:
 
     def __init__(self, nx: int, ny: int, dx: float, dy: float, eta: float = 0.0, *, cfl: float = 0.45,
                  cg_tol: float | None = None, cg_max_iter: int = 0, cg_fail_is_error: bool = False,
-                 march_rows: int = 0):
+                 march_rows: int = 0, bc: tuple = ("periodic", "periodic")):
+        """bc: boundary of the x and y axes, "periodic" (the reference's) or
+        "outflow" (zero-gradient; MHD_BC_OUTFLOW of the C ABI)."""
         self.lib = _lib.load()
         o = _lib.Opts()
         _lib.check(self.lib.mhd_default_opts(C.byref(o)))
@@ -82,6 +84,11 @@ class Context:
         o.cg_max_iter = cg_max_iter
         o.cg_fail_is_error = int(cg_fail_is_error)
         o.march_rows = march_rows
+        try:
+            o.bc_x, o.bc_y = (_lib.BC_NAMES[b] for b in bc)
+        except (KeyError, ValueError, TypeError):
+            raise ValueError(f"bc must be a pair of 'periodic' / 'outflow', got {bc!r}") from None
+        self.bc = tuple(bc)
         h = C.c_void_p()
         _lib.check(self.lib.mhd_create(nx, ny, dx, dy, eta, C.byref(o), C.byref(h)), None)
         self.h = h

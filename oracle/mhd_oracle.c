@@ -38,6 +38,7 @@
 struct og_grid {
   int nx, ny, ng, nxt, nyt;
   double dx, dy, eta;
+  int bc_x, bc_y;   /* 0 periodic (the reference), 1 outflow (zero-gradient ghost rule) */
   double* f[OG_NFIELDS];
 };
 
@@ -114,11 +115,29 @@ double* og_field(og_grid* g, int which) { return (which >= 0 && which < OG_NFIEL
 
 void og_grid_set_eta(og_grid* g, double eta) { g->eta = eta; }
 
+/* Boundary of each axis: 0 periodic (the reference's only one), 1 outflow.
+ * The reference names the missing piece (Grid.hpp:141-142): a non-periodic
+ * run needs another ghost-fill rule.  Outflow here is the zero-gradient rule:
+ *   - conserved and face fields: ghosts take the value of the nearest cell
+ *     (face) inside, wherever the reference fills ghosts;
+ *   - cell-centred B of every cell, ghosts included, is the mean of its two
+ *     ghost-filled faces (computeCellB, not a copy);
+ *   - the CT update of the last face line uses E_z of the boundary corner
+ *     line, which cell_corner_electric_fields computes over [-1, n] anyway;
+ *   - corner quantities of resistive_B_term (eta J and the CG unknown) past
+ *     the last corner take the last corner's value, and generate_A_matrix
+ *     folds a clamped neighbour into the diagonal (a Neumann Laplacian).
+ * This is the rule the library's kernels implement. */
+void og_grid_set_bc(og_grid* g, int bc_x, int bc_y) { g->bc_x = bc_x; g->bc_y = bc_y; }
+
+static inline int clamp_index(int k, int n) { return k < 0 ? 0 : (k >= n ? n - 1 : k); }
+
 /* Grid copy assignment (used at solvers.hpp:493,501-502 and integrator_main.cpp:43-44) */
 void og_grid_copy(og_grid* dst, const og_grid* src) {
   size_t n = (size_t)src->nxt * src->nyt;
   dst->nx = src->nx; dst->ny = src->ny; dst->ng = src->ng; dst->nxt = src->nxt; dst->nyt = src->nyt;
   dst->dx = src->dx; dst->dy = src->dy; dst->eta = src->eta;
+  dst->bc_x = src->bc_x; dst->bc_y = src->bc_y;
   for (int k = 0; k < OG_NFIELDS; ++k) memcpy(dst->f[k], src->f[k], n * sizeof(double));
 }
 
@@ -140,16 +159,45 @@ static void fill_one_field(og_grid* g, double* f) {
   const int nx = g->nx, ny = g->ny, ng = g->ng, nxt = g->nxt, nyt = g->nyt;
   for (int j = ng; j < nyt - ng; ++j) {
     for (int gg = 0; gg < ng; ++gg) {
-      f[IDX(g, gg, j)] = f[IDX(g, nx + gg, j)];
-      f[IDX(g, nx + ng + gg, j)] = f[IDX(g, ng + gg, j)];
+      f[IDX(g, gg, j)] = g->bc_x ? f[IDX(g, ng, j)] : f[IDX(g, nx + gg, j)];
+      f[IDX(g, nx + ng + gg, j)] = g->bc_x ? f[IDX(g, nx + ng - 1, j)] : f[IDX(g, ng + gg, j)];
     }
   }
   for (int i = 0; i < nxt; ++i) {
     for (int gg = 0; gg < ng; ++gg) {
-      f[IDX(g, i, gg)] = f[IDX(g, i, ny + gg)];
-      f[IDX(g, i, ny + ng + gg)] = f[IDX(g, i, ng + gg)];
+      f[IDX(g, i, gg)] = g->bc_y ? f[IDX(g, i, ng)] : f[IDX(g, i, ny + gg)];
+      f[IDX(g, i, ny + ng + gg)] = g->bc_y ? f[IDX(g, i, ny + ng - 1)] : f[IDX(g, i, ng + gg)];
     }
   }
+}
+
+/* Ghosts of a cell-centred B array after og_compute_cell_b on an outflow
+ * axis: computeCellB already formed every ghost cell from its faces except
+ * the last padded column (row), which takes its neighbour's value; a
+ * periodic axis copies as fill_one_field does. */
+static void fill_cell_b(og_grid* g, double* f) {
+  const int nx = g->nx, ny = g->ny, ng = g->ng, nxt = g->nxt, nyt = g->nyt;
+  for (int j = ng; j < nyt - ng; ++j) {
+    for (int gg = 0; gg < ng; ++gg) {
+      if (g->bc_x) {
+        f[IDX(g, nxt - 1, j)] = f[IDX(g, nxt - 2, j)];
+      } else {
+        f[IDX(g, gg, j)] = f[IDX(g, nx + gg, j)];
+        f[IDX(g, nx + ng + gg, j)] = f[IDX(g, ng + gg, j)];
+      }
+    }
+  }
+  for (int i = 0; i < nxt; ++i) {
+    for (int gg = 0; gg < ng; ++gg) {
+      if (g->bc_y) {
+        f[IDX(g, i, nyt - 1)] = f[IDX(g, i, nyt - 2)];
+      } else {
+        f[IDX(g, i, gg)] = f[IDX(g, i, ny + gg)];
+        f[IDX(g, i, ny + ng + gg)] = f[IDX(g, i, ng + gg)];
+      }
+    }
+  }
+  if (g->bc_x) for (int j = 0; j < nyt; ++j) f[IDX(g, nxt - 1, j)] = f[IDX(g, nxt - 2, j)];
 }
 
 /* Grid::computeCellB, Grid.hpp:82-97 */
@@ -162,8 +210,9 @@ void og_compute_cell_b(og_grid* g) {
   }
 }
 
-/* Grid::fillGhostPeriodic, Grid.hpp:64-80 */
-void og_fill_ghost_periodic(og_grid* g) {
+/* Grid::fillGhostPeriodic, Grid.hpp:64-80, with the grid's boundary rule
+ * (identical to the reference's on periodic axes) */
+void og_fill_ghost(og_grid* g) {
   fill_one_field(g, g->RHO);
   fill_one_field(g, g->MXF);
   fill_one_field(g, g->MYF);
@@ -171,9 +220,17 @@ void og_fill_ghost_periodic(og_grid* g) {
   fill_one_field(g, g->BXF);
   fill_one_field(g, g->BYF);
   og_compute_cell_b(g);
-  fill_one_field(g, g->BXC);
-  fill_one_field(g, g->BYC);
+  if (g->bc_x || g->bc_y) {
+    fill_cell_b(g, g->BXC);
+    fill_cell_b(g, g->BYC);
+  } else {
+    fill_one_field(g, g->BXC);
+    fill_one_field(g, g->BYC);
+  }
 }
+
+/* the reference's name; periodic grids are filled exactly as Grid.hpp:64-80 */
+void og_fill_ghost_periodic(og_grid* g) { og_fill_ghost(g); }
 
 /* Grid::computePrimitives, Grid.hpp:100-136.  Velocities use the density
  * before the floor; the floor then mutates rho; the E re-sync block at
@@ -423,8 +480,10 @@ void og_time_derivatives(og_ws* w, const og_grid* g, double* const d[6]) {
       d[3][cur] = -((w->fl[FL_E_H][cur] - w->fl[FL_E_H][lft]) / dx + (w->fl[FL_E_V][cur] - w->fl[FL_E_V][bot]) / dy);
       d[1][cur] = -((w->fl[FL_MX_H][cur] - w->fl[FL_MX_H][lft]) / dx + (w->fl[FL_MX_V][cur] - w->fl[FL_MX_V][bot]) / dy);
       d[2][cur] = -((w->fl[FL_MY_H][cur] - w->fl[FL_MY_H][lft]) / dx + (w->fl[FL_MY_V][cur] - w->fl[FL_MY_V][bot]) / dy);
-      int ir = (i + 1) % nx;
-      int jt = (j + 1) % ny;
+      /* periodic: the wrapped corner (solvers.hpp:278-290); outflow: the
+       * boundary corner line, computed from the ghost-filled fluxes */
+      int ir = g->bc_x ? i + 1 : (i + 1) % nx;
+      int jt = g->bc_y ? j + 1 : (j + 1) % ny;
       int rgt2 = IDX(g, ir + 2, j + 2);
       int top2 = IDX(g, i + 2, jt + 2);
       d[4][cur] = -(g->EZF[top2] - g->EZF[cur]) / dy;
@@ -435,7 +494,7 @@ void og_time_derivatives(og_ws* w, const og_grid* g, double* const d[6]) {
 
 /* explicit_rhs, solvers.hpp:393-402 */
 void og_explicit_rhs(og_grid* g, og_ws* w, double* const d[6]) {
-  og_fill_ghost_periodic(g);
+  og_fill_ghost(g);
   og_compute_primitives(g);
   og_muscl_all(g, w);
   og_flux_population(g, w);
@@ -459,13 +518,14 @@ void og_generate_A_matrix(const og_grid* g, og_ws* w, double alpha) {
   for (int j = 0; j < ys; j++) {
     for (int i = 0; i < xs; i++) {
       int c = j * xs + i;
-      if (j == 0) t[nt++] = (triplet){c, (ys - 1) * xs + i, -cy};
+      /* a neighbour past an outflow boundary is the corner itself (Neumann) */
+      if (j == 0) t[nt++] = (triplet){c, g->bc_y ? c : (ys - 1) * xs + i, -cy};
       else t[nt++] = (triplet){c, c - xs, -cy};
-      if (j == ys - 1) t[nt++] = (triplet){c, i, -cy};
+      if (j == ys - 1) t[nt++] = (triplet){c, g->bc_y ? c : i, -cy};
       else t[nt++] = (triplet){c, c + xs, -cy};
-      if (i == 0) t[nt++] = (triplet){c, c + xs - 1, -cx};
+      if (i == 0) t[nt++] = (triplet){c, g->bc_x ? c : c + xs - 1, -cx};
       else t[nt++] = (triplet){c, c - 1, -cx};
-      if (i == xs - 1) t[nt++] = (triplet){c, c - xs + 1, -cx};
+      if (i == xs - 1) t[nt++] = (triplet){c, g->bc_x ? c : c - xs + 1, -cx};
       else t[nt++] = (triplet){c, c + 1, -cx};
       t[nt++] = (triplet){c, c, 1 + 2 * (cx + cy)};
     }
@@ -605,8 +665,8 @@ void og_resistive_term(const og_grid* g, og_ws* w, int implicit, double* bx_rate
   if (err) *err = e;
   for (int j = 0; j < ny; j++) {
     for (int i = 0; i < nx; i++) {
-      int ir = (i + 1) % nx;
-      int jt = (j + 1) % ny;
+      int ir = g->bc_x ? clamp_index(i + 1, nx) : (i + 1) % nx;
+      int jt = g->bc_y ? clamp_index(j + 1, ny) : (j + 1) % ny;
       double ec = ez[j * nx + i], et = ez[jt * nx + i], er = ez[j * nx + ir];
       int gidx = IDX(g, i + 2, j + 2);
       bx_rate[gidx] = -(et - ec) / g->dy;
@@ -656,13 +716,13 @@ void og_imex(og_grid* gi, og_ws* w, double dt, og_step_stats* st) {
   axpy_phys(gi, inter->BYF, f1[5], ce1);
   axpy_phys(gi, inter->BXF, w->g1[0], ci1);
   axpy_phys(gi, inter->BYF, w->g1[1], ci1);
-  og_fill_ghost_periodic(inter);
+  og_fill_ghost(inter);
   og_compute_primitives(inter);
   og_resistive_term(inter, w, 1, w->solved1[0], w->solved1[1],
                     &st->cg_iters[0], &st->cg_spmv[0], &st->cg_error[0]);   /* :526 */
   axpy_phys(gi, inter->BXF, w->solved1[0], diag_coeff);
   axpy_phys(gi, inter->BYF, w->solved1[1], diag_coeff);
-  og_fill_ghost_periodic(inter);
+  og_fill_ghost(inter);
   og_compute_primitives(inter);
   og_explicit_rhs(inter, w, f2);                             /* :534-535 */
   og_resistive_term(inter, w, 0, w->g2[0], w->g2[1], NULL, NULL, NULL);   /* :537 */
@@ -684,13 +744,13 @@ void og_imex(og_grid* gi, og_ws* w, double dt, og_step_stats* st) {
   axpy_phys(gi, whole->BYF, w->g1[1], ci2);
   axpy_phys(gi, whole->BXF, w->g2[0], ci3);
   axpy_phys(gi, whole->BYF, w->g2[1], ci3);
-  og_fill_ghost_periodic(whole);
+  og_fill_ghost(whole);
   og_compute_primitives(whole);
   og_resistive_term(whole, w, 1, w->solved2[0], w->solved2[1],
                     &st->cg_iters[1], &st->cg_spmv[1], &st->cg_error[1]);   /* :571 */
   axpy_phys(gi, whole->BXF, w->solved2[0], diag_coeff);
   axpy_phys(gi, whole->BYF, w->solved2[1], diag_coeff);
-  og_fill_ghost_periodic(whole);
+  og_fill_ghost(whole);
   og_compute_primitives(whole);
   og_resistive_term(whole, w, 0, w->g3[0], w->g3[1], NULL, NULL, NULL);   /* :579 */
 
@@ -711,7 +771,7 @@ void og_imex(og_grid* gi, og_ws* w, double dt, og_step_stats* st) {
   axpy_phys(gi, gi->BYF, w->g2[1], g2c);
   axpy_phys(gi, gi->BXF, w->g3[0], g3c);
   axpy_phys(gi, gi->BYF, w->g3[1], g3c);
-  og_fill_ghost_periodic(gi);
+  og_fill_ghost(gi);
   og_compute_primitives(gi);
   og_grid_destroy(initial);
 }
@@ -949,7 +1009,7 @@ double* og_ws_flux(og_ws* w, int which) { return (which >= 0 && which < FL_COUNT
 int og_run(og_grid* g, og_ws* w, double t_end, double dt_override, double cfl,
            int max_steps, int snapshot_every, double* step_dt, int* iters, double* t_out) {
   const double A_im4 = (2.0 - sqrt(2.0)) / 2.0;
-  og_fill_ghost_periodic(g);                                 /* :183-184 */
+  og_fill_ghost(g);                                          /* :183-184 */
   og_compute_primitives(g);
   double opt = og_cfl(g, cfl);                               /* :186 */
   double dt0 = (dt_override == 0.0) ? opt : dt_override;
@@ -977,7 +1037,7 @@ int og_run(og_grid* g, og_ws* w, double t_end, double dt_override, double cfl,
     if (iters) { iters[2 * steps] = st.cg_iters[0]; iters[2 * steps + 1] = st.cg_iters[1]; }
     steps++;
     if (snapshot_every > 0 && frame % snapshot_every == 0) {   /* :233-241 */
-      og_fill_ghost_periodic(g);
+      og_fill_ghost(g);
       og_compute_primitives(g);
     }
   }

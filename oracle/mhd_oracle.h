@@ -52,6 +52,9 @@ void og_grid_destroy(og_grid* g);
 double* og_field(og_grid* g, int which);
 void og_grid_copy(og_grid* dst, const og_grid* src);
 void og_grid_set_eta(og_grid* g, double eta);
+/* 0 periodic (the reference), 1 outflow (zero-gradient ghost rule) per axis */
+void og_grid_set_bc(og_grid* g, int bc_x, int bc_y);
+void og_fill_ghost(og_grid* g);
 void og_fill_ghost_periodic(og_grid* g);
 void og_compute_cell_b(og_grid* g);
 void og_compute_primitives(og_grid* g);

@@ -44,6 +44,8 @@ def _bind(lib: C.CDLL) -> C.CDLL:
         "og_grid_copy": (None, [vp, vp]),
         "og_grid_set_eta": (None, [vp, C.c_double]),
         "og_fill_ghost_periodic": (None, [vp]),
+        "og_fill_ghost": (None, [vp]),
+        "og_grid_set_bc": (None, [vp, C.c_int, C.c_int]),
         "og_compute_cell_b": (None, [vp]),
         "og_compute_primitives": (None, [vp]),
         "og_ws_create": (vp, [C.c_int, C.c_int]),
@@ -105,12 +107,17 @@ class Sim:
     """One reference Grid plus its integrator workspace (oracle side)."""
 
     def __init__(self, nx: int, ny: int, dx: float, dy: float, eta: float = 0.0,
-                 variant: str = "strict", lib: C.CDLL | None = None):
+                 variant: str = "strict", lib: C.CDLL | None = None, bc: tuple = ("periodic", "periodic")):
+        """bc: boundary of the x and y axes, "periodic" (the reference's) or
+        "outflow" (the zero-gradient ghost rule of og_grid_set_bc)."""
         self.lib = lib or load(variant)
         self.nx, self.ny = nx, ny
         self.nxt, self.nyt = nx + 4, ny + 4
         self.dx, self.dy, self.eta = dx, dy, eta
+        codes = {"periodic": 0, "outflow": 1}
+        bcx, bcy = (codes[b] for b in bc)
         self.g = self.lib.og_grid_create(nx, ny, dx, dy, eta)
+        self.lib.og_grid_set_bc(self.g, bcx, bcy)
         self.w = self.lib.og_ws_create(nx, ny)
 
     def close(self):
@@ -141,7 +148,7 @@ class Sim:
         self.lib.og_ws_set_cg(self.w, tol, max_iter)
 
     def fill_ghost(self):
-        self.lib.og_fill_ghost_periodic(self.g)
+        self.lib.og_fill_ghost(self.g)
 
     def primitives(self):
         self.lib.og_compute_primitives(self.g)
