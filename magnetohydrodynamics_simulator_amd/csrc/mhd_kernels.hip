@@ -52,8 +52,19 @@ struct Carry {
   double bxl_b, bxl_a;                        // bxf(c, k-1), bxf(c, k-2)
 };
 
-__device__ __forceinline__ double sh_up(double v) { return __shfl_up(v, 1, 64); }     // value of lane l-1
-__device__ __forceinline__ double sh_dn(double v) { return __shfl_down(v, 1, 64); }   // value of lane l+1
+// One-lane shifts across the whole 64-lane wavefront as DPP moves (wave_shr:1,
+// wave_shl:1) on the two halves of the double: VALU moves instead of LDS
+// permutes.  The lane without a source (lane 0 up, lane 63 down) keeps its own
+// value, as __shfl_up / __shfl_down(v, 1, 64) do; those lanes are stencil halo.
+constexpr int kDppWaveShl1 = 0x130, kDppWaveShr1 = 0x138;
+template <int CTRL>
+__device__ __forceinline__ double wave_shift(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(__double2loint(v), __double2loint(v), CTRL, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(__double2hiint(v), __double2hiint(v), CTRL, 0xf, 0xf, false);
+  return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double sh_up(double v) { return wave_shift<kDppWaveShr1>(v); }   // value of lane l-1
+__device__ __forceinline__ double sh_dn(double v) { return wave_shift<kDppWaveShl1>(v); }   // value of lane l+1
 
 // One wavefront's work: strip `strip` (columns strip * 60 ...), segment `seg`
 // (rows seg * a.rows ...).
