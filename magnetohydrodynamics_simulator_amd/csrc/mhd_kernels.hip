@@ -636,27 +636,31 @@ __device__ __forceinline__ void cg_pass_b_body(const CgArgs& a, const double* __
   const double d = a.diag, cx = a.cx, cy = a.cy;
   auto at = [&](int col, int j) -> int { return wrap_row(j, ny, (OPEN ? a.open_y : 0)) * nx + col; };
   double acc[2] = {0.0, 0.0};
-  double pm = pv[at(t.ic, t.y0 - 1)];
-  double pc = pv[at(t.ic, t.y0)];
-  double pn = pv[at(t.ic, t.y0 + 1)];
-  double xc = FIRST ? 0.0 : a.x[t.y0 * nx + t.ic];
-  double rc = a.r[t.y0 * nx + t.ic];
-  double hl = left ? pv[at(xl, t.y0)] : 0.0;
-  double hr = right ? pv[at(xr, t.y0)] : 0.0;
-  for (int j = t.y0; j < t.y1; ++j) {
-    const bool more = j + 1 < t.y1;
-    // loads for the next row
-    const double pnn = more ? pv[at(t.ic, j + 2)] : 0.0;
-    const double xn = (!FIRST && more) ? a.x[(j + 1) * nx + t.ic] : 0.0;
-    const double rnx = more ? a.r[(j + 1) * nx + t.ic] : 0.0;
-    const double hl_n = (left && more) ? pv[at(xl, j + 1)] : 0.0;
-    const double hr_n = (right && more) ? pv[at(xr, j + 1)] : 0.0;
-    const int buf = (j - t.y0) & 1;
+  // rows are swept downwards (pass A sweeps upwards), so that this pass first
+  // reads the rows of p and r that the previous pass wrote last
+  const int j0 = t.y1 - 1, nrow = t.y1 - t.y0;
+  double pa = pv[at(t.ic, j0 + 1)];           // north of the current row
+  double pc = pv[at(t.ic, j0)];
+  double pb = pv[at(t.ic, j0 - 1)];           // south
+  double xc = FIRST ? 0.0 : a.x[j0 * nx + t.ic];
+  double rc = a.r[j0 * nx + t.ic];
+  double hl = left ? pv[at(xl, j0)] : 0.0;
+  double hr = right ? pv[at(xr, j0)] : 0.0;
+  for (int k = 0; k < nrow; ++k) {
+    const int j = j0 - k;
+    const bool more = k + 1 < nrow;
+    // loads for the next row (j - 1)
+    const double pbb = more ? pv[at(t.ic, j - 2)] : 0.0;
+    const double xn = (!FIRST && more) ? a.x[(j - 1) * nx + t.ic] : 0.0;
+    const double rnx = more ? a.r[(j - 1) * nx + t.ic] : 0.0;
+    const double hl_n = (left && more) ? pv[at(xl, j - 1)] : 0.0;
+    const double hr_n = (right && more) ? pv[at(xr, j - 1)] : 0.0;
+    const int buf = k & 1;
     if (t.valid) prow[buf][tid + 1] = pc;
     if (left) prow[buf][0] = hl;
     if (right) prow[buf][w + 1] = hr;
     __syncthreads();
-    const double q = stencil(pm, prow[buf][tid], pc, prow[buf][tid + 2], pn, cx, cy, d);
+    const double q = stencil(pb, prow[buf][tid], pc, prow[buf][tid + 2], pa, cx, cy, d);
     if (t.valid) {
       const int o = j * nx + t.ic;
       a.x[o] = xc + alpha * pc;
@@ -666,9 +670,9 @@ __device__ __forceinline__ void cg_pass_b_body(const CgArgs& a, const double* __
       acc[0] += rn * rn;
       acc[1] += rn * z;
     }
-    pm = pc;
-    pc = pn;
-    pn = pnn;
+    pa = pc;
+    pc = pb;
+    pb = pbb;
     xc = xn;
     rc = rnx;
     hl = hl_n;
