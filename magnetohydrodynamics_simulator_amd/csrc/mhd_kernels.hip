@@ -550,19 +550,19 @@ __device__ __forceinline__ void cg_pass_a_body(const CgArgs& a, const double* __
   };
   auto at = [&](int col, int j) -> int { return wrap_row(j, ny, oy) * nx + col; };
   double acc[1] = {0.0};
-  double pm = pv(pold[at(t.ic, t.y0 - 1)], a.r[at(t.ic, t.y0 - 1)], t.ic, t.y0 - 1);
-  double pc = pv(pold[at(t.ic, t.y0)], a.r[at(t.ic, t.y0)], t.ic, t.y0);
-  double n_po = pold[at(t.ic, t.y0 + 1)], n_ro = a.r[at(t.ic, t.y0 + 1)];
-  double l_po = left ? pold[at(xl, t.y0)] : 0.0, l_ro = left ? a.r[at(xl, t.y0)] : 0.0;
-  double r_po = right ? pold[at(xr, t.y0)] : 0.0, r_ro = right ? a.r[at(xr, t.y0)] : 0.0;
+  double pm = pv(__builtin_nontemporal_load(&pold[at(t.ic, t.y0 - 1)]), a.r[at(t.ic, t.y0 - 1)], t.ic, t.y0 - 1);
+  double pc = pv(__builtin_nontemporal_load(&pold[at(t.ic, t.y0)]), a.r[at(t.ic, t.y0)], t.ic, t.y0);
+  double n_po = __builtin_nontemporal_load(&pold[at(t.ic, t.y0 + 1)]), n_ro = a.r[at(t.ic, t.y0 + 1)];
+  double l_po = left ? __builtin_nontemporal_load(&pold[at(xl, t.y0)]) : 0.0, l_ro = left ? a.r[at(xl, t.y0)] : 0.0;
+  double r_po = right ? __builtin_nontemporal_load(&pold[at(xr, t.y0)]) : 0.0, r_ro = right ? a.r[at(xr, t.y0)] : 0.0;
   for (int j = t.y0; j < t.y1; ++j) {
     const bool more = j + 1 < t.y1;
     // loads for the next row
-    const double nn_po = more ? pold[at(t.ic, j + 2)] : 0.0;
+    const double nn_po = more ? __builtin_nontemporal_load(&pold[at(t.ic, j + 2)]) : 0.0;
     const double nn_ro = more ? a.r[at(t.ic, j + 2)] : 0.0;
-    const double ln_po = (left && more) ? pold[at(xl, j + 1)] : 0.0;
+    const double ln_po = (left && more) ? __builtin_nontemporal_load(&pold[at(xl, j + 1)]) : 0.0;
     const double ln_ro = (left && more) ? a.r[at(xl, j + 1)] : 0.0;
-    const double rn_po = (right && more) ? pold[at(xr, j + 1)] : 0.0;
+    const double rn_po = (right && more) ? __builtin_nontemporal_load(&pold[at(xr, j + 1)]) : 0.0;
     const double rn_ro = (right && more) ? a.r[at(xr, j + 1)] : 0.0;
     const double pn = pv(n_po, n_ro, t.ic, j + 1);
     const int buf = (j - t.y0) & 1;
@@ -642,7 +642,7 @@ __device__ __forceinline__ void cg_pass_b_body(const CgArgs& a, const double* __
   double pa = pv[at(t.ic, j0 + 1)];           // north of the current row
   double pc = pv[at(t.ic, j0)];
   double pb = pv[at(t.ic, j0 - 1)];           // south
-  double xc = FIRST ? 0.0 : a.x[j0 * nx + t.ic];
+  double xc = FIRST ? 0.0 : __builtin_nontemporal_load(&a.x[j0 * nx + t.ic]);
   double rc = a.r[j0 * nx + t.ic];
   double hl = left ? pv[at(xl, j0)] : 0.0;
   double hr = right ? pv[at(xr, j0)] : 0.0;
@@ -651,7 +651,7 @@ __device__ __forceinline__ void cg_pass_b_body(const CgArgs& a, const double* __
     const bool more = k + 1 < nrow;
     // loads for the next row (j - 1)
     const double pbb = more ? pv[at(t.ic, j - 2)] : 0.0;
-    const double xn = (!FIRST && more) ? a.x[(j - 1) * nx + t.ic] : 0.0;
+    const double xn = (!FIRST && more) ? __builtin_nontemporal_load(&a.x[(j - 1) * nx + t.ic]) : 0.0;
     const double rnx = more ? a.r[(j - 1) * nx + t.ic] : 0.0;
     const double hl_n = (left && more) ? pv[at(xl, j - 1)] : 0.0;
     const double hr_n = (right && more) ? pv[at(xr, j - 1)] : 0.0;
@@ -663,7 +663,7 @@ __device__ __forceinline__ void cg_pass_b_body(const CgArgs& a, const double* __
     const double q = stencil(pb, prow[buf][tid], pc, prow[buf][tid + 2], pa, cx, cy, d);
     if (t.valid) {
       const int o = j * nx + t.ic;
-      a.x[o] = xc + alpha * pc;
+      __builtin_nontemporal_store(xc + alpha * pc, &a.x[o]);
       const double rn = rc - alpha * q;
       a.r[o] = rn;
       const double z = invd_of<OPEN>(a, t.ic, j) * rn;
