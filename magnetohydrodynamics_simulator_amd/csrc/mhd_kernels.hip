@@ -196,10 +196,12 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, int strip, int se
         for (int q = 0; q < 6; ++q) un[q] = a.s[q][o];
 #pragma unroll
         for (int q = 0; q < 6; ++q) {
-          a.o1[q][o] = un[q] + d[q] * a.c_b;       // W = U^n + dt delta f1 (stage-3 base)
+          // W = U^n + dt delta f1 (stage-3 base): next read by stage 2
+          __builtin_nontemporal_store(un[q] + d[q] * a.c_b, &a.o1[q][o]);
           double u = un[q] + d[q] * a.c_a;         // U2* = U^n + dt gamma f1
           if (q == 0 && u < kFloor) u = kFloor;
-          a.o2[q][o] = u;
+          if (q < 4) __builtin_nontemporal_store(u, &a.o2[q][o]);
+          else a.o2[q][o] = u;                     // faces: read next by the solve's start
         }
       } else {
         double wv[6];
@@ -216,7 +218,8 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, int strip, int se
           if (q == 4 && a.eta != 0.0) u = u + g2x * a.c_g;
           if (q == 5 && a.eta != 0.0) u = u + g2y * a.c_g;
           if (q == 0 && u < kFloor) u = kFloor;
-          a.o1[q][o] = u;
+          if (q < 4) __builtin_nontemporal_store(u, &a.o1[q][o]);
+          else a.o1[q][o] = u;                     // faces: read next by the solve's start
         }
       }
     }
