@@ -431,10 +431,14 @@ __device__ __forceinline__ double invd_of(const CgArgs& a, int i, int j) {
   return invd_at(a, i, j);
 }
 
-// The 2D sweeps below march a tile upwards one row per iteration.  Global
-// loads are issued two rows ahead of their use, so each wave keeps a row of
-// loads in flight across the LDS exchange and the arithmetic of the current
-// row (the barrier only waits for LDS traffic).
+// The 2D sweeps below march a tile one row per iteration (pass B downwards,
+// the others upwards: all tiles run at once, so consecutive passes meet the
+// rows the previous pass touched last while the memory-side cache still holds
+// them).  Global loads are issued two rows ahead of their use, so each wave
+// keeps a row of loads in flight across the LDS exchange and the arithmetic
+// of the current row (the barrier only waits for LDS traffic).  Data that is
+// not read again before it would be evicted (x in pass B, p_old in pass A)
+// moves with nontemporal loads and stores.
 
 // Start of a solve: b = eta J(B) (or bin), r = b, p = invd b, and the first
 // product q = A p; partials |b|^2, b . p, p . q.
