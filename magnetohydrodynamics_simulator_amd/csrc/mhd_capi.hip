@@ -209,12 +209,13 @@ FinArgs fin_args(mhd_ctx* c, const double* bx, const double* by, double* bxo, do
   f.bx_out = bxo;
   f.by_out = byo;
   f.st = c->st;
-  f.ty = c->tile_rows;
-  f.tiles_y = c->tiles_y;
+  f.ty = c->fin_rows;
+  f.tiles_y = c->fin_tiles_y;
   return f;
 }
 
-DiagArgs diag_args(mhd_ctx* c) {
+// fin: the tiles of the CFL partials (else those of the CG sweeps)
+DiagArgs diag_args(mhd_ctx* c, bool fin) {
   DiagArgs d{};
   d.nx = c->nx;
   d.ny = c->ny;
@@ -224,18 +225,19 @@ DiagArgs diag_args(mhd_ctx* c) {
   d.dy = c->dy;
   for (int q = 0; q < 6; ++q) d.s[q] = c->U[q];
   d.part = c->part;
-  d.ty = c->tile_rows;
-  d.tiles_y = c->tiles_y;
+  d.ty = fin ? c->fin_rows : c->tile_rows;
+  d.tiles_y = fin ? c->fin_tiles_y : c->tiles_y;
   return d;
 }
 
 // Launches a per-tile diagnostic kernel and copies its partials to the host.
 template <typename K>
-int diag_partials(mhd_ctx* c, K kernel, int id) {
-  DiagArgs d = diag_args(c);
-  int rc = launch_k(c, id, kernel, dim3(c->tiles), dim3(kTileX), d);
+int diag_partials(mhd_ctx* c, K kernel, int id, bool fin) {
+  DiagArgs d = diag_args(c, fin);
+  const int nb = fin ? c->fin_tiles : c->tiles;
+  int rc = launch_k(c, id, kernel, dim3(nb), dim3(kTileX), d);
   if (rc) return rc;
-  HIP_OR_FAIL(c, hipMemcpyAsync(c->part_host, c->part, 2 * sizeof(double) * c->tiles, hipMemcpyDeviceToHost, c->stream));
+  HIP_OR_FAIL(c, hipMemcpyAsync(c->part_host, c->part, 2 * sizeof(double) * nb, hipMemcpyDeviceToHost, c->stream));
   HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
   return MHD_OK;
 }
@@ -338,7 +340,7 @@ int imex_step(mhd_ctx* c, double dt, mhd_step_stats* stats, StepRecord* rec) {
         if (rec->spec[0] == 0 && rec->st[0]) HIP_OR_FAIL(c, hipMemcpyAsync(rec->st[0], c->st, sizeof(CgState), hipMemcpyDeviceToDevice, c->stream));
       }
     }
-    rc = launch_k(c, 5, k_finalize1, dim3(c->tiles), dim3(kTileX), f);
+    rc = launch_k(c, 5, k_finalize1, dim3(c->fin_tiles), dim3(kTileX), f);
     if (rc) return rc;
   }
   if (rec && rec->S[0]) {
@@ -379,7 +381,7 @@ int imex_step(mhd_ctx* c, double dt, mhd_step_stats* stats, StepRecord* rec) {
         if (rec->spec[1] == 0 && rec->st[1]) HIP_OR_FAIL(c, hipMemcpyAsync(rec->st[1], c->st, sizeof(CgState), hipMemcpyDeviceToDevice, c->stream));
       }
     }
-    rc = launch_k(c, 6, k_finalize2, dim3(c->tiles), dim3(kTileX), f);
+    rc = launch_k(c, 6, k_finalize2, dim3(c->fin_tiles), dim3(kTileX), f);
     if (rc) return rc;
     std::swap(c->U[4], c->S[4]);
     std::swap(c->U[5], c->S[5]);
@@ -456,15 +458,23 @@ int mhd_create(int nx, int ny, double dx, double dy, double eta, const mhd_opts*
     c->tile_rows = rows;
     c->tiles_y = (ny + rows - 1) / rows;
     c->tiles = tiles_x * c->tiles_y;
+    int fin_y = kFinTargetTiles / tiles_x;
+    if (fin_y < 1) fin_y = 1;
+    int fin_rows = (ny + fin_y - 1) / fin_y;
+    if (fin_rows < 4) fin_rows = 4;
+    c->fin_rows = fin_rows;
+    c->fin_tiles_y = (ny + fin_rows - 1) / fin_rows;
+    c->fin_tiles = tiles_x * c->fin_tiles_y;
   }
   const size_t pb = (size_t)c->tiles * 2 * sizeof(double);
+  const size_t pd = (size_t)std::max(c->tiles, c->fin_tiles) * 2 * sizeof(double);   // part, part_host
   if (hipMalloc(&c->x, fb) != hipSuccess || hipMalloc(&c->r, fb) != hipSuccess || hipMalloc(&c->p[0], fb) != hipSuccess ||
-      hipMalloc(&c->p[1], fb) != hipSuccess || hipMalloc(&c->part, pb) != hipSuccess ||
+      hipMalloc(&c->p[1], fb) != hipSuccess || hipMalloc(&c->part, pd) != hipSuccess ||
       hipMalloc(&c->part_s, pb * 2) != hipSuccess || hipMalloc(&c->part_a, pb) != hipSuccess ||
       hipMalloc(&c->part_b, pb) != hipSuccess ||
       hipMalloc(&c->st, sizeof(CgState)) != hipSuccess)
     return bail("device allocation failed", MHD_ENOMEM);
-  if (hipHostMalloc(&c->st_host, sizeof(CgState)) != hipSuccess || hipHostMalloc(&c->part_host, pb) != hipSuccess ||
+  if (hipHostMalloc(&c->st_host, sizeof(CgState)) != hipSuccess || hipHostMalloc(&c->part_host, pd) != hipSuccess ||
       hipHostMalloc(&c->stage, fb) != hipSuccess)
     return bail("pinned allocation failed", MHD_ENOMEM);
   for (int q = 0; q < 6; ++q) {
@@ -565,7 +575,7 @@ int mhd_download(mhd_ctx* c, double* rho, double* mx, double* my, double* E, dou
   bool any = false;
   for (double* q : pd) any = any || q;
   if (any) {
-    DiagArgs d = diag_args(c);
+    DiagArgs d = diag_args(c, false);
     int rc = launch(c, 8, [&] {
       hipLaunchKernelGGL(k_prims, dim3(grid_1d(c->n)), dim3(256), 0, c->stream, d, c->F1[0], c->F1[1], c->F1[2], c->F1[3],
                          c->F1[4]);
@@ -612,11 +622,11 @@ int mhd_download_device(mhd_ctx* c, double* const* fields) {
 }
 
 static int max_speed(mhd_ctx* c, double* maxs, double* nbad) {
-  int rc = diag_partials(c, k_cfl, 7);
+  int rc = diag_partials(c, k_cfl, 7, true);
   if (rc) return rc;
   // exact: a max (NaN skipped as the reference's `>` skips it) and a count
   double m = 1e-8, bad = 0.0;
-  for (int b = 0; b < c->tiles; ++b) {
+  for (int b = 0; b < c->fin_tiles; ++b) {
     const double v = c->part_host[2 * b];
     if (v > m) m = v;
     bad += c->part_host[2 * b + 1];
@@ -720,7 +730,7 @@ int mhd_cg_solve(mhd_ctx* c, const double* b, double* x, int* iters, double* err
 
 int mhd_divb(mhd_ctx* c, double* rms, double* maxabs) {
   if (check_ctx(c) || !rms || !maxabs) return fail(c, MHD_EINVAL, "mhd_divb: bad argument");
-  int rc = diag_partials(c, k_divb, 8);
+  int rc = diag_partials(c, k_divb, 8, false);
   if (rc) return rc;
   double sum = 0.0, mx = 0.0;
   for (int b = 0; b < c->tiles; ++b) {      // fixed order: deterministic

@@ -78,6 +78,9 @@ struct mhd_ctx {
   int last_spmv[4] = {1, 1, 1, 1};   // products of the previous solve per slot (0, 1: the step's
                                       // solves; 2, 3: the adjoint's)
   int tiles = 1, tiles_y = 1, tile_rows = 4;   // 2D sweeps: kTileX columns x tile_rows rows per tile
+  // the recombinations and the CFL partials: about kFinTargetTiles tiles (their
+  // partials are a max and a count, exact in any grouping)
+  int fin_tiles = 1, fin_tiles_y = 1, fin_rows = 4;
   int stage_rows = 16;
   std::string err;
   unsigned prof_mask = 0;        // kernel ids whose launches are bracketed by events
@@ -148,7 +151,7 @@ int grid_1d(long long n);
 mhd::StageArgs stage_args(mhd_ctx* c);
 mhd::CgArgs cg_args(mhd_ctx* c, const double* bx, const double* by);
 mhd::FinArgs fin_args(mhd_ctx* c, const double* bx, const double* by, double* bxo, double* byo, double coef);
-mhd::DiagArgs diag_args(mhd_ctx* c);
+mhd::DiagArgs diag_args(mhd_ctx* c, bool fin);
 // One CG solve (Eigen's algorithm) of A x = eta J(bx, by) or A x = bin into
 // c->x.  spec == 0: chunks predicted from the slot's previous solve, the host
 // reads the state after each chunk until the solve is done, *out receives it.

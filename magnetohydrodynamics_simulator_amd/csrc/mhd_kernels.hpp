@@ -12,6 +12,10 @@ constexpr int kTileX = 256;       // columns (= threads) per tile of the 2D swee
 #define MHD_TARGET_TILES 2048
 #endif
 constexpr int kTargetTiles = MHD_TARGET_TILES;   // tiles per 2D sweep (compile-time tuning parameter)
+#ifndef MHD_FIN_TARGET_TILES
+#define MHD_FIN_TARGET_TILES 1024
+#endif
+constexpr int kFinTargetTiles = MHD_FIN_TARGET_TILES;   // tiles of the recombinations and CFL partials
 
 // Explicit-stage kernel arguments.  MODE 0: o1 = L(s).
 // MODE 1 (f1 = L(s)): o1 = W = s + c_b f1, o2 = U2* = s + c_a f1.

@@ -192,7 +192,7 @@ int enqueue_step(mhd_ctx* c, Loop& L, int from) {
   if (res && from <= 3) {
     FinArgs f = fin_args(c, c->S[4], c->S[5], c->S[4], c->S[5], 0.0);
     f.st = R->st;
-    rc = launch_k(c, 5, k_finalize1_res, dim3(c->tiles), dim3(kTileX), f, R->rs, c->o.cg_fail_is_error);
+    rc = launch_k(c, 5, k_finalize1_res, dim3(c->fin_tiles), dim3(kTileX), f, R->rs, c->o.cg_fail_is_error);
     if (rc) return rc;
   }
   if (from <= 4) {
@@ -216,18 +216,18 @@ int enqueue_step(mhd_ctx* c, Loop& L, int from) {
     // stage 2; the kernel forms the new B itself)
     FinArgs f = fin_args(c, c->U[4], c->U[5], c->S[4], c->S[5], 0.0);
     f.st = R->st + 1;
-    DiagArgs d = diag_args(c);
-    rc = L.fixed_dt ? launch_k(c, 6, k_finalize2_res<false>, dim3(c->tiles), dim3(kTileX), f, d, R->rs,
+    DiagArgs d = diag_args(c, true);
+    rc = L.fixed_dt ? launch_k(c, 6, k_finalize2_res<false>, dim3(c->fin_tiles), dim3(kTileX), f, d, R->rs,
                                (const CgState*)R->st, R->log, kLogCap, c->o.cg_fail_is_error)
-                    : launch_k(c, 6, k_finalize2_res<true>, dim3(c->tiles), dim3(kTileX), f, d, R->rs,
+                    : launch_k(c, 6, k_finalize2_res<true>, dim3(c->fin_tiles), dim3(kTileX), f, d, R->rs,
                                (const CgState*)R->st, R->log, kLogCap, c->o.cg_fail_is_error);
     if (rc) return rc;
     std::swap(c->U[4], c->S[4]);
     std::swap(c->U[5], c->S[5]);
     return MHD_OK;
   }
-  DiagArgs d = diag_args(c);
-  return launch_k(c, 7, k_cfl_res, dim3(L.fixed_dt ? 1 : c->tiles), dim3(kTileX), d, (const RunState*)R->rs,
+  DiagArgs d = diag_args(c, true);
+  return launch_k(c, 7, k_cfl_res, dim3(L.fixed_dt ? 1 : c->fin_tiles), dim3(kTileX), d, (const RunState*)R->rs,
                   (const CgState*)R->st, R->log, kLogCap, L.fixed_dt ? 0 : 1);
 }
 
@@ -291,8 +291,8 @@ int rebase(mhd_ctx* c, Loop& L, long long done) {
   }
   L.base = done;
   if (L.fixed_dt) return MHD_OK;
-  DiagArgs d = diag_args(c);
-  return launch_k(c, 7, k_cfl, dim3(c->tiles), dim3(kTileX), d);
+  DiagArgs d = diag_args(c, true);
+  return launch_k(c, 7, k_cfl, dim3(c->fin_tiles), dim3(kTileX), d);
 }
 
 // A halted step (solve `slot` unfinished after A(spec + 1)): continue the
@@ -370,7 +370,7 @@ int mhd_run_resident(mhd_ctx* c, double t_end, double dt_override, int max_steps
   g.strips = (c->nx + kStripOut - 1) / kStripOut;
   g.ty = c->tile_rows;
   g.tiles_y = c->tiles_y;
-  g.tiles = c->tiles;
+  g.tiles = c->fin_tiles;                    // kb_dt reduces the CFL partials
   g.part_base = 0;
   g.dx = c->dx;
   g.dy = c->dy;
