@@ -486,7 +486,11 @@ int mhd_create(int nx, int ny, double dx, double dy, double eta, const mhd_opts*
   // stage kernel: one wavefront per strip segment of stage_rows rows.  Pick
   // the segment length so that the launch fills the resident wavefront slots
   // of the device a whole number of times (a partial last round would idle
-  // most of the chip), with segments of at least 32 rows.
+  // most of the chip), with segments of at least 32 rows; a grid large enough
+  // to fill the slots with 32-row segments uses segments of at most 32 rows
+  // (measured at 4096^2: 31-32 rows in three rounds run both stages 6-7 %
+  // faster than 92 rows in one round -- the waves running at once then cover
+  // a narrower band of rows).
   if (o.march_rows > 0) {
     c->stage_rows = o.march_rows;
   } else {
@@ -504,11 +508,13 @@ int mhd_create(int nx, int ny, double dx, double dy, double eta, const mhd_opts*
       if (segs < 1) continue;
       long long rows = (ny + segs - 1) / segs;
       if (rows < 32 && rounds > 1) break;
+      if (rows > 32 && strips * ((ny + 31) / 32) >= slots) continue;
       long long waves = strips * ((ny + rows - 1) / rows);
       long long r_used = (waves + slots - 1) / slots;
       long long cost = r_used * (rows + 4);   // row steps of the slowest slot
       if (best_cost < 0 || cost < best_cost) { best_cost = cost; best_rows = rows; }
     }
+    if (best_cost < 0) best_rows = ny < 32 ? ny : 32;   // no whole-round candidate: 32-row segments
     c->stage_rows = (int)(best_rows < 4 ? 4 : best_rows);
   }
   if (hipStreamSynchronize(c->stream) != hipSuccess) return bail("device init failed", MHD_EHIP);
