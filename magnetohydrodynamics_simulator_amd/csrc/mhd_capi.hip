@@ -488,9 +488,9 @@ int mhd_create(int nx, int ny, double dx, double dy, double eta, const mhd_opts*
   // of the device a whole number of times (a partial last round would idle
   // most of the chip), with segments of at least 32 rows; a grid large enough
   // to fill the slots with 32-row segments uses segments of at most 32 rows
-  // (measured at 4096^2: 31-32 rows in three rounds run both stages 6-7 %
-  // faster than 92 rows in one round -- the waves running at once then cover
-  // a narrower band of rows).
+  // (measured at 4096^2: 32-row segments in about three rounds run both
+  // stages 6-7 % faster than 92-row segments in one round -- the waves running
+  // at once then cover a narrower band of rows).
   if (o.march_rows > 0) {
     c->stage_rows = o.march_rows;
   } else {
@@ -514,7 +514,7 @@ int mhd_create(int nx, int ny, double dx, double dy, double eta, const mhd_opts*
       long long cost = r_used * (rows + 4);   // row steps of the slowest slot
       if (best_cost < 0 || cost < best_cost) { best_cost = cost; best_rows = rows; }
     }
-    if (best_cost < 0) best_rows = ny < 32 ? ny : 32;   // no whole-round candidate: 32-row segments
+    if (best_cost < 0) best_rows = ny < 32 ? ny : 32;   // no candidate left by the rules above
     c->stage_rows = (int)(best_rows < 4 ? 4 : best_rows);
   }
   if (hipStreamSynchronize(c->stream) != hipSuccess) return bail("device init failed", MHD_EHIP);
