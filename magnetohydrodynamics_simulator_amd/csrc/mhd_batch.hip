@@ -78,6 +78,7 @@ void plan(mhd_batch* b, std::vector<int2>& stage_jobs, std::vector<int2>& tile_j
 // Lays the runs out with the given launch shape and uploads geometry and job
 // tables (the schedule fields of the geometry are kept).
 int replan(mhd_batch* b, int stage_rows, int tile_rows) {
+  b->cfl_fresh = false;
   b->stage_rows = stage_rows;
   b->tile_rows = tile_rows;
   std::vector<int2> sj, tj;
@@ -360,6 +361,7 @@ int mhd_batch_destroy(mhd_batch* b) {
 const char* mhd_batch_last_error(const mhd_batch* b) { return b ? b->err.c_str() : g_batch_create_error.c_str(); }
 
 int mhd_batch_set_schedule(mhd_batch* b, const double* t_end, const int* max_steps, const double* dt_fixed) {
+  if (b) b->cfl_fresh = false;
   if (!b) return MHD_EINVAL;
   for (int r = 0; r < b->nruns; ++r) {
     RunGeom& g = b->geom_h[r];
@@ -382,6 +384,7 @@ int mhd_batch_upload(mhd_batch* b, int run, const double* rho, const double* mx,
                      const double* bxf, const double* byf) {
   int rc = check_run(b, run);
   if (rc) return rc;
+  b->cfl_fresh = false;
   const double* src[6] = {rho, mx, my, E, bxf, byf};
   for (int q = 0; q < 6; ++q)
     if (!src[q]) return bfail(b, MHD_EINVAL, "mhd_batch_upload: NULL field");
@@ -418,7 +421,10 @@ int mhd_batch_step(mhd_batch* b, int* n_active) {
   d.jobs = b->tile_jobs;
   for (int q = 0; q < 6; ++q) d.s[q] = b->U[q];
   d.part = b->part;
-  BLAUNCH(b, kb_cfl, dim3(nt), dim3(kTileX), 0, b->stream, d);
+  // the previous step's final recombination formed the partials already
+  const bool fresh = b->cfl_fresh;
+  b->cfl_fresh = false;
+  if (!fresh) BLAUNCH(b, kb_cfl, dim3(nt), dim3(kTileX), 0, b->stream, d);
   BLAUNCH(b, kb_dt, dim3(b->nruns), dim3(256), 0, b->stream, (const RunGeom*)b->geom, b->rs, (const double*)b->part,
           b->st[0], b->st[1], b->nruns, step_factors(T), 0);
   BHIP(b, hipMemcpyAsync(b->rs_host, b->rs, sizeof(RunState) * b->nruns, hipMemcpyDeviceToHost, b->stream));
@@ -471,7 +477,7 @@ int mhd_batch_step(mhd_batch* b, int* n_active) {
     // Runs with eta = 0 have no implicit step: their B is copied by the
     // finalizer as well (x = 0 there, so g3 = 0 and c g3 = 0).
     BatchFinArgs f = fin_args(b, b->rs, b->x, b->st[1], b->U[4], b->U[5], b->S[4], b->S[5]);
-    BLAUNCH(b, kb_finalize2, dim3(nt), dim3(kTileX), 0, b->stream, f);
+    BLAUNCH(b, kb_finalize2_cfl, dim3(nt), dim3(kTileX), 0, b->stream, f, d);
     std::swap(b->U[4], b->S[4]);
     std::swap(b->U[5], b->S[5]);
   }
@@ -483,6 +489,7 @@ int mhd_batch_step(mhd_batch* b, int* n_active) {
     act += b->rs_host[r].active;
   }
   b->steps_taken++;
+  b->cfl_fresh = b->any_eta;
   if (n_active) *n_active = act;
   return MHD_OK;
 }

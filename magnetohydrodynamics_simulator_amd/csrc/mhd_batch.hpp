@@ -56,6 +56,10 @@ struct mhd_batch {
   int stage_rows = 32, tile_rows = 32;
   int last_spmv[3] = {1, 1, 1};     // longest solve of the last call per slot (0, 1: step, 2: adjoint)
   bool any_eta = false;
+  // part holds the CFL partials of the current state (formed by the last
+  // step's final recombination); cleared by anything that changes the state,
+  // the schedule or the tile plan
+  bool cfl_fresh = false;
   long long steps_taken = 0;
   std::vector<int> cg_iters_last;   // 2 per run: Eigen iteration counts of the last step
   double* stage = nullptr;          // pinned staging buffer of the largest run (host transfers)

@@ -714,6 +714,7 @@ extern "C" {
 
 int mhd_batch_adjoint_forward(mhd_batch* b, int nsteps, int checkpoint_every) {
   if (!b) return MHD_EINVAL;
+  b->cfl_fresh = false;
   if (nsteps < 1 || checkpoint_every < 1)
     return bfail(b, MHD_EINVAL, "mhd_batch_adjoint_forward: need nsteps >= 1 and checkpoint_every >= 1");
   for (int r = 0; r < b->nruns; ++r) {
@@ -766,6 +767,7 @@ int mhd_batch_adjoint_set_lambda(mhd_batch* b, int run, const double* const* lam
 
 int mhd_batch_adjoint_backward(mhd_batch* b) {
   if (int rc = check_adj(b)) return rc;
+  b->cfl_fresh = false;
   BatchAdjoint& A = *b->adj;
   const int nr = b->nruns;
   int rc = copy_pooled(b, A.lam_in, A.lam, 6);

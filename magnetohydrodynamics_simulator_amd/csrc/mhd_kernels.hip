@@ -1250,6 +1250,44 @@ __global__ __launch_bounds__(kTileX) void kb_cfl(BatchDiagArgs b) {
   cfl_body(d, job.y);
 }
 
+// DiagArgs of one run of a batch (its tiles, its slice of the pooled fields
+// and partials).
+__device__ __forceinline__ DiagArgs batch_diag_args(const BatchDiagArgs& b, int run) {
+  const RunGeom& g = b.geom[run];
+  DiagArgs d;
+  d.open_x = d.open_y = 0;
+  d.nx = g.nx;
+  d.ny = g.ny;
+  d.ty = g.ty;
+  d.tiles_y = g.tiles_y;
+  d.dx = g.dx;
+  d.dy = g.dy;
+#pragma unroll
+  for (int q = 0; q < 6; ++q) d.s[q] = b.s[q] + g.base;
+  d.part = b.part + 2LL * g.part_base;
+  return d;
+}
+
+// kb_finalize2 that also forms the next step's CFL partials of every active
+// run from the new state (rho, mx, my, E of U3*, the new B in registers), so
+// the next step needs no kb_cfl.  Runs that did not step copy their B and
+// keep no partials (a run that did not step this time does not step next).
+__global__ __launch_bounds__(kTileX) void kb_finalize2_cfl(BatchFinArgs b, BatchDiagArgs dg) {
+  const int2 job = b.jobs[xcd_block(blockIdx.x, gridDim.x)];
+  const FinArgs f = batch_fin_args(b, job.x);
+  if (b.rs[job.x].active) {
+    finalize2_body<true, false>(f, job.y, batch_diag_args(dg, job.x));
+    return;
+  }
+  const Tile t = tile_of(f.nx, f.ny, f.ty, f.tiles_y, job.y);
+  if (!t.valid) return;
+  for (int j = t.y0; j < t.y1; ++j) {
+    const int o = j * f.nx + t.ic;
+    f.bx_out[o] = f.bx[o];
+    f.by_out[o] = f.by[o];
+  }
+}
+
 // Driver step of every run (src/integrator_main.cpp:76-100 of the reference,
 // as mhd_run does it for one run): activity, CFL time step with the Courant
 // number of the run (CFL_condition_checker, diagnostics.hpp:52-86), the clamp

@@ -186,6 +186,7 @@ template <bool FIRST> __global__ void kb_cg_pass_b(BatchCgArgs b, const double* 
 __global__ void kb_finalize1(BatchFinArgs b);
 __global__ void kb_finalize2(BatchFinArgs b);
 __global__ void kb_cfl(BatchDiagArgs b);
+__global__ void kb_finalize2_cfl(BatchFinArgs b, BatchDiagArgs d);
 __global__ void kb_dt(const RunGeom* geom, RunState* rs, const double* part, CgState* st0, CgState* st1,
                       int nruns, StepFactors f, int halt_on_nonfinite);
 

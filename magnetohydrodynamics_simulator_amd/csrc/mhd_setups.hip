@@ -204,6 +204,7 @@ extern "C" {
 int mhd_batch_init_fourier(mhd_batch* b, int run, int nmodes, const double* modes) {
   if (!b) return MHD_EINVAL;
   if (run < 0 || run >= b->nruns) return mhd_batch_host::bfail(b, MHD_EINVAL, "mhd_batch_init_fourier: bad run index");
+  b->cfl_fresh = false;
   const RunGeom& g = b->geom_h[run];
   std::string why = check_square(g.nx, g.ny, g.dx, g.dy);
   if (why.empty()) why = check_fourier(nmodes, modes);
@@ -229,6 +230,7 @@ int mhd_batch_init_fourier(mhd_batch* b, int run, int nmodes, const double* mode
 int mhd_batch_init_column(mhd_batch* b, int run, int sausage, int nmodes, const double* amps, const double* phases) {
   if (!b) return MHD_EINVAL;
   if (run < 0 || run >= b->nruns) return mhd_batch_host::bfail(b, MHD_EINVAL, "mhd_batch_init_column: bad run index");
+  b->cfl_fresh = false;
   const RunGeom& g = b->geom_h[run];
   std::string why = check_square(g.nx, g.ny, g.dx, g.dy);
   if (why.empty()) why = check_column(nmodes, amps, phases);

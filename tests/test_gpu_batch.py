@@ -151,3 +151,31 @@ def test_autotune_keeps_state_and_results():
             assert b.run_info(r)[0] == steps
             for k in CONSERVED:
                 assert rel_l1(interior(got[k]), interior(ref[r][k])) <= 1e-12, (r, k)
+
+
+def test_state_and_schedule_changes_between_steps():
+    """A batch step forms the next step's CFL partials in its final
+    recombination; an upload or a new schedule between steps must make the
+    next step form them afresh.  Run 1 is replaced after two steps and every
+    run is extended to five: each run must equal a single-run context that
+    takes the same steps from the same states."""
+    _say("batch: upload and schedule change between steps vs single-run contexts")
+    tol, cfl = 1e-10, 0.4
+    states = _states()[:3]
+    new_ic = setups.ensemble_state(states[1][1], 4242)
+    with _batch(states, tol, cfl) as b:
+        b.set_schedule(max_steps=2)
+        b.run()
+        b.upload(1, new_ic)
+        b.set_schedule(max_steps=5)
+        b.run()
+        for r, (ic, nx, ny, dx, dy, eta) in enumerate(states):
+            got = b.download(r)
+            assert b.run_info(r)[0] == 5, r
+            start = new_ic if r == 1 else ic
+            with make_gpu(start, nx, ny, dx, dy, eta, tol=tol, cfl=cfl) as ctx:
+                ctx.run(1e9, max_steps=5)
+                ref = ctx.download()
+            for k in CONSERVED:
+                e = rel_l1(interior(got[k]), interior(ref[k]))
+                assert e <= 1e-12, (r, k, e)
