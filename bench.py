@@ -3,7 +3,7 @@
 Workload: resistive Orszag-Tang vortex, 4096 x 4096 cells, eta = 1e-3,
 Jacobi-PCG relative tolerance 1e-10, CFL 0.4 -- the largest single-GPU
 configuration of the "plain method at scale" workload.  A step is one pass of
-the reference driver loop (src/integrator_main.cpp:207-231): CFL time step,
+the reference driver loop (src/integrator_main.cpp:76-100): CFL time step,
 operator rebuild when dt changed, one IMEX step with its two implicit solves.
 Inputs are resident in HBM before the timed region.
 

@@ -94,7 +94,7 @@ MHD_API const char* mhd_last_error(const mhd_ctx* ctx);
 
 /* Copies the conserved state from padded host arrays (ghosts ignored) and
  * applies the density floor, as the driver's first fillGhostPeriodic +
- * computePrimitives does (src/integrator_main.cpp:183-184, Grid.hpp:115). */
+ * computePrimitives does (src/integrator_main.cpp:52-53, Grid.hpp:115). */
 MHD_API int mhd_upload(mhd_ctx* ctx, const double* rho, const double* mx, const double* my,
                const double* E, const double* bxf, const double* byf);
 
@@ -119,7 +119,7 @@ MHD_API int mhd_download_device(mhd_ctx* ctx, double* const* fields);
 MHD_API int mhd_cfl_dt(mhd_ctx* ctx, double* dt);
 
 /* generate_A_matrix + solver.compute (include/solvers.hpp:404-443,
- * src/integrator_main.cpp:189-191): sets alpha of the operator I - alpha L.
+ * src/integrator_main.cpp:58-60): sets alpha of the operator I - alpha L.
  * The driver passes alpha = dt * eta * gamma_ARS. */
 MHD_API int mhd_build_operator(mhd_ctx* ctx, double alpha);
 
@@ -142,7 +142,7 @@ MHD_API int mhd_cg_solve(mhd_ctx* ctx, const double* b, double* x, int* iters, d
 /* calculate_current_div_B (include/diagnostics.hpp:15-42). */
 MHD_API int mhd_divb(mhd_ctx* ctx, double* rms, double* maxabs);
 
-/* Driver loop (src/integrator_main.cpp:183-243): CFL time step (or the
+/* Driver loop (src/integrator_main.cpp:52-112): CFL time step (or the
  * constant dt_override when it is non-zero), the clamp of the last step to
  * t_end, the operator rebuild when dt changes by more than 1e-10 relative,
  * then IMEX; stops at t_end or after max_steps steps (max_steps < 0: no
@@ -160,7 +160,7 @@ MHD_API int mhd_run(mhd_ctx* ctx, double t_end, double dt_override, int max_step
  * context.  A non-zero return stops the loop and is returned. */
 typedef int (*mhd_snapshot_fn)(mhd_ctx* ctx, int frame, double t, double dt, void* user);
 
-/* The driver loop of mhd_run (src/integrator_main.cpp:183-243, same
+/* The driver loop of mhd_run (src/integrator_main.cpp:52-112, same
  * arguments and results, identical states, time steps and CG counts) with
  * the per-step decisions on the device: dt, the t_end clamp and the operator
  * rebuild rule are computed by a kernel, and CG solves run a launched
@@ -168,7 +168,7 @@ typedef int (*mhd_snapshot_fn)(mhd_ctx* ctx, int frame, double t, double dt, voi
  * driver state once per group of steps instead of three times per step.
  * With snapshot_every > 0 and on_snapshot non-NULL, on_snapshot runs before
  * the first step and after every snapshot_every-th step, like the
- * reference's snapshot cadence (integrator_main.cpp:197-203, 236-241).
+ * reference's snapshot cadence (integrator_main.cpp:65-70, 236-241).
  * No reference counterpart beyond the loop itself. */
 MHD_API int mhd_run_resident(mhd_ctx* ctx, double t_end, double dt_override, int max_steps, int snapshot_every,
                              mhd_snapshot_fn on_snapshot, void* user, int* steps_out, double* t_out,

@@ -16,7 +16,7 @@
 //   CFL_condition_checker(g)                            mhd_mi355x::CFL_condition_checker(g, ctx)
 //   calculate_current_div_B(g)                          mhd_mi355x::calculate_current_div_B(g, ctx)
 //   write_data_to_binary_file(g, frame, rate, dt)       mhd_mi355x::write_data_to_binary_file(g, ctx, frame, rate, dt)
-//   main's time loop (integrator_main.cpp:183-243)      mhd_mi355x::integrate(g, ctx, t_end, dt_override, rate)
+//   main's time loop (integrator_main.cpp:52-112)      mhd_mi355x::integrate(g, ctx, t_end, dt_override, rate)
 //
 // The Grid-taking functions copy the state to the device, run, and copy it
 // back (PCIe both ways).  Loops that keep the state on the GPU call
@@ -140,7 +140,7 @@ void write_data_to_binary_file(const GridT& g, Context& ctx, int current_frame_i
   check(mhd_write_snapshot(ctx.get(), name, snapshot_rate, time_step), ctx.get());
 }
 
-// The time loop of the reference's main (src/integrator_main.cpp:183-243) on
+// The time loop of the reference's main (src/integrator_main.cpp:52-112) on
 // the device: g's state is uploaded once, advanced to t_end (constant dt when
 // dt_override != 0) by mhd_run_resident, with a snapshot
 // output/snapshot_<frame>.dat before the first step and every

@@ -755,7 +755,7 @@ int mhd_run(mhd_ctx* c, double t_end, double dt_override, int max_steps, int* st
     return fail(c, MHD_EINVAL, "mhd_run: bad t_end or dt_override");
   const double A_im4 = c->tab.A_im4;
   double opt;
-  int rc = mhd_cfl_dt(c, &opt);                                   // integrator_main.cpp:186
+  int rc = mhd_cfl_dt(c, &opt);                                   // integrator_main.cpp:55
   if (rc) return rc;
   double dt0 = (dt_override == 0.0) ? opt : dt_override;
   double mc = dt0 * c->eta;

@@ -1,7 +1,7 @@
 // mhd_resident.hip -- the device-resident driver loop of a single context
 // (C ABI mhd_run_resident, mhd_resident_tune).
 //
-// Same semantics as mhd_run (the loop of src/integrator_main.cpp:183-243 of
+// Same semantics as mhd_run (the loop of src/integrator_main.cpp:52-112 of
 // the reference: CFL dt or the constant override, the clamp of the last step
 // to t_end, the operator rebuild when dt moves by more than 1e-10 relative,
 // IMEX), with the per-step decisions taken on the device:
@@ -387,7 +387,7 @@ int mhd_run_resident(mhd_ctx* c, double t_end, double dt_override, int max_steps
   HIP_OR_FAIL(c, hipMemcpyAsync(R->rs, R->rs_host, sizeof(RunState), hipMemcpyHostToDevice, c->stream));
   HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));   // the host copies above read pageable / pinned host memory
 
-  if (on_snapshot && snapshot_every > 0) {   // initial snapshot (integrator_main.cpp:186-203)
+  if (on_snapshot && snapshot_every > 0) {   // initial snapshot (integrator_main.cpp:55-70)
     double dt0 = dt_override;
     if (dt_override == 0.0 && (rc = mhd_cfl_dt(c, &dt0))) return rc;
     if ((rc = on_snapshot(c, 0, 0.0, dt0, user))) return rc;

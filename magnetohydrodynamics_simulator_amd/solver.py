@@ -189,7 +189,7 @@ class Context:
         return rms.value, mx.value
 
     def run(self, t_end: float, dt_override: float = 0.0, max_steps: int = -1):
-        """The driver loop of src/integrator_main.cpp:183-243 (no snapshots)."""
+        """The driver loop of src/integrator_main.cpp:52-112 (no snapshots)."""
         cap = max_steps if max_steps > 0 else 1_000_000
         dts = np.zeros(cap)
         its = np.zeros(2 * cap, dtype=np.int32)
@@ -475,5 +475,5 @@ class Batch:
 
 def operator_alpha(dt: float, eta: float) -> float:
     """alpha of generate_A_matrix as the driver forms it:
-    (dt * resistivity) * A_im[4] (src/integrator_main.cpp:189-190)."""
+    (dt * resistivity) * A_im[4] (src/integrator_main.cpp:58-59)."""
     return (dt * eta) * GAMMA_ARS

@@ -1004,7 +1004,7 @@ void og_ws_destroy(og_ws* w) {
 double* og_ws_rv(og_ws* w, int which) { return (which >= 0 && which < RV_COUNT) ? w->rv[which] : NULL; }
 double* og_ws_flux(og_ws* w, int which) { return (which >= 0 && which < FL_COUNT) ? w->fl[which] : NULL; }
 
-/* ---------------- driver (src/integrator_main.cpp:183-243) ---------------- */
+/* ---------------- driver (src/integrator_main.cpp:52-112) ---------------- */
 
 int og_run(og_grid* g, og_ws* w, double t_end, double dt_override, double cfl,
            int max_steps, int snapshot_every, double* step_dt, int* iters, double* t_out) {

@@ -103,7 +103,7 @@ void og_ic_sausage(og_grid* g);
 void og_ic_kink(og_grid* g);
 void og_ic_decaying_sine(og_grid* g, og_ws* w);
 
-/* driver loop, src/integrator_main.cpp:183-243.  Returns number of steps taken.
+/* driver loop, src/integrator_main.cpp:52-112.  Returns number of steps taken.
  * dt_override == 0 selects the CFL time step.  snapshot_every > 0 repeats the
  * driver's ghost fill + primitives at snapshot frames (no file is written).
  * If step_dt != NULL it receives each step's dt (capacity max_steps);

@@ -267,3 +267,56 @@ def test_floors_and_last_step_clamp():
     assert t == 2e-4 and abs(dts.sum() - 2e-4) < 1e-18
     assert dts[0] < 1e-8                        # the floored cell's wave speed limits dt
     assert np.isfinite(s.interior("E")).all() and s.interior("rho").min() >= 1e-8
+
+
+def _stepped_column(n, axis, u, lo=0.15, hi=-0.05, p=1e-2):
+    """A field component varying along one axis only, in plateaus at least
+    three cells wide (so every minmod slope is zero and MUSCL reconstruction is
+    piecewise constant), carried by a uniform flow u along that axis that is
+    faster than the fast wave (|u| = 2, c_f < 0.35), so HLL returns the
+    upwind state's physical flux.  axis 'x': B_y(x) with v = (u, 0);
+    axis 'y': B_x(y) with v = (0, u).  Both are divergence-free."""
+    prof = np.where((np.arange(n) // 4) % 2 == 0, lo, hi)
+    prof[n // 2 + 1: n // 2 + 4] = 0.3 * hi                       # a third level: unequal jumps
+    shape = (n + 4, n + 4)
+    b = np.zeros(shape)
+    k = (np.arange(n + 4) - 2) % n
+    if axis == "x":
+        b[...] = prof[k][None, :]
+        bx, by, mx, my = np.zeros(shape), b, np.full(shape, u), np.zeros(shape)
+    else:
+        b[...] = prof[k][:, None]
+        bx, by, mx, my = b, np.zeros(shape), np.zeros(shape), np.full(shape, u)
+    E = p / (2.0 / 3.0) + 0.5 * u * u + 0.5 * b * b
+    return {"rho": np.ones(shape), "mx": mx, "my": my, "E": E, "bxf": bx, "byf": by}, prof
+
+
+@pytest.mark.parametrize("axis,u", [("x", 2.0), ("x", -2.0), ("y", 2.0), ("y", -2.0)])
+def test_ct_corner_ez_is_upwind(axis, u):
+    """Pins the upwind selection of the corner E_z (solvers.hpp:314-340).
+    For the column above every face flux is the upwind cell's u * B_t, and the
+    upwind rule makes E_z at a corner that same value; the constrained-transport
+    update of the varying face component is then textbook first-order upwind
+    advection, dB/dt = -u (B_k - B_{k-1}) / h for u > 0 and
+    -u (B_{k+1} - B_k) / h for u < 0, while the other component stays 0.  A
+    downwind pick (eE for vx > 0, eN for vy > 0) or the four-face average give
+    a central-difference term at every jump and fail this by O(u jump / h)."""
+    n = 16
+    h = 2 * math.pi / n
+    ic, prof = _stepped_column(n, axis, u)
+    s = _sim_from(ic, n, n, h, h)
+    d = s.explicit_rhs()
+    if u > 0:
+        want = -u * (prof - np.roll(prof, 1)) / h
+    else:
+        want = -u * (np.roll(prof, -1) - prof) / h
+    scale = abs(u) * 0.2 / h
+    if axis == "x":
+        got, other = interior(d[5]), interior(d[4])
+        np.testing.assert_allclose(got, np.broadcast_to(want[None, :], got.shape), rtol=0, atol=1e-13 * scale)
+    else:
+        got, other = interior(d[4]), interior(d[5])
+        # d(bx)/dt = -dEz/dy: the same upwind form along y
+        np.testing.assert_allclose(got, np.broadcast_to(want[:, None], got.shape), rtol=0, atol=1e-13 * scale)
+    assert np.abs(other).max() <= 1e-13 * scale
+    assert np.abs(want).max() > 0.1 * scale          # the column has jumps, so the rule is exercised
