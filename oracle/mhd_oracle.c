@@ -1009,34 +1009,34 @@ double* og_ws_flux(og_ws* w, int which) { return (which >= 0 && which < FL_COUNT
 int og_run(og_grid* g, og_ws* w, double t_end, double dt_override, double cfl,
            int max_steps, int snapshot_every, double* step_dt, int* iters, double* t_out) {
   const double A_im4 = (2.0 - sqrt(2.0)) / 2.0;
-  og_fill_ghost(g);                                          /* :183-184 */
+  og_fill_ghost(g);                                          /* :52-53 */
   og_compute_primitives(g);
-  double opt = og_cfl(g, cfl);                               /* :186 */
+  double opt = og_cfl(g, cfl);                               /* :55 */
   double dt0 = (dt_override == 0.0) ? opt : dt_override;
   double mc = dt0 * g->eta;
-  og_generate_A_matrix(g, w, mc * A_im4);                    /* :189-191 */
-  og_compute_primitives(g);                                  /* :198 */
+  og_generate_A_matrix(g, w, mc * A_im4);                    /* :58-60 */
+  og_compute_primitives(g);                                  /* :67 */
   double prev = dt0, t = 0.0;
   int frame = 0, steps = 0;
-  while (t < t_end && (max_steps < 0 || steps < max_steps)) {   /* :207 */
+  while (t < t_end && (max_steps < 0 || steps < max_steps)) {   /* :76 */
     double copt = og_cfl(g, cfl);
     double dt = (dt_override == 0.0) ? copt : dt_override;
-    if (t + dt > t_end) dt = t_end - t;                      /* :214 */
+    if (t + dt > t_end) dt = t_end - t;                      /* :83 */
     frame++;
     if (g->eta > 0.0) {
-      if (fabs(dt - prev) > 1e-10 * dt) {                    /* :219 */
+      if (fabs(dt - prev) > 1e-10 * dt) {                    /* :88 */
         double m = dt * g->eta;
         og_generate_A_matrix(g, w, m * A_im4);
       }
     }
     prev = dt;
     og_step_stats st;
-    og_imex(g, w, dt, &st);                                  /* :230 */
+    og_imex(g, w, dt, &st);                                  /* :99 */
     t += dt;
     if (step_dt) step_dt[steps] = dt;
     if (iters) { iters[2 * steps] = st.cg_iters[0]; iters[2 * steps + 1] = st.cg_iters[1]; }
     steps++;
-    if (snapshot_every > 0 && frame % snapshot_every == 0) {   /* :233-241 */
+    if (snapshot_every > 0 && frame % snapshot_every == 0) {   /* :102-111 */
       og_fill_ghost(g);
       og_compute_primitives(g);
     }
