@@ -240,6 +240,26 @@ MHD_API int mhd_adjoint_backward(mhd_ctx* ctx, const double* const* lam_final, d
  * fields on the GPU (the LibTorch autograd Function of this package). */
 MHD_API int mhd_adjoint_backward_device(mhd_ctx* ctx, const double* const* lam_final, double* const* lam0,
                                 double* eta_bar);
+/* Branch-frozen forward map, for checking the adjoint against finite
+ * differences (no reference counterpart).  The step is piecewise smooth: the
+ * minmod choice (MUSCL.hpp:11-17), the HLL case and its wave-speed min / max
+ * (HLL.hpp:145-152), the upwind corner E_z (solvers.hpp:326-338) and the
+ * density / pressure floors (Grid.hpp:115,124) switch between smooth
+ * branches, and the adjoint differentiates each with the branch taken at the
+ * recorded forward state.  With on != 0 the next mhd_adjoint_backward also
+ * records those choices (one 64-bit word per cell and stage, 2 nsteps nx ny
+ * words on the device; the pass then runs without graph replay and with the
+ * unfused right-hand-side adjoint). */
+MHD_API int mhd_adjoint_record_branches(mhd_ctx* ctx, int on);
+/* Runs nsteps (<= the recorded steps) IMEX steps of the recorded constant dt
+ * from the current state with every branch choice read from the record, the
+ * implicit solves exact to rounding (Jacobi sweeps to below 2^-53 of the
+ * solution; operators with 2 (cx + cy) / (1 + 2 (cx + cy)) > 1/4 are refused)
+ * and resistivity eta (<= 0: the context's).  A smooth function of the
+ * initial state and eta whose derivative is what the adjoint computes; equal
+ * to the forward step wherever the state makes the recorded choices.  The
+ * context then holds the final state. */
+MHD_API int mhd_adjoint_forward_frozen(mhd_ctx* ctx, int nsteps, double eta);
 /* Unit entry point: out = L'(U)^T lam for the explicit right-hand side L of
  * explicit_rhs (solvers.hpp:393-402) at the current state U (padded arrays;
  * out's physical cells are written), by the fused single-pass kernel (the

@@ -30,8 +30,9 @@ def hipcc() -> str:
 
 
 def flags() -> list[str]:
+    """Compile flags of every source (gfx950 device code, host PIC)."""
     return [
-        "-O3", "-std=c++17", f"--offload-arch={ARCH}", "-fPIC", "-shared", "-Wl,-soname,libmhd_mi355x.so",
+        "-O3", "-std=c++17", f"--offload-arch={ARCH}", "-fPIC",
         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
         "-munsafe-fp-atomics",
         "-Xarch_host", "-fvisibility=hidden",     # export only the C ABI (MHD_API)
@@ -39,19 +40,39 @@ def flags() -> list[str]:
     ]
 
 
-def build(verbose: bool = False, extra: list[str] | None = None) -> str:
-    """Compile every HIP source into LIB; always recompiles."""
-    tmp = LIB + ".tmp"
-    cmd = [hipcc()] + flags() + (extra or []) + [os.path.join(CSRC, s) for s in SOURCES] + ["-o", tmp]
+def _run(cmd: list[str], verbose: bool) -> subprocess.CompletedProcess:
     if verbose:
         print(" ".join(cmd), flush=True)
-    res = subprocess.run(cmd, capture_output=True, text=True)
+    return subprocess.run(cmd, capture_output=True, text=True)
+
+
+def build(verbose: bool = False, extra: list[str] | None = None) -> str:
+    """Compile every HIP source into LIB; always recompiles.  The sources are
+    compiled to objects in parallel (each carries its own gfx950 code object)
+    and linked into one shared library."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    obj_dir = os.path.join(PKG, "_build")
+    os.makedirs(obj_dir, exist_ok=True)
+    objs = [os.path.join(obj_dir, os.path.splitext(s)[0] + ".o") for s in SOURCES]
+    cmds = [[hipcc()] + flags() + (extra or []) + ["-c", os.path.join(CSRC, s), "-o", o]
+            for s, o in zip(SOURCES, objs)]
+    jobs = max(1, min(len(cmds), os.cpu_count() or 1, 8))
+    with ThreadPoolExecutor(jobs) as ex:
+        results = list(ex.map(lambda c: _run(c, verbose), cmds))
+    for res in results:
+        if res.returncode != 0:
+            sys.stderr.write(res.stdout + res.stderr)
+            raise RuntimeError(f"hipcc failed with exit code {res.returncode}")
+        errs = [ln for ln in res.stderr.splitlines() if "error" in ln.lower()]
+        if errs:
+            sys.stderr.write("\n".join(errs) + "\n")
+    tmp = LIB + ".tmp"
+    res = _run([hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-Wl,-soname,libmhd_mi355x.so"] + objs +
+               ["-o", tmp], verbose)
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)
-        raise RuntimeError(f"hipcc failed with exit code {res.returncode}")
-    errs = [ln for ln in res.stderr.splitlines() if "error" in ln.lower()]
-    if errs:
-        sys.stderr.write("\n".join(errs) + "\n")
+        raise RuntimeError(f"hipcc link failed with exit code {res.returncode}")
     os.replace(tmp, LIB)
     return LIB
 

@@ -72,6 +72,8 @@ SIGNATURES = {
     "mhd_write_restart": (C.c_int, [_vp, C.c_char_p, C.c_double]),
     "mhd_read_restart": (C.c_int, [_vp, C.c_char_p, _dp]),
     "mhd_download_device": (C.c_int, [_vp, C.POINTER(_dp)]),
+    "mhd_adjoint_record_branches": (C.c_int, [_vp, C.c_int]),
+    "mhd_adjoint_forward_frozen": (C.c_int, [_vp, C.c_int, C.c_double]),
     "mhd_explicit_rhs_vjp": (C.c_int, [_vp, C.POINTER(_dp), C.POINTER(_dp)]),
     "mhd_explicit_rhs_vjp_unfused": (C.c_int, [_vp, C.POINTER(_dp), C.POINTER(_dp)]),
     "mhd_batch_create": (C.c_int, [C.c_int, _ip, _ip, _dp, _dp, _dp, C.POINTER(Opts), C.POINTER(_vp)]),

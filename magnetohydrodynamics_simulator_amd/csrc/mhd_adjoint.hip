@@ -38,6 +38,7 @@
 #include "mhd_adjoint_march.hpp"
 #include "mhd_adjoint_math.hpp"
 #include "mhd_device.hpp"
+#include "mhd_frozen.hpp"
 #include "mhd_kernels.hpp"
 
 using namespace mhd;
@@ -171,6 +172,32 @@ __global__ __launch_bounds__(256) void ka_zero(int n, double* y) {
   for (int o = blockIdx.x * blockDim.x + threadIdx.x; o < n; o += gridDim.x * blockDim.x) y[o] = 0.0;
 }
 
+// Branch words of a stage state whose primitives ka_prims has just written.
+__global__ __launch_bounds__(256) void ka_record(AdjArgs a, const double* rho_after, uint64_t* out) {
+  FOR_CELLS(a) out[o_] = record_cell(a, rho_after, i_, j_);
+}
+
+// The explicit right-hand side with recorded branch choices (mhd_frozen.hpp).
+__global__ __launch_bounds__(256) void kf_prims(FrozenArgs a) { FOR_CELLS(a) fz_prims_cell(a, i_, j_); }
+__global__ __launch_bounds__(256) void kf_faces(FrozenArgs a) { FOR_CELLS(a) fz_faces_cell(a, i_, j_); }
+__global__ __launch_bounds__(256) void kf_update(FrozenArgs a) { FOR_CELLS(a) fz_update_cell(a, i_, j_); }
+
+__global__ __launch_bounds__(256) void kf_comb1(int n, Fields6 u, Fields6 f, double c1, double cw, const uint64_t* br,
+                                                Fields6 s, Fields6 w) {
+  for (int o = blockIdx.x * blockDim.x + threadIdx.x; o < n; o += gridDim.x * blockDim.x)
+    fz_comb1_cell(o, u.f, f.f, c1, cw, br, s.f, w.f);
+}
+
+__global__ __launch_bounds__(256) void kf_comb2(LinArgs g, Fields6 w, Fields6 f, const double* q, double c2,
+                                                double c3, const uint64_t* br, Fields6 u) {
+  FOR_CELLS(g) fz_comb2_cell(g, i_, j_, w.f, f.f, q, c2, c3, br, u.f);
+}
+
+// q = eta J(bx, by)
+__global__ __launch_bounds__(256) void kf_J(LinArgs g, const double* bx, const double* by, double eta, double* q) {
+  FOR_CELLS(g) q[o_] = eta * corner_J(g, bx, by, i_, j_);
+}
+
 #undef FOR_CELLS
 
 }  // namespace mhd
@@ -212,6 +239,13 @@ struct AdjointStore {
   std::vector<double*> xs;            // the two implicit solutions of every forward step
   CgState* fwd_states = nullptr;      // their final solver states (2 per step)
   double* jac = nullptr;              // Jacobi ping-pong buffer
+  // branch words of every stage of the last recorded backward pass
+  // (2 per step, N each: mhd_adjoint_record_branches / mhd_adjoint_forward_frozen)
+  uint64_t* br = nullptr;
+  long long br_cap = 0;               // words allocated
+  bool record = false;                // record in the next backward pass
+  int br_steps = 0;                   // steps recorded (0: none)
+  double br_dt = 0.0;
   CgState* states = nullptr;          // final states of every solve of a backward pass (4 per step)
   CgState* states_host = nullptr;     // pinned mirror
   int nstates = 0;
@@ -232,6 +266,7 @@ void destroy_adjoint(mhd_ctx* c) {
   if (c->adj->states) (void)hipFree(c->adj->states);
   if (c->adj->fwd_states) (void)hipFree(c->adj->fwd_states);
   if (c->adj->graph) (void)hipGraphExecDestroy(c->adj->graph);
+  if (c->adj->br) (void)hipFree(c->adj->br);
   if (c->adj->states_host) (void)hipHostFree(c->adj->states_host);
   delete c->adj;
   c->adj = nullptr;
@@ -322,7 +357,7 @@ Fields6 f6(double* const* p) {
 
 // out += L'(s)^T lam for the state s (6 device fields), by the four unfused
 // gather kernels (the reference of the fused kernel).
-int explicit_rhs_vjp_unfused(mhd_ctx* c, double* const* s, double* const* lam, double* const* out) {
+AdjArgs adj_args(mhd_ctx* c, double* const* s, double* const* lam, double* const* out) {
   AdjointStore& A = *c->adj;
   AdjArgs a;
   a.nx = c->nx;
@@ -333,8 +368,8 @@ int explicit_rhs_vjp_unfused(mhd_ctx* c, double* const* s, double* const* lam, d
   a.idy = 1.0 / c->dy;
   for (int q = 0; q < 6; ++q) {
     a.s[q] = s[q];
-    a.lam[q] = lam[q];
-    a.out[q] = out[q];
+    a.lam[q] = lam ? lam[q] : nullptr;
+    a.out[q] = out ? out[q] : nullptr;
   }
   for (int q = 0; q < 7; ++q) a.P[q] = A.P[q];
   for (int q = 0; q < 5; ++q) {
@@ -347,6 +382,11 @@ int explicit_rhs_vjp_unfused(mhd_ctx* c, double* const* s, double* const* lam, d
   a.bny = A.bny;
   a.bcb[0] = A.bcb[0];
   a.bcb[1] = A.bcb[1];
+  return a;
+}
+
+int explicit_rhs_vjp_unfused(mhd_ctx* c, double* const* s, double* const* lam, double* const* out) {
+  const AdjArgs a = adj_args(c, s, lam, out);
   const int g = grid_1d(c->n);
   int rc = launch(c, 8, [&] { hipLaunchKernelGGL(ka_prims, dim3(g), dim3(256), 0, c->stream, a); });
   if (rc) return rc;
@@ -390,8 +430,20 @@ int explicit_rhs_vjp_fused(mhd_ctx* c, double* const* s, double* const* lam, dou
   return launch(c, 8, [&] { hipLaunchKernelGGL(ka_march, grid, dim3(64), 0, c->stream, a); });
 }
 
-// out += L'(s)^T lam, by whichever of the two is faster at this size.
-int explicit_rhs_vjp(mhd_ctx* c, double* const* s, double* const* lam, double* const* out) {
+// out += L'(s)^T lam, by whichever of the two is faster at this size.  With
+// words != nullptr also the branch words of s (the unfused kernels then run,
+// so that the words come from the primitives the product used); rho_after:
+// the density of the state the stage produces.
+int explicit_rhs_vjp(mhd_ctx* c, double* const* s, double* const* lam, double* const* out,
+                     uint64_t* words = nullptr, const double* rho_after = nullptr) {
+  if (words) {
+    int rc = explicit_rhs_vjp_unfused(c, s, lam, out);
+    if (rc) return rc;
+    const AdjArgs a = adj_args(c, s, nullptr, nullptr);
+    return launch(c, 8, [&] {
+      hipLaunchKernelGGL(ka_record, dim3(grid_1d(c->n)), dim3(256), 0, c->stream, a, rho_after, words);
+    });
+  }
   if (c->n < kFusedMinCells) return explicit_rhs_vjp_unfused(c, s, lam, out);
   return explicit_rhs_vjp_fused(c, s, lam, out);
 }
@@ -406,28 +458,32 @@ int jacobi_sweeps(mhd_ctx* c) {
 // the operator is close enough to the identity (see jacobi_sweeps), else by
 // CG (zero where the solve does no product; its final state is copied to
 // *state; spec > 0: speculative, see cg_run).
+// c->x = A^-1 b by `sweeps` Jacobi sweeps from x = 0 (rounded up to whole
+// launches of kJacSweeps) for A = I - alpha Lap with cx = alpha / dx^2,
+// cy = alpha / dy^2.
+int jacobi_run(mhd_ctx* c, double cx, double cy, int sweeps, const double* b) {
+  AdjointStore& A = *c->adj;
+  const double invd = 1.0 / (1 + 2 * (cx + cy));
+  LinArgs g = lin_args(c);
+  const int tiles = ((c->nx + kJacTile - 1) / kJacTile) * ((c->ny + kJacTile - 1) / kJacTile);
+  const int launches = (sweeps + kJacSweeps - 1) / kJacSweeps;
+  // ping-pong so that the last launch writes c->x
+  double* bufs[2] = {(launches & 1) ? c->x : A.jac, (launches & 1) ? A.jac : c->x};
+  const double* in = nullptr;
+  for (int k = 0; k < launches; ++k) {
+    double* out = bufs[k & 1];
+    int rc = launch(c, 8, [&] {
+      hipLaunchKernelGGL(ka_jacobi, dim3(tiles), dim3(256), 0, c->stream, g, cx, cy, invd, b, in, out);
+    });
+    if (rc) return rc;
+    in = out;
+  }
+  return MHD_OK;
+}
+
 int adjoint_solve(mhd_ctx* c, const double* xbar, int spec, CgState* state) {
   const int sweeps = jacobi_sweeps(c);
-  if (sweeps > 0) {
-    AdjointStore& A = *c->adj;
-    const double cx = c->alpha / (c->dx * c->dx), cy = c->alpha / (c->dy * c->dy);
-    const double invd = 1.0 / (1 + 2 * (cx + cy));
-    LinArgs g = lin_args(c);
-    const int tiles = ((c->nx + kJacTile - 1) / kJacTile) * ((c->ny + kJacTile - 1) / kJacTile);
-    const int launches = (sweeps + kJacSweeps - 1) / kJacSweeps;
-    // ping-pong so that the last launch writes c->x
-    double* bufs[2] = {(launches & 1) ? c->x : A.jac, (launches & 1) ? A.jac : c->x};
-    const double* in = nullptr;
-    for (int k = 0; k < launches; ++k) {
-      double* out = bufs[k & 1];
-      int rc = launch(c, 8, [&] {
-        hipLaunchKernelGGL(ka_jacobi, dim3(tiles), dim3(256), 0, c->stream, g, cx, cy, invd, xbar, in, out);
-      });
-      if (rc) return rc;
-      in = out;
-    }
-    return MHD_OK;
-  }
+  if (sweeps > 0) return jacobi_run(c, c->alpha / (c->dx * c->dx), c->alpha / (c->dy * c->dy), sweeps, xbar);
   CgState cs;
   int rc = cg_run(c, nullptr, nullptr, xbar, &cs, 2, spec, state);
   if (rc) return rc;
@@ -442,7 +498,8 @@ int adjoint_solve(mhd_ctx* c, const double* xbar, int spec, CgState* state) {
 // rec: the step's forward data; rho_next: density after the step; spec:
 // speculative iterations of the two adjoint solves (0: host-checked);
 // states: where their final states go.
-int adjoint_step(mhd_ctx* c, const StepRecord& rec, const double* rho_next, int spec, CgState* states) {
+// step: the step's index (for the branch words when they are recorded).
+int adjoint_step(mhd_ctx* c, const StepRecord& rec, const double* rho_next, int spec, CgState* states, int step) {
   AdjointStore& A = *c->adj;
   const Tableau& T = c->tab;
   const double dt = A.dt, cg = dt * T.A_im4, eta = c->eta;
@@ -478,7 +535,8 @@ int adjoint_step(mhd_ctx* c, const StepRecord& rec, const double* rho_next, int 
   KL(ka_Jt, dim3(gr), dim3(256), 0, c->stream, g, (const double*)A.q1, eta, A.sb[4], A.sb[5]);
   KL(ka_dot_J, dim3(nb), dim3(256), 0, c->stream, g, (const double*)A.q1, (const double*)rec.S[4],
      (const double*)rec.S[5], P + 3 * nb);
-  rc = explicit_rhs_vjp(c, rec.S, A.fb, A.sb);
+  uint64_t* words2 = A.record ? A.br + (2LL * step + 1) * c->n : nullptr;
+  rc = explicit_rhs_vjp(c, rec.S, A.fb, A.sb, words2, rho_next);
   if (rc) return rc;
   // ---- implicit 1: S.B = S*.B + cg R(x1), x1 = A^-1 eta J(S*.B)
   KL(ka_Rt, dim3(gr), dim3(256), 0, c->stream, g, (const double*)A.sb[4], (const double*)A.sb[5], cg, A.q1);
@@ -492,7 +550,8 @@ int adjoint_step(mhd_ctx* c, const StepRecord& rec, const double* rho_next, int 
   // ---- stage 1: S* = U + dt gamma f1, W = U + dt delta f1, f1 = L(U)
   KL(ka_adj_stage1_out, dim3(gr), dim3(256), 0, c->stream, n, f6(A.sb), f6(A.wb), f6(A.fb), f6(lam),
      (const double*)rec.S[0], dt * T.A_ex3, dt * T.A_ex6);
-  rc = explicit_rhs_vjp(c, rec.Un, A.fb, lam);
+  uint64_t* words1 = A.record ? A.br + 2LL * step * c->n : nullptr;
+  rc = explicit_rhs_vjp(c, rec.Un, A.fb, lam, words1, rec.S[0]);
   if (rc) return rc;
   // ---- d/d eta: <J(B3), R^T g3bar> + <J(U3*.B), b2bar> + dt gamma <b2bar, Lap x2>
   //               + <J(S.B), R^T g2bar> + <J(S*.B), b1bar> + dt gamma <b1bar, Lap x1>
@@ -540,7 +599,7 @@ int backward(mhd_ctx* c, bool spec) {
       // right-hand sides differ, so a margin is added to the forward counts
       const int sp = std::max(A.fwd_spmv[2 * s], A.fwd_spmv[2 * s + 1]);
       const int spec_adj = spec ? sp + 2 + sp / 4 : 0;
-      int rc = adjoint_step(c, r, rn, spec_adj, A.states + 4 * s + 2);
+      int rc = adjoint_step(c, r, rn, spec_adj, A.states + 4 * s + 2, s);
       if (rc) return rc;
     }
     rho_next = A.ckpt[6 * sg + 0];
@@ -664,7 +723,22 @@ int backward_entry(mhd_ctx* c, const double* const* lam_final, double* const* la
     }
     return MHD_OK;
   };
-  if (jacobi_sweeps(c) > 0 && c->prof_mask == 0) {
+  if (A.record) {
+    // the record is filled by this pass: allocate it for 2 words per step and cell
+    const long long need = 2LL * A.nsteps * c->n;
+    if (A.br_cap < need) {
+      if (A.br) (void)hipFree(A.br);
+      A.br = nullptr;
+      A.br_cap = 0;
+      if (hipMalloc(&A.br, sizeof(uint64_t) * need) != hipSuccess) {
+        A.br = nullptr;
+        return fail(c, MHD_ENOMEM, "mhd_adjoint_backward: branch record allocation failed");
+      }
+      A.br_cap = need;
+    }
+    A.br_steps = 0;
+  }
+  if (jacobi_sweeps(c) > 0 && c->prof_mask == 0 && !A.record) {
     // no host round trip anywhere in the sweep: replay it as one graph
     // (captured on the first call of a configuration, reused after)
     rc = upload_lam();
@@ -746,6 +820,70 @@ int backward_entry(mhd_ctx* c, const double* const* lam_final, double* const* la
   HIP_OR_FAIL(c, hipMemcpyAsync(&eb, A.eta_bar, sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
   if (eta_bar) *eta_bar = eb;
+  if (A.record) {
+    A.br_steps = A.nsteps;
+    A.br_dt = A.dt;
+  }
+  return MHD_OK;
+}
+
+// One step of the frozen map (mhd_adjoint_forward_frozen) on c->U, step s
+// of the record, resistivity eta, Jacobi solves with cx, cy and `sweeps`.
+int frozen_step(mhd_ctx* c, int s, double eta, double cx, double cy, int sweeps) {
+  AdjointStore& A = *c->adj;
+  const Tableau& T = c->tab;
+  const double dt = A.br_dt, cg = dt * T.A_im4;
+  const int n = (int)c->n;
+  const int gr = grid_1d(c->n);
+  LinArgs g = lin_args(c);
+  const uint64_t* w1 = A.br + 2LL * s * c->n;
+  const uint64_t* w2 = w1 + c->n;
+  int rc;
+#define KL(...) do { rc = launch(c, 8, [&] { hipLaunchKernelGGL(__VA_ARGS__); }); if (rc) return rc; } while (0)
+  auto rhs = [&](double* const* st, const uint64_t* w, double* const* out) -> int {
+    FrozenArgs a;
+    a.nx = c->nx;
+    a.ny = c->ny;
+    a.idx = 1.0 / c->dx;
+    a.idy = 1.0 / c->dy;
+    a.br = w;
+    for (int q = 0; q < 6; ++q) {
+      a.s[q] = st[q];
+      a.P[q] = A.P[q];
+      a.out[q] = out[q];
+    }
+    for (int q = 0; q < 5; ++q) {
+      a.fx[q] = A.xrb[q];
+      a.fy[q] = A.yrb[q];
+    }
+    int r;
+    if ((r = launch(c, 8, [&] { hipLaunchKernelGGL(kf_prims, dim3(gr), dim3(256), 0, c->stream, a); }))) return r;
+    if ((r = launch(c, 8, [&] { hipLaunchKernelGGL(kf_faces, dim3(gr), dim3(256), 0, c->stream, a); }))) return r;
+    return launch(c, 8, [&] { hipLaunchKernelGGL(kf_update, dim3(gr), dim3(256), 0, c->stream, a); });
+  };
+  // stage 1: f1 = L(U); S* = U + dt gamma f1, W = U + dt delta f1
+  if ((rc = rhs(c->U, w1, A.fb))) return rc;
+  KL(kf_comb1, dim3(gr), dim3(256), 0, c->stream, n, f6(c->U), f6(A.fb), dt * T.A_ex3, dt * T.A_ex6, w1, f6(A.sb),
+     f6(A.wb));
+  // implicit 1: x1 = A^-1 eta J(S*.B); S.B = S*.B + cg R(x1)
+  KL(kf_J, dim3(gr), dim3(256), 0, c->stream, g, (const double*)A.sb[4], (const double*)A.sb[5], eta, A.q1);
+  if ((rc = jacobi_run(c, cx, cy, sweeps, A.q1))) return rc;
+  KL(ka_R, dim3(gr), dim3(256), 0, c->stream, g, (const double*)A.sb[4], (const double*)A.sb[5], (const double*)c->x,
+     (const CgState*)nullptr, cg, A.sb[4], A.sb[5]);
+  // stage 2: U3* = W + dt (1 - delta) L(S) + dt (1 - gamma) R(eta J(S.B)) on B
+  if ((rc = rhs(A.sb, w2, A.fb))) return rc;
+  KL(kf_J, dim3(gr), dim3(256), 0, c->stream, g, (const double*)A.sb[4], (const double*)A.sb[5], eta, A.q1);
+  KL(kf_comb2, dim3(gr), dim3(256), 0, c->stream, g, f6(A.wb), f6(A.fb), (const double*)A.q1, dt * T.A_ex7,
+     dt * T.A_im7, w2, f6(c->U));
+  // implicit 2: B3 = U3*.B + cg R(x2), x2 = A^-1 eta J(U3*.B); U'.B = U3*.B + cg R(eta J(B3))
+  KL(kf_J, dim3(gr), dim3(256), 0, c->stream, g, (const double*)c->U[4], (const double*)c->U[5], eta, A.q1);
+  if ((rc = jacobi_run(c, cx, cy, sweeps, A.q1))) return rc;
+  KL(ka_R, dim3(gr), dim3(256), 0, c->stream, g, (const double*)c->U[4], (const double*)c->U[5], (const double*)c->x,
+     (const CgState*)nullptr, cg, A.gx, A.gy);
+  KL(kf_J, dim3(gr), dim3(256), 0, c->stream, g, (const double*)A.gx, (const double*)A.gy, eta, A.q1);
+  KL(ka_R, dim3(gr), dim3(256), 0, c->stream, g, (const double*)c->U[4], (const double*)c->U[5], (const double*)A.q1,
+     (const CgState*)nullptr, cg, c->U[4], c->U[5]);
+#undef KL
   return MHD_OK;
 }
 
@@ -763,6 +901,38 @@ int mhd_adjoint_backward_device(mhd_ctx* c, const double* const* lam_final, doub
 
 int mhd_explicit_rhs_vjp(mhd_ctx* c, const double* const* lam, double* const* out) {
   return rhs_vjp_entry(c, lam, out, true);
+}
+
+int mhd_adjoint_record_branches(mhd_ctx* c, int on) {
+  if (check(c)) return MHD_EINVAL;
+  int rc = ensure_store(c, 0, 0);
+  if (rc) return rc;
+  c->adj->record = (on != 0);
+  return MHD_OK;
+}
+
+int mhd_adjoint_forward_frozen(mhd_ctx* c, int nsteps, double eta) {
+  if (check(c)) return MHD_EINVAL;
+  if (!c->adj || c->adj->br_steps < 1)
+    return fail(c, MHD_EINVAL, "mhd_adjoint_forward_frozen: no branch record (mhd_adjoint_record_branches, then "
+                               "mhd_adjoint_backward)");
+  AdjointStore& A = *c->adj;
+  if (nsteps < 1 || nsteps > A.br_steps)
+    return fail(c, MHD_EINVAL, "mhd_adjoint_forward_frozen: need 1 <= nsteps <= the recorded steps");
+  if (eta <= 0) eta = c->eta;
+  if (!std::isfinite(eta)) return fail(c, MHD_EINVAL, "mhd_adjoint_forward_frozen: eta must be finite");
+  const double alpha = (A.br_dt * eta) * c->tab.A_im4;
+  const double cx = alpha / (c->dx * c->dx), cy = alpha / (c->dy * c->dy);
+  const int sweeps = jacobi_sweep_count(cx, cy);
+  if (sweeps < 1)
+    return fail(c, MHD_EINVAL, "mhd_adjoint_forward_frozen: the implicit operator is too far from the identity for "
+                               "exact Jacobi solves (2 (cx + cy) / (1 + 2 (cx + cy)) > 1/4)");
+  for (int s = 0; s < nsteps; ++s) {
+    int rc = frozen_step(c, s, eta, cx, cy, sweeps);
+    if (rc) return rc;
+  }
+  HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
+  return MHD_OK;
 }
 
 int mhd_explicit_rhs_vjp_unfused(mhd_ctx* c, const double* const* lam, double* const* out) {

@@ -298,6 +298,19 @@ class Context:
         self._check(self.lib.mhd_adjoint_backward(self.h, lp, op, C.byref(eb)))
         return outs, eb.value
 
+    def adjoint_record_branches(self, on: bool = True):
+        """Record, in the next adjoint_backward, every branch choice the
+        adjoint linearises at (minmod, HLL case and wave-speed min / max,
+        corner E_z upwinding, floors), for adjoint_forward_frozen."""
+        self._check(self.lib.mhd_adjoint_record_branches(self.h, int(bool(on))))
+
+    def adjoint_forward_frozen(self, nsteps: int, eta: float | None = None):
+        """nsteps steps of the recorded dt from the current state with the
+        recorded branch choices and exact implicit solves: the smooth map whose
+        derivative the adjoint computes (eta: resistivity, default the
+        context's)."""
+        self._check(self.lib.mhd_adjoint_forward_frozen(self.h, nsteps, 0.0 if eta is None else float(eta)))
+
     # ---- instrumentation ----
     def profile(self, kernels=None):
         """Bracket launches of the named kernels (all if True, none if

@@ -170,17 +170,17 @@ def _kink_loss(n, amps, phases, eta, dt, steps):
     return setups.window_magnetic_energy(fin, n, 1.0 / n)[0]
 
 
-def test_kink_column_gradient_matches_central_differences():
+def test_kink_column_gradient_unfrozen_differences():
     """Reduced differentiation workload: kink column 64x64, 8 displacement
-    modes (wavelengths of 8 cells and more: shorter ones move minmod slope
-    differences by the size of the difference step itself, and the
-    differences then cross limiter kinks), eta 1e-3, 12 steps of frozen dt,
-    checkpoints every 5; the loss is the magnetic energy of the central
-    window.  Gradient with respect to the amplitudes (lam0 contracted with
-    the analytic d(state)/d(a_k)) and
-    eta against central differences of the GPU forward (relative 1e-4 on the
-    amplitude gradient's norm, which is what the differences can resolve
-    here; 1e-5 on eta)."""
+    modes, eta 1e-3, 12 steps of frozen dt, checkpoints every 5; the loss is
+    the magnetic energy of the central window.  Gradient with respect to the
+    amplitudes (lam0 contracted with the analytic d(state)/d(a_k)) and eta
+    against central differences of the plain GPU forward -- the map with its
+    branch choices free to switch.  Its differences cross minmod / upwind
+    kinks, so the amplitude comparison resolves only about 1e-5 here (bar
+    1e-4 on the norm; the 1e-6 differentiation bar is checked per mode on
+    the branch-frozen map in test_gpu_frozen_fd.py); eta, whose differences
+    cross no kink, is held to 1e-6."""
     n, eta, steps = 64, 1e-3, 12
     _say(f"kink column {n}^2 gradient, 8 modes, {steps} steps")
     amps, phases = setups.kink_column_modes(8, 1e-3, seed=0)
@@ -219,7 +219,7 @@ def test_kink_column_gradient_matches_central_differences():
     he = 1e-4 * eta
     fde = (_kink_loss(n, amps, phases, eta + he, dt, steps) - _kink_loss(n, amps, phases, eta - he, dt, steps)) / (2 * he)
     print(f"eta: adjoint {eta_bar:.10e} differences {fde:.10e}", flush=True)
-    assert abs(eta_bar - fde) <= 1e-5 * abs(fde), (eta_bar, fde)
+    assert abs(eta_bar - fde) <= TOL * abs(fde), (eta_bar, fde)
 
 
 def test_adjoint_repeat_replays_the_same_sweep():
