@@ -214,6 +214,24 @@ MHD_API int mhd_profile_get(mhd_ctx* ctx, int kernel_id, long long* launches, do
 MHD_API int mhd_calibration_copy(mhd_ctx* ctx, long long nbytes);
 MHD_API int mhd_synchronize(mhd_ctx* ctx);
 
+/* Allocation-free CFL time step.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * serving loops that forbid hipMalloc).  The time step of mhd_cfl_dt
+ * (CFL_condition_checker, diagnostics.hpp:44-88; Courant number cfl, the
+ * reference's 0.45 reproduced bit for bit) of a state the caller holds in
+ * six compact nx*ny device arrays (rho, mx, my, E, bxf, byf, row-major
+ * j*nx+i, the context layout of mhd_upload_device), written to the device
+ * double *dt_device (NaN if a wave speed is not finite).  No context, no
+ * allocation, no host synchronisation: two kernels on `stream` (a
+ * hipStream_t; NULL: the null stream).  Scratch: a 64 KiB file-scope device
+ * array per GPU, held until the process ends, shared by every call on that
+ * GPU -- calls that overlap on one GPU race.  Limit: the per-tile partials
+ * of the grid must fit the scratch (at most 4096 tiles of 256 columns:
+ * nx <= 1048576), nx, ny >= 4, nx * ny <= 2^30; larger grids return
+ * MHD_EINVAL (mhd_last_error(NULL) says why) without launching. */
+MHD_API int mhd_cfl_dt_noalloc(int nx, int ny, double dx, double dy, double eta, double cfl, int bc_x, int bc_y,
+                               const double* const* fields, double* dt_device, void* stream);
+
 /* ------------------------------------------------------------------------
  * Discrete adjoint (no reference counterpart: the reference has no
  * derivatives).  The adjoint of the whole IMEX step: reverse-mode
