@@ -395,7 +395,8 @@ def bench_batch(a) -> dict:
     host_cells = int(sum(n * n for n in ns[:n_host]))
     for r in range(n_host):
         b.init_fourier(r, tables[r])
-    shape = b.autotune(trial_steps=2)           # launch shape (also warms up)
+    shape = b.autotune(trial_steps=2)           # launch shape, CFL fusion, graph replay (also warms up)
+    tuned = b.tuning()
     b.set_schedule(max_steps=a.batch_steps)
     b.synchronize()
     t0 = time.perf_counter()
@@ -403,6 +404,26 @@ def bench_batch(a) -> dict:
     b.synchronize()
     t_batch = time.perf_counter() - t0
     its = np.array([b.run_info(r)[3] for r in range(len(members))])
+    # the launch modes side by side with the tuned shape, from the initial states
+    modes = {"host_checked_steps": dict(group_steps=0),
+             "groups_direct_fused_cfl": dict(group_steps=16, graph=False, fused_cfl=True),
+             "groups_direct_separate_cfl": dict(group_steps=16, graph=False, fused_cfl=False),
+             "groups_graph_fused_cfl": dict(group_steps=16, graph=True, fused_cfl=True),
+             "groups_graph_separate_cfl": dict(group_steps=16, graph=True, fused_cfl=False)}
+    mode_steps = min(a.batch_steps, 50)
+    mode_ms = {}
+    for name, m in modes.items():
+        for r in range(len(members)):
+            b.init_fourier(r, tables[r])
+        b.tune(**m)
+        b.set_schedule(max_steps=2)
+        b.run()                                  # warm-up (captures the graphs)
+        b.set_schedule(max_steps=2 + mode_steps)
+        b.synchronize()
+        t0 = time.perf_counter()
+        nd = b.run()
+        b.synchronize()
+        mode_ms[name] = round((time.perf_counter() - t0) / max(nd, 1) * 1e3, 4)
     b.close()
     # plain path: one context per run, the driver loop of each in turn
     t_plain = 0.0
@@ -425,6 +446,11 @@ def bench_batch(a) -> dict:
             "plain_value": round(v_plain, 2), "plain_steps_per_run": a.plain_steps,
             "speedup_vs_plain": round(v_batch / v_plain, 2),
             "autotuned_shape": {"stage_rows": shape[0], "tile_rows": shape[1]},
+            "autotuned": tuned,
+            "launch_modes_ms_per_batch_step": dict(mode_ms, steps=mode_steps,
+                                                   note="same shape, from the initial states after 2 warm-up steps; "
+                                                        "groups: 16 steps per driver read-back, CG budgets on the "
+                                                        "device; host_checked_steps: one read-back per CG chunk"),
             "cg_iters_last_step_mean": round(float(its.mean()), 2),
             "setup": {"device_generation_s": round(t_init_dev, 4),
                       "device_Mcells_per_s": round(cells / t_init_dev / 1e6, 1),

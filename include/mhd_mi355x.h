@@ -337,14 +337,39 @@ MHD_API int mhd_batch_init_column(mhd_batch* b, int run, int sausage, int nmodes
 /* One driver step of every run that has not reached its end time or step
  * limit; *n_active (may be NULL) receives the number of runs that stepped. */
 MHD_API int mhd_batch_step(mhd_batch* b, int* n_active);
-/* mhd_batch_step until no run is active or max_batch_steps (< 0: no limit). */
+/* Steps every run until none is active or max_batch_steps batch steps (< 0:
+ * no limit) have been taken; *steps_done: the batch steps in which some run
+ * stepped.  In group mode (mhd_batch_tune, the default: groups of 16) the
+ * steps are enqueued in groups with the per-run driver on the device and one
+ * read-back of the driver state per group: each CG solve runs a launched
+ * iteration budget (the longest solve of that slot in the last group plus a
+ * margin; a solve's passes after its convergence return at once), a check
+ * after each solve halts the group if a run's solve is unfinished, and the
+ * host then finishes that solve with host-checked iterations and resumes.
+ * The same kernels in the same order as mhd_batch_step: states, time steps
+ * and iteration counts are those of calling mhd_batch_step repeatedly.  With
+ * group_steps = 0, mhd_batch_step in a loop. */
 MHD_API int mhd_batch_run(mhd_batch* b, int max_batch_steps, int* steps_done);
-/* Autotuner: times trial_steps steps of the uploaded batch for each candidate
- * launch shape (rows per explicit-stage segment x rows per CG / finalize
- * tile, each in {16, 32, 64}), restores the state, and keeps the fastest
- * shape; the chosen rows are returned (either pointer may be NULL).  Results
- * do not depend on the shape beyond the summation order of the CG dot
- * products. */
+/* Launch mode of mhd_batch_run: group_steps (0: one host-checked step at a
+ * time; 1..4096: steps per group), spec_margin (extra launched CG iterations
+ * per solve), graph (1: pairs of steps captured once as a hipGraph and
+ * replayed, per B-buffer parity, budgets and fusion), fused_cfl (1: the
+ * final recombination also forms the next step's CFL partials; 0: a separate
+ * CFL pass at the start of each step).  Results are identical in every mode. */
+MHD_API int mhd_batch_tune(mhd_batch* b, int group_steps, int spec_margin, int graph, int fused_cfl);
+/* The batch's current launch shape and mode (any pointer may be NULL). */
+MHD_API int mhd_batch_get_tuning(mhd_batch* b, int* stage_rows, int* tile_rows, int* group_steps, int* graph,
+                                 int* fused_cfl);
+/* Autotuner: times trial_steps steps of the uploaded batch (in the configured
+ * launch mode) for each candidate launch shape (rows per explicit-stage
+ * segment x rows per CG / finalize tile, each in {16, 32, 64}), then, in
+ * group mode with a resistive run, each stage fusion (CFL partials fused into
+ * the final recombination or a separate pass) x launch mode (direct launches
+ * or graph replay) with the best shape; restores the state after every trial
+ * and keeps the fastest configuration.  The chosen rows are returned (either
+ * pointer may be NULL; mhd_batch_get_tuning gives the rest).  Results do not
+ * depend on the configuration beyond the summation order of the CG dot
+ * products (the tile shape). */
 MHD_API int mhd_batch_autotune(mhd_batch* b, int trial_steps, int* stage_rows, int* tile_rows);
 /* Steps taken, time, last dt and the two CG iteration counts of the last step of one run. */
 MHD_API int mhd_batch_run_info(mhd_batch* b, int run, int* steps, double* t, double* last_dt, int* cg_iters);

@@ -88,6 +88,9 @@ SIGNATURES = {
     "mhd_batch_run": (C.c_int, [_vp, C.c_int, _ip]),
     "mhd_batch_run_info": (C.c_int, [_vp, C.c_int, _ip, _dp, _dp, _ip]),
     "mhd_batch_autotune": (C.c_int, [_vp, C.c_int, _ip, _ip]),
+    "mhd_batch_tune": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "mhd_batch_get_tuning": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                       C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "mhd_batch_synchronize": (C.c_int, [_vp]),
     "mhd_batch_adjoint_forward": (C.c_int, [_vp, C.c_int, C.c_int]),
     "mhd_batch_adjoint_set_lambda": (C.c_int, [_vp, C.c_int, C.POINTER(_dp)]),

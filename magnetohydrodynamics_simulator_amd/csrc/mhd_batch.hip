@@ -77,8 +77,16 @@ void plan(mhd_batch* b, std::vector<int2>& stage_jobs, std::vector<int2>& tile_j
 
 // Lays the runs out with the given launch shape and uploads geometry and job
 // tables (the schedule fields of the geometry are kept).
+void drop_graphs(mhd_batch* b) {
+  for (BatchGraph& g : b->graphs) {
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    g = BatchGraph{};
+  }
+}
+
 int replan(mhd_batch* b, int stage_rows, int tile_rows) {
   b->cfl_fresh = false;
+  drop_graphs(b);        // captured launches carry the old grid sizes
   b->stage_rows = stage_rows;
   b->tile_rows = tile_rows;
   std::vector<int2> sj, tj;
@@ -234,6 +242,318 @@ int check_run(mhd_batch* b, int run) {
   return MHD_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Device-resident groups of batch steps (mhd_batch_run in group mode).
+//
+// The host enqueues `group` steps of the whole batch and reads the driver
+// state back once per group.  Every CG solve gets a launched-iteration
+// budget (the longest solve of that slot in the last group, plus a margin);
+// a one-block check after each solve halts the group (control word
+// kCtlHalt) if a stepping run's solve is unfinished, and every later batched
+// kernel then returns at once.  At the read-back the host continues that
+// solve with host-checked iterations, finishes the step and enqueues the
+// group's remaining steps.  The kernels and their order are those of
+// mhd_batch_step, so the states, time steps and iteration counts are
+// identical to stepping one host-checked step at a time.
+// ---------------------------------------------------------------------------
+
+struct BLoop {
+  double* B0[2];     // the state's face B at the start of the call (U[4], U[5])
+  double* B1[2];     // the other pair (S[4], S[5])
+  int spec[2];       // launched iterations per solve: A(1) B(1) ... A(spec) B(spec) A(spec + 1)
+};
+
+// The B pointers after `completed` steps of the call (every step of a batch
+// with a resistive run ends with the swap of the final recombination).
+void set_bparity(mhd_batch* b, const BLoop& L, long long completed) {
+  const bool odd = b->any_eta && (completed & 1);
+  for (int k = 0; k < 2; ++k) {
+    b->U[4 + k] = odd ? L.B1[k] : L.B0[k];
+    b->S[4 + k] = odd ? L.B0[k] : L.B1[k];
+  }
+}
+
+BatchDiagArgs group_diag(mhd_batch* b) {
+  BatchDiagArgs d{};
+  d.geom = b->geom;
+  d.rs = b->rs;
+  d.jobs = b->tile_jobs;
+  for (int q = 0; q < 6; ++q) d.s[q] = b->U[q];
+  d.part = b->part;
+  d.ctl = b->ctl;
+  return d;
+}
+
+BatchCgArgs group_cg(mhd_batch* b, int slot) {
+  CgCall cc;
+  cc.slot = slot;
+  cc.bx = slot == 0 ? b->S[4] : b->U[4];
+  cc.by = slot == 0 ? b->S[5] : b->U[5];
+  cc.x = b->x;
+  cc.st = b->st[slot];
+  cc.rs = b->rs;
+  BatchCgArgs a = cg_args(b, cc);
+  a.ctl = b->ctl;
+  return a;
+}
+
+// Launched iterations of a solve: start, B(0), then A(1) B(1) ... A(spec) B(spec)
+// A(spec + 1) (the last A takes the convergence decision), then the check.
+int enqueue_group_cg(mhd_batch* b, int slot, int spec) {
+  const BatchCgArgs a = group_cg(b, slot);
+  const int nb = b->n_tile_jobs;
+  BLAUNCH(b, kb_cg_start, dim3(nb), dim3(kTileX), 0, b->stream, a);
+  BLAUNCH(b, kb_cg_pass_b<true>, dim3(nb), dim3(kTileX), 0, b->stream, a, (const double*)b->p[0], 0);
+  for (int it = 1; it <= spec + 1; ++it) {
+    BLAUNCH(b, kb_cg_pass_a, dim3(nb), dim3(kTileX), 0, b->stream, a, (const double*)b->p[(it - 1) & 1], b->p[it & 1],
+            it);
+    if (it == spec + 1) break;
+    BLAUNCH(b, kb_cg_pass_b<false>, dim3(nb), dim3(kTileX), 0, b->stream, a, (const double*)b->p[it & 1], it);
+  }
+  BLAUNCH(b, kb_check_solve, dim3(1), dim3(256), 0, b->stream, (const RunState*)b->rs, (const CgState*)b->st[slot],
+          b->nruns, slot, b->o.cg_fail_is_error, b->ctl, b->iters);
+  return MHD_OK;
+}
+
+// Phases of a step: 0 CFL partials (unless the last recombination formed
+// them) and kb_dt, 1 stage 1, 2 solve 1 and its check, 3 finaliser 1,
+// 4 stage 2, 5 solve 2 and its check, 6 final recombination (+ the B swap).
+int enqueue_group_step(mhd_batch* b, BLoop& L, int from) {
+  const BatchTableau& T = b->tab;
+  const int nt = b->n_tile_jobs;
+  const BatchDiagArgs d = group_diag(b);
+  const bool fused = b->any_eta && b->fused_cfl;
+  if (from <= 0) {
+    if (!fused) BLAUNCH(b, kb_cfl, dim3(nt), dim3(kTileX), 0, b->stream, d);
+    BLAUNCH(b, kb_dt, dim3(b->nruns), dim3(256), 0, b->stream, (const RunGeom*)b->geom, b->rs, (const double*)b->part,
+            b->st[0], b->st[1], b->nruns, step_factors(T), 1, b->ctl);
+  }
+  if (from <= 1) {
+    BatchStageArgs s1{};
+    s1.geom = b->geom;
+    s1.rs = b->rs;
+    s1.jobs = b->stage_jobs;
+    s1.ctl = b->ctl;
+    for (int q = 0; q < 6; ++q) {
+      s1.s[q] = b->U[q];
+      s1.o1[q] = b->W[q];
+      s1.o2[q] = b->S[q];
+    }
+    BLAUNCH(b, kb_stage<1>, dim3(b->n_stage_jobs), dim3(64), 0, b->stream, s1);
+  }
+  if (b->any_eta && from <= 2) {
+    if (int rc = enqueue_group_cg(b, 0, L.spec[0])) return rc;
+  }
+  if (b->any_eta && from <= 3) {
+    BatchFinArgs f = fin_args(b, b->rs, b->x, b->st[0], b->S[4], b->S[5], b->S[4], b->S[5]);
+    f.ctl = b->ctl;
+    BLAUNCH(b, kb_finalize1, dim3(nt), dim3(kTileX), 0, b->stream, f);
+  }
+  if (from <= 4) {
+    BatchStageArgs s2{};
+    s2.geom = b->geom;
+    s2.rs = b->rs;
+    s2.jobs = b->stage_jobs;
+    s2.ctl = b->ctl;
+    for (int q = 0; q < 6; ++q) {
+      s2.s[q] = b->S[q];
+      s2.u[q] = b->W[q];
+      s2.o1[q] = b->U[q];
+    }
+    BLAUNCH(b, kb_stage<2>, dim3(b->n_stage_jobs), dim3(64), 0, b->stream, s2);
+  }
+  if (b->any_eta && from <= 5) {
+    if (int rc = enqueue_group_cg(b, 1, L.spec[1])) return rc;
+  }
+  if (b->any_eta && from <= 6) {
+    BatchFinArgs f = fin_args(b, b->rs, b->x, b->st[1], b->U[4], b->U[5], b->S[4], b->S[5]);
+    f.ctl = b->ctl;
+    if (fused) BLAUNCH(b, kb_finalize2_cfl, dim3(nt), dim3(kTileX), 0, b->stream, f, d);
+    else BLAUNCH(b, kb_finalize2, dim3(nt), dim3(kTileX), 0, b->stream, f);
+    std::swap(b->U[4], b->S[4]);
+    std::swap(b->U[5], b->S[5]);
+  }
+  return MHD_OK;
+}
+
+// The two-step graph for the current B pointers, budgets and CFL fusion,
+// captured on first use (least recently used entry replaced).
+int group_graph(mhd_batch* b, BLoop& L, BatchGraph** out) {
+  double* key[4] = {b->U[4], b->U[5], b->S[4], b->S[5]};
+  const bool fused = b->fused_cfl != 0;
+  BatchGraph* victim = &b->graphs[0];
+  for (BatchGraph& g : b->graphs) {
+    if (g.exec && std::equal(key, key + 4, g.key) && g.spec[0] == L.spec[0] && g.spec[1] == L.spec[1] &&
+        g.fused == fused) {
+      g.last_use = ++b->gclock;
+      *out = &g;
+      return MHD_OK;
+    }
+    if (!g.exec || (victim->exec && g.last_use < victim->last_use)) victim = &g;
+  }
+  if (victim->exec) (void)hipGraphExecDestroy(victim->exec);
+  victim->exec = nullptr;
+  BHIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  int rc = enqueue_group_step(b, L, 0);
+  if (!rc) rc = enqueue_group_step(b, L, 0);     // two swaps: the B pointers are back where they started
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(b->stream, &g);
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) return bfail(b, MHD_EHIP, std::string("batch graph capture: ") + hipGetErrorString(e));
+  const hipError_t ei = hipGraphInstantiate(&victim->exec, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (ei != hipSuccess) {
+    victim->exec = nullptr;
+    return bfail(b, MHD_EHIP, std::string("batch graph instantiate: ") + hipGetErrorString(ei));
+  }
+  std::copy(key, key + 4, victim->key);
+  victim->spec[0] = L.spec[0];
+  victim->spec[1] = L.spec[1];
+  victim->fused = fused;
+  victim->last_use = ++b->gclock;
+  *out = victim;
+  return MHD_OK;
+}
+
+int read_group(mhd_batch* b) {
+  BHIP(b, hipMemcpyAsync(b->ctl_host, b->ctl, 4 * sizeof(int), hipMemcpyDeviceToHost, b->stream));
+  BHIP(b, hipMemcpyAsync(b->rs_host, b->rs, sizeof(RunState) * b->nruns, hipMemcpyDeviceToHost, b->stream));
+  BHIP(b, hipStreamSynchronize(b->stream));
+  return MHD_OK;
+}
+
+// Longest solve of each slot among the runs' current states.
+int read_budgets(mhd_batch* b, int most[2]) {
+  BHIP(b, hipMemcpyAsync(b->st_host, b->st[0], sizeof(CgState) * b->nruns, hipMemcpyDeviceToHost, b->stream));
+  BHIP(b, hipMemcpyAsync(b->st_host2, b->st[1], sizeof(CgState) * b->nruns, hipMemcpyDeviceToHost, b->stream));
+  BHIP(b, hipStreamSynchronize(b->stream));
+  most[0] = most[1] = 0;
+  for (int r = 0; r < b->nruns; ++r) {
+    if (b->st_host[r].done == 1 || b->st_host[r].done == 2) most[0] = std::max(most[0], b->st_host[r].spmv);
+    if (b->st_host2[r].done == 1 || b->st_host2[r].done == 2) most[1] = std::max(most[1], b->st_host2[r].spmv);
+  }
+  return MHD_OK;
+}
+
+// A halted solve: continue it with host-checked iterations from the last A
+// launched (A(spec + 1)), then record its counts.
+int resume_group_solve(mhd_batch* b, BLoop& L, int slot) {
+  const BatchCgArgs a = group_cg(b, slot);
+  const int nb = b->n_tile_jobs;
+  long long maxit = 0;
+  for (const RunGeom& g : b->geom_h) maxit = std::max(maxit, g.max_iters);
+  long long it = L.spec[slot] + 1;
+  for (;;) {
+    BLAUNCH(b, kb_cg_pass_b<false>, dim3(nb), dim3(kTileX), 0, b->stream, a, (const double*)b->p[it & 1], (int)it);
+    ++it;
+    BLAUNCH(b, kb_cg_pass_a, dim3(nb), dim3(kTileX), 0, b->stream, a, (const double*)b->p[(it - 1) & 1], b->p[it & 1],
+            (int)it);
+    BHIP(b, hipMemcpyAsync(b->st_host, b->st[slot], sizeof(CgState) * b->nruns, hipMemcpyDeviceToHost, b->stream));
+    BHIP(b, hipStreamSynchronize(b->stream));
+    bool all = true;
+    for (int r = 0; r < b->nruns; ++r) all = all && b->st_host[r].done;
+    if (all) break;
+    if (it > maxit + 4) return bfail(b, MHD_EHIP, "CG state machine did not terminate");
+  }
+  int most = 0;
+  for (int r = 0; r < b->nruns; ++r)
+    if (b->st_host[r].done == 1 || b->st_host[r].done == 2) most = std::max(most, b->st_host[r].spmv);
+  L.spec[slot] = std::max(L.spec[slot], most - 1 + b->margin);
+  BLAUNCH(b, kb_check_solve, dim3(1), dim3(256), 0, b->stream, (const RunState*)b->rs, (const CgState*)b->st[slot],
+          b->nruns, slot, b->o.cg_fail_is_error, b->ctl, b->iters);
+  return MHD_OK;
+}
+
+// Up to max_batch_steps (< 0: until every run has stopped) steps in groups.
+int run_groups(mhd_batch* b, int max_batch_steps, int* steps_done) {
+  const int nr = b->nruns;
+  BHIP(b, hipMemcpyAsync(b->rs_host, b->rs, sizeof(RunState) * nr, hipMemcpyDeviceToHost, b->stream));
+  BHIP(b, hipStreamSynchronize(b->stream));
+  std::vector<int> steps0(nr);
+  for (int r = 0; r < nr; ++r) steps0[r] = b->rs_host[r].steps;
+  BLoop L{};
+  for (int k = 0; k < 2; ++k) {
+    L.B0[k] = b->U[4 + k];
+    L.B1[k] = b->S[4 + k];
+    L.spec[k] = std::max(0, b->last_spmv[k] - 1) + b->margin;
+  }
+  BLAUNCH(b, kb_ctl_reset, dim3(1), dim3(64), 0, b->stream, b->ctl);
+  const bool fused = b->any_eta && b->fused_cfl;
+  if (fused && !b->cfl_fresh) {
+    const BatchDiagArgs d = group_diag(b);
+    BLAUNCH(b, kb_cfl, dim3(b->n_tile_jobs), dim3(kTileX), 0, b->stream, d);
+  }
+  b->cfl_fresh = false;
+  long long done = 0;
+  int rc = MHD_OK;
+  for (;;) {
+    if (max_batch_steps >= 0 && done >= max_batch_steps) break;
+    long long G = b->group;
+    if (max_batch_steps >= 0) G = std::min<long long>(G, max_batch_steps - done);
+    long long k = 0;
+    if (b->graph) {
+      for (; k + 2 <= G; k += 2) {
+        BatchGraph* g = nullptr;
+        if ((rc = group_graph(b, L, &g))) return rc;
+        BHIP(b, hipGraphLaunch(g->exec, b->stream));
+      }
+    }
+    for (; k < G; ++k)
+      if ((rc = enqueue_group_step(b, L, 0))) return rc;
+    if ((rc = read_group(b))) return rc;
+    long long began = b->ctl_host[kCtlSteps];
+    while (b->ctl_host[kCtlHalt] == kHaltSolve1 || b->ctl_host[kCtlHalt] == kHaltSolve2) {
+      const int slot = b->ctl_host[kCtlHalt] == kHaltSolve1 ? 0 : 1;
+      const long long kh = began - 1;                       // steps of the group finished before the halted one
+      set_bparity(b, L, done + kh);
+      BLAUNCH(b, kb_ctl_reset, dim3(1), dim3(64), 0, b->stream, b->ctl);
+      if ((rc = resume_group_solve(b, L, slot))) return rc;
+      if ((rc = enqueue_group_step(b, L, slot == 0 ? 3 : 6))) return rc;
+      for (long long q = kh + 1; q < G; ++q)
+        if ((rc = enqueue_group_step(b, L, 0))) return rc;
+      if ((rc = read_group(b))) return rc;
+      began = kh + 1 + b->ctl_host[kCtlSteps];
+    }
+    const int h = b->ctl_host[kCtlHalt];
+    if (h == kHaltNonfinite) return bfail(b, MHD_ENONFINITE, "non-finite wave speed (batched group of steps)");
+    if (h == kHaltCgLimit1 || h == kHaltCgLimit2)
+      return bfail(b, MHD_ECG, "a run's CG solve reached the iteration limit (batched group of steps)");
+    done += G;
+    set_bparity(b, L, done);
+    int act = 0;
+    for (int r = 0; r < nr; ++r) act += b->rs_host[r].active;
+    if (b->any_eta) {
+      int most[2];
+      if ((rc = read_budgets(b, most))) return rc;
+      for (int q = 0; q < 2; ++q)
+        if (most[q] > 0) {
+          L.spec[q] = most[q] - 1 + b->margin;
+          b->last_spmv[q] = most[q];
+        }
+    }
+    BLAUNCH(b, kb_ctl_reset, dim3(1), dim3(64), 0, b->stream, b->ctl);
+    if (act == 0) break;
+  }
+  BHIP(b, hipMemcpyAsync(b->iters_host, b->iters, 2 * sizeof(int) * nr, hipMemcpyDeviceToHost, b->stream));
+  BHIP(b, hipStreamSynchronize(b->stream));
+  int stepped = 0;
+  for (int r = 0; r < nr; ++r) {
+    const int ds = b->rs_host[r].steps - steps0[r];
+    stepped = std::max(stepped, ds);
+    if (ds > 0) {
+      b->cg_iters_last[2 * r] = b->iters_host[2 * r];
+      b->cg_iters_last[2 * r + 1] = b->iters_host[2 * r + 1];
+    }
+  }
+  b->steps_taken += stepped;
+  b->cfl_fresh = fused;
+  if (steps_done) *steps_done = stepped;
+  return MHD_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -320,8 +640,16 @@ int mhd_batch_create(int nruns, const int* nx, const int* ny, const double* dx, 
   for (int r = 0; r < nruns; ++r) largest = std::max(largest, (long long)nx[r] * ny[r]);
   if (hipHostMalloc(&b->rs_host, sizeof(RunState) * nruns) != hipSuccess ||
       hipHostMalloc(&b->st_host, sizeof(CgState) * nruns) != hipSuccess ||
+      hipHostMalloc(&b->st_host2, sizeof(CgState) * nruns) != hipSuccess ||
+      hipHostMalloc(&b->ctl_host, 4 * sizeof(int)) != hipSuccess ||
+      hipHostMalloc(&b->iters_host, 2 * sizeof(int) * nruns) != hipSuccess ||
       hipHostMalloc(&b->stage, sizeof(double) * largest) != hipSuccess)
     return bail("pinned allocation failed", MHD_ENOMEM);
+  if (hipMalloc(&b->ctl, 4 * sizeof(int)) != hipSuccess || hipMalloc(&b->iters, 2 * sizeof(int) * nruns) != hipSuccess)
+    return bail("device allocation failed", MHD_ENOMEM);
+  if (hipMemsetAsync(b->ctl, 0, 4 * sizeof(int), b->stream) != hipSuccess ||
+      hipMemsetAsync(b->iters, 0, 2 * sizeof(int) * nruns, b->stream) != hipSuccess)
+    return bail("hipMemset failed", MHD_EHIP);
   b->n_stage_jobs = (int)sj.size();
   b->n_tile_jobs = (int)tj.size();
   for (int q = 0; q < 6; ++q)
@@ -346,12 +674,16 @@ int mhd_batch_destroy(mhd_batch* b) {
     if (b->S[q]) (void)hipFree(b->S[q]);
     if (b->W[q]) (void)hipFree(b->W[q]);
   }
+  drop_graphs(b);
   void* bufs[] = {b->x, b->r, b->p[0], b->p[1], b->part, b->part_s, b->part_a, b->part_b, b->st[0], b->st[1],
-                  b->geom, b->rs, b->stage_jobs, b->tile_jobs};
+                  b->geom, b->rs, b->stage_jobs, b->tile_jobs, b->ctl, b->iters};
   for (void* p : bufs)
     if (p) (void)hipFree(p);
   if (b->rs_host) (void)hipHostFree(b->rs_host);
   if (b->st_host) (void)hipHostFree(b->st_host);
+  if (b->st_host2) (void)hipHostFree(b->st_host2);
+  if (b->ctl_host) (void)hipHostFree(b->ctl_host);
+  if (b->iters_host) (void)hipHostFree(b->iters_host);
   if (b->stage) (void)hipHostFree(b->stage);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
@@ -426,7 +758,7 @@ int mhd_batch_step(mhd_batch* b, int* n_active) {
   b->cfl_fresh = false;
   if (!fresh) BLAUNCH(b, kb_cfl, dim3(nt), dim3(kTileX), 0, b->stream, d);
   BLAUNCH(b, kb_dt, dim3(b->nruns), dim3(256), 0, b->stream, (const RunGeom*)b->geom, b->rs, (const double*)b->part,
-          b->st[0], b->st[1], b->nruns, step_factors(T), 0);
+          b->st[0], b->st[1], b->nruns, step_factors(T), 0, nullptr);
   BHIP(b, hipMemcpyAsync(b->rs_host, b->rs, sizeof(RunState) * b->nruns, hipMemcpyDeviceToHost, b->stream));
   // stage 1: W = U + dt delta f1 (into W), U2* = U + dt gamma f1 (into S)
   BatchStageArgs s1{};
@@ -494,8 +826,33 @@ int mhd_batch_step(mhd_batch* b, int* n_active) {
   return MHD_OK;
 }
 
+int mhd_batch_tune(mhd_batch* b, int group_steps, int spec_margin, int graph, int fused_cfl) {
+  if (!b) return MHD_EINVAL;
+  if (group_steps < 0 || group_steps > 4096 || spec_margin < 0 || (graph != 0 && graph != 1) ||
+      (fused_cfl != 0 && fused_cfl != 1))
+    return bfail(b, MHD_EINVAL, "mhd_batch_tune: group_steps in [0, 4096], spec_margin >= 0, graph and fused_cfl 0 or 1");
+  b->group = group_steps;
+  b->margin = spec_margin;
+  b->graph = graph;
+  if (b->fused_cfl != fused_cfl) b->cfl_fresh = false;
+  b->fused_cfl = fused_cfl;
+  return MHD_OK;
+}
+
+int mhd_batch_get_tuning(mhd_batch* b, int* stage_rows, int* tile_rows, int* group_steps, int* graph,
+                         int* fused_cfl) {
+  if (!b) return MHD_EINVAL;
+  if (stage_rows) *stage_rows = b->stage_rows;
+  if (tile_rows) *tile_rows = b->tile_rows;
+  if (group_steps) *group_steps = b->group;
+  if (graph) *graph = b->graph;
+  if (fused_cfl) *fused_cfl = b->fused_cfl;
+  return MHD_OK;
+}
+
 int mhd_batch_run(mhd_batch* b, int max_batch_steps, int* steps_done) {
   if (!b) return MHD_EINVAL;
+  if (b->group > 0) return run_groups(b, max_batch_steps, steps_done);
   int k = 0;
   for (; max_batch_steps < 0 || k < max_batch_steps; ++k) {
     int act = 0;
@@ -542,23 +899,46 @@ int mhd_batch_autotune(mhd_batch* b, int trial_steps, int* stage_rows, int* tile
   }
   double best = INFINITY;
   int best_s = b->stage_rows, best_t = b->tile_rows;
+  const int graph_saved = b->graph, fused_saved = b->fused_cfl;
+  int best_g = b->graph, best_f = b->fused_cfl;
   int rc = MHD_OK;
+  // time trial_steps steps of the current configuration after a warm-up
+  // (two steps: a graph configuration captures its pair there), then put the
+  // state back
+  auto trial = [&](double& total) -> int {
+    total = 0.0;
+    int r = MHD_OK;
+    if (b->group > 0) {
+      int nd = 0;
+      if ((r = mhd_batch_run(b, 2, &nd))) return r;
+      if (hipStreamSynchronize(b->stream) != hipSuccess) return bfail(b, MHD_EHIP, "autotune: sync failed");
+      auto t0 = std::chrono::steady_clock::now();
+      r = mhd_batch_run(b, trial_steps, &nd);
+      auto t1 = std::chrono::steady_clock::now();
+      total = std::chrono::duration<double>(t1 - t0).count();
+    } else {
+      for (int k = 0; k < trial_steps + 1 && r == MHD_OK; ++k) {
+        if (hipStreamSynchronize(b->stream) != hipSuccess) return bfail(b, MHD_EHIP, "autotune: sync failed");
+        auto t0 = std::chrono::steady_clock::now();
+        r = mhd_batch_step(b, nullptr);
+        auto t1 = std::chrono::steady_clock::now();
+        if (k > 0) total += std::chrono::duration<double>(t1 - t0).count();   // the first step warms the shape up
+      }
+    }
+    for (int q = 0; q < 6; ++q)
+      (void)hipMemcpyAsync(b->U[q], saved[q], fb, hipMemcpyDeviceToDevice, b->stream);
+    (void)hipMemcpyAsync(b->rs, rs_saved, sizeof(RunState) * b->nruns, hipMemcpyDeviceToDevice, b->stream);
+    for (int k = 0; k < 3; ++k) b->last_spmv[k] = spmv_saved[k];
+    b->cfl_fresh = false;
+    return r;
+  };
+  // launch shapes (in the configured launch mode)
   for (int sr : kStageRowChoices) {
     for (int tr : kTileRowChoices) {
       rc = replan(b, sr, tr);
       if (rc) break;
       double total = 0.0;
-      for (int k = 0; k < trial_steps + 1 && rc == MHD_OK; ++k) {
-        if (hipStreamSynchronize(b->stream) != hipSuccess) { rc = bfail(b, MHD_EHIP, "autotune: sync failed"); break; }
-        auto t0 = std::chrono::steady_clock::now();
-        rc = mhd_batch_step(b, nullptr);
-        auto t1 = std::chrono::steady_clock::now();
-        if (k > 0) total += std::chrono::duration<double>(t1 - t0).count();   // the first step warms the shape up
-      }
-      for (int q = 0; q < 6; ++q)
-        (void)hipMemcpyAsync(b->U[q], saved[q], fb, hipMemcpyDeviceToDevice, b->stream);
-      (void)hipMemcpyAsync(b->rs, rs_saved, sizeof(RunState) * b->nruns, hipMemcpyDeviceToDevice, b->stream);
-      for (int k = 0; k < 3; ++k) b->last_spmv[k] = spmv_saved[k];
+      rc = trial(total);
       if (rc) break;
       if (total < best) {
         best = total;
@@ -568,6 +948,27 @@ int mhd_batch_autotune(mhd_batch* b, int trial_steps, int* stage_rows, int* tile
     }
     if (rc) break;
   }
+  // stage fusion and graph replay of the device-resident groups, with the
+  // best shape: the next step's CFL partials formed by the final
+  // recombination or by a separate pass, direct launches or captured pairs
+  if (rc == MHD_OK && b->group > 0 && b->any_eta) {
+    rc = replan(b, best_s, best_t);
+    for (int f = 1; f >= 0 && rc == MHD_OK; --f)
+      for (int g = 0; g <= 1 && rc == MHD_OK; ++g) {
+        b->fused_cfl = f;
+        b->graph = g;
+        double total = 0.0;
+        rc = trial(total);
+        if (rc) break;
+        if (total < best) {
+          best = total;
+          best_f = f;
+          best_g = g;
+        }
+      }
+  }
+  b->graph = (rc == MHD_OK) ? best_g : graph_saved;
+  b->fused_cfl = (rc == MHD_OK) ? best_f : fused_saved;
   b->geom_h = geom_saved;
   b->cg_iters_last = its_saved;
   b->steps_taken = steps_saved;

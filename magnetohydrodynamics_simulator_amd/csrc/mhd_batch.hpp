@@ -23,6 +23,17 @@ struct BatchTableau {
 
 struct BatchAdjoint;   // mhd_batch_adjoint.hip
 
+// Two batch steps captured as a graph, valid for the B pointers, CG budgets
+// and CFL fusion it was captured with.
+struct BatchGraph {
+  hipGraphExec_t exec = nullptr;
+  double* key[4] = {};
+  int spec[2] = {0, 0};
+  bool fused = true;
+  unsigned long long last_use = 0;
+};
+constexpr int kBatchGraphCache = 4;
+
 }  // namespace mhd_batch_host
 
 struct mhd_batch {
@@ -63,6 +74,18 @@ struct mhd_batch {
   long long steps_taken = 0;
   std::vector<int> cg_iters_last;   // 2 per run: Eigen iteration counts of the last step
   double* stage = nullptr;          // pinned staging buffer of the largest run (host transfers)
+  // device-resident groups of steps (mhd_batch_run; mhd_batch_tune)
+  int group = 16;                   // steps enqueued between two read-backs (0: one host-checked step at a time)
+  int margin = 0;                   // extra launched CG iterations per solve
+  int graph = 0;                    // 1: replay pairs of steps as captured graphs
+  int fused_cfl = 1;                // 1: the final recombination forms the next CFL partials; 0: a separate pass
+  int* ctl = nullptr;               // device control word (kCtlHalt, kCtlSteps)
+  int* ctl_host = nullptr;          // pinned
+  int* iters = nullptr;             // device: 2 per run, the last step's Eigen iteration counts
+  int* iters_host = nullptr;        // pinned
+  mhd::CgState* st_host2 = nullptr; // pinned, the other solve slot
+  mhd_batch_host::BatchGraph graphs[mhd_batch_host::kBatchGraphCache];
+  unsigned long long gclock = 0;
   mhd_batch_host::BatchAdjoint* adj = nullptr;
   std::string err;
 };

@@ -507,7 +507,7 @@ int record_step(mhd_batch* b, int s, int k, bool given) {
     d.part = b->part;
     BLAUNCH(b, kb_cfl, dim3(nt), dim3(kTileX), 0, b->stream, d);
     BLAUNCH(b, kb_dt, dim3(nr), dim3(256), 0, b->stream, (const RunGeom*)b->geom, b->rs, (const double*)b->part, st0,
-            st1, nr, step_factors(T), 0);
+            st1, nr, step_factors(T), 0, nullptr);
     BHIP(b, hipMemcpyAsync(b->rs_host, b->rs, sizeof(RunState) * nr, hipMemcpyDeviceToHost, b->stream));
   }
   // stage 1: W = U + dt delta f1, S* = U + dt gamma f1 (S*.B into SsB)

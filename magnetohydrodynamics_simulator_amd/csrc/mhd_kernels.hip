@@ -1092,6 +1092,7 @@ __global__ __launch_bounds__(256) void k_copy(const double* __restrict__ src, do
 
 template <int MODE>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void kb_stage(BatchStageArgs b) {
+  if (b.ctl && b.ctl[kCtlHalt]) return;
   const int2 job = b.jobs[blockIdx.x];
   const RunState& rs = b.rs[job.x];
   if (!rs.active) return;
@@ -1151,6 +1152,7 @@ __device__ __forceinline__ CgArgs batch_cg_args(const BatchCgArgs& b, int run) {
 }
 
 __global__ __launch_bounds__(kTileX) void kb_cg_start(BatchCgArgs b) {
+  if (b.ctl && b.ctl[kCtlHalt]) return;
   const int2 job = b.jobs[xcd_block(blockIdx.x, gridDim.x)];
   if (b.st[job.x].done) return;            // stopped run, or eta = 0: no solve
   const CgArgs a = batch_cg_args(b, job.x);
@@ -1159,6 +1161,7 @@ __global__ __launch_bounds__(kTileX) void kb_cg_start(BatchCgArgs b) {
 
 // Start of a solve with the given right-hand side bin (pooled).
 __global__ __launch_bounds__(kTileX) void kb_cg_start_rhs(BatchCgArgs b, const double* __restrict__ bin) {
+  if (b.ctl && b.ctl[kCtlHalt]) return;
   const int2 job = b.jobs[xcd_block(blockIdx.x, gridDim.x)];
   if (b.st[job.x].done) return;
   const CgArgs a = batch_cg_args(b, job.x);
@@ -1167,6 +1170,7 @@ __global__ __launch_bounds__(kTileX) void kb_cg_start_rhs(BatchCgArgs b, const d
 
 __global__ __launch_bounds__(kTileX) void kb_cg_pass_a(BatchCgArgs b, const double* __restrict__ pold,
                                                        double* __restrict__ pnew, int it) {
+  if (b.ctl && b.ctl[kCtlHalt]) return;
   const int2 job = b.jobs[xcd_block(blockIdx.x, gridDim.x)];
   if (b.st[job.x].done) return;
   const CgArgs a = batch_cg_args(b, job.x);
@@ -1176,6 +1180,7 @@ __global__ __launch_bounds__(kTileX) void kb_cg_pass_a(BatchCgArgs b, const doub
 
 template <bool FIRST>
 __global__ __launch_bounds__(kTileX) void kb_cg_pass_b(BatchCgArgs b, const double* __restrict__ pv, int it) {
+  if (b.ctl && b.ctl[kCtlHalt]) return;
   const int2 job = b.jobs[xcd_block(blockIdx.x, gridDim.x)];
   if (b.st[job.x].done) return;
   const CgArgs a = batch_cg_args(b, job.x);
@@ -1206,6 +1211,7 @@ __device__ __forceinline__ FinArgs batch_fin_args(const BatchFinArgs& b, int run
 }
 
 __global__ __launch_bounds__(kTileX) void kb_finalize1(BatchFinArgs b) {
+  if (b.ctl && b.ctl[kCtlHalt]) return;
   const int2 job = b.jobs[xcd_block(blockIdx.x, gridDim.x)];
   if (!b.rs[job.x].active) return;
   finalize1_body(batch_fin_args(b, job.x), job.y);
@@ -1214,6 +1220,7 @@ __global__ __launch_bounds__(kTileX) void kb_finalize1(BatchFinArgs b) {
 // The host swaps the B arrays after this kernel for all runs, so a run that
 // no longer steps copies its B across unchanged.
 __global__ __launch_bounds__(kTileX) void kb_finalize2(BatchFinArgs b) {
+  if (b.ctl && b.ctl[kCtlHalt]) return;
   const int2 job = b.jobs[xcd_block(blockIdx.x, gridDim.x)];
   const FinArgs f = batch_fin_args(b, job.x);
   if (b.rs[job.x].active) {
@@ -1232,6 +1239,7 @@ __global__ __launch_bounds__(kTileX) void kb_finalize2(BatchFinArgs b) {
 // Per-tile CFL partials of every run that has not reached its end yet (the
 // driver state still holds the previous step's activity; kb_dt decides).
 __global__ __launch_bounds__(kTileX) void kb_cfl(BatchDiagArgs b) {
+  if (b.ctl && b.ctl[kCtlHalt]) return;
   const int2 job = b.jobs[xcd_block(blockIdx.x, gridDim.x)];
   const RunGeom& g = b.geom[job.x];
   const RunState& rs = b.rs[job.x];
@@ -1273,6 +1281,7 @@ __device__ __forceinline__ DiagArgs batch_diag_args(const BatchDiagArgs& b, int 
 // the next step needs no kb_cfl.  Runs that did not step copy their B and
 // keep no partials (a run that did not step this time does not step next).
 __global__ __launch_bounds__(kTileX) void kb_finalize2_cfl(BatchFinArgs b, BatchDiagArgs dg) {
+  if (b.ctl && b.ctl[kCtlHalt]) return;
   const int2 job = b.jobs[xcd_block(blockIdx.x, gridDim.x)];
   const FinArgs f = batch_fin_args(b, job.x);
   if (b.rs[job.x].active) {
@@ -1297,11 +1306,13 @@ __global__ __launch_bounds__(kTileX) void kb_finalize2_cfl(BatchFinArgs b, Batch
 __global__ __launch_bounds__(256) void kb_dt(const RunGeom* __restrict__ geom, RunState* __restrict__ rs,
                                              const double* __restrict__ part, CgState* __restrict__ st0,
                                              CgState* __restrict__ st1, int nruns, StepFactors f,
-                                             int halt_on_nonfinite) {
+                                             int halt_on_nonfinite, int* __restrict__ ctl) {
   const double a_im4 = f.im4;
   __shared__ double lds[16];
   const int run = blockIdx.x;
   if (run >= nruns) return;
+  if (ctl && ctl[kCtlHalt]) return;          // halted group: nothing moves until the host resumes
+  if (ctl && run == 0 && threadIdx.x == 0) ctl[kCtlSteps] += 1;
   const RunGeom& g = geom[run];
   RunState s = rs[run];
   if (s.halt) return;                        // halted resident loop: nothing moves until the host resumes
@@ -1325,7 +1336,8 @@ __global__ __launch_bounds__(256) void kb_dt(const RunGeom* __restrict__ geom, R
   } else {
     if (halt_on_nonfinite && g.dt_fixed <= 0.0 && bad[0] > 0.0) {
       s.nonfinite = 1;                       // the step is not taken (mhd_cfl_dt fails before it)
-      s.halt = kHaltNonfinite;
+      if (ctl) ctl[kCtlHalt] = kHaltNonfinite;
+      else s.halt = kHaltNonfinite;
       rs[run] = s;
       return;
     }
@@ -1496,6 +1508,29 @@ __global__ __launch_bounds__(kTileX) void k_cfl_res(DiagArgs a, const RunState* 
 
 __global__ void k_clear_halt(RunState* rs) {
   if (blockIdx.x == 0 && threadIdx.x == 0) rs->halt = 0;
+}
+
+__global__ __launch_bounds__(256) void kb_check_solve(const RunState* __restrict__ rs, const CgState* __restrict__ st,
+                                                      int nruns, int slot, int fail_is_error, int* __restrict__ ctl,
+                                                      int* __restrict__ iters) {
+  if (ctl[kCtlHalt]) return;
+  __shared__ int code;
+  if (threadIdx.x == 0) code = 0;
+  __syncthreads();
+  for (int r = threadIdx.x; r < nruns; r += blockDim.x) {
+    if (!rs[r].active) continue;
+    const int done = st[r].done;
+    if (done == 0) atomicMax(&code, 2);
+    else if (done == 2 && fail_is_error) atomicMax(&code, 1);
+    iters[2 * r + slot] = st[r].iters;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0 && code != 0)
+    ctl[kCtlHalt] = (code == 2) ? (slot == 0 ? kHaltSolve1 : kHaltSolve2) : (slot == 0 ? kHaltCgLimit1 : kHaltCgLimit2);
+}
+
+__global__ void kb_ctl_reset(int* ctl) {
+  if (blockIdx.x == 0 && threadIdx.x < 4) ctl[threadIdx.x] = 0;
 }
 
 }  // namespace mhd

@@ -140,6 +140,7 @@ struct BatchStageArgs {
   const double* u[6];
   double* o1[6];
   double* o2[6];
+  const int* ctl;          // batch control word (kCtlHalt, kCtlSteps); null: none
 };
 
 struct BatchCgArgs {
@@ -156,6 +157,7 @@ struct BatchCgArgs {
   double* part_b;
   CgState* st;             // one state per run for this solve slot
   double tol;
+  const int* ctl;
 };
 
 struct BatchFinArgs {
@@ -168,6 +170,7 @@ struct BatchFinArgs {
   double* bx_out;
   double* by_out;
   const CgState* st;
+  const int* ctl;
 };
 
 struct BatchDiagArgs {
@@ -176,7 +179,17 @@ struct BatchDiagArgs {
   const int2* jobs;
   const double* s[6];
   double* part;
+  const int* ctl;
 };
+
+// Control word of a batch's device-resident group of steps (an int[4] on the
+// device): [0] halt code (0 running; kHaltSolve1 / kHaltSolve2: a solve left
+// unfinished after its launched iterations; kHaltNonfinite; kHaltCgLimit1 /
+// 2: a solve at its iteration limit with opts.cg_fail_is_error), [1] batch
+// steps begun since the group started (kb_dt), [2], [3] unused.  Once [0] is
+// set every batched kernel returns at once until the host clears it.
+constexpr int kCtlHalt = 0;
+constexpr int kCtlSteps = 1;
 
 template <int MODE> __global__ void kb_stage(BatchStageArgs b);
 __global__ void kb_cg_start(BatchCgArgs b);
@@ -188,7 +201,13 @@ __global__ void kb_finalize2(BatchFinArgs b);
 __global__ void kb_cfl(BatchDiagArgs b);
 __global__ void kb_finalize2_cfl(BatchFinArgs b, BatchDiagArgs d);
 __global__ void kb_dt(const RunGeom* geom, RunState* rs, const double* part, CgState* st0, CgState* st1,
-                      int nruns, StepFactors f, int halt_on_nonfinite);
+                      int nruns, StepFactors f, int halt_on_nonfinite, int* ctl);
+// After a batch's solve `slot` (0, 1): halts the group (ctl[kCtlHalt]) if a
+// stepping run's solve is unfinished, or at its limit with fail_is_error;
+// records each stepping run's iteration count in iters[2 run + slot].
+__global__ void kb_check_solve(const RunState* rs, const CgState* st, int nruns, int slot, int fail_is_error,
+                               int* ctl, int* iters);
+__global__ void kb_ctl_reset(int* ctl);
 
 // Single-run resident loop: the plain kernels with their step coefficients
 // read from the device driver state (c = dt * f), returning at once once the

@@ -171,7 +171,7 @@ int enqueue_step(mhd_ctx* c, Loop& L, int from) {
   int rc;
   if (from <= 0) {
     rc = launch_k(c, 8, kb_dt, dim3(1), dim3(256), (const RunGeom*)R->geom, R->rs, (const double*)c->part, R->st,
-                  R->st + 1, 1, step_factors(T), 1);
+                  R->st + 1, 1, step_factors(T), 1, nullptr);
     if (rc) return rc;
   }
   if (from <= 1) {

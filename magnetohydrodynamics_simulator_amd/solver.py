@@ -446,6 +446,19 @@ class Batch:
         self._check(self.lib.mhd_batch_autotune(self.h, trial_steps, C.byref(sr), C.byref(tr)))
         return sr.value, tr.value
 
+    def tune(self, group_steps: int = 16, spec_margin: int = 0, graph: bool = False, fused_cfl: bool = True):
+        """Launch mode of run(): steps per device-resident group (0: one
+        host-checked step at a time), extra launched CG iterations per solve,
+        graph replay of step pairs, CFL partials fused into the final
+        recombination.  Results are identical in every mode."""
+        self._check(self.lib.mhd_batch_tune(self.h, group_steps, spec_margin, int(bool(graph)), int(bool(fused_cfl))))
+
+    def tuning(self) -> dict:
+        """The current launch shape and mode."""
+        v = [C.c_int() for _ in range(5)]
+        self._check(self.lib.mhd_batch_get_tuning(self.h, *[C.byref(x) for x in v]))
+        return dict(zip(("stage_rows", "tile_rows", "group_steps", "graph", "fused_cfl"), (x.value for x in v)))
+
     def run_info(self, run: int):
         steps, t, dt = C.c_int(), C.c_double(), C.c_double()
         its = (C.c_int * 2)()

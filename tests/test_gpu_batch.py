@@ -179,3 +179,75 @@ def test_state_and_schedule_changes_between_steps():
             for k in CONSERVED:
                 e = rel_l1(interior(got[k]), interior(ref[k]))
                 assert e <= 1e-12, (r, k, e)
+
+
+# launch modes of mhd_batch_run (mhd_batch_tune); group_steps = 0 is the
+# host-checked step-at-a-time path every other mode must reproduce bit for bit
+MODES = [
+    dict(group_steps=16),
+    dict(group_steps=16, graph=True),
+    dict(group_steps=5, graph=True, fused_cfl=False),
+    dict(group_steps=3, fused_cfl=False),
+    dict(group_steps=1),
+    dict(group_steps=7, spec_margin=2, graph=True),
+]
+
+
+def _mode_run(states, tol, cfl, mode, t_end, calls):
+    with _batch(states, tol, cfl) as b:
+        b.tune(**mode)
+        b.set_schedule(t_end=t_end)
+        done = [b.run(n) for n in calls]
+        out = [b.download(r) for r in range(len(states))]
+        info = [b.run_info(r) for r in range(len(states))]
+    return done, out, info
+
+
+def test_group_modes_match_step_path():
+    """Device-resident groups (CG budgets, halt and host resume, graph
+    replay, fused or separate CFL partials) against mhd_batch_step in a
+    loop: every field, the steps taken, time, last dt and CG counts of every
+    run identical.  The first group of a batch starts with a budget of one
+    product per solve, so it halts in its first solve and resumes; runs end
+    at different times inside groups."""
+    tol, cfl = 1e-10, 0.4
+    states = _states()
+    t_end = [0.03, 0.02, 0.045, 0.011, 1e9]
+    calls = [9, 6]
+    _say("batch launch modes: host-checked steps (reference)")
+    ref_done, ref, ref_info = _mode_run(states, tol, cfl, dict(group_steps=0), t_end, calls)
+    assert ref_done[0] == 9 and len({i[0] for i in ref_info}) > 1     # runs end at different steps
+    for mode in MODES:
+        _say(f"batch launch mode {mode} vs host-checked steps")
+        done, out, info = _mode_run(states, tol, cfl, mode, t_end, calls)
+        assert done == ref_done, (done, ref_done)
+        for r in range(len(states)):
+            assert info[r][0] == ref_info[r][0] and info[r][1] == ref_info[r][1], (r, info[r], ref_info[r])
+            assert info[r][2] == ref_info[r][2] and list(info[r][3]) == list(ref_info[r][3]), (r, info[r], ref_info[r])
+            for k in CONSERVED:
+                assert np.array_equal(interior(out[r][k]), interior(ref[r][k])), (mode, r, k)
+
+
+def test_autotune_chooses_fusion_and_launch_mode():
+    """The autotuner also picks the CFL fusion and graph replay of the
+    device-resident groups; whatever it picks, the run equals the untuned
+    one to the CG tiling's summation order."""
+    _say("batch autotune over shapes, CFL fusion and graph replay")
+    tol, cfl, steps = 1e-10, 0.4, 6
+    states = _states()
+    with _batch(states, tol, cfl) as b0:
+        b0.set_schedule(max_steps=steps)
+        b0.run()
+        ref = [b0.download(r) for r in range(len(states))]
+    with _batch(states, tol, cfl) as b:
+        b.set_schedule(max_steps=steps)
+        b.autotune(trial_steps=2)
+        t = b.tuning()
+        print(f"[case] chosen {t}", flush=True)
+        assert t["group_steps"] == 16 and t["graph"] in (0, 1) and t["fused_cfl"] in (0, 1)
+        b.run()
+        for r in range(len(states)):
+            assert b.run_info(r)[0] == steps
+            got = b.download(r)
+            for k in CONSERVED:
+                assert rel_l1(interior(got[k]), interior(ref[r][k])) <= 1e-12, (r, k)
