@@ -31,7 +31,9 @@ namespace mhd {
 //   dU(k-2)    flux divergence + CT of row k-2         (F, G, Ez)
 // so output row k-2 is final once row k has been loaded; a segment of H rows
 // takes H + 4 row steps.  Neighbouring columns move between lanes with
-// cross-lane shuffles; no LDS and no barriers are used.
+// cross-lane shuffles; no barriers are used (a workgroup is one wavefront).
+// Stage 1 keeps each row's point values for its output two rows later in a
+// per-wavefront LDS ring instead of re-reading them from memory.
 // ---------------------------------------------------------------------------
 
 struct Carry {
@@ -84,6 +86,12 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, int strip, int se
 
   Carry s = {};
   s.byf_c = a.s[5][bc_wrap(y0 - 2, ny, oy) * nx + c0];
+  // stage 1: the point values of row k (rho, mx, my, E, bx and by faces) are
+  // kept for two rows in a per-wave LDS ring instead of being re-read from
+  // memory when row k is output
+  __shared__ double ring[MODE == 1 ? 3 : 1][6][64];
+  int slot = 0;
+
   s.bxl_b = 0.0;
   // row loads are issued one row ahead of their use so that their latency
   // overlaps the flux arithmetic of the current row
@@ -106,8 +114,13 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, int strip, int se
       n_rho = a.s[0][bn + c0]; n_mx = a.s[1][bn + c0]; n_my = a.s[2][bn + c0]; n_E = a.s[3][bn + c0];
       n_bxl = a.s[4][bn + c0]; n_bxr = a.s[4][bn + c1]; n_byn = a.s[5][bnn + c0];
     }
+    if (MODE == 1) {
+      ring[slot][0][lane] = rho; ring[slot][1][lane] = mx; ring[slot][2][lane] = my;
+      ring[slot][3][lane] = E; ring[slot][4][lane] = bxl; ring[slot][5][lane] = s.byf_c;
+    }
     // ---- P(k) ----
     const double bxc = (bxl + bxr) / 2.0;
+
     const double byc = (s.byf_c + byn) / 2.0;
     double c_rho, c_vx, c_vy, c_p;
     primitives(rho, mx, my, E, bxc, byc, c_rho, c_vx, c_vy, c_p);
@@ -192,8 +205,10 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, int strip, int se
       } else if (MODE == 1) {
         // all point loads before any store: no load has to wait for a store
         double un[6];
+        const int old = (slot == 2) ? 0 : slot + 1;    // the slot of row k - 2
 #pragma unroll
-        for (int q = 0; q < 6; ++q) un[q] = a.s[q][o];
+        for (int q = 0; q < 6; ++q) un[q] = ring[old][q][lane];
+
 #pragma unroll
         for (int q = 0; q < 6; ++q) {
           // W = U^n + dt delta f1 (stage-3 base): next read by stage 2
@@ -239,8 +254,10 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, int strip, int se
     s.j_a = j_b;
     s.bxl_a = s.bxl_b;
     s.bxl_b = bxl;
+    slot = (slot == 2) ? 0 : slot + 1;
   }
 }
+
 
 #ifndef MHD_STAGE_WAVES
 #define MHD_STAGE_WAVES 3      // waves per SIMD the single-run stage kernel is compiled for (tuning parameter)
