@@ -698,7 +698,11 @@ int sweep(mhd_ctx* c, bool spec) {
   if (rc) return rc;
   rc = backward(c, spec);
   if (rc) return rc;
-  return copy_fields(c, c->U, A.keep, 6);
+  rc = copy_fields(c, c->U, A.keep, 6);
+  if (rc) return rc;
+  // the recompute overwrote the unfloored densities: the next CFL step reads
+  // the final state's primitives
+  return reset_raw_density(c);
 }
 
 // mhd_adjoint_backward with the final adjoint read from, and the initial one
@@ -931,6 +935,7 @@ int mhd_adjoint_forward_frozen(mhd_ctx* c, int nsteps, double eta) {
     int rc = frozen_step(c, s, eta, cx, cy, sweeps);
     if (rc) return rc;
   }
+  if (int rc = reset_raw_density(c)) return rc;
   HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
   return MHD_OK;
 }

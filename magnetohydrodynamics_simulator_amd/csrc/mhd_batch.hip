@@ -281,6 +281,7 @@ BatchDiagArgs group_diag(mhd_batch* b) {
   for (int q = 0; q < 6; ++q) d.s[q] = b->U[q];
   d.part = b->part;
   d.ctl = b->ctl;
+  d.rho_raw = b->rho_raw;
   return d;
 }
 
@@ -355,6 +356,7 @@ int enqueue_group_step(mhd_batch* b, BLoop& L, int from) {
     s2.rs = b->rs;
     s2.jobs = b->stage_jobs;
     s2.ctl = b->ctl;
+    s2.rho_raw = b->rho_raw;
     for (int q = 0; q < 6; ++q) {
       s2.s[q] = b->S[q];
       s2.u[q] = b->W[q];
@@ -628,6 +630,7 @@ int mhd_batch_create(int nruns, const int* nx, const int* ny, const double* dx, 
       return bail("device allocation failed", MHD_ENOMEM);
   const size_t pb = b->cap_tiles * sizeof(double);
   if (hipMalloc(&b->x, fb) != hipSuccess || hipMalloc(&b->r, fb) != hipSuccess || hipMalloc(&b->p[0], fb) != hipSuccess ||
+      hipMalloc(&b->rho_raw, fb) != hipSuccess ||
       hipMalloc(&b->p[1], fb) != hipSuccess || hipMalloc(&b->part, 2 * pb) != hipSuccess ||
       hipMalloc(&b->part_s, 3 * pb) != hipSuccess || hipMalloc(&b->part_a, pb) != hipSuccess ||
       hipMalloc(&b->part_b, 2 * pb) != hipSuccess || hipMalloc(&b->st[0], sizeof(CgState) * nruns) != hipSuccess ||
@@ -654,6 +657,7 @@ int mhd_batch_create(int nruns, const int* nx, const int* ny, const double* dx, 
   b->n_tile_jobs = (int)tj.size();
   for (int q = 0; q < 6; ++q)
     if (hipMemsetAsync(b->U[q], 0, fb, b->stream) != hipSuccess) return bail("hipMemset failed", MHD_EHIP);
+  if (hipMemsetAsync(b->rho_raw, 0, fb, b->stream) != hipSuccess) return bail("hipMemset failed", MHD_EHIP);
   std::memset(b->rs_host, 0, sizeof(RunState) * nruns);
   if (hipMemcpyAsync(b->geom, b->geom_h.data(), sizeof(RunGeom) * nruns, hipMemcpyHostToDevice, b->stream) != hipSuccess ||
       hipMemcpyAsync(b->rs, b->rs_host, sizeof(RunState) * nruns, hipMemcpyHostToDevice, b->stream) != hipSuccess ||
@@ -676,7 +680,7 @@ int mhd_batch_destroy(mhd_batch* b) {
   }
   drop_graphs(b);
   void* bufs[] = {b->x, b->r, b->p[0], b->p[1], b->part, b->part_s, b->part_a, b->part_b, b->st[0], b->st[1],
-                  b->geom, b->rs, b->stage_jobs, b->tile_jobs, b->ctl, b->iters};
+                  b->geom, b->rs, b->stage_jobs, b->tile_jobs, b->ctl, b->iters, b->rho_raw};
   for (void* p : bufs)
     if (p) (void)hipFree(p);
   if (b->rs_host) (void)hipHostFree(b->rs_host);
@@ -726,6 +730,9 @@ int mhd_batch_upload(mhd_batch* b, int run, const double* rho, const double* mx,
   const long long n = (long long)g.nx * g.ny;
   const int blocks = (int)std::min<long long>((n + 255) / 256, 4096);
   BLAUNCH(b, k_floor_rho, dim3(blocks), dim3(256), 0, b->stream, b->U[0] + g.base, (int)n);
+  // the run's first CFL step reads the primitives of the floored state
+  BLAUNCH(b, k_raw_reset, dim3(blocks), dim3(256), 0, b->stream, (const double*)b->U[0] + g.base, b->rho_raw + g.base,
+          (int)n);
   // a fresh state restarts the run's driver
   RunState z{};
   BHIP(b, hipMemcpyAsync(b->rs + run, &z, sizeof(RunState), hipMemcpyHostToDevice, b->stream));
@@ -753,6 +760,7 @@ int mhd_batch_step(mhd_batch* b, int* n_active) {
   d.jobs = b->tile_jobs;
   for (int q = 0; q < 6; ++q) d.s[q] = b->U[q];
   d.part = b->part;
+  d.rho_raw = b->rho_raw;
   // the previous step's final recombination formed the partials already
   const bool fresh = b->cfl_fresh;
   b->cfl_fresh = false;
@@ -789,6 +797,7 @@ int mhd_batch_step(mhd_batch* b, int* n_active) {
   s2.geom = b->geom;
   s2.rs = b->rs;
   s2.jobs = b->stage_jobs;
+  s2.rho_raw = b->rho_raw;
   for (int q = 0; q < 6; ++q) {
     s2.s[q] = b->S[q];
     s2.u[q] = b->W[q];

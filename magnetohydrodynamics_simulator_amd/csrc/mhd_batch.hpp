@@ -52,6 +52,7 @@ struct mhd_batch {
   double* W[6] = {};
   double* x = nullptr;
   double* r = nullptr;
+  double* rho_raw = nullptr;        // pooled: the unfloored density of floored cells (DiagArgs.rho_raw)
   double* p[2] = {};
   double* part = nullptr;           // CFL: two partials per tile
   double* part_s = nullptr;

@@ -76,6 +76,7 @@ StageArgs stage_args(mhd_ctx* c) {
   a.inv_dx = 1.0 / c->dx;
   a.inv_dy = 1.0 / c->dy;
   a.eta = c->eta;
+  a.rho_raw = c->rho_raw;
   return a;
 }
 
@@ -225,6 +226,7 @@ DiagArgs diag_args(mhd_ctx* c, bool fin) {
   d.dy = c->dy;
   for (int q = 0; q < 6; ++q) d.s[q] = c->U[q];
   d.part = c->part;
+  d.rho_raw = c->rho_raw;
   d.ty = fin ? c->fin_rows : c->tile_rows;
   d.tiles_y = fin ? c->fin_tiles_y : c->tiles_y;
   return d;
@@ -279,6 +281,13 @@ int from_host_padded(mhd_ctx* c, const double* host, double* dev) {
 }
 
 int check_ctx(mhd_ctx* c) { return c ? MHD_OK : MHD_EINVAL; }
+
+int reset_raw_density(mhd_ctx* c) {
+  return launch(c, 8, [&] {
+    hipLaunchKernelGGL(k_raw_reset, dim3(grid_1d(c->n)), dim3(256), 0, c->stream, (const double*)c->U[0], c->rho_raw,
+                       (int)c->n);
+  });
+}
 
 int copy_fields(mhd_ctx* c, double* const* dst, double* const* src, int n) {
   CopyArgs a{};
@@ -469,6 +478,7 @@ int mhd_create(int nx, int ny, double dx, double dy, double eta, const mhd_opts*
   const size_t pb = (size_t)c->tiles * 2 * sizeof(double);
   const size_t pd = (size_t)std::max(c->tiles, c->fin_tiles) * 2 * sizeof(double);   // part, part_host
   if (hipMalloc(&c->x, fb) != hipSuccess || hipMalloc(&c->r, fb) != hipSuccess || hipMalloc(&c->p[0], fb) != hipSuccess ||
+      hipMalloc(&c->rho_raw, fb) != hipSuccess ||
       hipMalloc(&c->p[1], fb) != hipSuccess || hipMalloc(&c->part, pd) != hipSuccess ||
       hipMalloc(&c->part_s, pb * 2) != hipSuccess || hipMalloc(&c->part_a, pb) != hipSuccess ||
       hipMalloc(&c->part_b, pb) != hipSuccess ||
@@ -479,6 +489,7 @@ int mhd_create(int nx, int ny, double dx, double dy, double eta, const mhd_opts*
     return bail("pinned allocation failed", MHD_ENOMEM);
   for (int q = 0; q < 6; ++q) {
     if (hipMemsetAsync(c->U[q], 0, fb, c->stream) != hipSuccess) return bail("hipMemset failed", MHD_EHIP);
+  if (hipMemsetAsync(c->rho_raw, 0, fb, c->stream) != hipSuccess) return bail("hipMemset failed", MHD_EHIP);
   }
   *c->st_host = CgState{};
   if (hipMemcpyAsync(c->st, c->st_host, sizeof(CgState), hipMemcpyHostToDevice, c->stream) != hipSuccess)
@@ -538,7 +549,7 @@ int mhd_destroy(mhd_ctx* c) {
     if (c->S[q]) (void)hipFree(c->S[q]);
     if (c->F1[q]) (void)hipFree(c->F1[q]);
   }
-  double* bufs[] = {c->x, c->r, c->p[0], c->p[1], c->part, c->part_s, c->part_a, c->part_b};
+  double* bufs[] = {c->x, c->r, c->p[0], c->p[1], c->part, c->part_s, c->part_a, c->part_b, c->rho_raw};
   for (double* b : bufs)
     if (b) (void)hipFree(b);
   if (c->st) (void)hipFree(c->st);
@@ -561,7 +572,7 @@ int mhd_upload(mhd_ctx* c, const double* rho, const double* mx, const double* my
     if (rc0) return rc0;
   }
   int rc = launch(c, 8, [&] {
-    hipLaunchKernelGGL(k_floor_rho, dim3(grid_1d(c->n)), dim3(256), 0, c->stream, c->U[0], (int)c->n);
+    hipLaunchKernelGGL(k_floor_rho_raw, dim3(grid_1d(c->n)), dim3(256), 0, c->stream, c->U[0], c->rho_raw, (int)c->n);
   });
   if (rc) return rc;
   HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
@@ -607,7 +618,7 @@ int mhd_upload_device(mhd_ctx* c, const double* const* fields) {
   int rc = copy_fields(c, c->U, src, 6);
   if (rc) return rc;
   rc = launch(c, 8, [&] {
-    hipLaunchKernelGGL(k_floor_rho, dim3(grid_1d(c->n)), dim3(256), 0, c->stream, c->U[0], (int)c->n);
+    hipLaunchKernelGGL(k_floor_rho_raw, dim3(grid_1d(c->n)), dim3(256), 0, c->stream, c->U[0], c->rho_raw, (int)c->n);
   });
   if (rc) return rc;
   HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
@@ -759,7 +770,10 @@ int mhd_run(mhd_ctx* c, double t_end, double dt_override, int max_steps, int* st
   if (rc) return rc;
   double dt0 = (dt_override == 0.0) ? opt : dt_override;
   double mc = dt0 * c->eta;
-  rc = mhd_build_operator(c, mc * A_im4);                         // :189-191
+  rc = mhd_build_operator(c, mc * A_im4);                         // :58-60
+  if (rc) return rc;
+  // :67: primitives of the floored state, which the first CFL step reads
+  rc = reset_raw_density(c);
   if (rc) return rc;
   double prev = dt0, t = 0.0;
   int steps = 0;

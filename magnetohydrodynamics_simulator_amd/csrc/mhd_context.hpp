@@ -64,6 +64,9 @@ struct mhd_ctx {
   double* F1[6] = {};
   double* x = nullptr;
   double* r = nullptr;
+  // the unfloored density of each cell whose stored density is the floor
+  // (the velocities of the reference's last computePrimitives come from it)
+  double* rho_raw = nullptr;
   double* p[2] = {};
   double* part_s = nullptr;      // CG start kernel: three partials per tile
   double* part_a = nullptr;      // CG pass A: one partial per tile
@@ -161,6 +164,10 @@ mhd::DiagArgs diag_args(mhd_ctx* c, bool fin);
 int cg_run(mhd_ctx* c, const double* bx, const double* by, const double* bin, mhd::CgState* out, int slot,
            int spec = 0, mhd::CgState* spec_out = nullptr);
 int to_host_padded(mhd_ctx* c, const double* dev, double* host, bool ghosts);
+// A computePrimitives pass over the stored state (Grid.hpp:111-115) as far as
+// the next CFL step sees it: the velocities of floored cells come from the
+// floored density from now on (k_raw_reset).
+int reset_raw_density(mhd_ctx* c);
 // dst[q] = src[q] for q < n (n <= 6 device fields of the context's size), one launch.
 int copy_fields(mhd_ctx* c, double* const* dst, double* const* src, int n);
 int from_host_padded(mhd_ctx* c, const double* host, double* dev);

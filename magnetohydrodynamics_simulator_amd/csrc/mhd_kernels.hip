@@ -232,7 +232,12 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, int strip, int se
           double u = wv[q] + d[q] * a.c_b;         // U3* = W + dt (1 - delta) f2
           if (q == 4 && a.eta != 0.0) u = u + g2x * a.c_g;
           if (q == 5 && a.eta != 0.0) u = u + g2y * a.c_g;
-          if (q == 0 && u < kFloor) u = kFloor;
+          if (q == 0 && u <= kFloor) {
+            // the final computePrimitives of the step forms the velocities
+            // (which the next CFL step reads) before the floor (Grid.hpp:111-115)
+            if (a.rho_raw) a.rho_raw[o] = u;
+            if (u < kFloor) u = kFloor;
+          }
           if (q < 4) __builtin_nontemporal_store(u, &a.o1[q][o]);
           else a.o1[q][o] = u;                     // faces: read next by the solve's start
         }
@@ -931,7 +936,8 @@ __device__ __forceinline__ void finalize2_body(const FinArgs& a, int vb, const D
         // above the last row of an outflow axis by is the last row's
         const double bxc = (bxo_prev + bxrow[buf ^ 1][tid + 2]) / 2.0;
         const double byc = (byo_prev + (((OPEN ? a.open_y : 0) && j >= ny) ? byo_prev : byo)) / 2.0;
-        cfl_cell(u_prev[0], u_prev[1], u_prev[2], u_prev[3], bxc, byc, m, bad);
+        const double rr = (d.rho_raw && u_prev[0] == kFloor) ? d.rho_raw[(j - 1) * nx + t.ic] : u_prev[0];
+        cfl_cell(rr, u_prev[1], u_prev[2], u_prev[3], bxc, byc, m, bad);
       }
       bxo_prev = bxo;
       byo_prev = byo;
@@ -1007,7 +1013,8 @@ __device__ __forceinline__ void cfl_body(const DiagArgs& a, int vb) {
       const double by_hi_n = more ? a.s[5][at(t.ic, j + 2)] : 0.0;
       const double bxc = (bxl + bxr) / 2.0;
       const double byc = (by_lo + by_hi) / 2.0;
-      cfl_cell(rho, mx, my, E, bxc, byc, m, bad[0]);
+      const double rr = (a.rho_raw && rho == kFloor) ? a.rho_raw[o] : rho;
+      cfl_cell(rr, mx, my, E, bxc, byc, m, bad[0]);
       by_lo = by_hi; by_hi = by_hi_n;
       rho = rho_n; mx = mx_n; my = my_n; E = E_n; bxl = bxl_n; bxr = bxr_n;
       o = on;
@@ -1093,6 +1100,21 @@ __global__ __launch_bounds__(256) void k_floor_rho(double* rho, int n) {
     if (rho[o] < kFloor) rho[o] = kFloor;
 }
 
+__global__ __launch_bounds__(256) void k_floor_rho_raw(double* rho, double* raw, int n) {
+  for (int o = blockIdx.x * blockDim.x + threadIdx.x; o < n; o += gridDim.x * blockDim.x) {
+    const double r = rho[o];
+    if (r <= kFloor) {
+      raw[o] = r;
+      if (r < kFloor) rho[o] = kFloor;
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_raw_reset(const double* rho, double* raw, int n) {
+  for (int o = blockIdx.x * blockDim.x + threadIdx.x; o < n; o += gridDim.x * blockDim.x)
+    if (rho[o] <= kFloor) raw[o] = rho[o];
+}
+
 __global__ __launch_bounds__(256) void k_copy(const double* __restrict__ src, double* __restrict__ dst, long long n) {
   for (long long o = blockIdx.x * (long long)blockDim.x + threadIdx.x; o < n; o += (long long)gridDim.x * blockDim.x)
     dst[o] = src[o];
@@ -1134,6 +1156,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void kb
   a.c_a = (MODE == 1) ? rs.coef[0] : 0.0;
   a.c_b = (MODE == 1) ? rs.coef[1] : rs.coef[2];
   a.c_g = (MODE == 2) ? rs.coef[3] : 0.0;
+  a.rho_raw = (MODE == 2 && b.rho_raw) ? b.rho_raw + g.base : nullptr;
   stage_body<MODE>(a, job.y % g.strips, job.y / g.strips);
 }
 template __global__ void kb_stage<1>(BatchStageArgs);
@@ -1272,6 +1295,7 @@ __global__ __launch_bounds__(kTileX) void kb_cfl(BatchDiagArgs b) {
 #pragma unroll
   for (int q = 0; q < 6; ++q) d.s[q] = b.s[q] + g.base;
   d.part = b.part + 2LL * g.part_base;
+  d.rho_raw = b.rho_raw ? b.rho_raw + g.base : nullptr;
   cfl_body(d, job.y);
 }
 
@@ -1290,6 +1314,7 @@ __device__ __forceinline__ DiagArgs batch_diag_args(const BatchDiagArgs& b, int 
 #pragma unroll
   for (int q = 0; q < 6; ++q) d.s[q] = b.s[q] + g.base;
   d.part = b.part + 2LL * g.part_base;
+  d.rho_raw = b.rho_raw ? b.rho_raw + g.base : nullptr;
   return d;
 }
 
@@ -1363,7 +1388,7 @@ __global__ __launch_bounds__(256) void kb_dt(const RunGeom* __restrict__ geom, R
     const double dt_hyp = (g.cfl / 0.9) * h / m;
     const double dt_cfl = ((dt_res < dt_hyp) ? dt_res : dt_hyp) * 0.9;
     const double dt0 = (g.dt_fixed > 0.0) ? g.dt_fixed : dt_cfl;
-    if (s.steps == 0) {                      // operator of the first dt, before the loop
+    if (s.steps == 0 && !(s.prev_dt > 0.0)) {   // operator of the first dt, before the loop (unless preset)
       s.alpha = (dt0 * g.eta) * a_im4;
       s.prev_dt = dt0;
     }

@@ -29,6 +29,8 @@ struct StageArgs {
   int open_x, open_y;   // outflow (zero-gradient) boundaries instead of periodic ones
   double dx, dy, inv_dx, inv_dy, eta;
   double c_a, c_b, c_g;
+  // MODE 2: the new density before its floor, where it is <= 1e-8 (null: not kept)
+  double* rho_raw;
 };
 
 // Conjugate-gradient state, device resident.  Written by block 0 of a pass
@@ -73,6 +75,9 @@ struct DiagArgs {
   double dx, dy;
   const double* s[6];
   double* part;    // two partials per tile
+  // the density a floored cell had before its floor (null: the stored one);
+  // read only where the stored density is the floor itself
+  const double* rho_raw;
 };
 
 // ---------------------------------------------------------------------------
@@ -141,6 +146,7 @@ struct BatchStageArgs {
   double* o1[6];
   double* o2[6];
   const int* ctl;          // batch control word (kCtlHalt, kCtlSteps); null: none
+  double* rho_raw;         // pooled: MODE 2 keeps the unfloored density of floored cells (null: none)
 };
 
 struct BatchCgArgs {
@@ -180,6 +186,7 @@ struct BatchDiagArgs {
   const double* s[6];
   double* part;
   const int* ctl;
+  const double* rho_raw;   // pooled (null: none)
 };
 
 // Control word of a batch's device-resident group of steps (an int[4] on the
@@ -241,6 +248,12 @@ __global__ void k_cfl(DiagArgs a);
 __global__ void k_divb(DiagArgs a);
 __global__ void k_prims(DiagArgs a, double* vx, double* vy, double* p, double* bxc, double* byc);
 __global__ void k_floor_rho(double* rho, int n);
+// The density floor of an uploaded state, keeping the unfloored value of each
+// floored cell in raw (Grid::computePrimitives forms the velocities from it).
+__global__ void k_floor_rho_raw(double* rho, double* raw, int n);
+// raw = rho where rho is at the floor: the velocities of the next CFL step come
+// from the floored density (a computePrimitives pass over the stored state).
+__global__ void k_raw_reset(const double* rho, double* raw, int n);
 __global__ void k_cg_reset(CgState* st, long long max_iters);
 struct CopyArgs {
   const double* src[6];

@@ -290,6 +290,9 @@ int rebase(mhd_ctx* c, Loop& L, long long done) {
     L.B1[k] = c->S[4 + k];
   }
   L.base = done;
+  // the reference recomputes the primitives of the stored state before the
+  // run (integrator_main.cpp:67) and at each snapshot (:107-108)
+  if (int rc = reset_raw_density(c)) return rc;
   if (L.fixed_dt) return MHD_OK;
   DiagArgs d = diag_args(c, true);
   return launch_k(c, 7, k_cfl, dim3(c->fin_tiles), dim3(kTileX), d);
@@ -383,13 +386,19 @@ int mhd_run_resident(mhd_ctx* c, double t_end, double dt_override, int max_steps
   g.max_iters = (c->o.cg_max_iter > 0) ? c->o.cg_max_iter : 2LL * c->n;
   g.max_steps = max_steps < 0 ? -1 : max_steps;
   *R->rs_host = RunState{};
+  // the operator of the first dt (integrator_main.cpp:55-60): the CFL step of
+  // the state as uploaded, before the primitives are recomputed from the
+  // floored density (:67) for the loop's own first CFL step
+  double dt_init = dt_override;
+  if (dt_override == 0.0 && (rc = mhd_cfl_dt(c, &dt_init))) return rc;
+  R->rs_host->prev_dt = dt_init;
+  R->rs_host->alpha = (dt_init * c->eta) * c->tab.A_im4;
   HIP_OR_FAIL(c, hipMemcpyAsync(R->geom, &g, sizeof(RunGeom), hipMemcpyHostToDevice, c->stream));
   HIP_OR_FAIL(c, hipMemcpyAsync(R->rs, R->rs_host, sizeof(RunState), hipMemcpyHostToDevice, c->stream));
   HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));   // the host copies above read pageable / pinned host memory
 
   if (on_snapshot && snapshot_every > 0) {   // initial snapshot (integrator_main.cpp:55-70)
-    double dt0 = dt_override;
-    if (dt_override == 0.0 && (rc = mhd_cfl_dt(c, &dt0))) return rc;
+    const double dt0 = dt_init;
     if ((rc = on_snapshot(c, 0, 0.0, dt0, user))) return rc;
   }
   if ((rc = rebase(c, L, 0))) return rc;   // B pointers and the CFL partials of the initial state

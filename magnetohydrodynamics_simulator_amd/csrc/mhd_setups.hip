@@ -224,6 +224,11 @@ int mhd_batch_init_fourier(mhd_batch* b, int run, int nmodes, const double* mode
   RunState z{};   // a fresh state restarts the run's driver (as mhd_batch_upload)
   e = hipMemcpy(b->rs + run, &z, sizeof(RunState), hipMemcpyHostToDevice);
   if (e != hipSuccess) return mhd_batch_host::bfail(b, MHD_EHIP, std::string("mhd_batch_init_fourier: ") + hipGetErrorString(e));
+  // the run's first CFL step reads the primitives of the new state
+  hipLaunchKernelGGL(k_raw_reset, dim3(init_blocks(n)), dim3(256), 0, b->stream, (const double*)b->U[0] + g.base,
+                     b->rho_raw + g.base, (int)n);
+  e = hipGetLastError();
+  if (e != hipSuccess) return mhd_batch_host::bfail(b, MHD_EHIP, std::string("mhd_batch_init_fourier: ") + hipGetErrorString(e));
   return MHD_OK;
 }
 
@@ -254,6 +259,11 @@ int mhd_batch_init_column(mhd_batch* b, int run, int sausage, int nmodes, const 
   RunState z{};
   e = hipMemcpy(b->rs + run, &z, sizeof(RunState), hipMemcpyHostToDevice);
   if (e != hipSuccess) return mhd_batch_host::bfail(b, MHD_EHIP, std::string("mhd_batch_init_column: ") + hipGetErrorString(e));
+  // the run's first CFL step reads the primitives of the new state
+  hipLaunchKernelGGL(k_raw_reset, dim3(init_blocks(n)), dim3(256), 0, b->stream, (const double*)b->U[0] + g.base,
+                     b->rho_raw + g.base, (int)n);
+  e = hipGetLastError();
+  if (e != hipSuccess) return mhd_batch_host::bfail(b, MHD_EHIP, std::string("mhd_batch_init_column: ") + hipGetErrorString(e));
   return MHD_OK;
 }
 
@@ -273,7 +283,7 @@ int mhd_init_fourier(mhd_ctx* c, int nmodes, const double* modes) {
                        (const FourierMode*)d, nmodes);
   });
   if (e != hipSuccess) return mhd_host::fail(c, MHD_EHIP, std::string("mhd_init_fourier: ") + hipGetErrorString(e));
-  return MHD_OK;
+  return mhd_host::reset_raw_density(c);   // the first CFL step reads the primitives of the new state
 }
 
 int mhd_init_column(mhd_ctx* c, int sausage, int nmodes, const double* amps, const double* phases) {
@@ -296,7 +306,7 @@ int mhd_init_column(mhd_ctx* c, int sausage, int nmodes, const double* amps, con
                        nmodes, sausage ? 1 : 0);
   });
   if (e != hipSuccess) return mhd_host::fail(c, MHD_EHIP, std::string("mhd_init_column: ") + hipGetErrorString(e));
-  return MHD_OK;
+  return mhd_host::reset_raw_density(c);   // the first CFL step reads the primitives of the new state
 }
 
 }  // extern "C"
