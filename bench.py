@@ -93,7 +93,13 @@ def parse():
     ap.add_argument("--eta", type=float, default=1e-3)
     ap.add_argument("--cg-tol", type=float, default=1e-10)
     ap.add_argument("--cfl", type=float, default=0.4)
-    ap.add_argument("--cpu-steps", type=int, default=2, help="CPU-baseline sample (steps at --n)")
+    ap.add_argument("--cpu-steps", type=int, default=5,
+                    help="CPU-baseline sample: driver steps at --n from the state the GPU's timed region starts from")
+    ap.add_argument("--cpu-steps-2048", type=int, default=5, help="CPU-baseline steps of the 2048^2 leg (0: skip)")
+    ap.add_argument("--n2", type=int, default=2048, help="grid side of the second resistive leg (0: skip)")
+    ap.add_argument("--n2-steps", type=int, default=40, help="timed steps of the second resistive leg")
+    ap.add_argument("--small-n", type=int, default=128,
+                    help="grid side of the ideal Orszag-Tang run to t = 0.5 on the GPU and the CPU (0: skip)")
     ap.add_argument("--pcie-calls", type=int, default=3,
                     help="Grid-taking calls (upload + step + download) timed for the PCIe-inclusive rate")
     ap.add_argument("--no-cpu", action="store_true")
@@ -125,6 +131,7 @@ def parse():
     ap.add_argument("--outflow-steps", type=int, default=20,
                     help="steps of the same workload with outflow boundaries on both axes (0: skip)")
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--cpu-child", default=None, help=argparse.SUPPRESS)
     return ap.parse_args()
 
 
@@ -322,44 +329,74 @@ def hbm_traffic(a, kernel: str) -> dict | None:
     return res
 
 
-def cpu_baseline(n: int, eta: float, tol: float, cfl: float, steps: int) -> dict:
-    """Time the oracle's reference-faithful driver loop on one host core."""
+NAMES = ("rho", "mx", "my", "E", "bxf", "byf")
+
+
+def cpu_child(job_file: str) -> None:
+    """The CPU baseline's legs, run in a process of their own on one host core
+    (concurrently with the GPU legs of the parent): the oracle's
+    reference-faithful driver loop (CFL + CSR rebuild + IMEX per step,
+    src/integrator_main.cpp:76-100), built with the reference's -O3
+    -ffast-math and -march=native for this host.  Each leg starts from a
+    saved state (the one the GPU leg's timed region started from) or from the
+    Orszag-Tang initial state."""
     from magnetohydrodynamics_simulator_amd import setups
     from oracle import pyoracle as po
 
+    jobs = json.load(open(job_file))
     lib_path, flags = None, "-O3 -ffast-math -march=x86-64-v3 (prebuilt)"
-    out_dir = tempfile.mkdtemp(prefix="mhd_oracle_")    # a native build of the oracle for this host
+    out_dir = tempfile.mkdtemp(prefix="mhd_oracle_")
     try:
         po.build(fast_native=True, out_dir=out_dir)
         lib_path, flags = os.path.join(out_dir, "libmhd_oracle_native.so"), "-O3 -ffast-math -march=native"
     except Exception:
         pass
     lib = po.load("fast", path=lib_path) if lib_path else po.load("fast")
-    dx = setups.grid_spacing(n)
-    ic = setups.orszag_tang(n)
-    sim = po.Sim(n, n, dx, dx, eta, lib=lib)
-    sim.set_cg(tol, -1)
-    for k in ("rho", "mx", "my", "E", "bxf", "byf"):
-        sim.field(k)[...] = ic[k]
-    del ic
-    t0 = time.perf_counter()
-    sim.run(1e9, cfl=cfl, max_steps=0, snapshot_every=0)           # driver set-up only
-    t_setup = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    nsteps, _, its, _ = sim.run(1e9, cfl=cfl, max_steps=steps, snapshot_every=0)
-    t_run = time.perf_counter() - t0 - t_setup
-    sim.close()
     cpu = subprocess.run(["lscpu"], capture_output=True, text=True).stdout
     model = next((ln.split(":", 1)[1].strip() for ln in cpu.splitlines() if ln.startswith("Model name")), "unknown")
-    return {
-        "value": n * n * nsteps / t_run / 1e6,
-        "unit": "Mcell-updates/s",
-        "cores": 1,
-        "kind": "port",
-        "sample": f"orszag_tang {n}x{n} eta={eta} cg_tol={tol} cfl={cfl}: {nsteps} driver steps "
-                  f"(CFL + CSR rebuild + IMEX), {t_run:.1f} s on 1 thread of {model} "
-                  f"(nproc {os.cpu_count()}); oracle built {flags}; CG iters {its.tolist()}",
-    }
+    out = {"oracle_flags": flags, "host": f"1 thread of {model} (nproc {os.cpu_count()})", "legs": {}}
+    for job in jobs:
+        n = job["n"]
+        dx = setups.grid_spacing(n)
+        sim = po.Sim(n, n, dx, dx, job["eta"], lib=lib)
+        if job["tol"] is not None:
+            sim.set_cg(job["tol"], -1)
+        if job.get("state"):
+            with np.load(job["state"]) as z:
+                for k in NAMES:
+                    sim.field(k)[...] = z[k]
+        else:
+            ic = setups.orszag_tang(n)
+            for k in NAMES:
+                sim.field(k)[...] = ic[k]
+        t0 = time.perf_counter()
+        sim.run(1e9, cfl=job["cfl"], max_steps=0, snapshot_every=0)            # driver set-up only
+        t_setup = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        nsteps, dts, its, t = sim.run(job.get("t_end", 1e9), cfl=job["cfl"], max_steps=job.get("steps", -1),
+                                      snapshot_every=0)
+        t_run = time.perf_counter() - t0 - t_setup
+        sim.close()
+        out["legs"][job["name"]] = {"steps": int(nsteps), "seconds": round(t_run, 3),
+                                    "value": n * n * nsteps / t_run / 1e6, "ms_per_step": t_run / nsteps * 1e3,
+                                    "cg_iters": its.tolist() if nsteps <= 8 else "n/a", "t": t}
+    print(json.dumps(out), flush=True)
+
+
+def start_cpu_baseline(jobs: list) -> tuple:
+    """Starts the CPU legs in a child process; returns (process, job file)."""
+    fd, path = tempfile.mkstemp(prefix="mhd_cpu_jobs_", suffix=".json")
+    with os.fdopen(fd, "w") as fh:
+        json.dump(jobs, fh)
+    proc = subprocess.Popen([sys.executable, os.path.abspath(__file__), "--cpu-child", path],
+                            stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, cwd=ROOT)
+    return proc, path
+
+
+def save_state(ctx, path: str) -> str:
+    st = ctx.download()
+    np.savez(path, **{k: st[k] for k in NAMES})
+    return path
 
 
 def bench_batch(a) -> dict:
@@ -657,6 +694,9 @@ def main():
     if a.pmc_child:
         pmc_child(a)
         return
+    if a.cpu_child:
+        cpu_child(a.cpu_child)
+        return
     if a.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {a.gpus}); multi-GPU runs are not part of this build",
               file=sys.stderr)
@@ -735,6 +775,9 @@ def main():
                    "note": "bc = (outflow, outflow), resident loop, no events; the periodic figure is driver_comparison"}
     if a.compare_steps > 0 or a.outflow_steps > 0:
         del st0
+    # the state the timed region starts from, for the CPU baseline
+    tmp_dir = tempfile.mkdtemp(prefix="mhd_bench_")
+    state_main = save_state(ctx, os.path.join(tmp_dir, "state_main.npz")) if not a.no_cpu else None
     # timed region: only the dominant kernel's launches carry events
     ctx.profile([dom])
     ctx.synchronize()
@@ -770,6 +813,66 @@ def main():
         pcie = {"value": round(N / tp / 1e6, 2), "unit": "Mcell-updates/s", "ms_per_call": round(tp * 1e3, 2),
                 "note": "upload 6 + step + download 11 padded host arrays per call (pageable host memory)"}
     ctx.close()
+
+    # the same resistive workload at n2 (2048^2: its working set is close to the
+    # 256 MB last-level cache, so its HBM figures come with that caveat)
+    second = None
+    state_2 = None
+    if a.n2 > 0:
+        n2 = a.n2
+        dx2 = setups.grid_spacing(n2)
+        ic2 = setups.orszag_tang(n2)
+        with Context(n2, n2, dx2, dx2, a.eta, cfl=a.cfl, cg_tol=a.cg_tol) as c2:
+            c2.upload(*[ic2[k] for k in NAMES])
+            del ic2
+            c2.run_resident(1e9, max_steps=max(a.warmup, 5))
+            if not a.no_cpu:
+                state_2 = save_state(c2, os.path.join(tmp_dir, "state_2.npz"))
+            c2.synchronize()
+            t0 = time.perf_counter()
+            ns2, _, its2, _ = c2.run_resident(1e9, max_steps=a.n2_steps)
+            c2.synchronize()
+            t2 = time.perf_counter() - t0
+        spmv2 = int(its2.sum()) + 2 * ns2
+        bytes2 = n2 * n2 * (336 + 144 + 64 * spmv2 / ns2)
+        second = {"workload": f"resistive Orszag-Tang {n2}x{n2}, eta={a.eta}, Jacobi-PCG tol {a.cg_tol}, CFL {a.cfl}",
+                  "steps": ns2, "value": round(n2 * n2 * ns2 / t2 / 1e6, 2), "unit": "Mcell-updates/s",
+                  "ms_per_step": round(t2 / ns2 * 1e3, 4), "cg_spmv_per_step": round(spmv2 / ns2, 2),
+                  "step_roofline_frac": round(bytes2 / (t2 / ns2) / 1e9 / HBM_PEAK_GBS, 4),
+                  "note": "resident loop after the same warm-up; the working set (~1 GB of state and CG vectors "
+                          "touched per pass ~200-300 MB) is comparable to the 256 MB last-level cache, so part of "
+                          "the traffic is served by it"}
+    # the CPU-runnable workload: ideal Orszag-Tang to t = 0.5 (CFL 0.4), whole run
+    small = None
+    if a.small_n > 0:
+        ns_ = a.small_n
+        dxs = setups.grid_spacing(ns_)
+        ics = setups.orszag_tang(ns_)
+        with Context(ns_, ns_, dxs, dxs, 0.0, cfl=a.cfl) as cs:
+            cs.upload(*[ics[k] for k in NAMES])
+            cs.run_resident(0.5)                       # warm-up run
+            cs.upload(*[ics[k] for k in NAMES])
+            cs.synchronize()
+            t0 = time.perf_counter()
+            nss, _, _, tss = cs.run_resident(0.5)
+            cs.synchronize()
+            ts_ = time.perf_counter() - t0
+            _, divs = cs.calculate_current_div_B()
+        small = {"workload": f"ideal Orszag-Tang {ns_}x{ns_} to t = 0.5, CFL {a.cfl}", "steps": nss, "t_end": tss,
+                 "value": round(ns_ * ns_ * nss / ts_ / 1e6, 2), "unit": "Mcell-updates/s",
+                 "ms_per_step": round(ts_ / nss * 1e3, 4), "divb_max": divs,
+                 "note": "whole run, resident loop; launch-bound at this size"}
+    cpu_proc = None
+    if not a.no_cpu:
+        jobs = [{"name": "main", "n": n, "eta": a.eta, "tol": a.cg_tol, "cfl": a.cfl, "steps": a.cpu_steps,
+                 "state": state_main}]
+        if a.n2 > 0 and a.cpu_steps_2048 > 0:
+            jobs.append({"name": "second", "n": a.n2, "eta": a.eta, "tol": a.cg_tol, "cfl": a.cfl,
+                         "steps": a.cpu_steps_2048, "state": state_2})
+        if a.small_n > 0:
+            jobs.append({"name": "small", "n": a.small_n, "eta": 0.0, "tol": None, "cfl": a.cfl, "t_end": 0.5,
+                         "state": None})
+        cpu_proc = start_cpu_baseline(jobs)
 
     solves = 2 * steps if a.eta > 0 else 0
     spmv = int(its.sum()) + solves          # converged solves: products = iterations + 1
@@ -825,8 +928,45 @@ def main():
         line["adjoint"] = bench_adjoint(a)
     if a.ba_runs > 0:
         line["batched_adjoint"] = bench_batch_adjoint(a)
+    if second is not None:
+        line["second_size"] = second
+    if small is not None:
+        line["cpu_runnable_workload"] = small
+    if cpu_proc is not None:
+        proc, job_path = cpu_proc
+        out, err = proc.communicate()
+        os.remove(job_path)
+        res = json.loads(out.strip().splitlines()[-1]) if proc.returncode == 0 and out.strip() else None
+        if res is None:
+            line["cpu_baseline"] = {"value": None, "unit": "Mcell-updates/s", "cores": 1, "kind": "port",
+                                    "sample": f"CPU baseline failed: {err.strip()[-300:]}"}
+        else:
+            m = res["legs"]["main"]
+            line["cpu_baseline"] = {
+                "value": m["value"], "unit": "Mcell-updates/s", "cores": 1, "kind": "port",
+                "sample": f"{m['steps']} driver steps (CFL + CSR rebuild + IMEX) of the {n}x{n} workload from the "
+                          f"state the GPU's timed region starts from (after {a.warmup} warm-up steps and the "
+                          f"instrumented pass), {m['seconds']} s on {res['host']}; oracle built "
+                          f"{res['oracle_flags']}; CG iterations {m['cg_iters']}; "
+                          f"{a.steps} steps at this rate: {m['ms_per_step'] * a.steps / 1e3:.1f} s",
+                "gpu_over_cpu": round(line["value"] / m["value"], 1)}
+            legs = {}
+            if "second" in res["legs"]:
+                l2 = res["legs"]["second"]
+                legs["second_size"] = {"n": a.n2, "steps": l2["steps"], "value": l2["value"],
+                                       "ms_per_step": round(l2["ms_per_step"], 1), "cg_iters": l2["cg_iters"],
+                                       "from": "the state the GPU leg's timed region starts from",
+                                       "gpu_over_cpu": round(second["value"] / l2["value"], 1) if second else None}
+            if "small" in res["legs"]:
+                ls = res["legs"]["small"]
+                legs["cpu_runnable_workload"] = {"n": a.small_n, "steps": ls["steps"], "t_end": ls["t"],
+                                                 "value": ls["value"], "seconds": ls["seconds"],
+                                                 "gpu_over_cpu": round(small["value"] / ls["value"], 1)
+                                                 if small else None}
+            line["cpu_baseline_legs"] = legs
     if not a.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(n, a.eta, a.cg_tol, a.cfl, a.cpu_steps)
+        import shutil
+        shutil.rmtree(tmp_dir, ignore_errors=True)
     print(json.dumps(line), flush=True)
 
 
