@@ -221,3 +221,93 @@ def test_batch_adjoint_non_square_runs_match_single_run():
             err = np.abs(interior(res[r][0][k]) - interior(lam0[k])).max()
             assert err <= TOL_PARITY * scale, (r, k, err, scale)
         assert abs(res[r][1] - eb) <= TOL_PARITY * abs(eb), (r, res[r][1], eb)
+
+
+def test_config5_full_size():
+    """The batched-differentiation workload at its full size: 64 seeded
+    sausage-column runs (sides 256..1024, 33.5 M cells, eta 1e-4..1e-2), each at
+    0.8 x its CFL step, 40 steps, checkpoints every 10, loss = each run's
+    window magnetic energy.
+      * the recorded forward is the plain batched forward bit for bit;
+      * the two smallest runs' dJ/dU(initial) and dJ/d eta against single-run
+        contexts: 1e-8 relative;
+      * three runs' gradients against central differences of the batched
+        forward (a batch of their 6 perturbed states, 16 fixed CG
+        iterations): <lam0, v> vs <lamN, (F(U + e v) - F(U - e v)) / 2e>,
+        1e-6 relative."""
+    steps, every, eps = 40, 10, 1e-5     # state step 1e-8 (noise grows as 1/eps below it: 3e-6 at 1e-10)
+    members = setups.sausage_ensemble(64, seed=0)
+    ns = [m[0] for m in members]
+    etas = [m[1] for m in members]
+    modes = [setups.sausage_column_modes(8, 1e-2, seed=s) for _, _, s in members]
+    _say(f"config 5: {len(members)} sausage runs, sides {min(ns)}..{max(ns)}, {sum(n * n for n in ns)} cells, "
+         f"{steps} steps, checkpoints every {every}")
+
+    def make():
+        b = Batch(ns, ns, [1.0 / n for n in ns], [1.0 / n for n in ns], etas, cfl=0.4, cg_tol=1e-10)
+        for r, md in enumerate(modes):
+            b.init_column(r, md[0], md[1], True)
+        return b
+
+    order = sorted(range(len(members)), key=lambda r: (ns[r], r))
+    small = order[:2]
+    fd_runs = order[:2] + [next(r for r in order if ns[r] == 512)]
+    with make() as b:
+        b.set_schedule(max_steps=1)
+        b.run()
+        dts = [0.8 * b.run_info(r)[2] for r in range(len(members))]
+    with make() as p:
+        p.set_schedule(max_steps=steps, dt_fixed=dts)
+        p.run()
+        plain = {r: p.download(r) for r in set(small + fd_runs)}
+    lams, res = {}, {}
+    with make() as b:
+        b.set_schedule(max_steps=steps, dt_fixed=dts)
+        b.adjoint_forward(steps, every)
+        for r in range(len(members)):
+            fin = b.download(r)
+            if r in plain:
+                for f in CONSERVED:
+                    assert np.array_equal(interior(fin[f]), interior(plain[r][f])), (r, f)
+            _, lam = setups.window_magnetic_energy(fin, ns[r], 1.0 / ns[r])
+            lams[r] = lam
+            b.adjoint_set_lambda(r, lam)
+        b.adjoint_backward()
+        for r in set(small + fd_runs):
+            res[r] = b.adjoint_result(r)
+    print(f"[case] recorded forward == plain batched forward on runs {sorted(plain)}", flush=True)
+    for r in small:
+        n = ns[r]
+        with Context(n, n, 1.0 / n, 1.0 / n, etas[r], cfl=0.4, cg_tol=1e-10) as c:
+            c.init_column(modes[r][0], modes[r][1], True)
+            c.adjoint_forward(dts[r], steps, every)
+            lam0_s, eb_s = c.adjoint_backward(lams[r])
+        lam0_b, eb_b = res[r]
+        worst = max(np.abs(interior(lam0_b[f]) - interior(lam0_s[f])).max() / np.abs(interior(lam0_s[f])).max()
+                    for f in CONSERVED)
+        print(f"[case] run {r} ({n}^2, eta {etas[r]:.3e}): lam0 max rel diff {worst:.2e}, "
+              f"eta_bar {eb_b:.12e} vs {eb_s:.12e}", flush=True)
+        assert worst <= TOL_PARITY and abs(eb_b - eb_s) <= TOL_PARITY * abs(eb_s), (r, worst, eb_b, eb_s)
+    # central differences of the batched forward for three runs
+    vs = {r: _smooth(ns[r], 500 + r, 1e-3) for r in fd_runs}
+    pert_ns, pert_etas, pert_dts, states = [], [], [], []
+    for r in fd_runs:
+        st = setups.sausage_column(ns[r], modes[r][0], modes[r][1])
+        for sgn in (1, -1):
+            pert_ns.append(ns[r])
+            pert_etas.append(etas[r])
+            pert_dts.append(dts[r])
+            states.append({f: st[f] + sgn * eps * vs[r][f] for f in CONSERVED})
+    with Batch(pert_ns, pert_ns, [1.0 / n for n in pert_ns], [1.0 / n for n in pert_ns], pert_etas,
+               **FIXED_CG) as q:
+        q.set_schedule(max_steps=steps, dt_fixed=pert_dts)
+        for k, st in enumerate(states):
+            q.upload(k, st)
+        q.run()
+        outs = [q.download(k) for k in range(len(states))]
+    for i, r in enumerate(fd_runs):
+        jv = {f: (outs[2 * i][f] - outs[2 * i + 1][f]) / (2 * eps) for f in CONSERVED}
+        lhs, rhs = _dot(res[r][0], vs[r]), _dot(lams[r], jv)
+        print(f"[case] run {r} ({ns[r]}^2): <lam0, v> {lhs:.12e} vs differences {rhs:.12e}, "
+              f"rel {abs(lhs - rhs) / abs(rhs):.2e}", flush=True)
+        assert abs(lhs - rhs) <= TOL_DOT * abs(rhs), (r, lhs, rhs)
