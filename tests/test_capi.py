@@ -121,3 +121,26 @@ def test_cpp_adapter_compiles(tmp_path):
                         "-o", str(tmp_path / "call_site"), os.path.join(pkg, "libmhd_mi355x.so"),
                         f"-Wl,-rpath,{pkg}"], capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+
+
+REFERENCE_INCLUDE = "/root/reference/include"
+
+
+def test_cpp_adapter_compiles_on_reference_grid(tmp_path):
+    """tests/cpp/reference_grid_call_site.cpp compiles against the reference's
+    own Grid (include/Grid.hpp) where the reference tree is present, with
+    -Wall -Werror: the adapter's templates take the real struct (the GPU test
+    test_gpu_reference_grid.py runs the binary tests/cpp/Makefile builds)."""
+    import shutil
+    import subprocess
+    if shutil.which("g++") is None:
+        pytest.skip("g++ not available")
+    if not os.path.exists(os.path.join(REFERENCE_INCLUDE, "Grid.hpp")):
+        pytest.skip("reference tree not present")
+    L.load()
+    pkg = os.path.dirname(L.LIB_PATH)
+    r = subprocess.run(["g++", "-std=c++17", "-O1", "-Wall", "-Werror", "-I", REFERENCE_INCLUDE,
+                        "-I", os.path.join(ROOT, "include"), os.path.join(ROOT, "tests", "cpp", "reference_grid_call_site.cpp"),
+                        "-o", str(tmp_path / "call_site"), os.path.join(pkg, "libmhd_mi355x.so"),
+                        f"-Wl,-rpath,{pkg}"], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
