@@ -49,6 +49,10 @@ HBM_PEAK_GBS = 8000.0
 # for the set-up and finish of the two implicit solves, 64 per CG iteration.
 STAGE_BYTES = {"stage1": 144, "stage2": 144, "cg_start": 32, "implicit_finalize1": 40,
                "implicit_finalize2": 40, "cfl": 48}
+# algorithmic bytes per cell of one working launch (cg_pass_b: the first
+# pass of a solve moves 32, the others 40; the final recombination of the
+# resident loop also reads rho, mx, my, E for the next CFL step: +32)
+ALGO_BYTES_PER_CELL = dict(STAGE_BYTES, cg_pass_a=24, cg_pass_b=40)
 KERNELS = ("stage1", "stage2", "cg_start", "cg_pass_a", "cg_pass_b", "implicit_finalize1",
            "implicit_finalize2", "cfl")
 # device symbol of each kernel id, for the hardware-counter pass
@@ -282,22 +286,26 @@ def _counter_rows(out_dir: str):
     return rows
 
 
-def hbm_traffic(a, kernel: str) -> dict | None:
-    """HBM bytes per launch of `kernel` from two rocprofv3 counter passes
+def hbm_traffic(a) -> dict | None:
+    """HBM bytes per working launch of every kernel of the step, from two
+    rocprofv3 counter passes over the same two-step child workload
     (FETCH_SIZE, then WRITE_SIZE; KiB per dispatch), each scaled by what the
-    same counter reads for the 1 GiB copy kernel.  None if rocprofv3 is
-    unavailable or a pass fails."""
+    same counter reads for the 1 GiB copy kernel.  Returns {kernel key:
+    {"FETCH_SIZE": bytes, "WRITE_SIZE": bytes}} plus the two scale factors
+    under "_scale"; None if rocprofv3 is unavailable or a pass fails."""
     import shutil
     import tempfile
 
     exe = shutil.which("rocprofv3")
     if exe is None:
         return None
-    sym = (KERNEL_SYMBOLS_RESIDENT if a.driver == "resident" else KERNEL_SYMBOLS)[kernel]
-    res = {}
+    syms = KERNEL_SYMBOLS_RESIDENT if a.driver == "resident" else KERNEL_SYMBOLS
+    regex = "|".join(sorted({v.split("<")[0] for v in syms.values()})) + "|k_copy"
+    res = {k: {} for k in syms}
+    res["_scale"] = {}
     for counter in ("FETCH_SIZE", "WRITE_SIZE"):
         out_dir = tempfile.mkdtemp(prefix="mhd_pmc_")
-        cmd = [exe, "--pmc", counter, "--kernel-include-regex", f"{sym}|k_copy", "-d", out_dir, "-o", "pmc",
+        cmd = [exe, "--pmc", counter, "--kernel-include-regex", regex, "-d", out_dir, "-o", "pmc",
                "--output-format", "csv", "--", sys.executable, os.path.abspath(__file__), "--pmc-child",
                "--n", str(a.n), "--eta", str(a.eta), "--cg-tol", str(a.cg_tol), "--cfl", str(a.cfl),
                "--driver", a.driver]
@@ -307,7 +315,8 @@ def hbm_traffic(a, kernel: str) -> dict | None:
             return None
         if r.returncode != 0:
             return None
-        vals, copy = [], []
+        vals = {k: [] for k in syms}
+        copy = []
         for row in _counter_rows(out_dir):
             name = row.get("Kernel_Name", "")
             if row.get("Counter_Name") != counter:
@@ -315,17 +324,24 @@ def hbm_traffic(a, kernel: str) -> dict | None:
             v = float(row.get("Counter_Value", "nan"))
             if "k_copy" in name:
                 copy.append(v)
-            elif sym in name:
-                vals.append(v)
+                continue
+            for k, sym in syms.items():
+                if sym in name:
+                    vals[k].append(v)
+                    break
         shutil.rmtree(out_dir, ignore_errors=True)
-        if not vals or not copy:
+        if not copy:
             return None
-        # launches that found their solve finished read nothing: average the working ones
-        top = max(vals)
-        vals = [v for v in vals if v > 0.05 * top]
         scale = CALIB_BYTES / (sum(copy) / len(copy) * 1024.0)     # true bytes per counted byte
-        res[counter] = sum(vals) / len(vals) * 1024.0 * scale
-        res[counter + "_scale"] = scale
+        res["_scale"][counter] = scale
+        for k, vs in vals.items():
+            if not vs:
+                continue
+            # launches that found their solve finished or their step halted read nothing:
+            # average the working ones
+            top = max(vs)
+            vs = [v for v in vs if v > 0.05 * top]
+            res[k][counter] = sum(vs) / len(vs) * 1024.0 * scale
     return res
 
 
@@ -520,10 +536,9 @@ def bench_adjoint(a) -> dict:
 
     with Context(n, n, dx, dx, eta, cfl=0.4) as c:
         if a.adjoint_no_graph:
-            # event timing on a kernel the adjoint does not launch: the
-            # backward then runs its kernels directly instead of replaying
+            # the backward launches its kernels directly instead of replaying
             # them as a graph (the kernels and their order are the same)
-            c.profile(["cfl"])
+            c.adjoint_use_graph(False)
         c.upload(*[st[k] for k in names])
         dt = 0.8 * c.CFL_condition_checker()
         # plain forward (warm, then timed)
@@ -915,13 +930,24 @@ def main():
         "pcie_inclusive": pcie,
     }
     if not a.no_traffic:
-        tr = hbm_traffic(a, dom)
-        if tr is not None:
-            line["roofline"]["traffic"] = int(tr["FETCH_SIZE"] + tr["WRITE_SIZE"])
+        tr = hbm_traffic(a)
+        if tr is not None and "FETCH_SIZE" in tr[dom] and "WRITE_SIZE" in tr[dom]:
+            sc = tr["_scale"]
+            line["roofline"]["traffic"] = int(tr[dom]["FETCH_SIZE"] + tr[dom]["WRITE_SIZE"])
             line["roofline"]["traffic_detail"] = {
-                "fetch_bytes": int(tr["FETCH_SIZE"]), "write_bytes": int(tr["WRITE_SIZE"]),
-                "fetch_scale": round(tr["FETCH_SIZE_scale"], 4), "write_scale": round(tr["WRITE_SIZE_scale"], 4),
+                "fetch_bytes": int(tr[dom]["FETCH_SIZE"]), "write_bytes": int(tr[dom]["WRITE_SIZE"]),
+                "fetch_scale": round(sc["FETCH_SIZE"], 4), "write_scale": round(sc["WRITE_SIZE"], 4),
                 "source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, 2 driver steps, scaled by a 1 GiB copy"}
+            # every kernel's traffic per working launch beside its algorithmic bytes per launch
+            for k, t in tr.items():
+                if k == "_scale" or k not in kb or "FETCH_SIZE" not in t or "WRITE_SIZE" not in t:
+                    continue
+                kb[k]["traffic_MB_per_working_launch"] = round((t["FETCH_SIZE"] + t["WRITE_SIZE"]) / 1e6, 1)
+                kb[k]["fetch_MB"] = round(t["FETCH_SIZE"] / 1e6, 1)
+                kb[k]["write_MB"] = round(t["WRITE_SIZE"] / 1e6, 1)
+                if k in ALGO_BYTES_PER_CELL:
+                    per_cell = ALGO_BYTES_PER_CELL[k] + (32 if k == "implicit_finalize2" and resident else 0)
+                    kb[k]["algorithmic_MB_per_launch"] = round(per_cell * N / 1e6, 1)
     if a.batch_runs > 0:
         line["batched"] = bench_batch(a)
     if a.adjoint_n > 0:

@@ -258,6 +258,13 @@ MHD_API int mhd_adjoint_backward(mhd_ctx* ctx, const double* const* lam_final, d
  * fields on the GPU (the LibTorch autograd Function of this package). */
 MHD_API int mhd_adjoint_backward_device(mhd_ctx* ctx, const double* const* lam_final, double* const* lam0,
                                 double* eta_bar);
+/* Launch mode of the backward pass (no reference counterpart).  on != 0
+ * (default): a pass with no host round trip (Jacobi adjoint solves) is
+ * captured once per configuration as a HIP graph and replayed; on == 0:
+ * its kernels are launched directly, in the same order with the same
+ * arguments (bit-identical results).  Turn it off under tools that trace
+ * graph-launched kernels unreliably. */
+MHD_API int mhd_adjoint_use_graph(mhd_ctx* ctx, int on);
 /* Branch-frozen forward map, for checking the adjoint against finite
  * differences (no reference counterpart).  The step is piecewise smooth: the
  * minmod choice (MUSCL.hpp:11-17), the HLL case and its wave-speed min / max

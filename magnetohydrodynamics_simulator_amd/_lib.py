@@ -75,6 +75,7 @@ SIGNATURES = {
     "mhd_cfl_dt_noalloc": (C.c_int, [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int,
                                      C.c_int, C.POINTER(_dp), _dp, _vp]),
     "mhd_adjoint_record_branches": (C.c_int, [_vp, C.c_int]),
+    "mhd_adjoint_use_graph": (C.c_int, [_vp, C.c_int]),
     "mhd_adjoint_forward_frozen": (C.c_int, [_vp, C.c_int, C.c_double]),
     "mhd_explicit_rhs_vjp": (C.c_int, [_vp, C.POINTER(_dp), C.POINTER(_dp)]),
     "mhd_explicit_rhs_vjp_unfused": (C.c_int, [_vp, C.POINTER(_dp), C.POINTER(_dp)]),

@@ -244,6 +244,7 @@ struct AdjointStore {
   uint64_t* br = nullptr;
   long long br_cap = 0;               // words allocated
   bool record = false;                // record in the next backward pass
+  bool use_graph = true;              // replay the sync-free backward sweep as a captured graph
   int br_steps = 0;                   // steps recorded (0: none)
   double br_dt = 0.0;
   CgState* states = nullptr;          // final states of every solve of a backward pass (4 per step)
@@ -261,11 +262,13 @@ struct AdjointStore {
 
 void destroy_adjoint(mhd_ctx* c) {
   if (!c->adj) return;
+  // the graph's nodes name the buffers below: destroy it first
+  if (c->adj->graph) (void)hipGraphExecDestroy(c->adj->graph);
+  c->adj->graph = nullptr;
   for (double* p : c->adj->owned)
     if (p) (void)hipFree(p);
   if (c->adj->states) (void)hipFree(c->adj->states);
   if (c->adj->fwd_states) (void)hipFree(c->adj->fwd_states);
-  if (c->adj->graph) (void)hipGraphExecDestroy(c->adj->graph);
   if (c->adj->br) (void)hipFree(c->adj->br);
   if (c->adj->states_host) (void)hipHostFree(c->adj->states_host);
   delete c->adj;
@@ -742,7 +745,7 @@ int backward_entry(mhd_ctx* c, const double* const* lam_final, double* const* la
     }
     A.br_steps = 0;
   }
-  if (jacobi_sweeps(c) > 0 && c->prof_mask == 0 && !A.record) {
+  if (jacobi_sweeps(c) > 0 && c->prof_mask == 0 && !A.record && A.use_graph) {
     // no host round trip anywhere in the sweep: replay it as one graph
     // (captured on the first call of a configuration, reused after)
     rc = upload_lam();
@@ -912,6 +915,19 @@ int mhd_adjoint_record_branches(mhd_ctx* c, int on) {
   int rc = ensure_store(c, 0, 0);
   if (rc) return rc;
   c->adj->record = (on != 0);
+  return MHD_OK;
+}
+
+int mhd_adjoint_use_graph(mhd_ctx* c, int on) {
+  if (check(c)) return MHD_EINVAL;
+  int rc = ensure_store(c, 0, 0);
+  if (rc) return rc;
+  AdjointStore& A = *c->adj;
+  A.use_graph = (on != 0);
+  if (!A.use_graph && A.graph) {
+    (void)hipGraphExecDestroy(A.graph);
+    A.graph = nullptr;
+  }
   return MHD_OK;
 }
 

@@ -298,6 +298,11 @@ class Context:
         self._check(self.lib.mhd_adjoint_backward(self.h, lp, op, C.byref(eb)))
         return outs, eb.value
 
+    def adjoint_use_graph(self, on: bool = True):
+        """Replay the backward pass as a captured HIP graph (default) or
+        launch its kernels directly (same kernels and order, same bits)."""
+        self._check(self.lib.mhd_adjoint_use_graph(self.h, int(bool(on))))
+
     def adjoint_record_branches(self, on: bool = True):
         """Record, in the next adjoint_backward, every branch choice the
         adjoint linearises at (minmod, HLL case and wave-speed min / max,

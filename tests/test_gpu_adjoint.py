@@ -281,22 +281,28 @@ def test_kink_column_dot_product_small_step():
 
 
 def test_adjoint_direct_launches_match_graph_replay():
-    """With kernel event timing on, the backward pass launches its kernels
-    directly (speculative solves checked afterwards) instead of replaying a
-    graph; with Jacobi adjoint solves there is no CG state to check, and the
-    result is the graph replay's bit for bit."""
+    """With graph replay off (mhd_adjoint_use_graph(ctx, 0)), and with kernel
+    event timing on, the backward pass launches its kernels directly
+    (speculative solves checked afterwards) instead of replaying a graph; with
+    Jacobi adjoint solves there is no CG state to check, and the result is
+    the graph replay's bit for bit."""
     n, eta, steps, every = 32, 2e-2, 5, 2
     _say(f"adjoint with direct launches vs graph replay, {n}x{n}, {steps} steps")
     ic = _state(n, 21)
     lamN = dict(zip(CONSERVED, _smooth_fields(n, 22)))
     res = []
-    for timed in (False, True):
+    for mode in ("graph", "timed", "direct"):
         with _ctx(ic, n, eta) as c:
-            if timed:
+            if mode == "timed":
                 c.profile(["cfl"])
+            if mode == "direct":
+                c.adjoint_use_graph(False)
             c.adjoint_forward(4e-3, steps, every)
             res.append(c.adjoint_backward(lamN))
-    (l1, e1), (l2, e2) = res
-    assert e1 == e2
-    for k in CONSERVED:
-        assert np.array_equal(l1[k], l2[k]), k
+            if mode == "direct":     # again on the same context: still direct, still the same
+                res.append(c.adjoint_backward(lamN))
+    l1, e1 = res[0]
+    for l2, e2 in res[1:]:
+        assert e1 == e2
+        for k in CONSERVED:
+            assert np.array_equal(l1[k], l2[k]), k
