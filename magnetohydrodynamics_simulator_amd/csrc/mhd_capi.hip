@@ -77,11 +77,18 @@ StageArgs stage_args(mhd_ctx* c) {
   a.inv_dy = 1.0 / c->dy;
   a.eta = c->eta;
   a.rho_raw = c->rho_raw;
+  a.strips = (c->nx + kStripOut - 1) / kStripOut;
+  a.segs = (c->ny + a.rows - 1) / a.rows;
+  a.band = kStageBands > 0 ? (a.strips + kStageBands - 1) / kStageBands : 0;
   return a;
 }
 
+dim3 stage_grid(const StageArgs& a) {
+  return a.band > 0 ? dim3((unsigned)(a.strips * a.segs)) : dim3((unsigned)a.strips, (unsigned)a.segs);
+}
+
 int run_stage(mhd_ctx* c, int mode, const StageArgs& a, int id) {
-  dim3 grid((c->nx + kStripOut - 1) / kStripOut, (c->ny + a.rows - 1) / a.rows);
+  const dim3 grid = stage_grid(a);
   if (mode == 0) return launch_k(c, id, k_stage<0>, grid, dim3(64), a);
   if (mode == 1) return launch_k(c, id, k_stage<1>, grid, dim3(64), a);
   return launch_k(c, id, k_stage<2>, grid, dim3(64), a);

@@ -152,6 +152,8 @@ int grid_1d(long long n);
 // the host-side dt / operator; the resident loop's kernels replace them with
 // the device driver state's).
 mhd::StageArgs stage_args(mhd_ctx* c);
+// the launch grid of a stage kernel with these arguments (StageArgs.band)
+dim3 stage_grid(const mhd::StageArgs& a);
 mhd::CgArgs cg_args(mhd_ctx* c, const double* bx, const double* by);
 mhd::FinArgs fin_args(mhd_ctx* c, const double* bx, const double* by, double* bxo, double* byo, double coef);
 mhd::DiagArgs diag_args(mhd_ctx* c, bool fin);

@@ -264,13 +264,36 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, int strip, int se
 }
 
 
+// The (strip, segment) of this workgroup.  With a.band > 0 the grid is 1D:
+// consecutive workgroup ids are dealt to the 8 XCDs in turn, so XCD k runs
+// ids k, k + 8, ...; those are renumbered into one contiguous run of virtual
+// ids per XCD, and virtual ids walk column bands of a.band strips left to
+// right, within a band the rows of strips bottom to top.  Horizontally and
+// vertically adjacent strips then share their halo lines in one L2.  Measured
+// at 4096^2 (both stages, results bitwise the same): one band (a.band =
+// strips: each XCD marches a band of whole rows) cuts the stages' L2 misses
+// by 19-22 % and their time by 1-4 % against the 2D grid; narrower bands
+// (2, 4, 8, 16 per row) run slower.
+__device__ __forceinline__ int2 stage_tile(const StageArgs& a) {
+  if (a.band <= 0) return make_int2(blockIdx.x, blockIdx.y);
+  const int nb = gridDim.x, b = blockIdx.x;
+  const int q = nb >> 3, rem = nb & 7, k = b & 7;
+  const int v = k * q + (k < rem ? k : rem) + (b >> 3);
+  const int per_band = a.band * a.segs;
+  const int bi = v / per_band, r = v - bi * per_band;
+  const int left = a.strips - bi * a.band;
+  const int w = left < a.band ? left : a.band;
+  return make_int2(bi * a.band + r % w, r / w);
+}
+
 #ifndef MHD_STAGE_WAVES
 #define MHD_STAGE_WAVES 3      // waves per SIMD the single-run stage kernel is compiled for (tuning parameter)
 #endif
 
 template <int MODE>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MHD_STAGE_WAVES))) void k_stage(StageArgs a) {
-  stage_body<MODE>(a, blockIdx.x, blockIdx.y);
+  const int2 t = stage_tile(a);
+  stage_body<MODE>(a, t.x, t.y);
 }
 
 template __global__ void k_stage<0>(StageArgs);
@@ -1439,7 +1462,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MHD_STAGE_WA
   b.c_a = (MODE == 1) ? rs->coef[0] : 0.0;
   b.c_b = (MODE == 1) ? rs->coef[1] : rs->coef[2];
   b.c_g = (MODE == 2) ? rs->coef[3] : 0.0;
-  stage_body<MODE>(b, blockIdx.x, blockIdx.y);
+  const int2 t = stage_tile(a);
+  stage_body<MODE>(b, t.x, t.y);
 }
 template __global__ void k_stage_res<1>(StageArgs, const RunState*);
 template __global__ void k_stage_res<2>(StageArgs, const RunState*);

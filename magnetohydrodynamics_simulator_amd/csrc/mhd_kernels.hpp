@@ -7,6 +7,10 @@
 namespace mhd {
 
 constexpr int kStripOut = 60;     // output columns per 64-lane strip of the stage kernel
+#ifndef MHD_STAGE_BANDS
+#define MHD_STAGE_BANDS 1          // column bands of the single-run stage launch (0: a 2D grid; tuning parameter)
+#endif
+constexpr int kStageBands = MHD_STAGE_BANDS;
 constexpr int kTileX = 256;       // columns (= threads) per tile of the 2D sweeps
 #ifndef MHD_TARGET_TILES
 #define MHD_TARGET_TILES 2048
@@ -31,6 +35,10 @@ struct StageArgs {
   double c_a, c_b, c_g;
   // MODE 2: the new density before its floor, where it is <= 1e-8 (null: not kept)
   double* rho_raw;
+  // launch shape: band == 0: a (strips, segs) grid; band > 0: a 1D grid of
+  // strips * segs workgroups, each XCD taking a contiguous run of the
+  // column bands of `band` strips (rows of strips bottom to top in a band)
+  int strips, segs, band;
 };
 
 // Conjugate-gradient state, device resident.  Written by block 0 of a pass

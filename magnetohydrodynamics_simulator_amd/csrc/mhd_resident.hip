@@ -181,8 +181,7 @@ int enqueue_step(mhd_ctx* c, Loop& L, int from) {
       a.o1[q] = c->F1[q];
       a.o2[q] = c->S[q];
     }
-    dim3 grid((c->nx + kStripOut - 1) / kStripOut, (c->ny + a.rows - 1) / a.rows);
-    rc = launch_k(c, 0, k_stage_res<1>, grid, dim3(64), a, (const RunState*)R->rs);
+    rc = launch_k(c, 0, k_stage_res<1>, stage_grid(a), dim3(64), a, (const RunState*)R->rs);
     if (rc) return rc;
   }
   if (res && from <= 2) {
@@ -202,8 +201,7 @@ int enqueue_step(mhd_ctx* c, Loop& L, int from) {
       b.u[q] = c->F1[q];
       b.o1[q] = c->U[q];
     }
-    dim3 grid((c->nx + kStripOut - 1) / kStripOut, (c->ny + b.rows - 1) / b.rows);
-    rc = launch_k(c, 1, k_stage_res<2>, grid, dim3(64), b, (const RunState*)R->rs);
+    rc = launch_k(c, 1, k_stage_res<2>, stage_grid(b), dim3(64), b, (const RunState*)R->rs);
     if (rc) return rc;
   }
   if (res && from <= 5) {
