@@ -8,7 +8,8 @@ the final time.  mhd_run itself is held to the oracle by test_gpu_parity.py.
 
 Cases: the adaptive CG budget; fixed budgets of 0 and 1 launched iterations,
 which halt the loop in (nearly) every solve and send those steps through the
-host's resume path, with group sizes 1, 3 and 16; pairs of steps replayed as
+host's resume path, with group sizes 1, 3 and 16; solves of 18-22 iterations
+halted and resumed mid-solve; pairs of steps replayed as
 captured graphs (every replay of a group checked through the final state); the t_end clamp; a constant
 dt; eta = 0 (no implicit solve); a non-square grid; the snapshot cadence
 (frame numbers, the state at each frame, files in the reference's format);
@@ -66,6 +67,32 @@ def test_resident_matches_run(tune):
     got = _resident(ic, n, n, dx, dy, eta, 10.0, 25, tune=tune, tol=1e-10, cfl=0.4)
     assert ref[2].min() > 0          # the solves iterate
     _same(got, ref)
+
+
+@pytest.mark.parametrize("tune", [None, (3, 0, 0), (4, 1, 1, True)], ids=["adaptive", "fixed0-g3", "graph-fixed1-g4"])
+def test_long_solves(tune):
+    """Solves of many iterations: resistivity-limited steps at tol 1e-25 (the
+    recursive residual keeps falling) take 18-22 iterations per solve, so the
+    fixed budgets halt the loop deep inside a solve and the host resumes it
+    there.  Resident loop = mhd_run bit for bit, and mhd_run = the oracle's
+    loop (same time steps, iteration counts within one, relative L1 <= 1e-10)."""
+    from helpers import interior, make_oracle, rel_l1
+    n, eta, tol, steps = 64, 0.5, 1e-25, 4
+    _say(f"long solves: resident vs mhd_run vs oracle, OT {n}^2 eta={eta} tol={tol}, tune={tune}")
+    ic, dx, dy = ic_arrays("orszag_tang", n, n)
+    ref = _plain(ic, n, n, dx, dy, eta, 10.0, steps, tol=tol, cfl=0.4)
+    assert ref[2].min() > 16, ref[2]
+    got = _resident(ic, n, n, dx, dy, eta, 10.0, steps, tune=tune, tol=tol, cfl=0.4)
+    _same(got, ref)
+    if tune is None:
+        sim = make_oracle(ic, n, n, dx, dy, eta, tol=tol)
+        nr, dts_r, its_r, _ = sim.run(1e9, cfl=0.4, max_steps=steps, snapshot_every=0)
+        assert nr == steps and np.allclose(ref[1], dts_r, rtol=1e-12, atol=0)
+        assert np.abs(ref[2] - its_r).max() <= 1, (ref[2], its_r)
+        for k in CONSERVED:
+            e = rel_l1(interior(ref[4][k]), sim.interior(k))
+            assert e <= 1e-10, (k, e)
+        sim.close()
 
 
 def test_resident_t_end_clamp_and_eta0():
