@@ -168,11 +168,26 @@ typedef int (*mhd_snapshot_fn)(mhd_ctx* ctx, int frame, double t, double dt, voi
  * driver state once per group of steps instead of three times per step.
  * With snapshot_every > 0 and on_snapshot non-NULL, on_snapshot runs before
  * the first step and after every snapshot_every-th step, like the
- * reference's snapshot cadence (integrator_main.cpp:65-70, 236-241).
+ * reference's snapshot cadence (integrator_main.cpp:65-70, 102-111).
  * No reference counterpart beyond the loop itself. */
 MHD_API int mhd_run_resident(mhd_ctx* ctx, double t_end, double dt_override, int max_steps, int snapshot_every,
                              mhd_snapshot_fn on_snapshot, void* user, int* steps_out, double* t_out,
                              double* step_dt, int* cg_iters);
+/* mhd_run_resident with the reference's snapshot files written while the
+ * loop runs on (write_data_to_binary_file, include/diagnostics.hpp:90-131,
+ * at the cadence of integrator_main.cpp:65-70, 102-111): before the first
+ * step and after every snapshot_every-th step (> 0), the device copies the
+ * file's arrays and div B partials into one of two staging buffers, a second
+ * stream moves them to pinned host memory, and a writer thread writes
+ * path_format with the frame number (one %d conversion, e.g.
+ * "output/snapshot_%06d.dat"; header snapshot_rate = snapshot_rate).  The
+ * files are byte for byte mhd_write_snapshot's from a mhd_run_resident
+ * callback.  Returns after the last file is written: the loop's error, or
+ * else the first write error.  Device and pinned host memory: 2 x (7 nx ny
+ * + 2 tiles) doubles each, for the call's duration. */
+MHD_API int mhd_run_resident_files(mhd_ctx* ctx, double t_end, double dt_override, int max_steps, int snapshot_every,
+                                   const char* path_format, int snapshot_rate, int* steps_out, double* t_out,
+                                   double* step_dt, int* cg_iters);
 /* Tuning of mhd_run_resident: steps enqueued between two read-backs (1..256,
  * default 16), extra CG iterations launched per solve beyond the most of the
  * previous group (default 0), fixed_budget >= 0: that many launched

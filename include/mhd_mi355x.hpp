@@ -144,24 +144,21 @@ void write_data_to_binary_file(const GridT& g, Context& ctx, int current_frame_i
 // the device: g's state is uploaded once, advanced to t_end (constant dt when
 // dt_override != 0) by mhd_run_resident, with a snapshot
 // output/snapshot_<frame>.dat before the first step and every
-// snapshot_frequency steps (0: none), and downloaded into g at the end.
-// Returns the number of steps taken.
+// snapshot_frequency steps (0: none), written while the loop runs on
+// (mhd_run_resident_files), and downloaded into g at the end.  Returns the
+// number of steps taken.
 template <class GridT>
 int integrate(GridT& g, Context& ctx, double t_end, double dt_override, int snapshot_frequency) {
   ctx.upload(g);
-  struct Cadence {
-    int rate;
-  } cad{snapshot_frequency};
-  auto write = [](mhd_ctx* c, int frame, double, double dt, void* user) -> int {
-    char name[64];
-    std::snprintf(name, sizeof(name), "output/snapshot_%06d.dat", frame);
-    return mhd_write_snapshot(c, name, static_cast<Cadence*>(user)->rate, dt);
-  };
   int steps = 0;
   double t = 0.0;
-  check(mhd_run_resident(ctx.get(), t_end, dt_override, -1, snapshot_frequency,
-                         snapshot_frequency > 0 ? +write : nullptr, &cad, &steps, &t, nullptr, nullptr),
-        ctx.get());
+  if (snapshot_frequency > 0)
+    check(mhd_run_resident_files(ctx.get(), t_end, dt_override, -1, snapshot_frequency, "output/snapshot_%06d.dat",
+                                 snapshot_frequency, &steps, &t, nullptr, nullptr),
+          ctx.get());
+  else
+    check(mhd_run_resident(ctx.get(), t_end, dt_override, -1, 0, nullptr, nullptr, &steps, &t, nullptr, nullptr),
+          ctx.get());
   ctx.download(g);
   return steps;
 }

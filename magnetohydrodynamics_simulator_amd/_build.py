@@ -18,7 +18,7 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libmhd_mi355x.so")
 SOURCES = ["mhd_kernels.hip", "mhd_capi.hip", "mhd_batch.hip", "mhd_adjoint.hip", "mhd_batch_adjoint.hip", "mhd_io.hip",
-           "mhd_resident.hip", "mhd_setups.hip", "mhd_noalloc.hip"]
+           "mhd_resident.hip", "mhd_setups.hip", "mhd_noalloc.hip", "mhd_snapshots.hip"]
 ARCH = "gfx950"
 
 

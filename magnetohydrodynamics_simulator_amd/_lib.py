@@ -55,6 +55,8 @@ SIGNATURES = {
     "mhd_divb": (C.c_int, [_vp, _dp, _dp]),
     "mhd_run": (C.c_int, [_vp, C.c_double, C.c_double, C.c_int, _ip, _dp, _dp, _ip]),
     "mhd_run_resident": (C.c_int, [_vp, C.c_double, C.c_double, C.c_int, C.c_int, _vp, _vp, _ip, _dp, _dp, _ip]),
+    "mhd_run_resident_files": (C.c_int, [_vp, C.c_double, C.c_double, C.c_int, C.c_int, C.c_char_p, C.c_int, _ip,
+                                         _dp, _dp, _ip]),
     "mhd_resident_tune": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int]),
     "mhd_init_fourier": (C.c_int, [_vp, C.c_int, _dp]),
     "mhd_init_column": (C.c_int, [_vp, C.c_int, C.c_int, _dp, _dp]),

@@ -166,6 +166,9 @@ mhd::DiagArgs diag_args(mhd_ctx* c, bool fin);
 int cg_run(mhd_ctx* c, const double* bx, const double* by, const double* bin, mhd::CgState* out, int slot,
            int spec = 0, mhd::CgState* spec_out = nullptr);
 int to_host_padded(mhd_ctx* c, const double* dev, double* host, bool ghosts);
+// Ghost layers of one padded (ny+4) x (nx+4) host array: periodic
+// (Grid::fillOneField) or, on an outflow axis, the nearest inside value.
+void fill_ghosts(double* f, int nx, int ny, int open_x, int open_y);
 // A computePrimitives pass over the stored state (Grid.hpp:111-115) as far as
 // the next CFL step sees it: the velocities of floored cells come from the
 // floored density from now on (k_raw_reset).

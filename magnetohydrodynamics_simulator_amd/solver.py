@@ -230,6 +230,24 @@ class Context:
         n = steps.value
         return n, dts[:n].copy(), its[: 2 * n].reshape(n, 2).copy(), t.value
 
+    def run_resident_files(self, t_end: float, snapshot_every: int, path_format: str, snapshot_rate: int | None = None,
+                           dt_override: float = 0.0, max_steps: int = -1):
+        """run_resident with the reference's snapshot files written while the
+        loop runs on (mhd_run_resident_files): path_format holds one %d (the
+        frame = steps taken), e.g. "output/snapshot_%06d.dat"; the header's
+        snapshot rate is snapshot_rate (default snapshot_every).  Returns
+        after the last file is written."""
+        cap = max_steps if max_steps > 0 else 1_000_000
+        dts = np.zeros(cap)
+        its = np.zeros(2 * cap, dtype=np.int32)
+        steps, t = C.c_int(), C.c_double()
+        rate = snapshot_every if snapshot_rate is None else snapshot_rate
+        self._check(self.lib.mhd_run_resident_files(self.h, t_end, dt_override, max_steps, snapshot_every,
+                                                    os.fsencode(path_format), rate, C.byref(steps), C.byref(t),
+                                                    _ptr(dts), its.ctypes.data_as(C.POINTER(C.c_int))))
+        n = steps.value
+        return n, dts[:n].copy(), its[: 2 * n].reshape(n, 2).copy(), t.value
+
     def resident_tune(self, group_steps: int = 16, spec_margin: int = 0, fixed_budget: int = -1,
                       graph: bool = False):
         """Group size, CG iteration margin, an optional fixed iteration budget

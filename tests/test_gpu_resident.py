@@ -159,6 +159,59 @@ def test_resident_snapshot_cadence(tmp_path):
         assert list(head[:4]) == [n, n, 2, every]
 
 
+@pytest.mark.parametrize("case", ["periodic", "outflow", "graph"])
+def test_async_snapshot_files(tmp_path, case):
+    """mhd_run_resident_files writes, while the loop runs on, byte for byte
+    the files a mhd_run_resident callback writes with mhd_write_snapshot at
+    the same frames (before the first step and every 3rd step), and leaves
+    the same run (state, time steps, CG counts)."""
+    nx, ny, eta, every, steps = 64, 48, 1e-2, 3, 14
+    bc = ("outflow", "periodic") if case == "outflow" else ("periodic", "periodic")
+    _say(f"asynchronous snapshot files, {nx}x{ny} {case}, every {every}, {steps} steps")
+    ic, dx, dy = ic_arrays("orszag_tang", nx, ny, seed=3, amp=1e-2)
+    sync_dir, async_dir = tmp_path / "sync", tmp_path / "async"
+    sync_dir.mkdir()
+    async_dir.mkdir()
+
+    def on_snap(ctx, frame, t, dt):
+        ctx.write_snapshot(os.fsdecode(sync_dir / f"snapshot_{frame:06d}.dat"), every, dt)
+
+    with make_gpu(ic, nx, ny, dx, dy, eta, cfl=0.4, tol=1e-10, bc=bc) as ctx:
+        if case == "graph":
+            ctx.resident_tune(4, 0, -1, True)
+        ref = ctx.run_resident(10.0, max_steps=steps, snapshot_every=every, on_snapshot=on_snap)
+        ref_state = ctx.download()
+    with make_gpu(ic, nx, ny, dx, dy, eta, cfl=0.4, tol=1e-10, bc=bc) as ctx:
+        if case == "graph":
+            ctx.resident_tune(4, 0, -1, True)
+        got = ctx.run_resident_files(10.0, every, os.fsdecode(async_dir / "snapshot_%06d.dat"), max_steps=steps)
+        got_state = ctx.download()
+    assert got[0] == ref[0] == steps
+    assert np.array_equal(got[1], ref[1]) and np.array_equal(got[2], ref[2]) and got[3] == ref[3]
+    for k in CONSERVED:
+        assert np.array_equal(got_state[k], ref_state[k]), k
+    names = sorted(p.name for p in sync_dir.iterdir())
+    assert names == [f"snapshot_{f:06d}.dat" for f in range(0, steps + 1, every)]
+    assert sorted(p.name for p in async_dir.iterdir()) == names
+    for name in names:
+        assert (async_dir / name).read_bytes() == (sync_dir / name).read_bytes(), name
+
+
+def test_async_snapshot_errors(tmp_path):
+    n = 32
+    ic, dx, dy = ic_arrays("orszag_tang", n, n)
+    _say("asynchronous snapshot files: error paths")
+    with make_gpu(ic, n, n, dx, dy, 1e-3, cfl=0.4) as ctx:
+        for fmt in ("snap_%s.dat", "snap_%d_%d.dat", "snap.dat"):
+            with pytest.raises(RuntimeError, match="path format"):
+                ctx.run_resident_files(1.0, 2, os.fsdecode(tmp_path / fmt), max_steps=2)
+        with pytest.raises(RuntimeError, match="cannot open"):
+            ctx.run_resident_files(1.0, 2, os.fsdecode(tmp_path / "missing_dir" / "snap_%04d.dat"), max_steps=6)
+        # the context still runs
+        ns, _, _, _ = ctx.run_resident(1.0, max_steps=2)
+        assert ns == 2
+
+
 def test_resident_errors():
     n = 32
     ic, dx, dy = ic_arrays("orszag_tang", n, n)
