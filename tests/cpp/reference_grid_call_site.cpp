@@ -3,19 +3,21 @@
 // own Grid (include/Grid.hpp, taken from the reference tree at build time)
 // through this library's C++ adapter (include/mhd_mi355x.hpp).
 //
-//   reference_grid_call_site IN OUT
+//   reference_grid_call_site IN OUT [SNAPSHOT_FREQUENCY]
 //
 // IN: doubles nx, ny, dx, dy, eta, t_end, then the six padded (ny+4) x (nx+4)
 // arrays rho, mx, my, E, bxf, byf.  The program sets up a Grid with them and
 // calls, in this order: explicit_rhs of the initial state,
 // CFL_condition_checker, generate_A_matrix for that time step, one IMEX step,
 // resistive_B_term (implicit) and calculate_current_div_B of the stepped
-// state, then integrate to t_end (no snapshots).
+// state, then integrate to t_end (snapshot files output/snapshot_<frame>.dat
+// every SNAPSHOT_FREQUENCY steps; none by default).
 // OUT: doubles dt, steps of the integration, rms and max |div B| after the
 // step, then padded arrays: the six explicit right-hand sides, the two
 // resistive rates, the six fields after the IMEX step, the six at t_end.
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <vector>
 
 #include "Grid.hpp"
@@ -33,10 +35,11 @@ std::vector<double>* field(Grid& g, int q) {
 }  // namespace
 
 int main(int argc, char** argv) {
-  if (argc != 3) {
-    std::fprintf(stderr, "usage: %s IN OUT\n", argv[0]);
+  if (argc != 3 && argc != 4) {
+    std::fprintf(stderr, "usage: %s IN OUT [SNAPSHOT_FREQUENCY]\n", argv[0]);
     return 2;
   }
+  const int snapshot_frequency = argc == 4 ? std::atoi(argv[3]) : 0;
   std::FILE* in = std::fopen(argv[1], "rb");
   if (!in) return 2;
   double hdr[6];
@@ -70,7 +73,7 @@ int main(int argc, char** argv) {
     for (int q = 0; q < 6; ++q) after_step[q] = *field(grid, q);
     mhd_mi355x::resistive_B_term(grid, ctx, rhs[6], rhs[7], true);
     const mhd_mi355x::divergence_checker_values div = mhd_mi355x::calculate_current_div_B(grid, ctx);
-    const int steps = mhd_mi355x::integrate(grid, ctx, t_end, 0.0, 0);
+    const int steps = mhd_mi355x::integrate(grid, ctx, t_end, 0.0, snapshot_frequency);
     const double head[4] = {dt, (double)steps, div.root_mean_square_div, div.max_norm_divergence};
     std::fwrite(head, sizeof(double), 4, out);
     for (int q = 0; q < 8; ++q) std::fwrite(rhs[q].data(), sizeof(double), rhs[q].size(), out);
