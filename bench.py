@@ -915,7 +915,11 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "bytes_per_launch": int(bytes_per_launch), "ms_per_launch": round(dom_ms / dom_cnt, 4),
                      "launches": dom_cnt,
-                     "working_launches": spmv if dom == "cg_pass_b" else None},
+                     "working_launches": spmv if dom == "cg_pass_b" else None,
+                     "note": "achieved = algorithmic bytes / mean launch time; the CG passes are ordered to re-read "
+                             "what the previous pass left in the 256 MB memory-side cache (MALL), which FETCH_SIZE "
+                             "also counts, so part of these bytes come from it; step_roofline is the "
+                             "cache-independent figure"},
         "step_roofline": {"algorithmic_bytes_per_step": int(step_bytes),
                           "frac": round(step_bytes / (elapsed / steps) / 1e9 / HBM_PEAK_GBS, 4)},
         "cg_spmv_per_step": round(spmv / steps, 2),
