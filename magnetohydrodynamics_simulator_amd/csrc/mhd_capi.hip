@@ -102,6 +102,8 @@ CgArgs cg_args(mhd_ctx* c, const double* bx, const double* by) {
   a.open_y = c->o.bc_y == MHD_BC_OUTFLOW;
   a.dx = c->dx;
   a.dy = c->dy;
+  a.inv_dx = 1.0 / c->dx;
+  a.inv_dy = 1.0 / c->dy;
   a.eta = c->eta;
   const double cx = c->alpha / (c->dx * c->dx);
   const double cy = c->alpha / (c->dy * c->dy);
@@ -209,6 +211,8 @@ FinArgs fin_args(mhd_ctx* c, const double* bx, const double* by, double* bxo, do
   f.open_y = c->o.bc_y == MHD_BC_OUTFLOW;
   f.dx = c->dx;
   f.dy = c->dy;
+  f.inv_dx = 1.0 / c->dx;
+  f.inv_dy = 1.0 / c->dy;
   f.eta = c->eta;
   f.c = coef;
   f.x = c->x;

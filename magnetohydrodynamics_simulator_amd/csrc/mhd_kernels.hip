@@ -176,7 +176,7 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, int strip, int se
     double j_b = 0.0;
     if (MODE == 2) {
       const double byl = sh_up(s.byf_b);
-      j_b = a.eta * ((s.byf_b - byl) / a.dx - (s.bxl_b - s.bxl_a) / a.dy);
+      j_b = a.eta * ((s.byf_b - byl) * a.inv_dx - (s.bxl_b - s.bxl_a) * a.inv_dy);
     }
 
     // ---- dU(k-2): flux divergence, CT update, stage combination ----
@@ -224,8 +224,8 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, int strip, int se
         for (int q = 0; q < 6; ++q) wv[q] = a.u[q][o];
         double g2x = 0.0, g2y = 0.0;
         if (a.eta != 0.0) {
-          g2x = -(j_above - s.j_a) / a.dy;
-          g2y = (j_right - s.j_a) / a.dx;
+          g2x = -(j_above - s.j_a) * a.inv_dy;
+          g2y = (j_right - s.j_a) * a.inv_dx;
         }
 #pragma unroll
         for (int q = 0; q < 6; ++q) {
@@ -438,8 +438,8 @@ __device__ __forceinline__ void tile_columns(int nx, int tiles_y, int vb, int& x
 // eta J(B) at corner (i, j) (resistive_B_term's first loop, solvers.hpp:450-463)
 // from the face values by(i, j), by(i-1, j), bx(i, j), bx(i, j-1).
 __device__ __forceinline__ double rhs_from(const CgArgs& a, double by_c, double by_w, double bx_c, double bx_s) {
-  const double dBy_dx = (by_c - by_w) / a.dx;
-  const double dBx_dy = (bx_c - bx_s) / a.dy;
+  const double dBy_dx = (by_c - by_w) * a.inv_dx;
+  const double dBx_dy = (bx_c - bx_s) * a.inv_dy;
   return a.eta * (dBy_dx - dBx_dy);
 }
 template <bool OPEN = true>
@@ -809,8 +809,8 @@ __device__ __forceinline__ void finalize1_body(const FinArgs& a, int vb) {
     const double bxn = more ? a.bx[(j + 1) * nx + t.ic] : 0.0;
     const double byn = more ? a.by[(j + 1) * nx + t.ic] : 0.0;
     const int o = j * nx + t.ic;
-    a.bx_out[o] = bxv + (-(xn - xc) / a.dy) * a.c;
-    a.by_out[o] = byv + ((xe - xc) / a.dx) * a.c;
+    a.bx_out[o] = bxv + (-(xn - xc) * a.inv_dy) * a.c;
+    a.by_out[o] = byv + ((xe - xc) * a.inv_dx) * a.c;
     xc = xn; xn = xnn; xe = xen; bxv = bxn; byv = byn;
   }
 }
@@ -841,13 +841,13 @@ __device__ __forceinline__ double fin2_j3(const FinArgs& a, bool xv, int col, in
   const double bxs = a.bx[row * nx + col];
   const double bys = a.by[row * nx + col];
   const double bysw = a.by[row * nx + cm];
-  const double bx3 = bxs + (-(xn - xc) / a.dy) * a.c;           // bx3(col, row)
-  const double by3 = bys + ((xe - xc) / a.dx) * a.c;            // by3(col, row)
+  const double bx3 = bxs + (-(xn - xc) * a.inv_dy) * a.c;       // bx3(col, row)
+  const double by3 = bys + ((xe - xc) * a.inv_dx) * a.c;        // by3(col, row)
   // by3(col - 1, row); left of the first column of an outflow axis B3 is
   // extended by its boundary value (the ghost rule on the recombined field)
-  const double by3w = ((OPEN ? a.open_x : 0) && col == 0) ? by3 : bysw + ((xc - xw) / a.dx) * a.c;
-  const double dBy_dx = (by3 - by3w) / a.dx;
-  const double dBx_dy = (bx3 - ch.bx3_below) / a.dy;
+  const double by3w = ((OPEN ? a.open_x : 0) && col == 0) ? by3 : bysw + ((xc - xw) * a.inv_dx) * a.c;
+  const double dBy_dx = (by3 - by3w) * a.inv_dx;
+  const double dBx_dy = (bx3 - ch.bx3_below) * a.inv_dy;
   ch.bx3_below = bx3;
   ch.bxs = bxs;
   ch.bys = bys;
@@ -864,7 +864,7 @@ __device__ __forceinline__ Fin2Chain fin2_begin(const FinArgs& a, bool xv, int c
   const double xc = xv ? a.x[rm * nx + col] : 0.0;
   const double xn = xv ? a.x[rmn * nx + col] : 0.0;
   Fin2Chain ch;
-  ch.bx3_below = a.bx[rm * nx + col] + (-(xn - xc) / a.dy) * a.c;
+  ch.bx3_below = a.bx[rm * nx + col] + (-(xn - xc) * a.inv_dy) * a.c;
   ch.bxs = ch.bys = 0.0;
   return ch;
 }
@@ -877,9 +877,15 @@ __device__ __forceinline__ void cfl_cell(double rho, double mx, double my, doubl
   primitives(rho, mx, my, E, bxc, byc, rf, vx, vy, p);
   rf = dmax(rf, 1e-8);
   p = dmax(p, 1e-8);
-  const double ss = kGamma * p / rf;
-  const double as = (bxc * bxc + byc * byc) / rf;
-  const double cur = sqrt(ss + as) + sqrt(vx * vx + vy * vy);
+  // the two divisions by rho as one refined reciprocal, the square roots by
+  // sqrt_nr (each within about an ulp; rf >= 1e-8 and ss + as >= 1.6e-16 are
+  // normal, a speed below 2^-500 adds nothing to them); non-finite inputs give
+  // a non-finite speed as the IEEE forms do
+  const double inv = rcp(rf);
+  const double ss = (kGamma * p) * inv;
+  const double as = (bxc * bxc + byc * byc) * inv;
+  const double v2 = vx * vx + vy * vy;
+  const double cur = sqrt_nr(ss + as) + ((v2 >= 0x1p-1000) ? sqrt_nr(v2) : 0.0);
   if (cur > m) m = cur;
   if (!(fabs(cur) <= 1.7976931348623157e308)) bad += 1.0;
 }
@@ -942,8 +948,8 @@ __device__ __forceinline__ void finalize2_body(const FinArgs& a, int vb, const D
       jrow[buf ^ 1][w + 1] = jhn;
     }
     const double jr = jrow[buf][tid + 2];        // J3(i + 1, j)
-    const double g3x = -((top ? jc : jn) - jc) / a.dy;
-    const double g3y = (jr - jc) / a.dx;
+    const double g3x = -((top ? jc : jn) - jc) * a.inv_dy;
+    const double g3y = (jr - jc) * a.inv_dx;
     const double bxo = bxs + g3x * a.c;
     const double byo = bys + g3y * a.c;
     if (t.valid && j < t.y1) {
@@ -953,7 +959,7 @@ __device__ __forceinline__ void finalize2_body(const FinArgs& a, int vb, const D
     }
     if (CFL) {
       if (t.valid) bxrow[buf][tid + 1] = bxo;
-      if (edge) bxrow[buf][w + 1] = hbxs + (-((top ? jhc : jhn) - jhc) / a.dy) * a.c;   // new bx(xr, j)
+      if (edge) bxrow[buf][w + 1] = hbxs + (-((top ? jhc : jhn) - jhc) * a.inv_dy) * a.c;   // new bx(xr, j)
       if (j > t.y0 && t.valid) {
         // cell (i, j - 1): bx faces (i, j - 1), (i + 1, j - 1); by faces (i, j - 1), (i, j);
         // above the last row of an outflow axis by is the last row's
@@ -998,11 +1004,11 @@ __global__ __launch_bounds__(256) void k_rate(FinArgs a, int zero) {
     const int jp = bc_wrap(j + 1, ny, a.open_y);
     const int ip = bc_wrap(i + 1, nx, a.open_x);
     if (zero) {
-      a.bx_out[o] = -(0.0 - 0.0) / a.dy;
-      a.by_out[o] = (0.0 - 0.0) / a.dx;
+      a.bx_out[o] = -(0.0 - 0.0) * a.inv_dy;
+      a.by_out[o] = (0.0 - 0.0) * a.inv_dx;
     } else {
-      a.bx_out[o] = -(a.x[jp * nx + i] - a.x[j * nx + i]) / a.dy;
-      a.by_out[o] = (a.x[j * nx + ip] - a.x[j * nx + i]) / a.dx;
+      a.bx_out[o] = -(a.x[jp * nx + i] - a.x[j * nx + i]) * a.inv_dy;
+      a.by_out[o] = (a.x[j * nx + ip] - a.x[j * nx + i]) * a.inv_dx;
     }
   }
 }
@@ -1196,6 +1202,8 @@ __device__ __forceinline__ CgArgs batch_cg_args(const BatchCgArgs& b, int run) {
   a.tiles_y = g.tiles_y;
   a.dx = g.dx;
   a.dy = g.dy;
+  a.inv_dx = g.inv_dx;
+  a.inv_dy = g.inv_dy;
   a.eta = g.eta;
   a.invd = rs.invd;
   a.diag = rs.diag;
@@ -1262,6 +1270,8 @@ __device__ __forceinline__ FinArgs batch_fin_args(const BatchFinArgs& b, int run
   f.tiles_y = g.tiles_y;
   f.dx = g.dx;
   f.dy = g.dy;
+  f.inv_dx = g.inv_dx;
+  f.inv_dy = g.inv_dy;
   f.eta = g.eta;
   f.c = b.rs[run].coef[4];
   f.x = b.x + g.base;

@@ -53,6 +53,7 @@ struct CgArgs {
   int nx, ny, ty, tiles_y;
   int open_x, open_y;
   double dx, dy, eta;
+  double inv_dx, inv_dy;   // 1 / dx, 1 / dy (the right-hand side eta J(B))
   double invd, diag, cx, cy, tol;
   const double* bx;  // face B for the right-hand side
   const double* by;
@@ -69,6 +70,7 @@ struct FinArgs {
   int nx, ny, ty, tiles_y;
   int open_x, open_y;
   double dx, dy, eta, c;
+  double inv_dx, inv_dy;   // 1 / dx, 1 / dy: the differences of the curl and J multiply by them
   const double* x;
   const double* bx;
   const double* by;
