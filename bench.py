@@ -505,6 +505,11 @@ def bench_batch(a) -> dict:
                                                         "groups: 16 steps per driver read-back, CG budgets on the "
                                                         "device; host_checked_steps: one read-back per CG chunk"),
             "cg_iters_last_step_mean": round(float(its.mean()), 2),
+            # the CG work per cell: runs of large eta take more iterations, and the
+            # largest runs carry most cells (the single 4096^2 run: ~4-5 per solve)
+            "cg_iters_last_step_cell_weighted": round(float(np.average(its.mean(axis=1),
+                                                                        weights=[n * n for n in ns])), 2),
+            "cg_iters_last_step_max": int(its.max()),
             "setup": {"device_generation_s": round(t_init_dev, 4),
                       "device_Mcells_per_s": round(cells / t_init_dev / 1e6, 1),
                       "host_generation_and_upload_Mcells_per_s": round(host_cells / t_init_host / 1e6, 2),
