@@ -462,7 +462,8 @@ def bench_batch(a) -> dict:
              "groups_direct_fused_cfl": dict(group_steps=16, graph=False, fused_cfl=True),
              "groups_direct_separate_cfl": dict(group_steps=16, graph=False, fused_cfl=False),
              "groups_graph_fused_cfl": dict(group_steps=16, graph=True, fused_cfl=True),
-             "groups_graph_separate_cfl": dict(group_steps=16, graph=True, fused_cfl=False)}
+             "groups_graph_separate_cfl": dict(group_steps=16, graph=True, fused_cfl=False),
+             "groups_one_graph_per_group_fused_cfl": dict(group_steps=16, graph=2, fused_cfl=True)}
     mode_steps = min(a.batch_steps, 50)
     mode_ms = {}
     for name, m in modes.items():

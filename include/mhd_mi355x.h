@@ -374,10 +374,12 @@ MHD_API int mhd_batch_step(mhd_batch* b, int* n_active);
 MHD_API int mhd_batch_run(mhd_batch* b, int max_batch_steps, int* steps_done);
 /* Launch mode of mhd_batch_run: group_steps (0: one host-checked step at a
  * time; 1..4096: steps per group), spec_margin (extra launched CG iterations
- * per solve), graph (1: pairs of steps captured once as a hipGraph and
- * replayed, per B-buffer parity, budgets and fusion), fused_cfl (1: the
- * final recombination also forms the next step's CFL partials; 0: a separate
- * CFL pass at the start of each step).  Results are identical in every mode. */
+ * per solve), graph (0: direct launches; 1: pairs of steps captured once as a
+ * hipGraph and replayed; 2: the even part of each group captured as one
+ * hipGraph and launched once per group; graphs are kept per step count,
+ * B-buffer parity, budgets and fusion), fused_cfl (1: the final
+ * recombination also forms the next step's CFL partials; 0: a separate CFL
+ * pass at the start of each step).  Results are identical in every mode. */
 MHD_API int mhd_batch_tune(mhd_batch* b, int group_steps, int spec_margin, int graph, int fused_cfl);
 /* The batch's current launch shape and mode (any pointer may be NULL). */
 MHD_API int mhd_batch_get_tuning(mhd_batch* b, int* stage_rows, int* tile_rows, int* group_steps, int* graph,
@@ -386,8 +388,9 @@ MHD_API int mhd_batch_get_tuning(mhd_batch* b, int* stage_rows, int* tile_rows, 
  * launch mode) for each candidate launch shape (rows per explicit-stage
  * segment x rows per CG / finalize tile, each in {16, 32, 64}), then, in
  * group mode with a resistive run, each stage fusion (CFL partials fused into
- * the final recombination or a separate pass) x launch mode (direct launches
- * or graph replay) with the best shape; restores the state after every trial
+ * the final recombination or a separate pass) x launch mode (direct launches,
+ * captured pairs or whole captured groups) with the best shape; restores the
+ * state after every trial
  * and keeps the fastest configuration.  The chosen rows are returned (either
  * pointer may be NULL; mhd_batch_get_tuning gives the rest).  Results do not
  * depend on the configuration beyond the summation order of the CG dot

@@ -378,15 +378,16 @@ int enqueue_group_step(mhd_batch* b, BLoop& L, int from) {
   return MHD_OK;
 }
 
-// The two-step graph for the current B pointers, budgets and CFL fusion,
-// captured on first use (least recently used entry replaced).
-int group_graph(mhd_batch* b, BLoop& L, BatchGraph** out) {
+// The graph of nsteps (even) batch steps for the current B pointers,
+// budgets and CFL fusion, captured on first use (least recently used entry
+// replaced).
+int group_graph(mhd_batch* b, BLoop& L, int nsteps, BatchGraph** out) {
   double* key[4] = {b->U[4], b->U[5], b->S[4], b->S[5]};
   const bool fused = b->fused_cfl != 0;
   BatchGraph* victim = &b->graphs[0];
   for (BatchGraph& g : b->graphs) {
     if (g.exec && std::equal(key, key + 4, g.key) && g.spec[0] == L.spec[0] && g.spec[1] == L.spec[1] &&
-        g.fused == fused) {
+        g.fused == fused && g.nsteps == nsteps) {
       g.last_use = ++b->gclock;
       *out = &g;
       return MHD_OK;
@@ -396,8 +397,9 @@ int group_graph(mhd_batch* b, BLoop& L, BatchGraph** out) {
   if (victim->exec) (void)hipGraphExecDestroy(victim->exec);
   victim->exec = nullptr;
   BHIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-  int rc = enqueue_group_step(b, L, 0);
-  if (!rc) rc = enqueue_group_step(b, L, 0);     // two swaps: the B pointers are back where they started
+  int rc = MHD_OK;
+  for (int k = 0; k < nsteps && !rc; ++k)       // an even number of swaps: the B pointers end where they started
+    rc = enqueue_group_step(b, L, 0);
   hipGraph_t g = nullptr;
   const hipError_t e = hipStreamEndCapture(b->stream, &g);
   if (rc) {
@@ -414,6 +416,7 @@ int group_graph(mhd_batch* b, BLoop& L, BatchGraph** out) {
   std::copy(key, key + 4, victim->key);
   victim->spec[0] = L.spec[0];
   victim->spec[1] = L.spec[1];
+  victim->nsteps = nsteps;
   victim->fused = fused;
   victim->last_use = ++b->gclock;
   *out = victim;
@@ -496,10 +499,15 @@ int run_groups(mhd_batch* b, int max_batch_steps, int* steps_done) {
     long long G = b->group;
     if (max_batch_steps >= 0) G = std::min<long long>(G, max_batch_steps - done);
     long long k = 0;
-    if (b->graph) {
+    if (b->graph == 2 && G >= 2) {         // the group's even part as one graph launch
+      BatchGraph* g = nullptr;
+      if ((rc = group_graph(b, L, (int)(G & ~1LL), &g))) return rc;
+      BHIP(b, hipGraphLaunch(g->exec, b->stream));
+      k = G & ~1LL;
+    } else if (b->graph == 1) {
       for (; k + 2 <= G; k += 2) {
         BatchGraph* g = nullptr;
-        if ((rc = group_graph(b, L, &g))) return rc;
+        if ((rc = group_graph(b, L, 2, &g))) return rc;
         BHIP(b, hipGraphLaunch(g->exec, b->stream));
       }
     }
@@ -837,9 +845,10 @@ int mhd_batch_step(mhd_batch* b, int* n_active) {
 
 int mhd_batch_tune(mhd_batch* b, int group_steps, int spec_margin, int graph, int fused_cfl) {
   if (!b) return MHD_EINVAL;
-  if (group_steps < 0 || group_steps > 4096 || spec_margin < 0 || (graph != 0 && graph != 1) ||
+  if (group_steps < 0 || group_steps > 4096 || spec_margin < 0 || graph < 0 || graph > 2 ||
       (fused_cfl != 0 && fused_cfl != 1))
-    return bfail(b, MHD_EINVAL, "mhd_batch_tune: group_steps in [0, 4096], spec_margin >= 0, graph and fused_cfl 0 or 1");
+    return bfail(b, MHD_EINVAL, "mhd_batch_tune: group_steps in [0, 4096], spec_margin >= 0, graph 0, 1 or 2, "
+                                "fused_cfl 0 or 1");
   b->group = group_steps;
   b->margin = spec_margin;
   b->graph = graph;
@@ -959,11 +968,12 @@ int mhd_batch_autotune(mhd_batch* b, int trial_steps, int* stage_rows, int* tile
   }
   // stage fusion and graph replay of the device-resident groups, with the
   // best shape: the next step's CFL partials formed by the final
-  // recombination or by a separate pass, direct launches or captured pairs
+  // recombination or by a separate pass; direct launches, captured pairs or
+  // each group as one captured graph
   if (rc == MHD_OK && b->group > 0 && b->any_eta) {
     rc = replan(b, best_s, best_t);
     for (int f = 1; f >= 0 && rc == MHD_OK; --f)
-      for (int g = 0; g <= 1 && rc == MHD_OK; ++g) {
+      for (int g = 0; g <= 2 && rc == MHD_OK; ++g) {
         b->fused_cfl = f;
         b->graph = g;
         double total = 0.0;

@@ -23,12 +23,14 @@ struct BatchTableau {
 
 struct BatchAdjoint;   // mhd_batch_adjoint.hip
 
-// Two batch steps captured as a graph, valid for the B pointers, CG budgets
-// and CFL fusion it was captured with.
+// An even number of batch steps (a pair, or a whole group) captured as a
+// graph, valid for the step count, B pointers, CG budgets and CFL fusion it
+// was captured with.
 struct BatchGraph {
   hipGraphExec_t exec = nullptr;
   double* key[4] = {};
   int spec[2] = {0, 0};
+  int nsteps = 0;
   bool fused = true;
   unsigned long long last_use = 0;
 };
@@ -78,7 +80,7 @@ struct mhd_batch {
   // device-resident groups of steps (mhd_batch_run; mhd_batch_tune)
   int group = 16;                   // steps enqueued between two read-backs (0: one host-checked step at a time)
   int margin = 0;                   // extra launched CG iterations per solve
-  int graph = 0;                    // 1: replay pairs of steps as captured graphs
+  int graph = 0;                    // 1: replay pairs of steps as captured graphs; 2: each group as one graph
   int fused_cfl = 1;                // 1: the final recombination forms the next CFL partials; 0: a separate pass
   int* ctl = nullptr;               // device control word (kCtlHalt, kCtlSteps)
   int* ctl_host = nullptr;          // pinned

@@ -469,12 +469,13 @@ class Batch:
         self._check(self.lib.mhd_batch_autotune(self.h, trial_steps, C.byref(sr), C.byref(tr)))
         return sr.value, tr.value
 
-    def tune(self, group_steps: int = 16, spec_margin: int = 0, graph: bool = False, fused_cfl: bool = True):
+    def tune(self, group_steps: int = 16, spec_margin: int = 0, graph: int = 0, fused_cfl: bool = True):
         """Launch mode of run(): steps per device-resident group (0: one
         host-checked step at a time), extra launched CG iterations per solve,
-        graph replay of step pairs, CFL partials fused into the final
-        recombination.  Results are identical in every mode."""
-        self._check(self.lib.mhd_batch_tune(self.h, group_steps, spec_margin, int(bool(graph)), int(bool(fused_cfl))))
+        graph replay (0 / False: direct launches, 1 / True: captured pairs of
+        steps, 2: each group as one captured graph), CFL partials fused into
+        the final recombination.  Results are identical in every mode."""
+        self._check(self.lib.mhd_batch_tune(self.h, group_steps, spec_margin, int(graph), int(bool(fused_cfl))))
 
     def tuning(self) -> dict:
         """The current launch shape and mode."""

@@ -190,6 +190,8 @@ MODES = [
     dict(group_steps=3, fused_cfl=False),
     dict(group_steps=1),
     dict(group_steps=7, spec_margin=2, graph=True),
+    dict(group_steps=16, graph=2),
+    dict(group_steps=7, graph=2, fused_cfl=False),
 ]
 
 
@@ -244,7 +246,7 @@ def test_autotune_chooses_fusion_and_launch_mode():
         b.autotune(trial_steps=2)
         t = b.tuning()
         print(f"[case] chosen {t}", flush=True)
-        assert t["group_steps"] == 16 and t["graph"] in (0, 1) and t["fused_cfl"] in (0, 1)
+        assert t["group_steps"] == 16 and t["graph"] in (0, 1, 2) and t["fused_cfl"] in (0, 1)
         b.run()
         for r in range(len(states)):
             assert b.run_info(r)[0] == steps
