@@ -767,17 +767,26 @@ def main():
         ctx.profile(False)
         st0 = ctx.download()
         cmp_ms = {}
-        for name, fn in (("host_loop", ctx.run), ("resident_loop", ctx.run_resident)):
+        for name, fn, graph in (("host_loop", ctx.run, 0), ("resident_loop", ctx.run_resident, 0),
+                                ("resident_loop_one_graph_per_group", ctx.run_resident, 2)):
+            ctx.resident_tune(graph=graph)
             ctx.upload(*[st0[k] for k in ("rho", "mx", "my", "E", "bxf", "byf")])
+            if graph:
+                ctx.run_resident(1e9, max_steps=a.compare_steps)      # captures the group's graph
+                ctx.upload(*[st0[k] for k in ("rho", "mx", "my", "E", "bxf", "byf")])
             ctx.synchronize()
             tc = time.perf_counter()
             nc, _, _, _ = fn(1e9, max_steps=a.compare_steps)
             ctx.synchronize()
             cmp_ms[name] = (time.perf_counter() - tc) / nc * 1e3
+        ctx.resident_tune()
         ctx.upload(*[st0[k] for k in ("rho", "mx", "my", "E", "bxf", "byf")])
         compare = {"steps": a.compare_steps, "host_loop_ms_per_step": round(cmp_ms["host_loop"], 4),
                    "resident_loop_ms_per_step": round(cmp_ms["resident_loop"], 4),
-                   "note": "no events; the same steps from the same state (mhd_run vs mhd_run_resident)"}
+                   "resident_loop_one_graph_per_group_ms_per_step":
+                       round(cmp_ms["resident_loop_one_graph_per_group"], 4),
+                   "note": "no events; the same steps from the same state (mhd_run vs mhd_run_resident with "
+                           "direct launches and with each group of 16 steps replayed as one captured graph)"}
     # the same workload with outflow (zero-gradient) boundaries on both axes
     outflow = None
     if a.outflow_steps > 0:

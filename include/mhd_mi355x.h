@@ -194,7 +194,9 @@ MHD_API int mhd_run_resident_files(mhd_ctx* ctx, double t_end, double dt_overrid
  * iterations per solve, not adapted (a solve that needs more halts the loop
  * and the host finishes its step; -1 restores the adaptive budget), and
  * graph = 1: pairs of steps are captured once per configuration (B pointers,
- * budgets, dt mode) as a HIP graph and replayed (0: direct launches). */
+ * budgets, dt mode) as a HIP graph and replayed; graph = 2: the even part of
+ * each group is captured as one graph, one launch per group (0: direct
+ * launches). */
 MHD_API int mhd_resident_tune(mhd_ctx* ctx, int group_steps, int spec_margin, int fixed_budget, int graph);
 
 /* write_data_to_binary_file (include/diagnostics.hpp:90-131): the current

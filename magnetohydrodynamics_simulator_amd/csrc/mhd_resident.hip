@@ -55,6 +55,7 @@ struct StepGraph {
   hipGraphExec_t exec = nullptr;
   double* key[4] = {};         // U[4], U[5], S[4], S[5] at capture
   int spec[2] = {0, 0};
+  int nsteps = 0;              // steps captured (even)
   bool fixed_dt = false;
   unsigned long long last_use = 0;
 };
@@ -229,15 +230,15 @@ int enqueue_step(mhd_ctx* c, Loop& L, int from) {
                   (const CgState*)R->st, R->log, kLogCap, L.fixed_dt ? 0 : 1);
 }
 
-// The two-step graph for the context's current B pointers and budgets,
-// captured on first use (least recently used entry replaced).
-int step_graph(mhd_ctx* c, Loop& L, StepGraph** out) {
+// The graph of nsteps (even) steps for the context's current B pointers and
+// budgets, captured on first use (least recently used entry replaced).
+int step_graph(mhd_ctx* c, Loop& L, int nsteps, StepGraph** out) {
   Resident* R = L.R;
   double* key[4] = {c->U[4], c->U[5], c->S[4], c->S[5]};
   StepGraph* victim = &R->graphs[0];
   for (StepGraph& g : R->graphs) {
     if (g.exec && std::equal(key, key + 4, g.key) && g.spec[0] == L.spec[0] && g.spec[1] == L.spec[1] &&
-        g.fixed_dt == L.fixed_dt) {
+        g.fixed_dt == L.fixed_dt && g.nsteps == nsteps) {
       g.last_use = ++R->clock;
       *out = &g;
       return MHD_OK;
@@ -247,8 +248,9 @@ int step_graph(mhd_ctx* c, Loop& L, StepGraph** out) {
   if (victim->exec) (void)hipGraphExecDestroy(victim->exec);
   victim->exec = nullptr;
   HIP_OR_FAIL(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-  int rc = enqueue_step(c, L, 0);
-  if (!rc) rc = enqueue_step(c, L, 0);     // two swaps: the B pointers are back where they started
+  int rc = MHD_OK;
+  for (int k = 0; k < nsteps && !rc; ++k)  // an even number of swaps: the B pointers end where they started
+    rc = enqueue_step(c, L, 0);
   hipGraph_t g = nullptr;
   const hipError_t e = hipStreamEndCapture(c->stream, &g);
   if (rc) {
@@ -265,6 +267,7 @@ int step_graph(mhd_ctx* c, Loop& L, StepGraph** out) {
   std::copy(key, key + 4, victim->key);
   victim->spec[0] = L.spec[0];
   victim->spec[1] = L.spec[1];
+  victim->nsteps = nsteps;
   victim->fixed_dt = L.fixed_dt;
   victim->last_use = ++R->clock;
   *out = victim;
@@ -332,8 +335,8 @@ extern "C" {
 
 int mhd_resident_tune(mhd_ctx* c, int group_steps, int spec_margin, int fixed_budget, int graph) {
   if (!c) return MHD_EINVAL;
-  if (group_steps < 1 || group_steps > kLogCap || spec_margin < 0 || (graph != 0 && graph != 1))
-    return fail(c, MHD_EINVAL, "mhd_resident_tune: group_steps in [1, 256], spec_margin >= 0, graph 0 or 1");
+  if (group_steps < 1 || group_steps > kLogCap || spec_margin < 0 || graph < 0 || graph > 2)
+    return fail(c, MHD_EINVAL, "mhd_resident_tune: group_steps in [1, 256], spec_margin >= 0, graph 0, 1 or 2");
   int rc = ensure_resident(c);
   if (rc) return rc;
   c->res->group = group_steps;
@@ -410,10 +413,15 @@ int mhd_run_resident(mhd_ctx* c, double t_end, double dt_override, int max_steps
     if (max_steps >= 0) G = std::min<long long>(G, max_steps - done);
     if (snapshot_every > 0) G = std::min<long long>(G, snapshot_every - done % snapshot_every);
     long long k = 0;
-    if (R->graph && c->prof_mask == 0) {   // pairs of steps as graphs (no timing events inside a capture)
+    if (R->graph == 2 && G >= 2 && c->prof_mask == 0) {   // the group's even part as one graph launch
+      StepGraph* sg = nullptr;
+      if ((rc = step_graph(c, L, (int)(G & ~1LL), &sg))) return rc;
+      HIP_OR_FAIL(c, hipGraphLaunch(sg->exec, c->stream));
+      k = G & ~1LL;
+    } else if (R->graph == 1 && c->prof_mask == 0) {      // pairs of steps (no timing events inside a capture)
       for (; k + 2 <= G; k += 2) {
         StepGraph* sg = nullptr;
-        if ((rc = step_graph(c, L, &sg))) return rc;
+        if ((rc = step_graph(c, L, 2, &sg))) return rc;
         HIP_OR_FAIL(c, hipGraphLaunch(sg->exec, c->stream));
       }
     }

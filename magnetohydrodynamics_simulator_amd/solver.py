@@ -249,9 +249,10 @@ class Context:
         return n, dts[:n].copy(), its[: 2 * n].reshape(n, 2).copy(), t.value
 
     def resident_tune(self, group_steps: int = 16, spec_margin: int = 0, fixed_budget: int = -1,
-                      graph: bool = False):
+                      graph: int = 0):
         """Group size, CG iteration margin, an optional fixed iteration budget
-        and graph replay of run_resident (mhd_resident_tune)."""
+        and graph replay of run_resident (mhd_resident_tune; graph 1 / True:
+        captured pairs of steps, 2: each group as one captured graph)."""
         self._check(self.lib.mhd_resident_tune(self.h, group_steps, spec_margin, fixed_budget, int(graph)))
 
     # ---- adjoint ----

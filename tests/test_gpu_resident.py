@@ -56,9 +56,9 @@ def _same(a, b):
 
 
 @pytest.mark.parametrize("tune", [None, (16, 1, 0), (3, 1, 1), (1, 0, -1), (5, 0, 0), (16, 0, -1, True),
-                                  (5, 0, 0, True), (4, 1, 1, True)],
+                                  (5, 0, 0, True), (4, 1, 1, True), (16, 0, -1, 2), (5, 0, 0, 2)],
                          ids=["adaptive", "fixed0-g16", "fixed1-g3", "margin0-g1", "fixed0-g5", "graph-g16",
-                              "graph-fixed0-g5", "graph-fixed1-g4"])
+                              "graph-fixed0-g5", "graph-fixed1-g4", "groupgraph-g16", "groupgraph-fixed0-g5"])
 def test_resident_matches_run(tune):
     n, eta = 96, 1e-2
     _say(f"resident vs mhd_run, OT {n}^2 eta={eta}, 25 steps, tune={tune}")
@@ -159,7 +159,7 @@ def test_resident_snapshot_cadence(tmp_path):
         assert list(head[:4]) == [n, n, 2, every]
 
 
-@pytest.mark.parametrize("case", ["periodic", "outflow", "graph"])
+@pytest.mark.parametrize("case", ["periodic", "outflow", "graph", "groupgraph"])
 def test_async_snapshot_files(tmp_path, case):
     """mhd_run_resident_files writes, while the loop runs on, byte for byte
     the files a mhd_run_resident callback writes with mhd_write_snapshot at
@@ -177,13 +177,13 @@ def test_async_snapshot_files(tmp_path, case):
         ctx.write_snapshot(os.fsdecode(sync_dir / f"snapshot_{frame:06d}.dat"), every, dt)
 
     with make_gpu(ic, nx, ny, dx, dy, eta, cfl=0.4, tol=1e-10, bc=bc) as ctx:
-        if case == "graph":
-            ctx.resident_tune(4, 0, -1, True)
+        if case in ("graph", "groupgraph"):
+            ctx.resident_tune(4, 0, -1, 2 if case == "groupgraph" else 1)
         ref = ctx.run_resident(10.0, max_steps=steps, snapshot_every=every, on_snapshot=on_snap)
         ref_state = ctx.download()
     with make_gpu(ic, nx, ny, dx, dy, eta, cfl=0.4, tol=1e-10, bc=bc) as ctx:
-        if case == "graph":
-            ctx.resident_tune(4, 0, -1, True)
+        if case in ("graph", "groupgraph"):
+            ctx.resident_tune(4, 0, -1, 2 if case == "groupgraph" else 1)
         got = ctx.run_resident_files(10.0, every, os.fsdecode(async_dir / "snapshot_%06d.dat"), max_steps=steps)
         got_state = ctx.download()
     assert got[0] == ref[0] == steps
