@@ -847,31 +847,49 @@ struct Fin2Chain {
   double bxs, bys;    // B3* faces of the last row computed
 };
 
-// J3(col, row) given chain.bx3_below = bx3(col, row - 1); advances the chain.
+// The loads of J3(col, row): x at (col, row), (col, row + 1), (col + 1, row),
+// (col - 1, row) and the B3* faces bx(col, row), by(col, row), by(col - 1, row).
+struct Fin2Loads {
+  double xc, xn, xe, xw, bxs, bys, bysw;
+};
+
 template <bool OPEN>
-__device__ __forceinline__ double fin2_j3(const FinArgs& a, bool xv, int col, int row, Fin2Chain& ch) {
+__device__ __forceinline__ Fin2Loads fin2_load(const FinArgs& a, bool xv, int col, int row) {
   const int nx = a.nx, ny = a.ny;
   const int cm = bc_wrap(col - 1, nx, (OPEN ? a.open_x : 0));
   const int cp = bc_wrap(col + 1, nx, (OPEN ? a.open_x : 0));
   const int rp = bc_wrap(row + 1, ny, (OPEN ? a.open_y : 0));
-  const double xc = xv ? a.x[row * nx + col] : 0.0;
-  const double xn = xv ? a.x[rp * nx + col] : 0.0;
-  const double xe = xv ? a.x[row * nx + cp] : 0.0;
-  const double xw = xv ? a.x[row * nx + cm] : 0.0;
-  const double bxs = a.bx[row * nx + col];
-  const double bys = a.by[row * nx + col];
-  const double bysw = a.by[row * nx + cm];
-  const double bx3 = bxs + (-(xn - xc) * a.inv_dy) * a.c;       // bx3(col, row)
-  const double by3 = bys + ((xe - xc) * a.inv_dx) * a.c;        // by3(col, row)
+  Fin2Loads v;
+  v.xc = xv ? a.x[row * nx + col] : 0.0;
+  v.xn = xv ? a.x[rp * nx + col] : 0.0;
+  v.xe = xv ? a.x[row * nx + cp] : 0.0;
+  v.xw = xv ? a.x[row * nx + cm] : 0.0;
+  v.bxs = a.bx[row * nx + col];
+  v.bys = a.by[row * nx + col];
+  v.bysw = a.by[row * nx + cm];
+  return v;
+}
+
+// J3(col, row) from its loads, given chain.bx3_below = bx3(col, row - 1);
+// advances the chain.
+template <bool OPEN>
+__device__ __forceinline__ double fin2_j3_of(const FinArgs& a, int col, const Fin2Loads& v, Fin2Chain& ch) {
+  const double bx3 = v.bxs + (-(v.xn - v.xc) * a.inv_dy) * a.c;     // bx3(col, row)
+  const double by3 = v.bys + ((v.xe - v.xc) * a.inv_dx) * a.c;      // by3(col, row)
   // by3(col - 1, row); left of the first column of an outflow axis B3 is
   // extended by its boundary value (the ghost rule on the recombined field)
-  const double by3w = ((OPEN ? a.open_x : 0) && col == 0) ? by3 : bysw + ((xc - xw) * a.inv_dx) * a.c;
+  const double by3w = ((OPEN ? a.open_x : 0) && col == 0) ? by3 : v.bysw + ((v.xc - v.xw) * a.inv_dx) * a.c;
   const double dBy_dx = (by3 - by3w) * a.inv_dx;
   const double dBx_dy = (bx3 - ch.bx3_below) * a.inv_dy;
   ch.bx3_below = bx3;
-  ch.bxs = bxs;
-  ch.bys = bys;
+  ch.bxs = v.bxs;
+  ch.bys = v.bys;
   return a.eta * (dBy_dx - dBx_dy);
+}
+
+template <bool OPEN>
+__device__ __forceinline__ double fin2_j3(const FinArgs& a, bool xv, int col, int row, Fin2Chain& ch) {
+  return fin2_j3_of<OPEN>(a, col, fin2_load<OPEN>(a, xv, col, row), ch);
 }
 
 template <bool OPEN>
@@ -945,6 +963,13 @@ __device__ __forceinline__ void finalize2_body(const FinArgs& a, int vb, const D
   double bxo_prev = 0.0, byo_prev = 0.0;
   double u_prev[4] = {0.0, 0.0, 0.0, 0.0};     // rho, mx, my, E of row j - 1 (loaded one row ahead)
   const int y_end = CFL ? t.y1 + 1 : t.y1;
+  // row j wrapped (the extra row of a top tile; the last row on an outflow axis)
+  auto wrapped = [&](int j) -> int { return (j >= ny) ? ((OPEN ? a.open_y : 0) ? ny - 1 : j - ny) : j; };
+  auto above = [&](int j) -> int { return bc_wrap(wrapped(j) + 1, ny, (OPEN ? a.open_y : 0)); };
+  // the loads of J3 one row ahead of their use
+  Fin2Loads nxt = fin2_load<OPEN>(a, xv, t.ic, above(t.y0));
+  Fin2Loads hnxt{};
+  if (edge) hnxt = fin2_load<OPEN>(a, xv, xr, above(t.y0));
   for (int j = t.y0; j < y_end; ++j) {
     double u_row[4] = {0.0, 0.0, 0.0, 0.0};
     if (CFL && j < t.y1) {
@@ -952,19 +977,22 @@ __device__ __forceinline__ void finalize2_body(const FinArgs& a, int vb, const D
 #pragma unroll
       for (int q = 0; q < 4; ++q) u_row[q] = d.s[q][o];
     }
+    const Fin2Loads cur = nxt, hcur = hnxt;
+    if (j + 1 < y_end) {
+      nxt = fin2_load<OPEN>(a, xv, t.ic, above(j + 1));
+      if (edge) hnxt = fin2_load<OPEN>(a, xv, xr, above(j + 1));
+    }
     const int buf = (j - t.y0) & 1;
-    // row j wrapped (the extra row of a top tile; the last row on an outflow axis)
-    const int jw = (j >= ny) ? ((OPEN ? a.open_y : 0) ? ny - 1 : j - ny) : j;
-    const int jp = bc_wrap(jw + 1, ny, (OPEN ? a.open_y : 0));
+    const int jw = wrapped(j);
     // J3 above the last corner row of an outflow axis is the last row's
     const bool top = (OPEN ? a.open_y : 0) && jw == ny - 1;
     const double bxs = own.bxs, bys = own.bys;   // B3* faces of row j
     const double hbxs = hal.bxs;                 // B3* bx of row j, halo column (edge thread)
-    const double jn = fin2_j3<OPEN>(a, xv, t.ic, jp, own);
+    const double jn = fin2_j3_of<OPEN>(a, t.ic, cur, own);
     if (t.valid) jrow[buf ^ 1][tid + 1] = jn;
     double jhn = 0.0;
     if (edge) {
-      jhn = fin2_j3<OPEN>(a, xv, xr, jp, hal);
+      jhn = fin2_j3_of<OPEN>(a, xr, hcur, hal);
       jrow[buf ^ 1][w + 1] = jhn;
     }
     const double jr = jrow[buf][tid + 2];        // J3(i + 1, j)
