@@ -195,8 +195,8 @@ MHD_API int mhd_run_resident_files(mhd_ctx* ctx, double t_end, double dt_overrid
  * and the host finishes its step; -1 restores the adaptive budget), and
  * graph = 1: pairs of steps are captured once per configuration (B pointers,
  * budgets, dt mode) as a HIP graph and replayed; graph = 2: the even part of
- * each group is captured as one graph, one launch per group (0: direct
- * launches). */
+ * each group is captured as one graph (at most 64 steps per graph), one
+ * launch per group (0: direct launches). */
 MHD_API int mhd_resident_tune(mhd_ctx* ctx, int group_steps, int spec_margin, int fixed_budget, int graph);
 
 /* write_data_to_binary_file (include/diagnostics.hpp:90-131): the current
@@ -378,7 +378,8 @@ MHD_API int mhd_batch_run(mhd_batch* b, int max_batch_steps, int* steps_done);
  * time; 1..4096: steps per group), spec_margin (extra launched CG iterations
  * per solve), graph (0: direct launches; 1: pairs of steps captured once as a
  * hipGraph and replayed; 2: the even part of each group captured as one
- * hipGraph and launched once per group; graphs are kept per step count,
+ * hipGraph (at most 64 steps per graph) and launched once per group; graphs
+ * are kept per step count,
  * B-buffer parity, budgets and fusion), fused_cfl (1: the final
  * recombination also forms the next step's CFL partials; 0: a separate CFL
  * pass at the start of each step).  Results are identical in every mode. */

@@ -381,6 +381,8 @@ int enqueue_group_step(mhd_batch* b, BLoop& L, int from) {
 // The graph of nsteps (even) batch steps for the current B pointers,
 // budgets and CFL fusion, captured on first use (least recently used entry
 // replaced).
+constexpr int kMaxGraphSteps = 64;   // steps per captured graph (graph = 2)
+
 int group_graph(mhd_batch* b, BLoop& L, int nsteps, BatchGraph** out) {
   double* key[4] = {b->U[4], b->U[5], b->S[4], b->S[5]};
   const bool fused = b->fused_cfl != 0;
@@ -499,11 +501,14 @@ int run_groups(mhd_batch* b, int max_batch_steps, int* steps_done) {
     long long G = b->group;
     if (max_batch_steps >= 0) G = std::min<long long>(G, max_batch_steps - done);
     long long k = 0;
-    if (b->graph == 2 && G >= 2) {         // the group's even part as one graph launch
-      BatchGraph* g = nullptr;
-      if ((rc = group_graph(b, L, (int)(G & ~1LL), &g))) return rc;
-      BHIP(b, hipGraphLaunch(g->exec, b->stream));
-      k = G & ~1LL;
+    if (b->graph == 2) {                   // the group's even part as one graph launch (chunks of at most 64 steps)
+      while (k + 2 <= G) {
+        const int n = (int)std::min<long long>((G - k) & ~1LL, kMaxGraphSteps);
+        BatchGraph* g = nullptr;
+        if ((rc = group_graph(b, L, n, &g))) return rc;
+        BHIP(b, hipGraphLaunch(g->exec, b->stream));
+        k += n;
+      }
     } else if (b->graph == 1) {
       for (; k + 2 <= G; k += 2) {
         BatchGraph* g = nullptr;

@@ -232,6 +232,8 @@ int enqueue_step(mhd_ctx* c, Loop& L, int from) {
 
 // The graph of nsteps (even) steps for the context's current B pointers and
 // budgets, captured on first use (least recently used entry replaced).
+constexpr int kMaxGraphSteps = 64;   // steps per captured graph (graph = 2)
+
 int step_graph(mhd_ctx* c, Loop& L, int nsteps, StepGraph** out) {
   Resident* R = L.R;
   double* key[4] = {c->U[4], c->U[5], c->S[4], c->S[5]};
@@ -413,11 +415,14 @@ int mhd_run_resident(mhd_ctx* c, double t_end, double dt_override, int max_steps
     if (max_steps >= 0) G = std::min<long long>(G, max_steps - done);
     if (snapshot_every > 0) G = std::min<long long>(G, snapshot_every - done % snapshot_every);
     long long k = 0;
-    if (R->graph == 2 && G >= 2 && c->prof_mask == 0) {   // the group's even part as one graph launch
-      StepGraph* sg = nullptr;
-      if ((rc = step_graph(c, L, (int)(G & ~1LL), &sg))) return rc;
-      HIP_OR_FAIL(c, hipGraphLaunch(sg->exec, c->stream));
-      k = G & ~1LL;
+    if (R->graph == 2 && c->prof_mask == 0) {   // the group's even part as one graph launch (<= 64 steps each)
+      while (k + 2 <= G) {
+        const int n = (int)std::min<long long>((G - k) & ~1LL, kMaxGraphSteps);
+        StepGraph* sg = nullptr;
+        if ((rc = step_graph(c, L, n, &sg))) return rc;
+        HIP_OR_FAIL(c, hipGraphLaunch(sg->exec, c->stream));
+        k += n;
+      }
     } else if (R->graph == 1 && c->prof_mask == 0) {      // pairs of steps (no timing events inside a capture)
       for (; k + 2 <= G; k += 2) {
         StepGraph* sg = nullptr;

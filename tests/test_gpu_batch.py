@@ -192,6 +192,7 @@ MODES = [
     dict(group_steps=7, spec_margin=2, graph=True),
     dict(group_steps=16, graph=2),
     dict(group_steps=7, graph=2, fused_cfl=False),
+    dict(group_steps=200, graph=2),
 ]
 
 

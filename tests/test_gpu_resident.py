@@ -95,6 +95,18 @@ def test_long_solves(tune):
         sim.close()
 
 
+def test_resident_group_graph_chunks():
+    """A group longer than one captured graph (graph = 2 captures at most 64
+    steps): a 140-step group runs as graphs of 64, 64 and 12 steps, bitwise
+    mhd_run."""
+    n, eta, steps = 48, 1e-2, 140
+    _say(f"resident group of {steps} steps as chunked graphs, OT {n}^2 eta={eta}")
+    ic, dx, dy = ic_arrays("orszag_tang", n, n)
+    ref = _plain(ic, n, n, dx, dy, eta, 10.0, steps, tol=1e-10, cfl=0.4)
+    got = _resident(ic, n, n, dx, dy, eta, 10.0, steps, tune=(200, 1, -1, 2), tol=1e-10, cfl=0.4)
+    _same(got, ref)
+
+
 def test_resident_t_end_clamp_and_eta0():
     n = 64
     ic, dx, dy = ic_arrays("orszag_tang", n, n)
