@@ -73,6 +73,16 @@ typedef struct mhd_opts {
 #define MHD_BC_PERIODIC 0
 #define MHD_BC_OUTFLOW 1
 
+/* The reference's problem setups (include/benchmarks.hpp), for
+ * mhd_init_setup / mhd_batch_init_setup. */
+#define MHD_SETUP_ORSZAG_TANG 0    /* orszag_tang_initial_conditions, benchmarks.hpp:15-76 */
+#define MHD_SETUP_DECAYING_SINE 1  /* decaying_magnetic_sine_wave_initial_conditions, :352-417 */
+#define MHD_SETUP_ALFVEN 2         /* alfven_wave_test_initial_values, :78-126 */
+#define MHD_SETUP_GAUSSIAN 3       /* gaussian_density_pulse_test_initial_values, :128-176 */
+#define MHD_SETUP_ACOUSTIC 4       /* acoustic_wave_test_initial_values, :178-231 */
+#define MHD_SETUP_SAUSAGE 5        /* sausage_instability_initial_conditions, :233-293 */
+#define MHD_SETUP_KINK 6           /* kink_instability_initial_conditions, :295-350 */
+
 /* Per-step statistics (the reference computes none of these; CG failure is
  * silent there, solvers.hpp:469). */
 typedef struct mhd_step_stats {
@@ -210,6 +220,15 @@ MHD_API int mhd_write_snapshot(mhd_ctx* ctx, const char* path, int snapshot_rate
  * (declared with the batch entry points) into a context's state. */
 MHD_API int mhd_init_fourier(mhd_ctx* ctx, int nmodes, const double* modes);
 MHD_API int mhd_init_column(mhd_ctx* ctx, int sausage, int nmodes, const double* amps, const double* phases);
+/* Replaces the reference's setup call test_selection(global_state, ...)
+ * (src/integrator_main.cpp:50; the switch over the seven setups of
+ * include/benchmarks.hpp:419-451, which fill the padded Grid on the host): the setup
+ * evaluated per interior cell on the device, at the context's nx, ny, dx, dy
+ * (centres (i + 1/2) dx, corners i dx, with the reference's pi =
+ * 3.14159265359), straight into its state.  The fields agree with the
+ * reference's host loops to a few ulps (the same operations in the same order;
+ * the device's sin, cos, exp).  setup: MHD_SETUP_*. */
+MHD_API int mhd_init_setup(mhd_ctx* ctx, int setup);
 /* Exact-restart files (no reference counterpart: the reference cannot
  * resume): "MHDRST01", 7 doubles (nx, ny, ng, dx, dy, eta, t) and the six
  * padded conserved arrays.  Reading checks the grid and uploads the state;
@@ -358,6 +377,9 @@ MHD_API int mhd_batch_download(mhd_batch* b, int run, double* rho, double* mx, d
 MHD_API int mhd_batch_init_fourier(mhd_batch* b, int run, int nmodes, const double* modes);
 MHD_API int mhd_batch_init_column(mhd_batch* b, int run, int sausage, int nmodes, const double* amps,
                                   const double* phases);
+/* mhd_init_setup into run `run` of a batch (at the run's nx, ny, dx, dy; the
+ * run restarts as after an upload). */
+MHD_API int mhd_batch_init_setup(mhd_batch* b, int run, int setup);
 /* One driver step of every run that has not reached its end time or step
  * limit; *n_active (may be NULL) receives the number of runs that stepped. */
 MHD_API int mhd_batch_step(mhd_batch* b, int* n_active);

@@ -16,6 +16,7 @@
 //   CFL_condition_checker(g)                            mhd_mi355x::CFL_condition_checker(g, ctx)
 //   calculate_current_div_B(g)                          mhd_mi355x::calculate_current_div_B(g, ctx)
 //   write_data_to_binary_file(g, frame, rate, dt)       mhd_mi355x::write_data_to_binary_file(g, ctx, frame, rate, dt)
+//   test_selection(g, mem) (benchmarks.hpp:419-451)     mhd_mi355x::test_selection(g, ctx, choice)
 //   main's time loop (integrator_main.cpp:52-112)      mhd_mi355x::integrate(g, ctx, t_end, dt_override, rate)
 //
 // The Grid-taking functions copy the state to the device, run, and copy it
@@ -113,6 +114,30 @@ template <class GridT, class Vec>
 void resistive_B_term(const GridT& g, Context& ctx, Vec& bx_rate, Vec& by_rate, bool solve_implicitly) {
   ctx.upload(g);
   check(mhd_resistive_term(ctx.get(), solve_implicitly ? 1 : 0, bx_rate.data(), by_rate.data(), nullptr), ctx.get());
+}
+
+// test_selection (benchmarks.hpp:419-451) with the menu choice as an argument
+// in place of the prompt on std::cin: 1 Orszag-Tang, 2 sausage, 3 kink,
+// 4 decaying sine, 5 Alfven wave, 6 Gaussian pulse, 7 acoustic wave; any
+// other value runs Orszag-Tang, as the reference's default does.  The setup
+// is generated on the device into ctx (the context's grid: g's nx, ny,
+// cell_width, cell_height) and downloaded into g.
+inline int setup_of_choice(int choice) {
+  switch (choice) {
+    case 2: return MHD_SETUP_SAUSAGE;
+    case 3: return MHD_SETUP_KINK;
+    case 4: return MHD_SETUP_DECAYING_SINE;
+    case 5: return MHD_SETUP_ALFVEN;
+    case 6: return MHD_SETUP_GAUSSIAN;
+    case 7: return MHD_SETUP_ACOUSTIC;
+    default: return MHD_SETUP_ORSZAG_TANG;
+  }
+}
+
+template <class GridT>
+void test_selection(GridT& g, Context& ctx, int choice) {
+  check(mhd_init_setup(ctx.get(), setup_of_choice(choice)), ctx.get());
+  ctx.download(g);
 }
 
 template <class GridT>

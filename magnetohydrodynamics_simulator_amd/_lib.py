@@ -62,6 +62,8 @@ SIGNATURES = {
     "mhd_init_column": (C.c_int, [_vp, C.c_int, C.c_int, _dp, _dp]),
     "mhd_batch_init_fourier": (C.c_int, [_vp, C.c_int, C.c_int, _dp]),
     "mhd_batch_init_column": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _dp, _dp]),
+    "mhd_init_setup": (C.c_int, [_vp, C.c_int]),
+    "mhd_batch_init_setup": (C.c_int, [_vp, C.c_int, C.c_int]),
     "mhd_profile_enable": (C.c_int, [_vp, C.c_int]),
     "mhd_profile_get": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_longlong), _dp]),
     "mhd_calibration_copy": (C.c_int, [_vp, C.c_longlong]),

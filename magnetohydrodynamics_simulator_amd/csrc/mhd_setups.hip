@@ -1,6 +1,7 @@
-// mhd_setups.hip -- initial states of the ensemble workloads generated on the
-// device (C ABI mhd_batch_init_fourier, mhd_batch_init_column,
-// mhd_init_fourier, mhd_init_column).
+// mhd_setups.hip -- initial states generated on the device: the reference's
+// problem setups and those of the ensemble workloads (C ABI
+// mhd_init_setup, mhd_batch_init_setup, mhd_batch_init_fourier,
+// mhd_batch_init_column, mhd_init_fourier, mhd_init_column).
 //
 // The reference's setups are host loops over a padded Grid
 // (include/benchmarks.hpp:15-417); the seeded setups of the ensemble
@@ -145,6 +146,100 @@ __global__ __launch_bounds__(256) void k_init_column(InitArgs a, const double* _
   }
 }
 
+// The reference's problem setups (include/benchmarks.hpp), as the package's
+// host restatement (setups.py) evaluates them on the reference's 2 pi domain:
+// cell centres X = (i + 0.5) dx, corners XC = i dx of the compact cell (i, j).
+constexpr double kPiRef = 3.14159265359;            // the reference's truncated pi (benchmarks.hpp)
+constexpr double kGamma = 5.0 / 3.0;
+
+// orszag_tang_initial_conditions, benchmarks.hpp:15-76
+__device__ __forceinline__ double ot_az(double xc, double yc) {
+#pragma clang fp contract(off)
+  return 0.5 * cos(2 * xc) + cos(yc);
+}
+
+__global__ __launch_bounds__(256) void k_init_reference(InitArgs a, int setup) {
+#pragma clang fp contract(off)
+  const long long n = (long long)a.nx * a.ny;
+  const double dx = a.dx, dy = a.dy;
+  const int nx = a.nx, ny = a.ny;
+  for (long long o = blockIdx.x * (long long)blockDim.x + threadIdx.x; o < n; o += (long long)gridDim.x * blockDim.x) {
+    const int i = (int)(o % nx), j = (int)(o / nx);
+    const double X = (i + 0.5) * dx, Y = (j + 0.5) * dy;
+    const double XC = i * dx, YC = j * dy, XE = (i + 1) * dx, YN = (j + 1) * dy;
+    double rho = 1.0, vx = 0.0, vy = 0.0, p = 1.0, bx = 0.0, by = 0.0;
+    switch (setup) {
+      case MHD_SETUP_ORSZAG_TANG: {
+        const double az = ot_az(XC, YC);
+        bx = (ot_az(XC, YN) - az) / dy;
+        by = -(ot_az(XE, YC) - az) / dx;
+        rho = 25.0 / 9.0;
+        vx = -sin(Y);
+        vy = sin(X);
+        p = 5.0 / 3.0;
+        break;
+      }
+      case MHD_SETUP_DECAYING_SINE: {   // decaying_magnetic_sine_wave_initial_conditions, benchmarks.hpp:352-417
+        const double kx = 2 * kPiRef / (nx * dx), ky = 2 * kPiRef / (ny * dy);
+        auto az = [&](double xc, double yc) -> double { return (1.0 / kx) * cos(kx * xc) * sin(ky * yc); };
+        const double a0 = az(XC, YC);
+        bx = (az(XC, YN) - a0) / dy;
+        by = -(az(XE, YC) - a0) / dx;
+        break;
+      }
+      case MHD_SETUP_ALFVEN: {          // alfven_wave_test_initial_values, benchmarks.hpp:78-126
+        const double kx = 2.0 * kPiRef / (nx * dx);
+        by = 1e-3 * sin(kx * X);
+        vy = -by / sqrt(1.0);
+        bx = 1.0;
+        break;
+      }
+      case MHD_SETUP_GAUSSIAN: {        // gaussian_density_pulse_test_initial_values, benchmarks.hpp:128-176
+        const double x_mid = 0.5 * nx * dx, y_mid = 0.5 * nx * dy;   // y_mid from nx, as in the reference
+        const double d2 = (X - x_mid) * (X - x_mid) + (Y - y_mid) * (Y - y_mid);
+        rho = 1.0 + exp(-5 * d2 / 4.0);
+        vx = vy = 1.0;
+        break;
+      }
+      case MHD_SETUP_ACOUSTIC: {        // acoustic_wave_test_initial_values, benchmarks.hpp:178-231
+        const double x = i * dx;       // the reference evaluates this setup at the cell's left corner
+        const double W = dx * nx;
+        const double p_bg = 1.0 / kGamma;
+        const double cs = sqrt(kGamma * p_bg / 1.0);
+        const double xs = 1e-4 * sin(4 * kPiRef * x / W);
+        rho = 1.0 + xs + 0.0;
+        vx = (cs / 1.0) * xs;
+        p = p_bg + (cs * cs) * (xs + 0.0);
+        break;
+      }
+      case MHD_SETUP_SAUSAGE: {         // sausage_instability_initial_conditions, benchmarks.hpp:233-293
+        const double k = 2.0 * kPiRef / (ny * dy);
+        const double x_mid = 0.5 * (nx + 4) * dx;   // from the padded width, as in the reference
+        const double r = X - x_mid;
+        const double pert = 1.0 - 0.15 * cos(k * Y);
+        rho = 1.0 * exp(-(r * r) / (0.04 * pert)) + 0.1;
+        rho = (rho < 1e-4) ? 1e-4 : rho;
+        p = 0.1 * exp(-(r * r) / (0.04 * pert)) + 0.1;
+        p = (p < 1e-4) ? 1e-4 : p;
+        by = 5.0 * (r / sqrt(0.04)) * exp(-(r * r) / 0.04);
+        break;
+      }
+      case MHD_SETUP_KINK: {            // kink_instability_initial_conditions, benchmarks.hpp:295-350
+        const double r = X - 0.5 * (nx + 4) * dx;
+        const double env = exp(-(r * r) / 0.04);
+        rho = env + 0.1;
+        vx = 0.05 * sin(2.0 * kPiRef * Y) * env;
+        p = env + 0.1;
+        by = env;
+        break;
+      }
+      default:
+        break;
+    }
+    store_state(a, o, rho, vx, vy, p, bx, by);
+  }
+}
+
 int init_blocks(long long n) { return (int)std::min<long long>((n + 255) / 256, 8192); }
 
 }  // namespace
@@ -265,6 +360,52 @@ int mhd_batch_init_column(mhd_batch* b, int run, int sausage, int nmodes, const 
   e = hipGetLastError();
   if (e != hipSuccess) return mhd_batch_host::bfail(b, MHD_EHIP, std::string("mhd_batch_init_column: ") + hipGetErrorString(e));
   return MHD_OK;
+}
+
+int mhd_batch_init_setup(mhd_batch* b, int run, int setup) {
+  if (!b) return MHD_EINVAL;
+  if (run < 0 || run >= b->nruns) return mhd_batch_host::bfail(b, MHD_EINVAL, "mhd_batch_init_setup: bad run index");
+  if (setup < MHD_SETUP_ORSZAG_TANG || setup > MHD_SETUP_KINK)
+    return mhd_batch_host::bfail(b, MHD_EINVAL, "mhd_batch_init_setup: unknown setup");
+  b->cfl_fresh = false;
+  const RunGeom& g = b->geom_h[run];
+  InitArgs a{};
+  for (int q = 0; q < 6; ++q) a.f[q] = b->U[q] + g.base;
+  a.nx = g.nx;
+  a.ny = g.ny;
+  a.dx = g.dx;
+  a.dy = g.dy;
+  const long long n = (long long)g.nx * g.ny;
+  hipError_t e = run_generator(b->stream, nullptr, 0, [&](void*) {
+    hipLaunchKernelGGL(k_init_reference, dim3(init_blocks(n)), dim3(256), 0, b->stream, a, setup);
+  });
+  if (e != hipSuccess) return mhd_batch_host::bfail(b, MHD_EHIP, std::string("mhd_batch_init_setup: ") + hipGetErrorString(e));
+  RunState z{};
+  e = hipMemcpy(b->rs + run, &z, sizeof(RunState), hipMemcpyHostToDevice);
+  if (e != hipSuccess) return mhd_batch_host::bfail(b, MHD_EHIP, std::string("mhd_batch_init_setup: ") + hipGetErrorString(e));
+  // the run's first CFL step reads the primitives of the new state
+  hipLaunchKernelGGL(k_raw_reset, dim3(init_blocks(n)), dim3(256), 0, b->stream, (const double*)b->U[0] + g.base,
+                     b->rho_raw + g.base, (int)n);
+  e = hipGetLastError();
+  if (e != hipSuccess) return mhd_batch_host::bfail(b, MHD_EHIP, std::string("mhd_batch_init_setup: ") + hipGetErrorString(e));
+  return MHD_OK;
+}
+
+int mhd_init_setup(mhd_ctx* c, int setup) {
+  if (!c) return MHD_EINVAL;
+  if (setup < MHD_SETUP_ORSZAG_TANG || setup > MHD_SETUP_KINK)
+    return mhd_host::fail(c, MHD_EINVAL, "mhd_init_setup: unknown setup");
+  InitArgs a{};
+  for (int q = 0; q < 6; ++q) a.f[q] = c->U[q];
+  a.nx = c->nx;
+  a.ny = c->ny;
+  a.dx = c->dx;
+  a.dy = c->dy;
+  hipError_t e = run_generator(c->stream, nullptr, 0, [&](void*) {
+    hipLaunchKernelGGL(k_init_reference, dim3(init_blocks(c->n)), dim3(256), 0, c->stream, a, setup);
+  });
+  if (e != hipSuccess) return mhd_host::fail(c, MHD_EHIP, std::string("mhd_init_setup: ") + hipGetErrorString(e));
+  return mhd_host::reset_raw_density(c);   // the first CFL step reads the primitives of the new state
 }
 
 int mhd_init_fourier(mhd_ctx* c, int nmodes, const double* modes) {

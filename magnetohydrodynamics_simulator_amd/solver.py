@@ -26,6 +26,18 @@ PI_REF = 3.14159265359                         # integrator_main.cpp:36
 
 _dp = C.POINTER(C.c_double)
 
+# The reference's setups by name (the keys of setups.SETUPS) and their
+# MHD_SETUP_* ids; benchmarks.hpp:419-451 selects the same seven.
+SETUP_IDS = {"orszag_tang": 0, "decaying_sine": 1, "alfven": 2, "gaussian": 3, "acoustic": 4, "sausage": 5,
+             "kink": 6}
+
+
+def _setup_id(name: str) -> int:
+    try:
+        return SETUP_IDS[name]
+    except KeyError:
+        raise ValueError(f"unknown setup {name!r}; one of {sorted(SETUP_IDS)}") from None
+
 
 def _ptr(a: np.ndarray | None):
     if a is None:
@@ -264,6 +276,12 @@ class Context:
         return keep, (_dp * 6)(*[_ptr(a) for a in keep])
 
     # ---- device-generated initial states ----
+    def init_setup(self, name: str):
+        """One of the reference's setups (include/benchmarks.hpp; names as in
+        setups.SETUPS) evaluated on the device at this context's grid
+        (mhd_init_setup)."""
+        self._check(self.lib.mhd_init_setup(self.h, _setup_id(name)))
+
     def init_fourier(self, modes: np.ndarray):
         """Fourier ensemble member generated on the device (mhd_init_fourier)."""
         t = np.ascontiguousarray(modes, dtype=np.float64).reshape(-1, 8)
@@ -432,6 +450,11 @@ class Batch:
             if a.shape != shape:
                 raise ValueError(f"run {run}: expected padded shape {shape}, got {a.shape}")
         self._check(self.lib.mhd_batch_upload(self.h, run, *[_ptr(a) for a in arrs]))
+
+    def init_setup(self, run: int, name: str):
+        """One of the reference's setups generated on the device at the run's
+        grid (mhd_batch_init_setup)."""
+        self._check(self.lib.mhd_batch_init_setup(self.h, run, _setup_id(name)))
 
     def init_fourier(self, run: int, modes: np.ndarray):
         """Device-generated Fourier ensemble member (mhd_batch_init_fourier);
