@@ -115,6 +115,8 @@ def parse():
     ap.add_argument("--adjoint-steps", type=int, default=50)
     ap.add_argument("--adjoint-modes", type=int, default=64)
     ap.add_argument("--fd-modes", type=int, default=64, help="modes checked against central differences")
+    ap.add_argument("--batch-no-graph", action="store_true",
+                    help="run the batched workload with direct launches only (for kernel traces)")
     ap.add_argument("--adjoint-no-graph", action="store_true",
                     help="run the adjoint backward without graph replay (for kernel traces)")
     ap.add_argument("--ba-runs", type=int, default=64, help="runs of the batched differentiation workload (0: skip)")
@@ -448,7 +450,18 @@ def bench_batch(a) -> dict:
     host_cells = int(sum(n * n for n in ns[:n_host]))
     for r in range(n_host):
         b.init_fourier(r, tables[r])
-    shape = b.autotune(trial_steps=2)           # launch shape, CFL fusion, graph replay (also warms up)
+    if a.batch_no_graph:
+        # no graph replay anywhere (the autotuner also tries it): the default
+        # launch shape, groups of 16 direct launches, CFL partials fused
+        b.tune(group_steps=16, graph=0, fused_cfl=True)
+        b.set_schedule(max_steps=2)
+        b.run()                                  # warm-up
+        for r in range(len(members)):
+            b.init_fourier(r, tables[r])
+        t_ = b.tuning()
+        shape = (t_["stage_rows"], t_["tile_rows"])
+    else:
+        shape = b.autotune(trial_steps=2)       # launch shape, CFL fusion, graph replay (also warms up)
     tuned = b.tuning()
     b.set_schedule(max_steps=a.batch_steps)
     b.synchronize()
@@ -465,6 +478,8 @@ def bench_batch(a) -> dict:
              "groups_graph_separate_cfl": dict(group_steps=16, graph=True, fused_cfl=False),
              "groups_one_graph_per_group_fused_cfl": dict(group_steps=16, graph=2, fused_cfl=True)}
     mode_steps = min(a.batch_steps, 50)
+    if a.batch_no_graph:
+        modes = {k: m for k, m in modes.items() if not m.get("graph")}
     mode_ms = {}
     for name, m in modes.items():
         for r in range(len(members)):

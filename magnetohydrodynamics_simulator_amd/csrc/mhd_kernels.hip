@@ -56,13 +56,17 @@ struct Carry {
 
 // One-lane shifts across the whole 64-lane wavefront as DPP moves (wave_shr:1,
 // wave_shl:1) on the two halves of the double: VALU moves instead of LDS
-// permutes.  The lane without a source (lane 0 up, lane 63 down) keeps its own
-// value, as __shfl_up / __shfl_down(v, 1, 64) do; those lanes are stencil halo.
+// permutes.  The lane without a source (lane 0 up, lane 63 down) receives 0
+// (bound_ctrl); those lanes are stencil halo, and what they compute from it
+// reaches only halo lanes (lanes 0-1 and 62-63 of a strip store nothing).
+// A plain DPP move has no previous destination value to preserve, so it needs
+// no copy of the source into the destination first (32 fewer VALU
+// instructions per row of the march; the stored results are bitwise the same).
 constexpr int kDppWaveShl1 = 0x130, kDppWaveShr1 = 0x138;
 template <int CTRL>
 __device__ __forceinline__ double wave_shift(double v) {
-  const int lo = __builtin_amdgcn_update_dpp(__double2loint(v), __double2loint(v), CTRL, 0xf, 0xf, false);
-  const int hi = __builtin_amdgcn_update_dpp(__double2hiint(v), __double2hiint(v), CTRL, 0xf, 0xf, false);
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, 0xf, 0xf, true);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, 0xf, 0xf, true);
   return __hiloint2double(hi, lo);
 }
 __device__ __forceinline__ double sh_up(double v) { return wave_shift<kDppWaveShr1>(v); }   // value of lane l-1
