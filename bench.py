@@ -65,6 +65,10 @@ KERNEL_SYMBOLS_RESIDENT = {"stage1": "k_stage_res<1>", "stage2": "k_stage_res<2>
                            "implicit_finalize1": "k_finalize1_res", "implicit_finalize2": "k_finalize2_res",
                            "cfl": "k_cfl"}
 CALIB_BYTES = 1 << 30          # the copy kernel of the counter calibration reads and writes 1 GiB
+# (arrays read, arrays written) of each kernel's algorithmic traffic, for the
+# streaming reference of the same mix (mhd_calibration_mix)
+KERNEL_MIX = {"stage1": (6, 12), "stage2": (12, 6), "cg_start": (2, 2), "cg_pass_a": (2, 1), "cg_pass_b": (3, 2),
+              "implicit_finalize1": (3, 2), "implicit_finalize2": (3, 2)}
 
 
 def kernel_bytes(k: str, N: int, launches: int, steps: int, solves: int, spmv: int, fused_cfl: bool = False) -> float:
@@ -841,6 +845,20 @@ def main():
     ncopy, ms_copy = ctx.profile_get("other")
     copy_gbs = 2 * (1 << 30) / (ms_copy / ncopy * 1e-3) / 1e9
     ctx.profile(False)
+    # the streaming rate of each kernel's read/write mix: a plain kernel that
+    # reads and writes the same number of N-double arrays once per element
+    mix_gbs = {}
+    for k, (ni, no) in KERNEL_MIX.items():
+        if k == "implicit_finalize2" and fused:
+            ni = 7                               # + rho, mx, my, E for the next step's CFL partials
+        key = f"{ni}in_{no}out"
+        if key not in mix_gbs:
+            ms_mix = ctx.calibration_mix(ni, no, N, reps=5)
+            mix_gbs[key] = round((ni + no) * N * 8 / (ms_mix * 1e-3) / 1e9, 1)
+        if k in kb and kb[k].get("GBs"):
+            kb[k]["stream_mix"] = key
+            kb[k]["stream_mix_GBs"] = mix_gbs[key]
+            kb[k]["frac_of_stream_mix"] = round(kb[k]["GBs"] / mix_gbs[key], 3)
     # the PCIe-inclusive rate of a Grid-taking call, IMEX(grid, ctx, dt) of the
     # C++ adapter: upload of the 6 conserved padded arrays, one step, download
     # of the 11 padded arrays (state and primitives) -- not the metric
@@ -946,6 +964,9 @@ def main():
                      "bytes_per_launch": int(bytes_per_launch), "ms_per_launch": round(dom_ms / dom_cnt, 4),
                      "launches": dom_cnt,
                      "working_launches": spmv if dom == "cg_pass_b" else None,
+                     "stream_mix_GBs": kb.get(dom, {}).get("stream_mix_GBs"),
+                     "frac_of_stream_mix": round(achieved / kb[dom]["stream_mix_GBs"], 3)
+                     if kb.get(dom, {}).get("stream_mix_GBs") else None,
                      "note": "achieved = algorithmic bytes / mean launch time; the CG passes are ordered to re-read "
                              "what the previous pass left in the 256 MB memory-side cache (MALL), which FETCH_SIZE "
                              "also counts, so part of these bytes come from it; step_roofline is the "
@@ -954,6 +975,10 @@ def main():
                           "frac": round(step_bytes / (elapsed / steps) / 1e9 / HBM_PEAK_GBS, 4)},
         "cg_spmv_per_step": round(spmv / steps, 2),
         "copy_kernel_GBs": round(copy_gbs, 1),
+        "stream_mix_GBs": dict(mix_gbs, note=f"plain kernels reading / writing N-double arrays ({n}x{n}) once per "
+                                             "element, 5 launches each (mhd_calibration_mix): the streaming rate "
+                                             "of each kernel's mix, against which kernels[k].frac_of_stream_mix "
+                                             "sets its algorithmic GB/s"),
         "divb_max": divmax,
         "kernels": kb,
         "kernels_note": "launches include the resident loop's launches that do no work (steps enqueued after a "

@@ -248,6 +248,13 @@ MHD_API int mhd_profile_get(mhd_ctx* ctx, int kernel_id, long long* launches, do
 /* Bytes-calibration helper for hardware counters: one plain device copy
  * kernel of nbytes (rounded down to doubles), run synchronously. */
 MHD_API int mhd_calibration_copy(mhd_ctx* ctx, long long nbytes);
+/* Streaming reference for a read/write mix (no reference counterpart): a
+ * plain kernel that reads n_in and writes n_out arrays of n doubles once per
+ * element, launched twice untimed and `reps` times between HIP events on the
+ * context's stream; *ms_per_launch receives the mean.  Mixes (n_in, n_out):
+ * (1,1) (2,1) (2,2) (3,2) (7,2) (6,12) (12,6) -- those of the step's kernels.
+ * Allocates and frees its (n_in + n_out) n doubles. */
+MHD_API int mhd_calibration_mix(mhd_ctx* ctx, int n_in, int n_out, long long n, int reps, double* ms_per_launch);
 MHD_API int mhd_synchronize(mhd_ctx* ctx);
 
 /* Allocation-free CFL time step.  No counterpart in the reference: it is for

@@ -67,6 +67,7 @@ SIGNATURES = {
     "mhd_profile_enable": (C.c_int, [_vp, C.c_int]),
     "mhd_profile_get": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_longlong), _dp]),
     "mhd_calibration_copy": (C.c_int, [_vp, C.c_longlong]),
+    "mhd_calibration_mix": (C.c_int, [_vp, C.c_int, C.c_int, C.c_longlong, C.c_int, _dp]),
     "mhd_synchronize": (C.c_int, [_vp]),
     "mhd_adjoint_forward": (C.c_int, [_vp, C.c_double, C.c_int, C.c_int]),
     "mhd_adjoint_backward": (C.c_int, [_vp, C.POINTER(_dp), C.POINTER(_dp), _dp]),

@@ -14,6 +14,7 @@
 // (A_im[3] = A_im[6] = b_im[0] = 0) and is not evaluated.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cfloat>
 #include <cmath>
 #include <cstdio>
@@ -852,6 +853,66 @@ int mhd_calibration_copy(mhd_ctx* c, long long nbytes) {
   HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
   (void)hipFree(src);
   (void)hipFree(dst);
+  return rc;
+}
+
+int mhd_calibration_mix(mhd_ctx* c, int n_in, int n_out, long long n, int reps, double* ms_per_launch) {
+  if (check_ctx(c) || n <= 0 || reps <= 0 || !ms_per_launch)
+    return fail(c, MHD_EINVAL, "mhd_calibration_mix: bad argument");
+  void (*k)(MixArgs, long long) = nullptr;
+  const int mix = n_in * 100 + n_out;
+  switch (mix) {
+    case 101: k = k_stream_mix<1, 1>; break;
+    case 201: k = k_stream_mix<2, 1>; break;
+    case 202: k = k_stream_mix<2, 2>; break;
+    case 302: k = k_stream_mix<3, 2>; break;
+    case 702: k = k_stream_mix<7, 2>; break;
+    case 612: k = k_stream_mix<6, 12>; break;
+    case 1206: k = k_stream_mix<12, 6>; break;
+    default:
+      return fail(c, MHD_EINVAL, "mhd_calibration_mix: mixes (in, out): (1,1) (2,1) (2,2) (3,2) (7,2) (6,12) (12,6)");
+  }
+  const int na = n_in + n_out;
+  std::vector<double*> arr(na, nullptr);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = MHD_OK;
+  auto release = [&] {
+    for (double* p : arr)
+      if (p) (void)hipFree(p);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+  };
+  for (int q = 0; q < na; ++q) {
+    if (hipMalloc(&arr[q], (size_t)n * sizeof(double)) != hipSuccess) {
+      arr[q] = nullptr;
+      release();
+      return fail(c, MHD_ENOMEM, "mhd_calibration_mix: allocation failed");
+    }
+  }
+  MixArgs a{};
+  for (int q = 0; q < n_in; ++q) a.in[q] = arr[q];
+  for (int q = 0; q < n_out; ++q) a.out[q] = arr[n_in + q];
+  hipError_t e = hipSuccess;
+  for (int q = 0; q < n_in && e == hipSuccess; ++q) e = hipMemsetAsync(arr[q], 0, (size_t)n * sizeof(double), c->stream);
+  if (e == hipSuccess) e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  const dim3 grid((unsigned)std::min<long long>((n + 255) / 256, 65536LL));
+  for (int w = 0; w < 2 && e == hipSuccess; ++w) {
+    hipLaunchKernelGGL(k, grid, dim3(256), 0, c->stream, a, n);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipEventRecord(e0, c->stream);
+  for (int r = 0; r < reps && e == hipSuccess; ++r) {
+    hipLaunchKernelGGL(k, grid, dim3(256), 0, c->stream, a, n);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipEventRecord(e1, c->stream);
+  if (e == hipSuccess) e = hipEventSynchronize(e1);
+  float ms = 0.0f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+  if (e != hipSuccess) rc = fail(c, MHD_EHIP, std::string("mhd_calibration_mix: ") + hipGetErrorString(e));
+  else *ms_per_launch = (double)ms / reps;
+  release();
   return rc;
 }
 

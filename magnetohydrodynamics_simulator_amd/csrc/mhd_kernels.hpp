@@ -272,5 +272,10 @@ struct CopyArgs {
 };
 __global__ void k_copy_fields(CopyArgs a, long long n);
 __global__ void k_copy(const double* src, double* dst, long long n);
+struct MixArgs {
+  const double* in[12];
+  double* out[12];
+};
+template <int NI, int NO> __global__ void k_stream_mix(MixArgs a, long long n);
 
 }  // namespace mhd

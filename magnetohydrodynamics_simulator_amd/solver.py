@@ -378,6 +378,13 @@ class Context:
     def calibration_copy(self, nbytes: int):
         self._check(self.lib.mhd_calibration_copy(self.h, nbytes))
 
+    def calibration_mix(self, n_in: int, n_out: int, n: int, reps: int = 5) -> float:
+        """Mean ms per launch of a plain kernel reading n_in and writing
+        n_out arrays of n doubles (mhd_calibration_mix)."""
+        ms = C.c_double()
+        self._check(self.lib.mhd_calibration_mix(self.h, n_in, n_out, n, reps, C.byref(ms)))
+        return ms.value
+
 
 class Batch:
     """Batched ensemble of independent periodic runs (mhd_batch_* of the C
