@@ -924,7 +924,8 @@ def main():
         small = {"workload": f"ideal Orszag-Tang {ns_}x{ns_} to t = 0.5, CFL {a.cfl}", "steps": nss, "t_end": tss,
                  "value": round(ns_ * ns_ * nss / ts_ / 1e6, 2), "unit": "Mcell-updates/s",
                  "ms_per_step": round(ts_ / nss * 1e3, 4), "divb_max": divs,
-                 "note": "whole run, resident loop; launch-bound at this size"}
+                 "note": "whole run, resident loop; too small to fill the device: one-row stage segments, "
+                         "each wave alone on its SIMD, bound by its dependency chain and the launches"}
     cpu_proc = None
     if not a.no_cpu:
         jobs = [{"name": "main", "n": n, "eta": a.eta, "tol": a.cg_tol, "cfl": a.cfl, "steps": a.cpu_steps,
