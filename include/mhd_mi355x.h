@@ -340,7 +340,7 @@ MHD_API int mhd_explicit_rhs_vjp_unfused(mhd_ctx* ctx, const double* const* lam,
 
 /* ------------------------------------------------------------------------
  * Batched ensembles (no reference counterpart: the reference advances one
- * Grid per process).  R independent periodic runs, each with its own size,
+ * Grid per process).  R independent runs, each with its own size,
  * cell size and resistivity, advance together: one launch per kernel covers
  * every run, and each run follows the reference driver loop
  * (src/integrator_main.cpp:76-112) on the device -- its own CFL time step
@@ -351,7 +351,9 @@ MHD_API int mhd_explicit_rhs_vjp_unfused(mhd_ctx* ctx, const double* const* lam,
 typedef struct mhd_batch mhd_batch;
 
 /* nx, ny, dx, dy, eta: arrays of nruns entries.  Every run starts at t = 0
- * with no end time and no step limit (see mhd_batch_set_schedule). */
+ * with no end time and no step limit (see mhd_batch_set_schedule).  The
+ * boundaries of opts (bc_x, bc_y: periodic or MHD_BC_OUTFLOW) apply to every
+ * run; the batched adjoint (mhd_batch_adjoint_*) takes periodic runs only. */
 MHD_API int mhd_batch_create(int nruns, const int* nx, const int* ny, const double* dx, const double* dy,
                      const double* eta, const mhd_opts* opts, mhd_batch** out);
 MHD_API int mhd_batch_destroy(mhd_batch* b);

@@ -129,18 +129,18 @@ namespace mhd_batch_host {
 int cg_batch(mhd_batch* b, const CgCall& c, int* most_spmv) {
   BatchCgArgs a = cg_args(b, c);
   const int nb = b->n_tile_jobs;
-  if (c.bin) BLAUNCH(b, kb_cg_start_rhs, dim3(nb), dim3(kTileX), 0, b->stream, a, c.bin);
-  else BLAUNCH(b, kb_cg_start, dim3(nb), dim3(kTileX), 0, b->stream, a);
-  BLAUNCH(b, kb_cg_pass_b<true>, dim3(nb), dim3(kTileX), 0, b->stream, a, (const double*)b->p[0], 0);
+  if (c.bin) BLAUNCH(b, kb_start_rhs(b), dim3(nb), dim3(kTileX), 0, b->stream, a, c.bin);
+  else BLAUNCH(b, kb_start(b), dim3(nb), dim3(kTileX), 0, b->stream, a);
+  BLAUNCH(b, kb_pass_b<true>(b), dim3(nb), dim3(kTileX), 0, b->stream, a, (const double*)b->p[0], 0);
   long long it = 1;
   if (c.spec > 0) {
     // A(1) B(1) ... A(spec) B(spec) A(spec + 1): the last A takes the decision
     for (; it <= c.spec + 1; ++it) {
       const double* pold = b->p[(it - 1) & 1];
       double* pnew = b->p[it & 1];
-      BLAUNCH(b, kb_cg_pass_a, dim3(nb), dim3(kTileX), 0, b->stream, a, pold, pnew, (int)it);
+      BLAUNCH(b, kb_pass_a(b), dim3(nb), dim3(kTileX), 0, b->stream, a, pold, pnew, (int)it);
       if (it == c.spec + 1) break;
-      BLAUNCH(b, kb_cg_pass_b<false>, dim3(nb), dim3(kTileX), 0, b->stream, a, (const double*)pnew, (int)it);
+      BLAUNCH(b, kb_pass_b<false>(b), dim3(nb), dim3(kTileX), 0, b->stream, a, (const double*)pnew, (int)it);
     }
     return MHD_OK;
   }
@@ -151,9 +151,9 @@ int cg_batch(mhd_batch* b, const CgCall& c, int* most_spmv) {
     for (int k = 0; k < chunk; ++k, ++it) {
       const double* pold = b->p[(it - 1) & 1];
       double* pnew = b->p[it & 1];
-      BLAUNCH(b, kb_cg_pass_a, dim3(nb), dim3(kTileX), 0, b->stream, a, pold, pnew, (int)it);
+      BLAUNCH(b, kb_pass_a(b), dim3(nb), dim3(kTileX), 0, b->stream, a, pold, pnew, (int)it);
       if (k + 1 == chunk) break;
-      BLAUNCH(b, kb_cg_pass_b<false>, dim3(nb), dim3(kTileX), 0, b->stream, a, (const double*)pnew, (int)it);
+      BLAUNCH(b, kb_pass_b<false>(b), dim3(nb), dim3(kTileX), 0, b->stream, a, (const double*)pnew, (int)it);
     }
     BHIP(b, hipMemcpyAsync(b->st_host, c.st, sizeof(CgState) * b->nruns, hipMemcpyDeviceToHost, b->stream));
     BHIP(b, hipStreamSynchronize(b->stream));
@@ -161,7 +161,7 @@ int cg_batch(mhd_batch* b, const CgCall& c, int* most_spmv) {
     for (int r = 0; r < b->nruns; ++r) all = all && b->st_host[r].done;
     if (all) break;
     const double* pv = b->p[it & 1];
-    BLAUNCH(b, kb_cg_pass_b<false>, dim3(nb), dim3(kTileX), 0, b->stream, a, pv, (int)it);
+    BLAUNCH(b, kb_pass_b<false>(b), dim3(nb), dim3(kTileX), 0, b->stream, a, pv, (int)it);
     ++it;
     if (it > maxit + 4) return bfail(b, MHD_EHIP, "CG state machine did not terminate");
   }
@@ -179,19 +179,19 @@ int cg_batch(mhd_batch* b, const CgCall& c, int* most_spmv) {
   return MHD_OK;
 }
 
-// Periodic ghosts of a padded (ny+4) x (nx+4) field (Grid::fillOneField,
-// Grid.hpp:143-161).
-void fill_ghosts_padded(double* f, int nx, int ny) {
+// Ghosts of a padded (ny+4) x (nx+4) field: periodic (Grid::fillOneField,
+// Grid.hpp:143-161) or, on an outflow axis, the nearest inside value.
+void fill_ghosts_padded(double* f, int nx, int ny, int open_x, int open_y) {
   const size_t nxt = (size_t)nx + 4;
   for (int j = 2; j < ny + 2; ++j)
     for (int k = 0; k < 2; ++k) {
-      f[j * nxt + k] = f[j * nxt + nx + k];
-      f[j * nxt + nx + 2 + k] = f[j * nxt + 2 + k];
+      f[j * nxt + k] = open_x ? f[j * nxt + 2] : f[j * nxt + nx + k];
+      f[j * nxt + nx + 2 + k] = open_x ? f[j * nxt + nx + 1] : f[j * nxt + 2 + k];
     }
   for (size_t i = 0; i < nxt; ++i)
     for (int k = 0; k < 2; ++k) {
-      f[k * nxt + i] = f[(ny + k) * nxt + i];
-      f[(ny + 2 + k) * nxt + i] = f[(2 + k) * nxt + i];
+      f[k * nxt + i] = open_y ? f[2 * nxt + i] : f[(ny + k) * nxt + i];
+      f[(ny + 2 + k) * nxt + i] = open_y ? f[(ny + 1) * nxt + i] : f[(2 + k) * nxt + i];
     }
 }
 
@@ -213,7 +213,7 @@ int stage_out(mhd_batch* b, int run, const double* pooled, double* host) {
   BHIP(b, hipMemcpyAsync(b->stage, pooled + g.base, nx * g.ny * sizeof(double), hipMemcpyDeviceToHost, b->stream));
   BHIP(b, hipStreamSynchronize(b->stream));
   for (int j = 0; j < g.ny; ++j) std::memcpy(host + (j + 2) * nxt + 2, b->stage + (size_t)j * nx, nx * sizeof(double));
-  fill_ghosts_padded(host, g.nx, g.ny);
+  fill_ghosts_padded(host, g.nx, g.ny, g.open & 1, (g.open >> 1) & 1);
   return MHD_OK;
 }
 
@@ -303,13 +303,13 @@ BatchCgArgs group_cg(mhd_batch* b, int slot) {
 int enqueue_group_cg(mhd_batch* b, int slot, int spec) {
   const BatchCgArgs a = group_cg(b, slot);
   const int nb = b->n_tile_jobs;
-  BLAUNCH(b, kb_cg_start, dim3(nb), dim3(kTileX), 0, b->stream, a);
-  BLAUNCH(b, kb_cg_pass_b<true>, dim3(nb), dim3(kTileX), 0, b->stream, a, (const double*)b->p[0], 0);
+  BLAUNCH(b, kb_start(b), dim3(nb), dim3(kTileX), 0, b->stream, a);
+  BLAUNCH(b, kb_pass_b<true>(b), dim3(nb), dim3(kTileX), 0, b->stream, a, (const double*)b->p[0], 0);
   for (int it = 1; it <= spec + 1; ++it) {
-    BLAUNCH(b, kb_cg_pass_a, dim3(nb), dim3(kTileX), 0, b->stream, a, (const double*)b->p[(it - 1) & 1], b->p[it & 1],
+    BLAUNCH(b, kb_pass_a(b), dim3(nb), dim3(kTileX), 0, b->stream, a, (const double*)b->p[(it - 1) & 1], b->p[it & 1],
             it);
     if (it == spec + 1) break;
-    BLAUNCH(b, kb_cg_pass_b<false>, dim3(nb), dim3(kTileX), 0, b->stream, a, (const double*)b->p[it & 1], it);
+    BLAUNCH(b, kb_pass_b<false>(b), dim3(nb), dim3(kTileX), 0, b->stream, a, (const double*)b->p[it & 1], it);
   }
   BLAUNCH(b, kb_check_solve, dim3(1), dim3(256), 0, b->stream, (const RunState*)b->rs, (const CgState*)b->st[slot],
           b->nruns, slot, b->o.cg_fail_is_error, b->ctl, b->iters);
@@ -370,8 +370,8 @@ int enqueue_group_step(mhd_batch* b, BLoop& L, int from) {
   if (b->any_eta && from <= 6) {
     BatchFinArgs f = fin_args(b, b->rs, b->x, b->st[1], b->U[4], b->U[5], b->S[4], b->S[5]);
     f.ctl = b->ctl;
-    if (fused) BLAUNCH(b, kb_finalize2_cfl, dim3(nt), dim3(kTileX), 0, b->stream, f, d);
-    else BLAUNCH(b, kb_finalize2, dim3(nt), dim3(kTileX), 0, b->stream, f);
+    if (fused) BLAUNCH(b, kb_fin2_cfl(b), dim3(nt), dim3(kTileX), 0, b->stream, f, d);
+    else BLAUNCH(b, kb_fin2(b), dim3(nt), dim3(kTileX), 0, b->stream, f);
     std::swap(b->U[4], b->S[4]);
     std::swap(b->U[5], b->S[5]);
   }
@@ -454,9 +454,9 @@ int resume_group_solve(mhd_batch* b, BLoop& L, int slot) {
   for (const RunGeom& g : b->geom_h) maxit = std::max(maxit, g.max_iters);
   long long it = L.spec[slot] + 1;
   for (;;) {
-    BLAUNCH(b, kb_cg_pass_b<false>, dim3(nb), dim3(kTileX), 0, b->stream, a, (const double*)b->p[it & 1], (int)it);
+    BLAUNCH(b, kb_pass_b<false>(b), dim3(nb), dim3(kTileX), 0, b->stream, a, (const double*)b->p[it & 1], (int)it);
     ++it;
-    BLAUNCH(b, kb_cg_pass_a, dim3(nb), dim3(kTileX), 0, b->stream, a, (const double*)b->p[(it - 1) & 1], b->p[it & 1],
+    BLAUNCH(b, kb_pass_a(b), dim3(nb), dim3(kTileX), 0, b->stream, a, (const double*)b->p[(it - 1) & 1], b->p[it & 1],
             (int)it);
     BHIP(b, hipMemcpyAsync(b->st_host, b->st[slot], sizeof(CgState) * b->nruns, hipMemcpyDeviceToHost, b->stream));
     BHIP(b, hipStreamSynchronize(b->stream));
@@ -583,8 +583,9 @@ int mhd_batch_create(int nruns, const int* nx, const int* ny, const double* dx, 
   if (opts) o = *opts;
   if (o.gamma != 5.0 / 3.0) return bfail(nullptr, MHD_EINVAL, "only gamma = 5/3 is compiled in");
   if (!(o.cfl > 0) || !(o.cg_tol >= 0) || o.march_rows < 0) return bfail(nullptr, MHD_EINVAL, "bad options");
-  if (o.bc_x != MHD_BC_PERIODIC || o.bc_y != MHD_BC_PERIODIC)
-    return bfail(nullptr, MHD_EINVAL, "batched runs are periodic (bc_x = bc_y = MHD_BC_PERIODIC)");
+  const bool bc_ok = [](int v) { return v == MHD_BC_PERIODIC || v == MHD_BC_OUTFLOW; }(o.bc_x) &&
+                     [](int v) { return v == MHD_BC_PERIODIC || v == MHD_BC_OUTFLOW; }(o.bc_y);
+  if (!bc_ok) return bfail(nullptr, MHD_EINVAL, "bc_x and bc_y must be MHD_BC_PERIODIC or MHD_BC_OUTFLOW");
   long long total = 0;
   for (int r = 0; r < nruns; ++r) {
     if (nx[r] < 4 || ny[r] < 4) return bfail(nullptr, MHD_EINVAL, "run " + std::to_string(r) + ": nx, ny must be >= 4");
@@ -617,6 +618,7 @@ int mhd_batch_create(int nruns, const int* nx, const int* ny, const double* dx, 
     g.dt_fixed = 0.0;
     g.max_steps = -1;
     g.max_iters = (o.cg_max_iter > 0) ? o.cg_max_iter : 2LL * nx[r] * ny[r];
+    g.open = (o.bc_x == MHD_BC_OUTFLOW ? 1 : 0) | (o.bc_y == MHD_BC_OUTFLOW ? 2 : 0);
     b->any_eta = b->any_eta || eta[r] > 0.0;
   }
   std::vector<int2> sj, tj;
@@ -831,7 +833,7 @@ int mhd_batch_step(mhd_batch* b, int* n_active) {
     // Runs with eta = 0 have no implicit step: their B is copied by the
     // finalizer as well (x = 0 there, so g3 = 0 and c g3 = 0).
     BatchFinArgs f = fin_args(b, b->rs, b->x, b->st[1], b->U[4], b->U[5], b->S[4], b->S[5]);
-    BLAUNCH(b, kb_finalize2_cfl, dim3(nt), dim3(kTileX), 0, b->stream, f, d);
+    BLAUNCH(b, kb_fin2_cfl(b), dim3(nt), dim3(kTileX), 0, b->stream, f, d);
     std::swap(b->U[4], b->S[4]);
     std::swap(b->U[5], b->S[5]);
   }

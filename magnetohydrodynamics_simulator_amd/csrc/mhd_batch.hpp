@@ -134,7 +134,18 @@ int cg_batch(mhd_batch* b, const CgCall& call, int* most_spmv = nullptr);
 mhd::BatchFinArgs fin_args(mhd_batch* b, const mhd::RunState* rs, const double* x, const mhd::CgState* st,
                            const double* bx, const double* by, double* bxo, double* byo);
 
-void fill_ghosts_padded(double* f, int nx, int ny);
+void fill_ghosts_padded(double* f, int nx, int ny, int open_x, int open_y);
+
+// The batch's boundary mode (all runs share it) selects the kernels built for
+// it: periodic, or with an outflow axis (the `OPEN` variants).
+inline bool batch_open(const mhd_batch* b) { return b->o.bc_x == MHD_BC_OUTFLOW || b->o.bc_y == MHD_BC_OUTFLOW; }
+inline auto kb_start(const mhd_batch* b) { return batch_open(b) ? mhd::kb_cg_start<true> : mhd::kb_cg_start<false>; }
+inline auto kb_start_rhs(const mhd_batch* b) { return batch_open(b) ? mhd::kb_cg_start_rhs<true> : mhd::kb_cg_start_rhs<false>; }
+inline auto kb_pass_a(const mhd_batch* b) { return batch_open(b) ? mhd::kb_cg_pass_a<true> : mhd::kb_cg_pass_a<false>; }
+template <bool FIRST>
+inline auto kb_pass_b(const mhd_batch* b) { return batch_open(b) ? mhd::kb_cg_pass_b<FIRST, true> : mhd::kb_cg_pass_b<FIRST, false>; }
+inline auto kb_fin2(const mhd_batch* b) { return batch_open(b) ? mhd::kb_finalize2<true> : mhd::kb_finalize2<false>; }
+inline auto kb_fin2_cfl(const mhd_batch* b) { return batch_open(b) ? mhd::kb_finalize2_cfl<true> : mhd::kb_finalize2_cfl<false>; }
 int stage_in(mhd_batch* b, int run, const double* host, double* pooled);
 int stage_out(mhd_batch* b, int run, const double* pooled, double* host);
 

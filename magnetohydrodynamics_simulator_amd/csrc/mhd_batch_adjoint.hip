@@ -558,7 +558,7 @@ int record_step(mhd_batch* b, int s, int k, bool given) {
     if (int rc = cg_batch(b, cc, &most)) return rc;
     A.fwd_spmv[2 * s + 1] = most;
   }
-  BLAUNCH(b, kb_finalize2, dim3(nt), dim3(kTileX), 0, b->stream,
+  BLAUNCH(b, kb_finalize2<false>, dim3(nt), dim3(kTileX), 0, b->stream,
           fin_args(b, rs, x2, st1, A.U3sB[k][0], A.U3sB[k][1], out[4], out[5]));
   if (!given) {
     BHIP(b, hipStreamSynchronize(b->stream));
@@ -714,6 +714,7 @@ extern "C" {
 
 int mhd_batch_adjoint_forward(mhd_batch* b, int nsteps, int checkpoint_every) {
   if (!b) return MHD_EINVAL;
+  if (batch_open(b)) return bfail(b, MHD_EINVAL, "mhd_batch_adjoint_forward: the batched adjoint supports periodic runs only");
   b->cfl_fresh = false;
   if (nsteps < 1 || checkpoint_every < 1)
     return bfail(b, MHD_EINVAL, "mhd_batch_adjoint_forward: need nsteps >= 1 and checkpoint_every >= 1");

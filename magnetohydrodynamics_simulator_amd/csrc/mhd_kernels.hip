@@ -1267,7 +1267,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void kb
   if (!rs.active) return;
   const RunGeom& g = b.geom[job.x];
   StageArgs a;
-  a.open_x = a.open_y = 0;     // batched runs are periodic
+  a.open_x = g.open & 1;
+  a.open_y = (g.open >> 1) & 1;
 #pragma unroll
   for (int q = 0; q < 6; ++q) {
     a.s[q] = b.s[q] + g.base;
@@ -1296,7 +1297,8 @@ __device__ __forceinline__ CgArgs batch_cg_args(const BatchCgArgs& b, int run) {
   const RunGeom& g = b.geom[run];
   const RunState& rs = b.rs[run];
   CgArgs a;
-  a.open_x = a.open_y = 0;
+  a.open_x = g.open & 1;
+  a.open_y = (g.open >> 1) & 1;
   a.nx = g.nx;
   a.ny = g.ny;
   a.ty = g.ty;
@@ -1323,23 +1325,30 @@ __device__ __forceinline__ CgArgs batch_cg_args(const BatchCgArgs& b, int run) {
   return a;
 }
 
+template <bool OPEN>
 __global__ __launch_bounds__(kTileX) void kb_cg_start(BatchCgArgs b) {
   if (b.ctl && b.ctl[kCtlHalt]) return;
   const int2 job = b.jobs[xcd_block(blockIdx.x, gridDim.x)];
   if (b.st[job.x].done) return;            // stopped run, or eta = 0: no solve
   const CgArgs a = batch_cg_args(b, job.x);
-  cg_start_body<true, false>(a, nullptr, job.y);
+  cg_start_body<true, OPEN>(a, nullptr, job.y);
 }
+template __global__ void kb_cg_start<false>(BatchCgArgs);
+template __global__ void kb_cg_start<true>(BatchCgArgs);
 
 // Start of a solve with the given right-hand side bin (pooled).
+template <bool OPEN>
 __global__ __launch_bounds__(kTileX) void kb_cg_start_rhs(BatchCgArgs b, const double* __restrict__ bin) {
   if (b.ctl && b.ctl[kCtlHalt]) return;
   const int2 job = b.jobs[xcd_block(blockIdx.x, gridDim.x)];
   if (b.st[job.x].done) return;
   const CgArgs a = batch_cg_args(b, job.x);
-  cg_start_body<false, false>(a, bin + b.geom[job.x].base, job.y);
+  cg_start_body<false, OPEN>(a, bin + b.geom[job.x].base, job.y);
 }
+template __global__ void kb_cg_start_rhs<false>(BatchCgArgs, const double*);
+template __global__ void kb_cg_start_rhs<true>(BatchCgArgs, const double*);
 
+template <bool OPEN>
 __global__ __launch_bounds__(kTileX) void kb_cg_pass_a(BatchCgArgs b, const double* __restrict__ pold,
                                                        double* __restrict__ pnew, int it) {
   if (b.ctl && b.ctl[kCtlHalt]) return;
@@ -1347,24 +1356,29 @@ __global__ __launch_bounds__(kTileX) void kb_cg_pass_a(BatchCgArgs b, const doub
   if (b.st[job.x].done) return;
   const CgArgs a = batch_cg_args(b, job.x);
   const long long base = b.geom[job.x].base;
-  cg_pass_a_body<false>(a, pold + base, pnew + base, it, job.y, b.geom[job.x].tiles);
+  cg_pass_a_body<OPEN>(a, pold + base, pnew + base, it, job.y, b.geom[job.x].tiles);
 }
+template __global__ void kb_cg_pass_a<false>(BatchCgArgs, const double*, double*, int);
+template __global__ void kb_cg_pass_a<true>(BatchCgArgs, const double*, double*, int);
 
-template <bool FIRST>
+template <bool FIRST, bool OPEN>
 __global__ __launch_bounds__(kTileX) void kb_cg_pass_b(BatchCgArgs b, const double* __restrict__ pv, int it) {
   if (b.ctl && b.ctl[kCtlHalt]) return;
   const int2 job = b.jobs[xcd_block(blockIdx.x, gridDim.x)];
   if (b.st[job.x].done) return;
   const CgArgs a = batch_cg_args(b, job.x);
-  cg_pass_b_body<FIRST, false>(a, pv + b.geom[job.x].base, it, job.y, b.geom[job.x].tiles);
+  cg_pass_b_body<FIRST, OPEN>(a, pv + b.geom[job.x].base, it, job.y, b.geom[job.x].tiles);
 }
-template __global__ void kb_cg_pass_b<true>(BatchCgArgs, const double*, int);
-template __global__ void kb_cg_pass_b<false>(BatchCgArgs, const double*, int);
+template __global__ void kb_cg_pass_b<true, false>(BatchCgArgs, const double*, int);
+template __global__ void kb_cg_pass_b<false, false>(BatchCgArgs, const double*, int);
+template __global__ void kb_cg_pass_b<true, true>(BatchCgArgs, const double*, int);
+template __global__ void kb_cg_pass_b<false, true>(BatchCgArgs, const double*, int);
 
 __device__ __forceinline__ FinArgs batch_fin_args(const BatchFinArgs& b, int run) {
   const RunGeom& g = b.geom[run];
   FinArgs f;
-  f.open_x = f.open_y = 0;
+  f.open_x = g.open & 1;
+  f.open_y = (g.open >> 1) & 1;
   f.nx = g.nx;
   f.ny = g.ny;
   f.ty = g.ty;
@@ -1393,12 +1407,13 @@ __global__ __launch_bounds__(kTileX) void kb_finalize1(BatchFinArgs b) {
 
 // The host swaps the B arrays after this kernel for all runs, so a run that
 // no longer steps copies its B across unchanged.
+template <bool OPEN>
 __global__ __launch_bounds__(kTileX) void kb_finalize2(BatchFinArgs b) {
   if (b.ctl && b.ctl[kCtlHalt]) return;
   const int2 job = b.jobs[xcd_block(blockIdx.x, gridDim.x)];
   const FinArgs f = batch_fin_args(b, job.x);
   if (b.rs[job.x].active) {
-    finalize2_body<false, false>(f, job.y, DiagArgs{});
+    finalize2_body<false, OPEN>(f, job.y, DiagArgs{});
     return;
   }
   const Tile t = tile_of(f.nx, f.ny, f.ty, f.tiles_y, job.y);
@@ -1409,6 +1424,8 @@ __global__ __launch_bounds__(kTileX) void kb_finalize2(BatchFinArgs b) {
     f.by_out[o] = f.by[o];
   }
 }
+template __global__ void kb_finalize2<false>(BatchFinArgs);
+template __global__ void kb_finalize2<true>(BatchFinArgs);
 
 // Per-tile CFL partials of every run that has not reached its end yet (the
 // driver state still holds the previous step's activity; kb_dt decides).
@@ -1419,7 +1436,8 @@ __global__ __launch_bounds__(kTileX) void kb_cfl(BatchDiagArgs b) {
   const RunState& rs = b.rs[job.x];
   if (!(rs.t < g.t_end) || (g.max_steps >= 0 && rs.steps >= g.max_steps)) return;
   DiagArgs d;
-  d.open_x = d.open_y = 0;
+  d.open_x = g.open & 1;
+  d.open_y = (g.open >> 1) & 1;
   d.nx = g.nx;
   d.ny = g.ny;
   d.ty = g.ty;
@@ -1438,7 +1456,8 @@ __global__ __launch_bounds__(kTileX) void kb_cfl(BatchDiagArgs b) {
 __device__ __forceinline__ DiagArgs batch_diag_args(const BatchDiagArgs& b, int run) {
   const RunGeom& g = b.geom[run];
   DiagArgs d;
-  d.open_x = d.open_y = 0;
+  d.open_x = g.open & 1;
+  d.open_y = (g.open >> 1) & 1;
   d.nx = g.nx;
   d.ny = g.ny;
   d.ty = g.ty;
@@ -1456,12 +1475,13 @@ __device__ __forceinline__ DiagArgs batch_diag_args(const BatchDiagArgs& b, int 
 // run from the new state (rho, mx, my, E of U3*, the new B in registers), so
 // the next step needs no kb_cfl.  Runs that did not step copy their B and
 // keep no partials (a run that did not step this time does not step next).
+template <bool OPEN>
 __global__ __launch_bounds__(kTileX) void kb_finalize2_cfl(BatchFinArgs b, BatchDiagArgs dg) {
   if (b.ctl && b.ctl[kCtlHalt]) return;
   const int2 job = b.jobs[xcd_block(blockIdx.x, gridDim.x)];
   const FinArgs f = batch_fin_args(b, job.x);
   if (b.rs[job.x].active) {
-    finalize2_body<true, false>(f, job.y, batch_diag_args(dg, job.x));
+    finalize2_body<true, OPEN>(f, job.y, batch_diag_args(dg, job.x));
     return;
   }
   const Tile t = tile_of(f.nx, f.ny, f.ty, f.tiles_y, job.y);
@@ -1472,6 +1492,8 @@ __global__ __launch_bounds__(kTileX) void kb_finalize2_cfl(BatchFinArgs b, Batch
     f.by_out[o] = f.by[o];
   }
 }
+template __global__ void kb_finalize2_cfl<false>(BatchFinArgs, BatchDiagArgs);
+template __global__ void kb_finalize2_cfl<true>(BatchFinArgs, BatchDiagArgs);
 
 // Driver step of every run (src/integrator_main.cpp:76-100 of the reference,
 // as mhd_run does it for one run): activity, CFL time step with the Courant

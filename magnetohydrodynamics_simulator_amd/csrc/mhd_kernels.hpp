@@ -110,7 +110,7 @@ struct RunGeom {
   double dt_fixed;         // > 0: constant time step (the driver's override)
   long long max_iters;     // CG iteration limit
   int max_steps;           // < 0: no step limit
-  int pad_;
+  int open;                // outflow axes: bit 0 x, bit 1 y (the batch's mhd_opts.bc_x / bc_y)
 };
 
 // Driver state of one run, device resident; kb_dt writes it once per step.
@@ -209,14 +209,16 @@ constexpr int kCtlHalt = 0;
 constexpr int kCtlSteps = 1;
 
 template <int MODE> __global__ void kb_stage(BatchStageArgs b);
-__global__ void kb_cg_start(BatchCgArgs b);
-__global__ void kb_cg_start_rhs(BatchCgArgs b, const double* bin);
-__global__ void kb_cg_pass_a(BatchCgArgs b, const double* pold, double* pnew, int it);
-template <bool FIRST> __global__ void kb_cg_pass_b(BatchCgArgs b, const double* pv, int it);
+// OPEN: the batch has an outflow axis (RunGeom.open); the kernels are built
+// per boundary mode like the single-run ones.
+template <bool OPEN> __global__ void kb_cg_start(BatchCgArgs b);
+template <bool OPEN> __global__ void kb_cg_start_rhs(BatchCgArgs b, const double* bin);
+template <bool OPEN> __global__ void kb_cg_pass_a(BatchCgArgs b, const double* pold, double* pnew, int it);
+template <bool FIRST, bool OPEN> __global__ void kb_cg_pass_b(BatchCgArgs b, const double* pv, int it);
 __global__ void kb_finalize1(BatchFinArgs b);
-__global__ void kb_finalize2(BatchFinArgs b);
+template <bool OPEN> __global__ void kb_finalize2(BatchFinArgs b);
 __global__ void kb_cfl(BatchDiagArgs b);
-__global__ void kb_finalize2_cfl(BatchFinArgs b, BatchDiagArgs d);
+template <bool OPEN> __global__ void kb_finalize2_cfl(BatchFinArgs b, BatchDiagArgs d);
 __global__ void kb_dt(const RunGeom* geom, RunState* rs, const double* part, CgState* st0, CgState* st1,
                       int nruns, StepFactors f, int halt_on_nonfinite, int* ctl);
 // After a batch's solve `slot` (0, 1): halts the group (ctl[kCtlHalt]) if a

@@ -394,7 +394,10 @@ class Batch:
     FIELDS = ("rho", "mx", "my", "E", "bxf", "byf")
 
     def __init__(self, nx, ny, dx, dy, eta, *, cfl: float = 0.45, cg_tol: float | None = None,
-                 cg_max_iter: int = 0, cg_fail_is_error: bool = False, march_rows: int = 0):
+                 cg_max_iter: int = 0, cg_fail_is_error: bool = False, march_rows: int = 0,
+                 bc: tuple = ("periodic", "periodic")):
+        """bc: boundary of the x and y axes of every run, "periodic" or
+        "outflow" (the batched adjoint takes periodic runs only)."""
         self.lib = _lib.load()
         self.nx = np.ascontiguousarray(nx, dtype=np.int32)
         self.ny = np.ascontiguousarray(ny, dtype=np.int32)
@@ -413,6 +416,11 @@ class Batch:
         o.cg_max_iter = cg_max_iter
         o.cg_fail_is_error = int(cg_fail_is_error)
         o.march_rows = march_rows
+        try:
+            o.bc_x, o.bc_y = (_lib.BC_NAMES[b] for b in bc)
+        except (KeyError, ValueError, TypeError):
+            raise ValueError(f"bc must be a pair of 'periodic' / 'outflow', got {bc!r}") from None
+        self.bc = tuple(bc)
         h = C.c_void_p()
         ip = C.POINTER(C.c_int)
         _lib.check_batch(self.lib.mhd_batch_create(self.nruns, self.nx.ctypes.data_as(ip), self.ny.ctypes.data_as(ip),

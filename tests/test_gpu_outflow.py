@@ -135,7 +135,7 @@ def test_resident_loop_equals_run_with_outflow():
 
 def test_download_ghosts_and_periodic_only_paths():
     nx, ny = 20, 16
-    _say("outflow ghosts of a download; periodic-only paths refuse outflow")
+    _say("outflow ghosts of a download; the periodic-only adjoint refuses outflow; boundary codes checked")
     ic, dx, dy = ic_arrays("orszag_tang", nx, ny, seed=1, amp=1e-2)
     with Context(nx, ny, dx, dy, 1e-3, bc=OUT) as c:
         c.upload(*[ic[k] for k in CONSERVED])
@@ -150,11 +150,14 @@ def test_download_ghosts_and_periodic_only_paths():
     lib = _lib.load()
     o = _lib.Opts()
     lib.mhd_default_opts(C.byref(o))
-    o.bc_x = _lib.BC_OUTFLOW
     n = (C.c_int * 1)(32)
     h = (C.c_double * 1)(0.1)
     e = (C.c_double * 1)(1e-3)
     b = C.c_void_p()
+    o.bc_x = 7                                   # neither periodic nor outflow
     assert lib.mhd_batch_create(1, n, n, h, h, e, C.byref(o), C.byref(b)) == _lib.MHD_EINVAL
+    o.bc_x = _lib.BC_OUTFLOW                     # batches take outflow axes (tests/test_gpu_batch_outflow.py)
+    assert lib.mhd_batch_create(1, n, n, h, h, e, C.byref(o), C.byref(b)) == _lib.MHD_OK
+    assert lib.mhd_batch_destroy(b) == _lib.MHD_OK
     with pytest.raises(ValueError):
         Context(nx, ny, dx, dy, bc=("outflow", "reflecting"))
