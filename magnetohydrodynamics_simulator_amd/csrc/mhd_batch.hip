@@ -500,6 +500,20 @@ int run_groups(mhd_batch* b, int max_batch_steps, int* steps_done) {
     if (max_batch_steps >= 0 && done >= max_batch_steps) break;
     long long G = b->group;
     if (max_batch_steps >= 0) G = std::min<long long>(G, max_batch_steps - done);
+    if (b->graph != 2) {
+      // no more steps than the longest remaining step limit when every active
+      // run has one (later steps would be launches that return at once); a
+      // whole-group graph keeps its size, so that its capture is reused
+      long long most = 0;
+      bool limited = true;
+      for (int r = 0; r < nr && limited; ++r) {
+        if (!b->rs_host[r].active) continue;
+        const int ms = b->geom_h[r].max_steps;
+        limited = ms >= 0;
+        most = std::max<long long>(most, (long long)ms - b->rs_host[r].steps);
+      }
+      if (limited) G = std::min(G, std::max(most, 1LL));
+    }
     long long k = 0;
     if (b->graph == 2) {                   // the group's even part as one graph launch (chunks of at most 64 steps)
       while (k + 2 <= G) {
@@ -914,6 +928,7 @@ int mhd_batch_autotune(mhd_batch* b, int trial_steps, int* stage_rows, int* tile
     (void)hipMemcpyAsync(saved[q], b->U[q], fb, hipMemcpyDeviceToDevice, b->stream);
   (void)hipMemcpyAsync(rs_saved, b->rs, sizeof(RunState) * b->nruns, hipMemcpyDeviceToDevice, b->stream);
   const int spmv_saved[3] = {b->last_spmv[0], b->last_spmv[1], b->last_spmv[2]};
+  int spmv_seen[3] = {spmv_saved[0], spmv_saved[1], spmv_saved[2]};
   const std::vector<int> its_saved = b->cg_iters_last;
   const long long steps_saved = b->steps_taken;
   std::vector<RunGeom> geom_saved = b->geom_h;
@@ -953,7 +968,10 @@ int mhd_batch_autotune(mhd_batch* b, int trial_steps, int* stage_rows, int* tile
     for (int q = 0; q < 6; ++q)
       (void)hipMemcpyAsync(b->U[q], saved[q], fb, hipMemcpyDeviceToDevice, b->stream);
     (void)hipMemcpyAsync(b->rs, rs_saved, sizeof(RunState) * b->nruns, hipMemcpyDeviceToDevice, b->stream);
-    for (int k = 0; k < 3; ++k) b->last_spmv[k] = spmv_saved[k];
+    for (int k = 0; k < 3; ++k) {
+      spmv_seen[k] = std::max(spmv_seen[k], b->last_spmv[k]);
+      b->last_spmv[k] = spmv_saved[k];
+    }
     b->cfl_fresh = false;
     return r;
   };
@@ -996,6 +1014,10 @@ int mhd_batch_autotune(mhd_batch* b, int trial_steps, int* stage_rows, int* tile
   b->graph = (rc == MHD_OK) ? best_g : graph_saved;
   b->fused_cfl = (rc == MHD_OK) ? best_f : fused_saved;
   b->geom_h = geom_saved;
+  // the trials' longest solves seed the device-resident groups' CG budgets (a
+  // budget only sets how many iterations a group launches before the device
+  // checks; one too short halts the group for a host-checked resume)
+  for (int k = 0; k < 3; ++k) b->last_spmv[k] = spmv_seen[k];
   b->cg_iters_last = its_saved;
   b->steps_taken = steps_saved;
   int rc2 = replan(b, best_s, best_t);
