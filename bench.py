@@ -131,8 +131,8 @@ def parse():
                     help="smallest runs of the batched differentiation workload checked against single-run contexts")
     ap.add_argument("--ba-plain-runs", type=int, default=16,
                     help="runs of the batched differentiation workload timed through single-run contexts")
-    ap.add_argument("--only", choices=("batch", "adjoint", "badjoint"), default=None,
-                    help="run only the batched, the differentiation or the batched differentiation leg")
+    ap.add_argument("--only", choices=("batch", "adjoint", "badjoint", "power"), default=None,
+                    help="run only the batched, the differentiation, the batched differentiation or the power leg")
     ap.add_argument("--driver", choices=("resident", "host"), default="resident",
                     help="time loop of the main workload: mhd_run_resident (per-step decisions on the device) "
                          "or mhd_run (host reads the CFL value and the CG state each step)")
@@ -140,6 +140,8 @@ def parse():
                     help="steps of the un-instrumented host-loop / resident-loop comparison (0: skip)")
     ap.add_argument("--outflow-steps", type=int, default=20,
                     help="steps of the same workload with outflow boundaries on both axes (0: skip)")
+    ap.add_argument("--power-seconds", type=float, default=4.0,
+                    help="seconds of each load in the power and clock record (0: skip)")
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--cpu-child", default=None, help=argparse.SUPPRESS)
     return ap.parse_args()
@@ -419,6 +421,78 @@ def save_state(ctx, path: str) -> str:
     st = ctx.download()
     np.savez(path, **{k: st[k] for k in NAMES})
     return path
+
+
+def smi_sample(stop, out: list) -> None:
+    """Socket power (W, `amd-smi metric --power`) and graphics clock (MHz,
+    `rocm-smi --showclocks`) about every 0.25 s until stop is set."""
+    import re
+    while not stop.is_set():
+        t, pw, clk = time.time(), None, None
+        try:
+            r = subprocess.run(["amd-smi", "metric", "--power", "--csv"], capture_output=True, text=True, timeout=5)
+            rows = [l.split(",") for l in r.stdout.splitlines() if "," in l]
+            if len(rows) >= 2:
+                col = rows[0].index("socket_power") if "socket_power" in rows[0] else 1
+                pw = float(rows[1][col])
+        except (OSError, ValueError, IndexError, subprocess.SubprocessError):
+            pass
+        try:
+            r = subprocess.run(["rocm-smi", "--showclocks"], capture_output=True, text=True, timeout=5)
+            m = re.search(r"sclk clock level: \S+: \((\d+)Mhz", r.stdout)
+            clk = int(m.group(1)) if m else None
+        except (OSError, subprocess.SubprocessError):
+            pass
+        out.append((t, pw, clk))
+        stop.wait(0.2)
+
+
+def bench_power(a, seconds: float = 4.0) -> dict:
+    """Board power and graphics clock while the device runs, back to back for
+    a few seconds each: ideal 4096^2 steps (the two fused explicit stages and
+    the CFL partials only), the resistive steps of the headline workload,
+    and a streaming copy of 1 GiB (`mhd_calibration_mix`, one array in, one
+    out)."""
+    import threading
+    from magnetohydrodynamics_simulator_amd import Context, setups
+
+    n = a.n
+    dx = setups.grid_spacing(n)
+    ic = setups.orszag_tang(n)
+    samples, marks = [], {}
+    stop = threading.Event()
+    th = threading.Thread(target=smi_sample, args=(stop, samples), daemon=True)
+    th.start()
+    try:
+        for eta, name in ((0.0, "ideal_steps"), (a.eta, "resistive_steps")):
+            with Context(n, n, dx, dx, eta, cfl=a.cfl, cg_tol=a.cg_tol) as c:
+                c.upload(*[ic[k] for k in NAMES])
+                c.run_resident(1e9, max_steps=5)
+                c.synchronize()
+                t0, steps = time.time(), 0
+                while time.time() - t0 < seconds:
+                    steps += c.run_resident(1e9, max_steps=40)[0]
+                    c.synchronize()
+                marks[name] = (t0, time.time(), steps)
+        with Context(64, 64, 0.1, 0.1, 0.0) as c:
+            t0, k = time.time(), 0
+            while time.time() - t0 < seconds:   # a streaming copy of 1 GiB, 200 launches per call
+                c.calibration_mix(1, 1, 1 << 27, reps=200)
+                k += 200
+            marks["stream_copy_1GiB"] = (t0, time.time(), k)
+    finally:
+        stop.set()
+        th.join(timeout=10)
+    out = {}
+    for name, (t0, t1, cnt) in marks.items():
+        sel = [(p, c) for t, p, c in samples if t0 + 0.5 <= t <= t1]
+        pw = [p for p, _ in sel if p is not None]
+        ck = [c for _, c in sel if c is not None]
+        out[name] = {"samples": len(sel), "socket_power_W": round(float(np.median(pw)), 1) if pw else None,
+                     "sclk_MHz": round(float(np.median(ck))) if ck else None, "units_done": cnt}
+    out["note"] = ("medians of amd-smi socket power and rocm-smi graphics clock sampled every ~0.25 s while each "
+                   "load runs; MI355X's board power limit is 1400 W and its peak graphics clock 2400 MHz")
+    return out
 
 
 def bench_batch(a) -> dict:
@@ -750,6 +824,9 @@ def main():
     if a.only == "badjoint":
         print(json.dumps({"batched_adjoint": bench_batch_adjoint(a)}), flush=True)
         return
+    if a.only == "power":
+        print(json.dumps({"power_and_clock": bench_power(a, a.power_seconds)}), flush=True)
+        return
     from magnetohydrodynamics_simulator_amd import Context, setups
 
     n = a.n
@@ -1053,6 +1130,8 @@ def main():
     if not a.no_cpu:
         import shutil
         shutil.rmtree(tmp_dir, ignore_errors=True)
+    if a.power_seconds > 0:                      # last: its full-power phases must not warm the timed legs
+        line["power_and_clock"] = bench_power(a, a.power_seconds)
     print(json.dumps(line), flush=True)
 
 
