@@ -14,7 +14,11 @@ Prints one JSON line: Mcell-updates/s (nx * ny * steps / s / 1e6) with
   cpu_baseline  the CPU restatement of the reference step (oracle/, built with
                 the reference's -O3 -ffast-math and -march=native on this host),
                 single-threaded like the reference, on a bounded sample of the
-                same workload.
+                same workload;
+  power_and_clock  board power and graphics clock while ideal steps (the
+                stage kernels), the resistive steps and a streaming copy run,
+                measured last so that its full-power phases do not warm the
+                timed legs.
 """
 from __future__ import annotations
 
