@@ -231,6 +231,38 @@ def test_group_modes_match_step_path():
                 assert np.array_equal(interior(out[r][k]), interior(ref[r][k])), (mode, r, k)
 
 
+def test_group_modes_with_per_run_step_limits():
+    """Per-run step limits and no limit on the call: a group launches no more
+    steps than the longest remaining limit (direct launches and captured
+    pairs), or its full size (one graph per group); every mode equals the
+    host-checked step path, and each run stops at its own limit."""
+    tol, cfl = 1e-10, 0.4
+    states = _states()
+    limits = [5, 9, 3, 7, 1]
+
+    def run(mode):
+        with _batch(states, tol, cfl) as b:
+            b.tune(**mode)
+            b.set_schedule(max_steps=limits)
+            done = b.run()
+            again = b.run()                      # every run has stopped: nothing more happens
+            return done, again, [b.download(r) for r in range(len(states))], [b.run_info(r) for r in range(len(states))]
+
+    _say("batch per-run step limits: host-checked steps (reference)")
+    ref_done, ref_again, ref, ref_info = run(dict(group_steps=0))
+    assert ref_done == max(limits) and ref_again == 0
+    assert [i[0] for i in ref_info] == limits
+    for mode in (dict(group_steps=16), dict(group_steps=4, graph=True), dict(group_steps=16, graph=2),
+                 dict(group_steps=2, fused_cfl=False)):
+        _say(f"batch per-run step limits, mode {mode}")
+        done, again, out, info = run(mode)
+        assert (done, again) == (ref_done, ref_again), (mode, done, again)
+        for r in range(len(states)):
+            assert info[r][:3] == ref_info[r][:3] and list(info[r][3]) == list(ref_info[r][3]), (mode, r, info[r])
+            for k in CONSERVED:
+                assert np.array_equal(out[r][k], ref[r][k]), (mode, r, k)
+
+
 def test_autotune_chooses_fusion_and_launch_mode():
     """The autotuner also picks the CFL fusion and graph replay of the
     device-resident groups; whatever it picks, the run equals the untuned
