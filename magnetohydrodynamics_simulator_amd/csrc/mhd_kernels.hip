@@ -895,6 +895,27 @@ __device__ __forceinline__ Fin2Loads fin2_load(const FinArgs& a, bool xv, int co
   return v;
 }
 
+// The loads of the row after the one prev was loaded for: x there is prev's
+// x of the row above (the row sequence of finalize2_body advances by one,
+// wrapped or clamped as bc_wrap does), so it is not loaded again.
+template <bool OPEN>
+__device__ __forceinline__ Fin2Loads fin2_load_after(const FinArgs& a, bool xv, int col, int row,
+                                                     const Fin2Loads& prev) {
+  const int nx = a.nx, ny = a.ny;
+  const int cm = bc_wrap(col - 1, nx, (OPEN ? a.open_x : 0));
+  const int cp = bc_wrap(col + 1, nx, (OPEN ? a.open_x : 0));
+  const int rp = bc_wrap(row + 1, ny, (OPEN ? a.open_y : 0));
+  Fin2Loads v;
+  v.xc = prev.xn;
+  v.xn = xv ? a.x[rp * nx + col] : 0.0;
+  v.xe = xv ? a.x[row * nx + cp] : 0.0;
+  v.xw = xv ? a.x[row * nx + cm] : 0.0;
+  v.bxs = a.bx[row * nx + col];
+  v.bys = a.by[row * nx + col];
+  v.bysw = a.by[row * nx + cm];
+  return v;
+}
+
 // J3(col, row) from its loads, given chain.bx3_below = bx3(col, row - 1);
 // advances the chain.
 template <bool OPEN>
@@ -1006,8 +1027,8 @@ __device__ __forceinline__ void finalize2_body(const FinArgs& a, int vb, const D
     }
     const Fin2Loads cur = nxt, hcur = hnxt;
     if (j + 1 < y_end) {
-      nxt = fin2_load<OPEN>(a, xv, t.ic, above(j + 1));
-      if (edge) hnxt = fin2_load<OPEN>(a, xv, xr, above(j + 1));
+      nxt = fin2_load_after<OPEN>(a, xv, t.ic, above(j + 1), cur);
+      if (edge) hnxt = fin2_load_after<OPEN>(a, xv, xr, above(j + 1), hcur);
     }
     const int buf = (j - t.y0) & 1;
     const int jw = wrapped(j);
