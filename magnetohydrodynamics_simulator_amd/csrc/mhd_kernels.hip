@@ -554,21 +554,30 @@ __device__ __forceinline__ void cg_start_body(const CgArgs& a, const double* __r
     b_last = v;
     return v;
   };
-  auto halo_b = [&](int col, int j) -> double {
+  // b of a halo column (rows visited in order from the tile's first), with bx
+  // of the row below carried in cb as for the own column
+  auto halo_b = [&](int col, int j, double& cb) -> double {
     const int r = wrap_row(j, ny, oy);
-    return FROM_B ? rhs_at<OPEN>(a, col, r) : bin[r * nx + col];
+    if (!FROM_B) return bin[r * nx + col];
+    const int cm = bc_wrap(col - 1, nx, (OPEN ? a.open_x : 0));
+    const double bxc = a.bx[r * nx + col];
+    const double v = rhs_from(a, a.by[r * nx + col], a.by[r * nx + cm], bxc, cb);
+    cb = bxc;
+    return v;
   };
+  double cbl = (FROM_B && left) ? a.bx[wrap_row(t.y0 - 1, ny, oy) * nx + xl] : 0.0;
+  double cbr = (FROM_B && right) ? a.bx[wrap_row(t.y0 - 1, ny, oy) * nx + xr] : 0.0;
   double acc[3] = {0.0, 0.0, 0.0};
   double bm = own_b(t.y0 - 1);
   double bc = own_b(t.y0);
   double bn = own_b(t.y0 + 1);
-  double hl = left ? halo_b(xl, t.y0) : 0.0;
-  double hr = right ? halo_b(xr, t.y0) : 0.0;
+  double hl = left ? halo_b(xl, t.y0, cbl) : 0.0;
+  double hr = right ? halo_b(xr, t.y0, cbr) : 0.0;
   for (int j = t.y0; j < t.y1; ++j) {
     // loads for the next row
     const double bnn = (j + 1 < t.y1) ? own_b(j + 2) : 0.0;
-    const double hl_n = (left && j + 1 < t.y1) ? halo_b(xl, j + 1) : 0.0;
-    const double hr_n = (right && j + 1 < t.y1) ? halo_b(xr, j + 1) : 0.0;
+    const double hl_n = (left && j + 1 < t.y1) ? halo_b(xl, j + 1, cbl) : 0.0;
+    const double hr_n = (right && j + 1 < t.y1) ? halo_b(xr, j + 1, cbr) : 0.0;
     const int buf = (j - t.y0) & 1;
     const double pc = invd_of<OPEN>(a, t.ic, j) * bc;
     if (t.valid) prow[buf][tid + 1] = pc;
